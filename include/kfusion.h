@@ -1,0 +1,166 @@
+/*
+ * kfusion.h -- C ABI of the MI355X (gfx950) KinectFusion frame loop.
+ *
+ * Drop-in boundary for the hot path of philipNoonan/oglFusionKinectV2: the
+ * GL textures / SSBOs owned by `class gFusion` (include/gFusion.h:124-276)
+ * become device buffers owned by an opaque session; every GLSL dispatch of the
+ * frame loop becomes a hand-written HIP kernel launched on the session's
+ * stream.  Conventions:
+ *   - every entry point returns an int status (KF_OK = 0, negative = error);
+ *     kf_last_error() gives a message for the calling thread;
+ *   - pointers are HOST pointers unless the name ends in _dev;
+ *   - matrices are 16 floats, column-major (glm layout: m[c*4 + r]);
+ *   - "pose" is camera->world (gFusion::m_pose);
+ *   - a session is driven by one thread at a time; sessions are independent;
+ *   - no exceptions cross the ABI.
+ */
+#ifndef KFUSION_H
+#define KFUSION_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define KF_OK 0
+#define KF_ERR_INVALID (-1) /* bad argument or configuration              */
+#define KF_ERR_HIP (-2)     /* a HIP runtime call failed                   */
+#define KF_ERR_OOM (-3)     /* device allocation failed                    */
+#define KF_ERR_BUDGET (-4)  /* admission refused: HBM budget exceeded      */
+#define KF_ERR_STATE (-5)   /* call not valid in the current session state */
+
+/* ---- TSDF storage (voxel = value + weight, stored as two planes) ---------- */
+#define KF_STORAGE_RG16I 0 /* int16 value*32766 + int16 weight (reference RG16I, 4 B) */
+#define KF_STORAGE_F32 1   /* fp32 value + fp32 weight (8 B)                            */
+#define KF_STORAGE_F16 2   /* fp16 value + fp16 weight (4 B), fp32 update arithmetic   */
+
+/* ---- combine rule for the 8x32 ICP partials (gFusion.cpp:1143-1149) ------- */
+#define KF_RULE_LAST_ROW 0 /* reference behaviour: only work group 7 survives */
+#define KF_RULE_SUM_ALL 1  /* sum the 8 rows                                 */
+
+/* ---- map selectors for kf_read_map / kf_write_map ------------------------- */
+#define KF_MAP_DEPTH 0      /* float mm, level l                    */
+#define KF_MAP_VERTEX 1     /* float4 camera-space vertex, level l  */
+#define KF_MAP_NORMAL 2     /* float4 normal (or (2,0,0,0)), level l */
+#define KF_MAP_REF_VERTEX 3 /* float4 raycast world vertex, level 0  */
+#define KF_MAP_REF_NORMAL 4 /* float4 raycast world normal, level 0  */
+
+/* ---- integrate gating ------------------------------------------------------ */
+#define KF_INTEGRATE_ALWAYS 0 /* gFusion::integrate() */
+#define KF_INTEGRATE_GATED 1  /* only if the last kf_frame/kf_track_frame decided so (device flag) */
+
+/* gFusionConfig (include/gFusion.h:42-82) plus the camera and depth size. */
+typedef struct kf_config {
+    int volume_size[3];     /* voxels                        (volumeSize)       */
+    float volume_dim[3];    /* metres                        (volumeDimensions) */
+    float mu;               /* truncation distance, metres                      */
+    float max_weight;
+    float near_plane, far_plane;
+    float dist_threshold, normal_threshold;
+    int iterations[3];      /* ICP iterations per level, 0 = finest             */
+    float cam[4];           /* fx, fy, cx, cy of the depth camera (camPamsDepth) */
+    int width, height;      /* depth image size (512 x 424)                     */
+    int storage;            /* KF_STORAGE_*                                     */
+    int reduce_rule;        /* KF_RULE_*                                        */
+} kf_config;
+
+typedef struct kf_session kf_session;
+
+/* Fills the defaults of gFusionConfig() (include/gFusion.h:55-70), the Kinect
+ * v2 intrinsics used by the synthetic sequences and a 512x424 depth image. */
+int kf_config_default(kf_config *cfg);
+
+/* Device bytes a session with this configuration holds (admission control). */
+int kf_session_bytes(const kf_config *cfg, size_t *bytes);
+
+/* Creates a session: allocates the device buffers that initTextures /
+ * initVolume / allocateBuffers create (gFusion.cpp:200-330).  `stream` is a
+ * hipStream_t (NULL: the session creates its own).  The volume is zeroed and
+ * the pose set to identity. */
+int kf_session_create(const kf_config *cfg, void *stream, kf_session **out);
+int kf_session_destroy(kf_session *s);
+int kf_get_config(const kf_session *s, kf_config *cfg);
+
+/* Reset(pose, false) + raycast(), as the UI does (gFusion.cpp:226-287,
+ * kinect.cpp:446-477): zero the volume, set the pose, raycast the empty
+ * volume, and arm the 4-frame integrate window of the frame loop. */
+int kf_reset(kf_session *s, const float pose[16]);
+int kf_reset_volume(kf_session *s);              /* resetVolume (gFusion.cpp:243) */
+int kf_set_pose(kf_session *s, const float pose[16]); /* setPose (gFusion.h:184)  */
+int kf_get_pose(kf_session *s, float pose[16]);  /* getPose (gFusion.h:180), syncs */
+
+/* depthToVertex(float*) (gFusion.cpp:496): upload one 512x424 depth frame in
+ * millimetres.  _f32 takes the reference's float mm array, _u16 the sensor's
+ * uint16 mm, _dev variants read a device buffer already in HBM.  The upload
+ * also builds the pyramid (mip + vertex + normal maps, gFusion.cpp:496-547). */
+int kf_depth_to_vertex_f32(kf_session *s, const float *depth_mm);
+int kf_depth_to_vertex_u16(kf_session *s, const uint16_t *depth_mm);
+int kf_depth_to_vertex_u16_dev(kf_session *s, const uint16_t *depth_mm_dev);
+int kf_depth_to_vertex_f32_dev(kf_session *s, const float *depth_mm_dev);
+
+/* Track() (gFusion.cpp:648-740): projective point-to-plane ICP over the 3
+ * levels, coarse to fine, entirely on the device (no host round trip per
+ * iteration).  If `tracked` is non-NULL the call synchronises and returns the
+ * accept/reject decision (0 < AE <= 0.5); the pose is restored on reject. */
+int kf_track(kf_session *s, int *tracked);
+
+/* Track() as the frame loop runs it: the ICP loop, the accept/reject step,
+ * then the frame-loop gate of kinect.cpp:677-691 (integrate if tracked or
+ * within the reset window; frame counter) -- all on the device, no sync. */
+int kf_track_frame(kf_session *s);
+
+/* One ICP iteration at `level` (track(level) + reduce(level) + getReduction +
+ * solve + pose update, gFusion.cpp:669-708).  `first` = 1 starts a level
+ * (clears the early-exit flag); `save_pose` = 1 records the pose to restore. */
+int kf_icp_iteration(kf_session *s, int level, int first, int save_pose);
+/* The accept/reject step of Track() after the last iteration (gFusion.cpp:715-723). */
+int kf_icp_finish(kf_session *s);
+
+/* integrate() (gFusion.cpp:1234-1287).  mode = KF_INTEGRATE_ALWAYS or _GATED. */
+int kf_integrate(kf_session *s, int mode);
+/* raycast() (gFusion.cpp:1289-1366): refreshes the reference vertex/normal maps. */
+int kf_raycast(kf_session *s);
+
+/* The frame block of kinect.cpp:654-695 with point-to-plane tracking and
+ * integration on: pyramid, Track, integrate if tracked or within the reset
+ * window, raycast.  No host synchronisation; the tracking result stays on the
+ * device (kf_read_status). */
+int kf_frame_u16_dev(kf_session *s, const uint16_t *depth_mm_dev);
+int kf_frame_f32(kf_session *s, const float *depth_mm);
+
+/* Read-back surface (gFusion.h:180-274).  All synchronise the session stream. */
+int kf_read_map(kf_session *s, int which, int level, void *host, size_t bytes);
+int kf_write_map(kf_session *s, int which, int level, const void *host, size_t bytes);
+/* TSDF planes in the session's storage type, x fastest then y then z.  Either
+ * pointer may be NULL. */
+int kf_read_tsdf(kf_session *s, void *value, void *weight);
+int kf_write_tsdf(kf_session *s, const void *value, const void *weight);
+/* The 8x32 partial sums of the last ICP iteration (reduce.cs output layout),
+ * its alignment energy, and the frame state: tracked flag, ICP iterations run,
+ * frame counter and reset window flag. */
+int kf_read_reduction(kf_session *s, float partials[256], float *alignment_energy);
+int kf_read_status(kf_session *s, int *tracked, int *iterations_run, int *counter, int *reset);
+/* Per-pixel output of the last ICP iteration at `level`
+ * ({int result; float error; float J[6]} of track.cs:18-27, as planes).
+ * Only recorded when enabled with kf_set_debug(s, 1). */
+int kf_set_debug(kf_session *s, int record_track_pixels);
+int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error, float *J);
+
+/* Synchronise the session stream / get its stream (hipStream_t). */
+int kf_synchronize(kf_session *s);
+int kf_get_stream(kf_session *s, void **stream);
+
+/* Algorithmic traffic of integrate for the current depth frame and pose:
+ * voxels read and voxels written under the reference semantics (no timing). */
+int kf_count_integrate_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *voxels_written);
+
+const char *kf_last_error(void);
+const char *kf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KFUSION_H */

@@ -1,0 +1,673 @@
+// kf_capi.hip -- session management and the C ABI declared in include/kfusion.h.
+//
+// A session owns the device buffers that gFusion creates in initTextures /
+// initVolume / allocateBuffers (gFusion.cpp:200-330):
+//   depth0            float  W*H            (R32F depth level 0, mm)
+//   vert[l], norm[l]  float4 (W>>l)*(H>>l)  (RGBA32F vertex / normal pyramids)
+//   refV, refN        float4 W*H            (raycast reference maps)
+//   tsdf, weight      planes of N0*N1*N2 voxels, x fastest (RG16I volume)
+//   state             DevState (pose, ICP partials, frame flags)
+// All work is queued on the session's stream; nothing synchronises except the
+// read-back calls and kf_track with a non-NULL result pointer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "kf_device.hpp"
+#include "kf_internal.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define KF_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(KF_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+size_t voxel_plane_bytes(int storage)
+{
+    switch (storage) {
+    case KF_STORAGE_RG16I: return 2;
+    case KF_STORAGE_F32: return 4;
+    case KF_STORAGE_F16: return 2;
+    }
+    return 0;
+}
+
+int validate(const kf_config *c)
+{
+    if (!c) return fail(KF_ERR_INVALID, "config is NULL");
+    for (int k = 0; k < 3; ++k) {
+        if (c->volume_size[k] < 1 || c->volume_size[k] > 2048)
+            return fail(KF_ERR_INVALID, "volume_size must be in [1, 2048]");
+        if (!(c->volume_dim[k] > 0.0f) || !std::isfinite(c->volume_dim[k]))
+            return fail(KF_ERR_INVALID, "volume_dim must be positive");
+        if (c->iterations[k] < 0 || c->iterations[k] > 64)
+            return fail(KF_ERR_INVALID, "iterations must be in [0, 64]");
+    }
+    if (c->width < 8 || c->height < 8 || c->width % 4 || c->height % 4 || c->width > 8192 || c->height > 8192)
+        return fail(KF_ERR_INVALID, "depth size must be multiples of 4 in [8, 8192]");
+    if (!(c->mu > 0.0f) || !std::isfinite(c->mu)) return fail(KF_ERR_INVALID, "mu must be positive");
+    if (!(c->max_weight >= 1.0f) || c->max_weight > 32767.0f)
+        return fail(KF_ERR_INVALID, "max_weight must be in [1, 32767]");
+    if (!(c->near_plane >= 0.0f) || !(c->far_plane > c->near_plane) || c->far_plane > 1e4f)
+        return fail(KF_ERR_INVALID, "need 0 <= near_plane < far_plane <= 1e4");
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(c->cam[k])) return fail(KF_ERR_INVALID, "camera parameters must be finite");
+    if (!(c->cam[0] > 0.0f) || !(c->cam[1] > 0.0f)) return fail(KF_ERR_INVALID, "fx, fy must be positive");
+    if (c->storage < 0 || c->storage > 2) return fail(KF_ERR_INVALID, "unknown storage");
+    if (c->reduce_rule < 0 || c->reduce_rule > 1) return fail(KF_ERR_INVALID, "unknown reduce_rule");
+    // stepSize() (gFusion.h:72-81) must advance t at every sample of the march
+    const float minxyz = std::min(std::min(c->volume_dim[0], c->volume_dim[1]), c->volume_dim[2]);
+    const float maxxyz = (float)std::max(std::max(c->volume_size[0], c->volume_size[1]), c->volume_size[2]);
+    if (!(minxyz / maxxyz >= 1e-5f)) return fail(KF_ERR_INVALID, "voxel step below 1e-5 m");
+    if (!(0.75f * c->mu >= 1e-5f)) return fail(KF_ERR_INVALID, "0.75*mu below 1e-5 m");
+    return KF_OK;
+}
+
+}  // namespace
+
+struct kf_session {
+    kf_config cfg;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+    float *depth0 = nullptr;
+    uint16_t *stage_u16 = nullptr;
+    float *stage_f32 = nullptr;
+    float4 *vert[3] = {nullptr, nullptr, nullptr};
+    float4 *norm[3] = {nullptr, nullptr, nullptr};
+    float4 *refV = nullptr, *refN = nullptr;
+    void *tsdf = nullptr, *weight = nullptr;
+    kf::DevState *st = nullptr;
+    float *block_partials = nullptr;
+    unsigned long long *counts = nullptr;
+    int debug = 0;
+    int dbg_level = -1;
+    int32_t *dbg_result = nullptr;
+    float *dbg_error = nullptr, *dbg_J = nullptr;
+    size_t voxels = 0;
+};
+
+namespace {
+
+size_t session_bytes(const kf_config *c)
+{
+    const size_t p0 = (size_t)c->width * c->height;
+    const size_t p1 = (size_t)(c->width >> 1) * (c->height >> 1);
+    const size_t p2 = (size_t)(c->width >> 2) * (c->height >> 2);
+    const size_t vox = (size_t)c->volume_size[0] * c->volume_size[1] * c->volume_size[2];
+    size_t b = 0;
+    b += p0 * 4 + p0 * 2 + p0 * 4;                 // depth0 + staging
+    b += 2 * 16 * (p0 + p1 + p2);                  // vertex + normal pyramids
+    b += 2 * 16 * p0;                              // reference maps
+    b += vox * 2 * voxel_plane_bytes(c->storage);  // TSDF value + weight planes
+    b += sizeof(kf::DevState) + (size_t)c->height * 28 * 4 + 64;
+    return b;
+}
+
+void free_all(kf_session *s)
+{
+    auto f = [](void *p) {
+        if (p) (void)hipFree(p);
+    };
+    f(s->depth0); f(s->stage_u16); f(s->stage_f32);
+    for (int l = 0; l < 3; ++l) {
+        f(s->vert[l]);
+        f(s->norm[l]);
+    }
+    f(s->refV); f(s->refN); f(s->tsdf); f(s->weight); f(s->st); f(s->block_partials); f(s->counts);
+    f(s->dbg_result); f(s->dbg_error); f(s->dbg_J);
+    if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+int level_w(const kf_session *s, int l) { return s->cfg.width >> l; }
+int level_h(const kf_session *s, int l) { return s->cfg.height >> l; }
+
+kf::PyramidArgs pyramid_args(kf_session *s)
+{
+    kf::PyramidArgs a{};
+    a.w = s->cfg.width;
+    a.h = s->cfg.height;
+    for (int k = 0; k < 4; ++k) a.cam[k] = s->cfg.cam[k];
+    int start = 0;
+    for (int l = 0; l < 3; ++l) {
+        a.block_start[l] = start;
+        start += (level_w(s, l) * level_h(s, l) + 255) / 256;
+        a.vert[l] = s->vert[l];
+        a.norm[l] = s->norm[l];
+    }
+    a.blocks_total = start;
+    a.depth0 = s->depth0;
+    return a;
+}
+
+kf::IntegrateArgs integrate_args(kf_session *s, int gated)
+{
+    const kf_config &c = s->cfg;
+    kf::IntegrateArgs a{};
+    for (int k = 0; k < 3; ++k) {
+        a.N[k] = c.volume_size[k];
+        a.dim[k] = c.volume_dim[k];
+    }
+    for (int k = 0; k < 4; ++k) a.cam[k] = c.cam[k];
+    a.mu = c.mu;
+    a.max_weight = c.max_weight;
+    a.depth = s->depth0;
+    a.dw = c.width;
+    a.dh = c.height;
+    a.tsdf = s->tsdf;
+    a.weight = s->weight;
+    // enough z segments that the launch holds >= 256K columns-segments
+    const long cols = (long)c.volume_size[0] * c.volume_size[1];
+    int segs = (int)std::min<long>(16, std::max<long>(1, (262144 + cols - 1) / cols));
+    int zlen = (c.volume_size[2] + segs - 1) / segs;
+    zlen = (zlen + 7) / 8 * 8;
+    a.zlen = std::max(zlen, 8);
+    a.gated = gated;
+    return a;
+}
+
+kf::RaycastArgs raycast_args(kf_session *s)
+{
+    const kf_config &c = s->cfg;
+    kf::RaycastArgs a{};
+    for (int k = 0; k < 3; ++k) {
+        a.N[k] = c.volume_size[k];
+        a.dim[k] = c.volume_dim[k];
+    }
+    for (int k = 0; k < 4; ++k) a.cam[k] = c.cam[k];
+    a.near_plane = c.near_plane;
+    a.far_plane = c.far_plane;
+    // stepSize(): include/gFusion.h:72-81 ; largeStep = 0.75 * mu (gFusion.cpp:1307)
+    const float minxyz = std::min(std::min(c.volume_dim[0], c.volume_dim[1]), c.volume_dim[2]);
+    const float maxxyz = std::max(std::max((float)c.volume_size[0], (float)c.volume_size[1]), (float)c.volume_size[2]);
+    a.step = minxyz / maxxyz;
+    a.large_step = 0.75f * c.mu;
+    a.tsdf = s->tsdf;
+    a.w = c.width;
+    a.h = c.height;
+    a.tiles_x = (c.width + 15) / 16;
+    a.tiles_y = (c.height + 15) / 16;
+    a.blocks = (a.tiles_x * a.tiles_y + 7) / 8 * 8;
+    a.refV = s->refV;
+    a.refN = s->refN;
+    return a;
+}
+
+int run_pyramid_from_stage(kf_session *s, const void *dev, int is_u16)
+{
+    kf::PyramidArgs a = pyramid_args(s);
+    hipError_t e = is_u16 ? kf::launch_pyramid_u16(a, (const uint16_t *)dev, s->stream)
+                          : kf::launch_pyramid_f32(a, (const float *)dev, s->stream);
+    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("pyramid launch: ") + hipGetErrorString(e));
+    return KF_OK;
+}
+
+int run_icp(kf_session *s, int level, int first, int save_pose)
+{
+    kf::IcpArgs a{};
+    a.vert = s->vert[level];
+    a.norm = s->norm[level];
+    a.w = level_w(s, level);
+    a.h = level_h(s, level);
+    a.refV = s->refV;
+    a.refN = s->refN;
+    a.rw = s->cfg.width;
+    a.rh = s->cfg.height;
+    for (int k = 0; k < 4; ++k) a.cam[k] = s->cfg.cam[k];
+    a.dist_th = s->cfg.dist_threshold;
+    a.normal_th = s->cfg.normal_threshold;
+    a.first = first;
+    a.save_pose = save_pose;
+    a.rule = s->cfg.reduce_rule;
+    a.block_partials = s->block_partials;
+    if (s->debug) {
+        a.dbg_result = s->dbg_result;
+        a.dbg_error = s->dbg_error;
+        a.dbg_J = s->dbg_J;
+        s->dbg_level = level;
+    }
+    hipError_t e = kf::launch_icp_iteration(a, s->st, s->stream);
+    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("icp launch: ") + hipGetErrorString(e));
+    return KF_OK;
+}
+
+// Track() loop (gFusion.cpp:661-709): levels coarse -> fine, iterations[level]
+// each; the early exit is a device flag, so every launch is queued up front.
+int run_track(kf_session *s, int frame)
+{
+    bool first_any = true;
+    for (int level = 2; level >= 0; --level)
+        for (int it = 0; it < s->cfg.iterations[level]; ++it) {
+            int rc = run_icp(s, level, it == 0, first_any ? 1 : 0);
+            if (rc) return rc;
+            first_any = false;
+        }
+    if (first_any) {
+        // no iteration configured: AE stays 0 -> rejected, pose unchanged
+        hipError_t e = kf::launch_state_set(s->st, nullptr, 0, s->stream);
+        if (e != hipSuccess) return fail(KF_ERR_HIP, hipGetErrorString(e));
+    }
+    hipError_t e = kf::launch_icp_finish(s->st, frame, s->stream);
+    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("finish launch: ") + hipGetErrorString(e));
+    return KF_OK;
+}
+
+int run_integrate(kf_session *s, int gated)
+{
+    kf::IntegrateArgs a = integrate_args(s, gated);
+    hipError_t e = kf::launch_integrate(a, s->cfg.storage, s->st, s->stream);
+    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("integrate launch: ") + hipGetErrorString(e));
+    return KF_OK;
+}
+
+int run_raycast(kf_session *s)
+{
+    kf::RaycastArgs a = raycast_args(s);
+    hipError_t e = kf::launch_raycast(a, s->cfg.storage, s->st, s->stream);
+    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("raycast launch: ") + hipGetErrorString(e));
+    return KF_OK;
+}
+
+int map_ptr(kf_session *s, int which, int level, void **ptr, size_t *bytes)
+{
+    if (level < 0 || level > 2) return fail(KF_ERR_INVALID, "level must be 0..2");
+    const size_t n = (size_t)level_w(s, level) * level_h(s, level);
+    switch (which) {
+    case KF_MAP_DEPTH:
+        if (level != 0) return fail(KF_ERR_INVALID, "only depth level 0 is stored");
+        *ptr = s->depth0; *bytes = n * 4; return KF_OK;
+    case KF_MAP_VERTEX: *ptr = s->vert[level]; *bytes = n * 16; return KF_OK;
+    case KF_MAP_NORMAL: *ptr = s->norm[level]; *bytes = n * 16; return KF_OK;
+    case KF_MAP_REF_VERTEX:
+        if (level != 0) return fail(KF_ERR_INVALID, "reference maps are level 0");
+        *ptr = s->refV; *bytes = n * 16; return KF_OK;
+    case KF_MAP_REF_NORMAL:
+        if (level != 0) return fail(KF_ERR_INVALID, "reference maps are level 0");
+        *ptr = s->refN; *bytes = n * 16; return KF_OK;
+    }
+    return fail(KF_ERR_INVALID, "unknown map");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *kf_last_error(void) { return g_err.c_str(); }
+const char *kf_version(void) { return "kfusion-gfx950 0.1"; }
+
+int kf_config_default(kf_config *c)
+{
+    if (!c) return fail(KF_ERR_INVALID, "config is NULL");
+    std::memset(c, 0, sizeof(*c));
+    for (int k = 0; k < 3; ++k) {
+        c->volume_size[k] = 256;   // gFusion.h:57
+        c->volume_dim[k] = 1.0f;   // gFusion.h:58
+        c->iterations[k] = 5;      // gFusion.h:66-68
+    }
+    c->mu = 0.05f;
+    c->max_weight = 100.0f;
+    c->near_plane = 0.1f;
+    c->far_plane = 3.0f;
+    c->dist_threshold = 0.05f;
+    c->normal_threshold = 0.8f;
+    c->cam[0] = 365.5f;
+    c->cam[1] = 365.5f;
+    c->cam[2] = 255.5f;
+    c->cam[3] = 211.5f;
+    c->width = 512;
+    c->height = 424;
+    c->storage = KF_STORAGE_RG16I;
+    c->reduce_rule = KF_RULE_LAST_ROW;
+    return KF_OK;
+}
+
+int kf_session_bytes(const kf_config *cfg, size_t *bytes)
+{
+    int rc = validate(cfg);
+    if (rc) return rc;
+    if (!bytes) return fail(KF_ERR_INVALID, "bytes is NULL");
+    *bytes = session_bytes(cfg);
+    return KF_OK;
+}
+
+int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
+{
+    int rc = validate(cfg);
+    if (rc) return rc;
+    if (!out) return fail(KF_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    kf_session *s = new (std::nothrow) kf_session();
+    if (!s) return fail(KF_ERR_OOM, "host allocation failed");
+    s->cfg = *cfg;
+    auto cleanup = [&](int code, const std::string &msg) {
+        free_all(s);
+        delete s;
+        return fail(code, msg);
+    };
+    if (hipGetDevice(&s->device) != hipSuccess) return cleanup(KF_ERR_HIP, "no HIP device");
+    if (stream) {
+        s->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(KF_ERR_HIP, "hipStreamCreate failed");
+        s->own_stream = true;
+    }
+    const size_t p0 = (size_t)cfg->width * cfg->height;
+    s->voxels = (size_t)cfg->volume_size[0] * cfg->volume_size[1] * cfg->volume_size[2];
+    const size_t vb = s->voxels * voxel_plane_bytes(cfg->storage);
+    auto alloc = [&](void **p, size_t n) { return hipMalloc(p, n) == hipSuccess; };
+    bool ok = alloc((void **)&s->depth0, p0 * 4) && alloc((void **)&s->stage_u16, p0 * 2) &&
+              alloc((void **)&s->stage_f32, p0 * 4) && alloc((void **)&s->refV, p0 * 16) &&
+              alloc((void **)&s->refN, p0 * 16) && alloc(&s->tsdf, vb) && alloc(&s->weight, vb) &&
+              alloc((void **)&s->st, sizeof(kf::DevState)) &&
+              alloc((void **)&s->block_partials, (size_t)cfg->height * 28 * 4) &&
+              alloc((void **)&s->counts, 2 * sizeof(unsigned long long));
+    for (int l = 0; l < 3 && ok; ++l) {
+        const size_t n = (size_t)(cfg->width >> l) * (cfg->height >> l);
+        ok = alloc((void **)&s->vert[l], n * 16) && alloc((void **)&s->norm[l], n * 16);
+    }
+    if (!ok) return cleanup(KF_ERR_OOM, "hipMalloc failed (session needs " + std::to_string(session_bytes(cfg)) + " B)");
+    // initial contents: zero maps / volume / state, identity pose
+    bool z = hipMemsetAsync(s->tsdf, 0, vb, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->weight, 0, vb, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->refV, 0, p0 * 16, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->refN, 0, p0 * 16, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->depth0, 0, p0 * 4, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->st, 0, sizeof(kf::DevState), s->stream) == hipSuccess;
+    for (int l = 0; l < 3 && z; ++l) {
+        const size_t n = (size_t)(cfg->width >> l) * (cfg->height >> l);
+        z = hipMemsetAsync(s->vert[l], 0, n * 16, s->stream) == hipSuccess &&
+            hipMemsetAsync(s->norm[l], 0, n * 16, s->stream) == hipSuccess;
+    }
+    const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    if (!z || kf::launch_state_set(s->st, I, 0, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return cleanup(KF_ERR_HIP, "session initialisation failed");
+    *out = s;
+    return KF_OK;
+}
+
+int kf_session_destroy(kf_session *s)
+{
+    if (!s) return KF_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return KF_OK;
+}
+
+int kf_get_config(const kf_session *s, kf_config *cfg)
+{
+    if (!s || !cfg) return fail(KF_ERR_INVALID, "NULL argument");
+    *cfg = s->cfg;
+    return KF_OK;
+}
+
+int kf_reset_volume(kf_session *s)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    const size_t vb = s->voxels * voxel_plane_bytes(s->cfg.storage);
+    KF_HIP(hipMemsetAsync(s->tsdf, 0, vb, s->stream));
+    KF_HIP(hipMemsetAsync(s->weight, 0, vb, s->stream));
+    return KF_OK;
+}
+
+int kf_set_pose(kf_session *s, const float pose[16])
+{
+    if (!s || !pose) return fail(KF_ERR_INVALID, "NULL argument");
+    KF_HIP(kf::launch_state_set(s->st, pose, 0, s->stream));
+    return KF_OK;
+}
+
+int kf_reset(kf_session *s, const float pose[16])
+{
+    if (!s || !pose) return fail(KF_ERR_INVALID, "NULL argument");
+    int rc = kf_reset_volume(s);
+    if (rc) return rc;
+    KF_HIP(kf::launch_state_set(s->st, pose, 1, s->stream));
+    return run_raycast(s);
+}
+
+int kf_get_pose(kf_session *s, float pose[16])
+{
+    if (!s || !pose) return fail(KF_ERR_INVALID, "NULL argument");
+    KF_HIP(hipMemcpyAsync(pose, s->st->pose, 64, hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+int kf_depth_to_vertex_f32(kf_session *s, const float *depth_mm)
+{
+    if (!s || !depth_mm) return fail(KF_ERR_INVALID, "NULL argument");
+    const size_t n = (size_t)s->cfg.width * s->cfg.height;
+    KF_HIP(hipMemcpyAsync(s->stage_f32, depth_mm, n * 4, hipMemcpyHostToDevice, s->stream));
+    return run_pyramid_from_stage(s, s->stage_f32, 0);
+}
+
+int kf_depth_to_vertex_u16(kf_session *s, const uint16_t *depth_mm)
+{
+    if (!s || !depth_mm) return fail(KF_ERR_INVALID, "NULL argument");
+    const size_t n = (size_t)s->cfg.width * s->cfg.height;
+    KF_HIP(hipMemcpyAsync(s->stage_u16, depth_mm, n * 2, hipMemcpyHostToDevice, s->stream));
+    return run_pyramid_from_stage(s, s->stage_u16, 1);
+}
+
+int kf_depth_to_vertex_u16_dev(kf_session *s, const uint16_t *depth_mm_dev)
+{
+    if (!s || !depth_mm_dev) return fail(KF_ERR_INVALID, "NULL argument");
+    return run_pyramid_from_stage(s, depth_mm_dev, 1);
+}
+
+int kf_depth_to_vertex_f32_dev(kf_session *s, const float *depth_mm_dev)
+{
+    if (!s || !depth_mm_dev) return fail(KF_ERR_INVALID, "NULL argument");
+    return run_pyramid_from_stage(s, depth_mm_dev, 0);
+}
+
+int kf_icp_iteration(kf_session *s, int level, int first, int save_pose)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    if (level < 0 || level > 2) return fail(KF_ERR_INVALID, "level must be 0..2");
+    return run_icp(s, level, first ? 1 : 0, save_pose ? 1 : 0);
+}
+
+int kf_icp_finish(kf_session *s)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    KF_HIP(kf::launch_icp_finish(s->st, 0, s->stream));
+    return KF_OK;
+}
+
+int kf_track(kf_session *s, int *tracked)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    int rc = run_track(s, 0);
+    if (rc) return rc;
+    if (tracked) {
+        KF_HIP(hipMemcpyAsync(tracked, &s->st->tracked, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        KF_HIP(hipStreamSynchronize(s->stream));
+    }
+    return KF_OK;
+}
+
+int kf_track_frame(kf_session *s)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    return run_track(s, 1);
+}
+
+int kf_integrate(kf_session *s, int mode)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    if (mode != KF_INTEGRATE_ALWAYS && mode != KF_INTEGRATE_GATED) return fail(KF_ERR_INVALID, "unknown mode");
+    return run_integrate(s, mode == KF_INTEGRATE_GATED);
+}
+
+int kf_raycast(kf_session *s)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    return run_raycast(s);
+}
+
+int kf_frame_u16_dev(kf_session *s, const uint16_t *depth_mm_dev)
+{
+    if (!s || !depth_mm_dev) return fail(KF_ERR_INVALID, "NULL argument");
+    int rc = run_pyramid_from_stage(s, depth_mm_dev, 1);
+    if (!rc) rc = run_track(s, 1);
+    if (!rc) rc = run_integrate(s, 1);
+    if (!rc) rc = run_raycast(s);
+    return rc;
+}
+
+int kf_frame_f32(kf_session *s, const float *depth_mm)
+{
+    int rc = kf_depth_to_vertex_f32(s, depth_mm);
+    if (!rc) rc = run_track(s, 1);
+    if (!rc) rc = run_integrate(s, 1);
+    if (!rc) rc = run_raycast(s);
+    return rc;
+}
+
+int kf_read_map(kf_session *s, int which, int level, void *host, size_t bytes)
+{
+    if (!s || !host) return fail(KF_ERR_INVALID, "NULL argument");
+    void *p;
+    size_t n;
+    int rc = map_ptr(s, which, level, &p, &n);
+    if (rc) return rc;
+    if (bytes != n) return fail(KF_ERR_INVALID, "map size is " + std::to_string(n) + " bytes");
+    KF_HIP(hipMemcpyAsync(host, p, n, hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+int kf_write_map(kf_session *s, int which, int level, const void *host, size_t bytes)
+{
+    if (!s || !host) return fail(KF_ERR_INVALID, "NULL argument");
+    void *p;
+    size_t n;
+    int rc = map_ptr(s, which, level, &p, &n);
+    if (rc) return rc;
+    if (bytes != n) return fail(KF_ERR_INVALID, "map size is " + std::to_string(n) + " bytes");
+    KF_HIP(hipMemcpyAsync(p, host, n, hipMemcpyHostToDevice, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+int kf_read_tsdf(kf_session *s, void *value, void *weight)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    const size_t vb = s->voxels * voxel_plane_bytes(s->cfg.storage);
+    if (value) KF_HIP(hipMemcpyAsync(value, s->tsdf, vb, hipMemcpyDeviceToHost, s->stream));
+    if (weight) KF_HIP(hipMemcpyAsync(weight, s->weight, vb, hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+int kf_write_tsdf(kf_session *s, const void *value, const void *weight)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    const size_t vb = s->voxels * voxel_plane_bytes(s->cfg.storage);
+    if (value) KF_HIP(hipMemcpyAsync(s->tsdf, value, vb, hipMemcpyHostToDevice, s->stream));
+    if (weight) KF_HIP(hipMemcpyAsync(s->weight, weight, vb, hipMemcpyHostToDevice, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+int kf_read_reduction(kf_session *s, float partials[256], float *ae)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    if (partials) KF_HIP(hipMemcpyAsync(partials, s->st->partials, 256 * 4, hipMemcpyDeviceToHost, s->stream));
+    if (ae) KF_HIP(hipMemcpyAsync(ae, &s->st->ae, 4, hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+int kf_read_status(kf_session *s, int *tracked, int *iterations_run, int *counter, int *reset)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    kf::DevState h;
+    KF_HIP(hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    if (tracked) *tracked = h.tracked;
+    if (iterations_run) *iterations_run = h.iters_run;
+    if (counter) *counter = h.counter;
+    if (reset) *reset = h.reset;
+    return KF_OK;
+}
+
+int kf_set_debug(kf_session *s, int record)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    if (record && !s->dbg_result) {
+        const size_t p0 = (size_t)s->cfg.width * s->cfg.height;
+        if (hipMalloc((void **)&s->dbg_result, p0 * 4) != hipSuccess ||
+            hipMalloc((void **)&s->dbg_error, p0 * 4) != hipSuccess ||
+            hipMalloc((void **)&s->dbg_J, p0 * 24) != hipSuccess)
+            return fail(KF_ERR_OOM, "debug buffers");
+        KF_HIP(hipMemsetAsync(s->dbg_result, 0, p0 * 4, s->stream));
+        KF_HIP(hipMemsetAsync(s->dbg_error, 0, p0 * 4, s->stream));
+        KF_HIP(hipMemsetAsync(s->dbg_J, 0, p0 * 24, s->stream));
+    }
+    s->debug = record ? 1 : 0;
+    return KF_OK;
+}
+
+int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error, float *J)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    if (!s->dbg_result) return fail(KF_ERR_STATE, "per-pixel recording is off (kf_set_debug)");
+    if (level != s->dbg_level) return fail(KF_ERR_STATE, "last recorded iteration was at another level");
+    const size_t n = (size_t)level_w(s, level) * level_h(s, level);
+    if (result) KF_HIP(hipMemcpyAsync(result, s->dbg_result, n * 4, hipMemcpyDeviceToHost, s->stream));
+    if (error) KF_HIP(hipMemcpyAsync(error, s->dbg_error, n * 4, hipMemcpyDeviceToHost, s->stream));
+    if (J) KF_HIP(hipMemcpyAsync(J, s->dbg_J, n * 24, hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+int kf_synchronize(kf_session *s)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+int kf_get_stream(kf_session *s, void **stream)
+{
+    if (!s || !stream) return fail(KF_ERR_INVALID, "NULL argument");
+    *stream = (void *)s->stream;
+    return KF_OK;
+}
+
+int kf_count_integrate_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *voxels_written)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    KF_HIP(hipMemsetAsync(s->counts, 0, 2 * sizeof(unsigned long long), s->stream));
+    kf::IntegrateArgs a = integrate_args(s, 0);
+    KF_HIP(kf::launch_count_integrate(a, s->st, s->counts, s->stream));
+    unsigned long long h[2];
+    KF_HIP(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    if (voxels_read) *voxels_read = h[0];
+    if (voxels_written) *voxels_written = h[1];
+    return KF_OK;
+}
+
+}  // extern "C"

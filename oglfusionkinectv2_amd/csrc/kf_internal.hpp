@@ -1,0 +1,70 @@
+// kf_internal.hpp -- kernel argument blocks and launchers shared by the
+// kernels (kf_kernels.hip) and the session / C-ABI layer (kf_capi.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/kfusion.h"
+
+namespace kf {
+
+struct DevState;
+
+struct PyramidArgs {
+    int w, h;               // level-0 size
+    float cam[4];           // level-0 fx, fy, cx, cy
+    int block_start[3];     // first block of each level in the 1-D grid
+    int blocks_total;
+    float *depth0;          // level-0 depth (float mm), read by integrate
+    float4 *vert[3];
+    float4 *norm[3];
+};
+
+struct IcpArgs {
+    const float4 *vert, *norm;  // level maps
+    int w, h;                   // level size
+    const float4 *refV, *refN;  // level-0 raycast maps
+    int rw, rh;
+    float cam[4];
+    float dist_th, normal_th;
+    int first, save_pose, rule;
+    float *block_partials;      // [h][28]
+    int32_t *dbg_result;        // optional per-pixel output
+    float *dbg_error, *dbg_J;
+};
+
+struct IntegrateArgs {
+    int N[3];
+    float dim[3];
+    float cam[4];
+    float mu, max_weight;
+    const float *depth;  // level-0 depth, float mm
+    int dw, dh;
+    void *tsdf, *weight; // planes
+    int zlen;            // z voxels per segment (blockIdx.z)
+    int gated;
+};
+
+struct RaycastArgs {
+    int N[3];
+    float dim[3];
+    float cam[4];
+    float near_plane, far_plane, step, large_step;
+    const void *tsdf;
+    int w, h;
+    int tiles_x, tiles_y, blocks;
+    float4 *refV, *refN;
+};
+
+hipError_t launch_pyramid_u16(const PyramidArgs &a, const uint16_t *in, hipStream_t s);
+hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t s);
+hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s);
+hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s);
+hipError_t launch_state_set(DevState *st, const float *pose, int arm_reset, hipStream_t s);
+hipError_t launch_integrate(const IntegrateArgs &a, int storage, const DevState *st, hipStream_t s);
+hipError_t launch_count_integrate(const IntegrateArgs &a, const DevState *st, unsigned long long *counts,
+                                  hipStream_t s);
+hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s);
+
+}  // namespace kf

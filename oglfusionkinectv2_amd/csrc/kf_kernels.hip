@@ -1,0 +1,893 @@
+// kf_kernels.hip -- HIP kernels of the KinectFusion frame loop for gfx950.
+//
+//   k_pyramid        depth (u16 / f32 mm) -> 3-level depth mip, vertex and
+//                    normal maps in one launch      (gFusion.cpp:496-547,
+//                    depthToVert.cs:33-72, vertToNorm.cs:9-57)
+//   k_icp_iteration  fused track + reduce + combine + FP64 solve + pose update
+//                    (gFusion.cpp:669-708, track.cs:47-163, reduce.cs:27-127,
+//                    gFusion.cpp:1127-1168)
+//   k_icp_finish     accept / reject of Track() + frame-loop gate
+//                    (gFusion.cpp:715-723, kinect.cpp:677-691)
+//   k_integrate      z-column TSDF update   (gFusion.cpp:1234-1287, integrate.cs:52-169)
+//   k_raycast        ray march + gradient normals (gFusion.cpp:1289-1366, raycast.cs)
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "kf_device.hpp"
+#include "kf_internal.hpp"
+
+namespace kf {
+
+// ---------------------------------------------------------------------------
+// TSDF storage traits.  The volume is two planes (value, weight), x fastest.
+// ---------------------------------------------------------------------------
+struct StoreRG16I {  // integrate.cs:37-50 (RG16I texture)
+    using V = int16_t;
+    static constexpr bool kScaled = true;
+    __device__ static float value(V v) { return (float)v * 0.00003051944088f; }
+    __device__ static float weight(V v) { return (float)v; }
+    __device__ static float raw(V v) { return (float)v; }
+    __device__ static V from_value(float t) { return (V)(int)(t * 32766.0f); }
+    __device__ static V from_weight(float w) { return (V)(int)w; }
+};
+struct StoreF32 {
+    using V = float;
+    static constexpr bool kScaled = false;
+    __device__ static float value(V v) { return v; }
+    __device__ static float weight(V v) { return v; }
+    __device__ static float raw(V v) { return v; }
+    __device__ static V from_value(float t) { return t; }
+    __device__ static V from_weight(float w) { return w; }
+};
+struct StoreF16 {
+    using V = _Float16;
+    static constexpr bool kScaled = false;
+    __device__ static float value(V v) { return (float)v; }
+    __device__ static float weight(V v) { return (float)v; }
+    __device__ static float raw(V v) { return (float)v; }
+    __device__ static V from_value(float t) { return (V)t; }
+    __device__ static V from_weight(float w) { return (V)w; }
+};
+
+// ---------------------------------------------------------------------------
+// Pyramid
+// ---------------------------------------------------------------------------
+template <typename DT>
+__device__ __forceinline__ float depth0_at(const DT *in, int w, int x, int y)
+{
+    return (float)in[y * w + x];
+}
+
+// depth at level l via the 2x2 box mean of glGenerateMipmap (gFusion.cpp:508)
+template <typename DT>
+__device__ __forceinline__ float depth_at(const DT *in, int w0, int l, int x, int y)
+{
+    if (l == 0) return depth0_at(in, w0, x, y);
+    if (l == 1) {
+        const int X = 2 * x, Y = 2 * y;
+        return ((depth0_at(in, w0, X, Y) + depth0_at(in, w0, X + 1, Y)) +
+                (depth0_at(in, w0, X, Y + 1) + depth0_at(in, w0, X + 1, Y + 1))) * 0.25f;
+    }
+    float q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int X = 4 * x + 2 * (k & 1), Y = 4 * y + 2 * (k >> 1);
+        q[k] = ((depth0_at(in, w0, X, Y) + depth0_at(in, w0, X + 1, Y)) +
+                (depth0_at(in, w0, X, Y + 1) + depth0_at(in, w0, X + 1, Y + 1))) * 0.25f;
+    }
+    return ((q[0] + q[1]) + (q[2] + q[3])) * 0.25f;
+}
+
+// depthToVert.cs:56 -- (d / 1000) * rotate(invK_l, (x, y, 1))
+__device__ __forceinline__ void vertex_of(float d, const float iK[16], int x, int y, float v[3])
+{
+    float r[3];
+    rotate3(iK, (float)x, (float)y, 1.0f, r);
+    const float s = d / 1000.0f;
+    v[0] = s * r[0];
+    v[1] = s * r[1];
+    v[2] = s * r[2];
+}
+
+template <typename DT>
+__global__ __launch_bounds__(256) void k_pyramid(PyramidArgs a, const DT *__restrict__ in)
+{
+    int b = blockIdx.x;
+    int l = 0;
+    if (b >= a.block_start[2]) l = 2;
+    else if (b >= a.block_start[1]) l = 1;
+    const int w = a.w >> l, h = a.h >> l;
+    const int idx = (b - a.block_start[l]) * 256 + threadIdx.x;
+    if (idx >= w * h) return;
+    const int x = idx % w, y = idx / w;
+
+    float cam[4];
+    const float sc = (float)(1 << l);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cam[k] = a.cam[k] / sc;  // camPamsDepth / float(1 << i)
+    float iK[16];
+    inverse_camera_matrix(cam, iK);
+
+    const float dc = depth_at(in, a.w, l, x, y);
+    if (l == 0) a.depth0[idx] = dc;
+    float v[3];
+    vertex_of(dc, iK, x, y, v);
+    a.vert[l][idx] = make_float4(v[0], v[1], v[2], 0.0f);
+
+    // vertToNorm.cs:20-24: left/up wrap to -1 at the border (out of bounds -> 0),
+    // down reads (x-1, y+1)
+    float L[3] = {0, 0, 0}, Rr[3], U[3] = {0, 0, 0}, D[3] = {0, 0, 0};
+    if (x > 0) vertex_of(depth_at(in, a.w, l, x - 1, y), iK, x - 1, y, L);
+    {
+        const int rx = min(x + 1, w - 1);
+        vertex_of(depth_at(in, a.w, l, rx, y), iK, rx, y, Rr);
+    }
+    if (y > 0) vertex_of(depth_at(in, a.w, l, x, y - 1), iK, x, y - 1, U);
+    if (x > 0) {
+        const int dy = min(y + 1, h - 1);
+        vertex_of(depth_at(in, a.w, l, x - 1, dy), iK, x - 1, dy, D);
+    }
+    float4 n;
+    if (L[2] == 0.0f || Rr[2] == 0.0f || U[2] == 0.0f || D[2] == 0.0f) {
+        n = make_float4(2.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+        const float dxv[3] = {Rr[0] - L[0], Rr[1] - L[1], Rr[2] - L[2]};
+        const float dyv[3] = {D[0] - U[0], D[1] - U[1], D[2] - U[2]};
+        const float c0 = dyv[1] * dxv[2] - dyv[2] * dxv[1];
+        const float c1 = dyv[2] * dxv[0] - dyv[0] * dxv[2];
+        const float c2 = dyv[0] * dxv[1] - dyv[1] * dxv[0];
+        const float len = sqrtf(dot3(c0, c1, c2, c0, c1, c2));
+        n = make_float4(c0 / len, c1 / len, c2 / len, 1.0f);
+    }
+    a.norm[l][idx] = n;
+}
+
+// ---------------------------------------------------------------------------
+// ICP: fused track + reduce + solve + update
+// ---------------------------------------------------------------------------
+
+// Minimum-norm least-squares solve of the symmetric 6x6 system (the contract
+// of Eigen::JacobiSVD::solve at gFusion.cpp:696-697): LDL^T in FP64 for the
+// positive-definite case, cyclic Jacobi eigen-decomposition otherwise.
+// Runs on one thread; `ws` (>= 120 doubles) is LDS scratch so that the solve
+// does not inflate the register budget of the kernel that calls it.
+constexpr int kSolveWs = 120;
+__device__ void solve6(const double *Cin, const double *b, double *x, double *ws)
+{
+    double(*A)[6] = (double(*)[6])ws;
+    double(*L)[6] = (double(*)[6])(ws + 36);
+    double(*V)[6] = (double(*)[6])(ws + 72);
+    double *Dg = ws + 108, *y = ws + 114;
+    double dmax = 0.0;
+    for (int i = 0; i < 6; ++i) {
+        for (int j = 0; j < 6; ++j) A[i][j] = 0.5 * (Cin[i * 6 + j] + Cin[j * 6 + i]);
+        dmax = fmax(dmax, fabs(A[i][i]));
+    }
+    // LDL^T
+    bool spd = dmax > 0.0;
+    const double tol = dmax * 1e-12;
+    for (int j = 0; j < 6 && spd; ++j) {
+        double d = A[j][j];
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k] * Dg[k];
+        if (!(d > tol)) {
+            spd = false;
+            break;
+        }
+        Dg[j] = d;
+        L[j][j] = 1.0;
+        for (int i = j + 1; i < 6; ++i) {
+            double s = A[i][j];
+            for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k] * Dg[k];
+            L[i][j] = s / d;
+        }
+    }
+    if (spd) {
+        for (int i = 0; i < 6; ++i) {
+            double s = b[i];
+            for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+            y[i] = s;
+        }
+        for (int i = 0; i < 6; ++i) y[i] /= Dg[i];
+        for (int i = 5; i >= 0; --i) {
+            double s = y[i];
+            for (int k = i + 1; k < 6; ++k) s -= L[k][i] * x[k];
+            x[i] = s;
+        }
+        return;
+    }
+    // Jacobi eigen-decomposition, pseudo-inverse with threshold 6*eps*sigma_max
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < 6; ++i) {
+            diag += A[i][i] * A[i][i];
+            for (int j = i + 1; j < 6; ++j) off += A[i][j] * A[i][j];
+        }
+        if (off == 0.0 || off <= 1e-34 * diag) break;
+        for (int p = 0; p < 5; ++p)
+            for (int q = p + 1; q < 6; ++q) {
+                const double apq = A[p][q];
+                if (apq == 0.0) continue;
+                const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 6; ++k) {
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 6; ++k) {
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 6; ++k) {
+                    const double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - s * vkq;
+                    V[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    double smax = 0.0;
+    for (int i = 0; i < 6; ++i) smax = fmax(smax, fabs(A[i][i]));
+    const double thr = fmax(smax * 6.0 * DBL_EPSILON, DBL_MIN);
+    for (int k = 0; k < 6; ++k) x[k] = 0.0;
+    for (int i = 0; i < 6; ++i) {
+        const double lam = A[i][i];
+        if (fabs(lam) < thr) continue;
+        double proj = 0.0;
+        for (int k = 0; k < 6; ++k) proj += V[k][i] * b[k];
+        proj /= lam;
+        for (int k = 0; k < 6; ++k) x[k] += V[k][i] * proj;
+    }
+}
+
+// glm::eulerAngleXYZ(x3,x4,x5) in double, cast to float, translation x0..2,
+// then m_pose = delta * m_pose in float (gFusion.cpp:700-705)
+__device__ void pose_update(const double x[6], float pose[16])
+{
+    const double ca = cos(x[3]), sa = sin(x[3]);
+    const double cb = cos(x[4]), sb = sin(x[4]);
+    const double cc = cos(x[5]), sc = sin(x[5]);
+    const double Rm[3][3] = {
+        {cb * cc, -cb * sc, sb},
+        {sa * sb * cc + ca * sc, -sa * sb * sc + ca * cc, -sa * cb},
+        {-ca * sb * cc + sa * sc, ca * sb * sc + sa * cc, ca * cb},
+    };
+    float d[16];
+    for (int c = 0; c < 3; ++c) {
+        for (int r = 0; r < 3; ++r) d[c * 4 + r] = (float)Rm[r][c];
+        d[c * 4 + 3] = 0.0f;
+    }
+    d[12] = (float)x[0];
+    d[13] = (float)x[1];
+    d[14] = (float)x[2];
+    d[15] = 1.0f;
+    mul4(d, pose, pose);
+}
+
+// track.cs:47-163 for one pixel.  Returns the result code; error/J valid for 1.
+__device__ __forceinline__ int track_pixel(const IcpArgs &a, const float *T, const float *view, int x, int y,
+                                           float &err, float J[6])
+{
+    const int i = y * a.w + x;
+    const float4 n = a.norm[i];
+    if (n.x == 2.0f) return -1;
+    const float4 v = a.vert[i];
+    const float v4[4] = {v.x, v.y, v.z, 1.0f};
+    float pv[4], pp[4];
+    matvec4(T, v4, pv);
+    matvec4(view, pv, pp);
+    const float px = pp[0] / pp[2] + 0.5f;
+    const float py = pp[1] / pp[2] + 0.5f;
+    if (px < 0.0f || px > (float)(a.rw - 1) || py < 0.0f || py > (float)(a.rh - 1)) return -2;
+    float rn[3] = {0.0f, 0.0f, 0.0f}, rv[3] = {0.0f, 0.0f, 0.0f};
+    // NaN coordinates pass the bounds test; their image load is out of bounds (zero)
+    if (px == px && py == py) {
+        const int ri = (int)py * a.rw + (int)px;
+        const float4 N4 = a.refN[ri];
+        const float4 V4 = a.refV[ri];
+        rn[0] = N4.x; rn[1] = N4.y; rn[2] = N4.z;
+        rv[0] = V4.x; rv[1] = V4.y; rv[2] = V4.z;
+    }
+    if (rn[0] == -2.0f) return -3;
+    const float diff[3] = {rv[0] - pv[0], rv[1] - pv[1], rv[2] - pv[2]};
+    float pn[3];
+    rotate3(T, n.x, n.y, n.z, pn);
+    if (sqrtf(dot3(diff[0], diff[1], diff[2], diff[0], diff[1], diff[2])) > a.dist_th) return -4;
+    if (dot3(pn[0], pn[1], pn[2], rn[0], rn[1], rn[2]) < a.normal_th) return -5;
+    err = dot3(rn[0], rn[1], rn[2], diff[0], diff[1], diff[2]);
+    J[0] = rn[0];
+    J[1] = rn[1];
+    J[2] = rn[2];
+    J[3] = pv[1] * rn[2] - pv[2] * rn[1];
+    J[4] = pv[2] * rn[0] - pv[0] * rn[2];
+    J[5] = pv[0] * rn[1] - pv[1] * rn[0];
+    return 1;
+}
+
+constexpr int kIcpThreads = 256;
+constexpr int kTerms = 28;  // e^2, e*J[6], upper-triangle J J^T [21]
+
+// One block per image row; the block's row group is y mod 8 (the work group
+// of reduce.cs that owns the row).  Per-thread FP32 products, DPP wave sums,
+// block partials in FP32, and the last block to finish sums the partials of
+// each group in FP64, applies the combine rule, solves and updates the pose.
+__global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevState *__restrict__ st)
+{
+    __shared__ float wsum[kIcpThreads / 64][kTerms];
+    __shared__ int last;
+    __shared__ double gsum[8][kTerms];
+    __shared__ double solve_ws[kSolveWs + 36 + 6 + 6];
+
+    if (!a.first && st->level_done) return;  // early exit of this level (uniform)
+
+    const int y = blockIdx.x;
+    float T[16], view[16];
+    {
+        float K[16], ip[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) T[i] = st->pose[i];
+        camera_matrix(a.cam, K);
+        inverse4(T, ip);
+        mul4(K, ip, view);  // getCameraMatrix(camPams) * inverse(m_pose) (gFusion.cpp:747)
+    }
+
+    float acc[kTerms];
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) acc[k] = 0.0f;
+
+    for (int x = threadIdx.x; x < a.w; x += kIcpThreads) {
+        float e = 0.0f, J[6] = {0, 0, 0, 0, 0, 0};
+        const int r = track_pixel(a, T, view, x, y, e, J);
+        if (a.dbg_result) {
+            const int i = y * a.w + x;
+            a.dbg_result[i] = r;
+            if (r == 1) {
+                a.dbg_error[i] = e;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) a.dbg_J[6 * i + k] = J[k];
+            }
+        }
+        if (r == 1) {
+            acc[0] += e * e;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[1 + k] += e * J[k];
+            int q = 7;
+#pragma unroll
+            for (int p = 0; p < 6; ++p)
+#pragma unroll
+                for (int s = p; s < 6; ++s) acc[q++] += J[p] * J[s];
+        }
+    }
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) {
+        const float s = wave_sum(acc[k]);
+        if (lane == 0) wsum[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kTerms) {
+        const int k = threadIdx.x;
+        const float s = (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]);
+        a.block_partials[y * kTerms + k] = s;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned t = atomicAdd(&st->ticket, 1u);
+        last = (t == (unsigned)(a.h - 1));
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+
+    // second pass: FP64 sums of the block partials per row group
+    if (threadIdx.x < 8 * kTerms) {
+        const int g = threadIdx.x / kTerms, k = threadIdx.x % kTerms;
+        double s = 0.0;
+        for (int row = g; row < a.h; row += 8)
+            s += (double)__hip_atomic_load(&a.block_partials[row * kTerms + k], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+        gsum[g][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+
+    // counts: reduce.cs counts every pixel of the group's rows once (:47-62, :101)
+    double cnt[8];
+    for (int g = 0; g < 8; ++g) {
+        const int rows = (a.h > g) ? (a.h - g + 7) / 8 : 0;
+        cnt[g] = (double)rows * (double)a.w;
+    }
+    for (int g = 0; g < 8; ++g) {
+        for (int k = 0; k < kTerms; ++k) st->partials[g * 32 + k] = (float)gsum[g][k];
+        st->partials[g * 32 + 28] = (float)cnt[g];
+        st->partials[g * 32 + 29] = 0.0f;
+        st->partials[g * 32 + 30] = 0.0f;
+        st->partials[g * 32 + 31] = 0.0f;
+    }
+    double *v = solve_ws + kSolveWs;  // 28 combined terms (LDS)
+    double *C = v;                     // reused after b is extracted: 36
+    double *b = solve_ws + kSolveWs + 36;
+    double *x = b + 6;
+    double count;
+    if (a.rule == 1) {  // sum all rows
+        count = 0.0;
+        for (int k = 0; k < kTerms; ++k) v[k] = 0.0;
+        for (int g = 0; g < 8; ++g) {
+            for (int k = 0; k < kTerms; ++k) v[k] += gsum[g][k];
+            count += cnt[g];
+        }
+    } else {  // reference: rows 1..7 assigned over row 0, only row 7 survives
+        for (int k = 0; k < kTerms; ++k) v[k] = gsum[7][k];
+        count = cnt[7];
+    }
+    const float ae = sqrtf((float)v[0] / (float)count);  // gFusion.cpp:1167
+    for (int k = 0; k < 6; ++k) b[k] = v[1 + k];
+    {
+        double jtj[21];
+        for (int k = 0; k < 21; ++k) jtj[k] = v[7 + k];
+        int q = 0;
+        for (int p = 0; p < 6; ++p)
+            for (int s2 = p; s2 < 6; ++s2) {
+                C[p * 6 + s2] = jtj[q];
+                C[s2 * 6 + p] = jtj[q];
+                ++q;
+            }
+    }
+    solve6(C, b, x, solve_ws);
+    if (a.save_pose)
+        for (int i = 0; i < 16; ++i) st->old_pose[i] = st->pose[i];
+    float P[16];
+    for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
+    pose_update(x, P);
+    for (int i = 0; i < 16; ++i) st->pose[i] = P[i];
+    double nrm = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        nrm += x[k] * x[k];
+        st->x[k] = x[k];
+    }
+    nrm = sqrt(nrm);
+    st->ae = ae;
+    st->level_done = (nrm < 1e-5 && nrm != 0.0) ? 1 : 0;
+    st->iters_run = (a.save_pose ? 0 : st->iters_run) + 1;
+    st->ticket = 0u;
+}
+
+// Track() accept / reject (gFusion.cpp:715-723); with `frame` the frame-loop
+// gate of kinect.cpp:677-691: integrate if tracked or within the reset window.
+__global__ void k_icp_finish(DevState *st, int frame)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const float ae = st->ae;
+    int tracked;
+    if (ae > 0.5f || ae == 0.0f) {
+        for (int i = 0; i < 16; ++i) st->pose[i] = st->old_pose[i];
+        tracked = 0;
+    } else {
+        tracked = 1;
+    }
+    st->tracked = tracked;
+    if (frame) {
+        const int go = tracked || st->reset;
+        st->integrate_now = go;
+        if (go && st->counter > 2) st->reset = 0;
+        st->counter += 1;
+    }
+}
+
+__global__ void k_state_set(DevState *st, Mat16 pose, int set_pose, int arm_reset)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (set_pose)
+        for (int i = 0; i < 16; ++i) st->pose[i] = pose.m[i];
+    if (arm_reset) {
+        st->counter = 0;
+        st->reset = 1;
+        st->integrate_now = 0;
+    }
+    st->ticket = 0u;
+    st->level_done = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Integrate
+// ---------------------------------------------------------------------------
+constexpr int kIntU = 8;  // z voxels per pipelined chunk
+
+struct ColumnSetup {
+    float pos[3], cx[3], dl[3], cd[3];
+};
+
+__device__ __forceinline__ void column_setup(const IntegrateArgs &a, const float *pose, int x, int y, ColumnSetup &c)
+{
+    float inv[16], K[16];
+    inverse4(pose, inv);  // inverse(m_pose) (gFusion.cpp:1242)
+    camera_matrix(a.cam, K);
+    const float vz = a.dim[2] / (float)a.N[2];
+    // getVolumePosition(pix) with pix.z = 0 (integrate.cs:16-19, 57)
+    const float c0 = ((float)x + 0.5f) * a.dim[0] / (float)a.N[0];
+    const float c1 = ((float)y + 0.5f) * a.dim[1] / (float)a.N[1];
+    const float c2 = (0.0f + 0.5f) * a.dim[2] / (float)a.N[2];
+    opmul3(inv, c0, c1, c2, c.pos);
+    opmul3(K, c.pos[0], c.pos[1], c.pos[2], c.cx);
+    rotate3(inv, 0.0f, 0.0f, vz, c.dl);
+    rotate3(K, c.dl[0], c.dl[1], c.dl[2], c.cd);
+}
+
+// One thread per (x, y) z-column, 64 consecutive x per wavefront (coalesced
+// plane accesses).  blockIdx.z splits the column into segments; a segment
+// replays the reference's incremental stepping up to its start (ALU only) so
+// every voxel sees exactly the positions of integrate.cs:62.  Each chunk of
+// kIntU voxels first resolves projection + depth (loads batched), then issues
+// the voxel loads it needs, then the updates and stores.
+template <class S>
+__global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevState *__restrict__ st)
+{
+    if (a.gated && !st->integrate_now) return;
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= a.N[0] || y >= a.N[1]) return;
+
+    ColumnSetup c;
+    column_setup(a, st->pose, x, y, c);
+    const int z0 = blockIdx.z * a.zlen;
+    const int z1 = min(z0 + a.zlen, a.N[2]);
+    for (int z = 0; z < z0; ++z) {
+        c.pos[0] += c.dl[0]; c.pos[1] += c.dl[1]; c.pos[2] += c.dl[2];
+        c.cx[0] += c.cd[0]; c.cx[1] += c.cd[1]; c.cx[2] += c.cd[2];
+    }
+    typename S::V *__restrict__ tv = (typename S::V *)a.tsdf;
+    typename S::V *__restrict__ wv = (typename S::V *)a.weight;
+    const size_t plane = (size_t)a.N[0] * a.N[1];
+    const size_t col = (size_t)y * a.N[0] + x;
+    const float dmax_x = (float)(a.dw - 1), dmax_y = (float)(a.dh - 1);
+
+    for (int zb = z0; zb < z1; zb += kIntU) {
+        int pix[kIntU];
+        float cz[kIntU], fac[kIntU];
+#pragma unroll
+        for (int k = 0; k < kIntU; ++k) {
+            bool ok = (zb + k) < z1;
+            ok = ok && !(c.pos[2] < 0.0001f);
+            const float px = c.cx[0] / c.cx[2] + 0.5f;
+            const float py = c.cx[1] / c.cx[2] + 0.5f;
+            ok = ok && !(px < 0.0f || px > dmax_x || py < 0.0f || py > dmax_y);
+            const int ix = (int)(ok ? px : 0.0f), iy = (int)(ok ? py : 0.0f);
+            pix[k] = ok ? iy * a.dw + ix : -1;
+            cz[k] = c.cx[2];
+            const float qx = c.pos[0] / c.pos[2], qy = c.pos[1] / c.pos[2];
+            fac[k] = sqrtf((1.0f + qx * qx) + qy * qy);
+            c.pos[0] += c.dl[0]; c.pos[1] += c.dl[1]; c.pos[2] += c.dl[2];
+            c.cx[0] += c.cd[0]; c.cx[1] += c.cd[1]; c.cx[2] += c.cd[2];
+        }
+        float dep[kIntU];
+#pragma unroll
+        for (int k = 0; k < kIntU; ++k) dep[k] = a.depth[pix[k] < 0 ? 0 : pix[k]];
+        int kind[kIntU];
+        float sdf[kIntU];
+#pragma unroll
+        for (int k = 0; k < kIntU; ++k) {
+            const float d = dep[k] / 1000.0f;
+            const float diff = (d - cz[k]) * fac[k];
+            const bool live = pix[k] >= 0 && !(d == 0.0f);
+            kind[k] = !live ? 0 : (diff > -a.mu ? 2 : 1);
+            sdf[k] = gmin(1.0f, diff / a.mu);
+        }
+        typename S::V ot[kIntU], ow[kIntU];
+#pragma unroll
+        for (int k = 0; k < kIntU; ++k) {
+            if (kind[k] == 2) {
+                const size_t vi = col + (size_t)(zb + k) * plane;
+                ot[k] = tv[vi];
+                ow[k] = wv[vi];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kIntU; ++k) {
+            if (kind[k] == 0) continue;
+            const size_t vi = col + (size_t)(zb + k) * plane;
+            float t = 0.0f, w = 0.0f;
+            if (kind[k] == 2) {
+                const float t0 = S::value(ot[k]), w0 = S::weight(ow[k]);
+                const float wd = (w0 * t0 + sdf[k]) / (w0 + 1.0f);
+                if (wd < 0.2f) {
+                    t = gmin(gmax(wd, -0.1f), 0.2f);
+                    w = gmin(w0 + 1.0f, a.max_weight);
+                }
+            }
+            tv[vi] = S::from_value(t);
+            wv[vi] = S::from_weight(w);
+        }
+    }
+}
+
+// Same walk as k_integrate without memory traffic on the volume: counts the
+// voxels the reference semantics read (diff > -mu) and write (any live voxel).
+__global__ __launch_bounds__(256) void k_count_integrate(IntegrateArgs a, const DevState *__restrict__ st,
+                                                        unsigned long long *counts)
+{
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= a.N[0] || y >= a.N[1]) return;
+    ColumnSetup c;
+    column_setup(a, st->pose, x, y, c);
+    unsigned long long nr = 0, nw = 0;
+    const float dmax_x = (float)(a.dw - 1), dmax_y = (float)(a.dh - 1);
+    for (int z = 0; z < a.N[2]; ++z) {
+        bool ok = !(c.pos[2] < 0.0001f);
+        const float px = c.cx[0] / c.cx[2] + 0.5f;
+        const float py = c.cx[1] / c.cx[2] + 0.5f;
+        ok = ok && !(px < 0.0f || px > dmax_x || py < 0.0f || py > dmax_y);
+        if (ok) {
+            const float d = a.depth[(int)py * a.dw + (int)px] / 1000.0f;
+            if (!(d == 0.0f)) {
+                const float qx = c.pos[0] / c.pos[2], qy = c.pos[1] / c.pos[2];
+                const float diff = (d - c.cx[2]) * sqrtf((1.0f + qx * qx) + qy * qy);
+                nw += 1;
+                if (diff > -a.mu) nr += 1;
+            }
+        }
+        c.pos[0] += c.dl[0]; c.pos[1] += c.dl[1]; c.pos[2] += c.dl[2];
+        c.cx[0] += c.cd[0]; c.cx[1] += c.cd[1]; c.cx[2] += c.cd[2];
+    }
+    atomicAdd(&counts[0], nr);
+    atomicAdd(&counts[1], nw);
+}
+
+// ---------------------------------------------------------------------------
+// Raycast
+// ---------------------------------------------------------------------------
+template <class S>
+struct VolView {
+    const typename S::V *__restrict__ v;
+    int N0, N1, N2;
+    size_t plane;
+    __device__ __forceinline__ float raw(int x, int y, int z) const
+    {
+        if ((unsigned)x >= (unsigned)N0 || (unsigned)y >= (unsigned)N1 || (unsigned)z >= (unsigned)N2) return 0.0f;
+        return S::raw(v[(size_t)z * plane + (size_t)y * N0 + x]);
+    }
+};
+
+// float -> int of an ivec3(floor(.)) cast; NaN / huge map out of bounds
+__device__ __forceinline__ int floor_index(float fl)
+{
+    return (fl > -2147483648.0f && fl < 2147483648.0f) ? (int)fl : INT_MIN;
+}
+
+// interpVol: raycast.cs:69-82
+template <class S>
+__device__ __forceinline__ float interp_vol(const VolView<S> &V, const RaycastArgs &a, float p0, float p1, float p2)
+{
+    const float s[3] = {(p0 * (float)a.N[0] / a.dim[0]) - 0.5f, (p1 * (float)a.N[1] / a.dim[1]) - 0.5f,
+                        (p2 * (float)a.N[2] / a.dim[2]) - 0.5f};
+    int lo[3], hi[3];
+    float f[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float fl = floorf(s[k]);
+        const int b = floor_index(fl);
+        f[k] = s[k] - fl;
+        lo[k] = (b == INT_MIN) ? b : max(b, 0);
+        hi[k] = (b == INT_MIN) ? b : min(b + 1, a.N[k] - 1);
+    }
+    const float r =
+        ((V.raw(lo[0], lo[1], lo[2]) * (1 - f[0]) + V.raw(hi[0], lo[1], lo[2]) * f[0]) * (1 - f[1]) +
+         (V.raw(lo[0], hi[1], lo[2]) * (1 - f[0]) + V.raw(hi[0], hi[1], lo[2]) * f[0]) * f[1]) * (1 - f[2]) +
+        ((V.raw(lo[0], lo[1], hi[2]) * (1 - f[0]) + V.raw(hi[0], lo[1], hi[2]) * f[0]) * (1 - f[1]) +
+         (V.raw(lo[0], hi[1], hi[2]) * (1 - f[0]) + V.raw(hi[0], hi[1], hi[2]) * f[0]) * f[1]) * f[2];
+    return S::kScaled ? r * 0.00003051944088f : r;
+}
+
+// getGradient: raycast.cs:229-276
+template <class S>
+__device__ void gradient(const VolView<S> &V, const RaycastArgs &a, const float hit[3], float g[3])
+{
+    int ll[3], lu[3], ul[3], uu[3];
+    float f[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float s = (hit[k] * (float)a.N[k] / a.dim[k]) - 0.5f;
+        const float fl = floorf(s);
+        const int b = floor_index(fl);
+        f[k] = s - fl;
+        if (b == INT_MIN) {
+            ll[k] = lu[k] = ul[k] = uu[k] = b;
+        } else {
+            ll[k] = max(b - 1, 0);
+            lu[k] = max(b, 0);
+            ul[k] = min(b + 1, a.N[k] - 1);
+            uu[k] = min(b + 2, a.N[k] - 1);
+        }
+    }
+    const int *lo = lu, *hi = ul;
+    const float fx = f[0], fy = f[1], fz = f[2];
+#define VR(x_, y_, z_) V.raw(x_, y_, z_)
+    g[0] = (((VR(ul[0], lo[1], lo[2]) - VR(ll[0], lo[1], lo[2])) * (1 - fx) +
+             (VR(uu[0], lo[1], lo[2]) - VR(lu[0], lo[1], lo[2])) * fx) * (1 - fy) +
+            ((VR(ul[0], hi[1], lo[2]) - VR(ll[0], hi[1], lo[2])) * (1 - fx) +
+             (VR(uu[0], hi[1], lo[2]) - VR(lu[0], hi[1], lo[2])) * fx) * fy) * (1 - fz) +
+           (((VR(ul[0], lo[1], hi[2]) - VR(ll[0], lo[1], hi[2])) * (1 - fx) +
+             (VR(uu[0], lo[1], hi[2]) - VR(lu[0], lo[1], hi[2])) * fx) * (1 - fy) +
+            ((VR(ul[0], hi[1], hi[2]) - VR(ll[0], hi[1], hi[2])) * (1 - fx) +
+             (VR(uu[0], hi[1], hi[2]) - VR(lu[0], hi[1], hi[2])) * fx) * fy) * fz;
+    g[1] = (((VR(lo[0], ul[1], lo[2]) - VR(lo[0], ll[1], lo[2])) * (1 - fx) +
+             (VR(hi[0], ul[1], lo[2]) - VR(hi[0], ll[1], lo[2])) * fx) * (1 - fy) +
+            ((VR(lo[0], uu[1], lo[2]) - VR(lo[0], lu[1], lo[2])) * (1 - fx) +
+             (VR(hi[0], uu[1], lo[2]) - VR(hi[0], lu[1], lo[2])) * fx) * fy) * (1 - fz) +
+           (((VR(lo[0], ul[1], hi[2]) - VR(lo[0], ll[1], hi[2])) * (1 - fx) +
+             (VR(hi[0], ul[1], hi[2]) - VR(hi[0], ll[1], hi[2])) * fx) * (1 - fy) +
+            ((VR(lo[0], uu[1], hi[2]) - VR(lo[0], lu[1], hi[2])) * (1 - fx) +
+             (VR(hi[0], uu[1], hi[2]) - VR(hi[0], lu[1], hi[2])) * fx) * fy) * fz;
+    g[2] = (((VR(lo[0], lo[1], ul[2]) - VR(lo[0], lo[1], ll[2])) * (1 - fx) +
+             (VR(hi[0], lo[1], ul[2]) - VR(hi[0], lo[1], ll[2])) * fx) * (1 - fy) +
+            ((VR(lo[0], hi[1], ul[2]) - VR(lo[0], hi[1], ll[2])) * (1 - fx) +
+             (VR(hi[0], hi[1], ul[2]) - VR(hi[0], hi[1], ll[2])) * fx) * fy) * (1 - fz) +
+           (((VR(lo[0], lo[1], uu[2]) - VR(lo[0], lo[1], lu[2])) * (1 - fx) +
+             (VR(hi[0], lo[1], uu[2]) - VR(hi[0], lo[1], lu[2])) * fx) * (1 - fy) +
+            ((VR(lo[0], hi[1], uu[2]) - VR(lo[0], hi[1], lu[2])) * (1 - fx) +
+             (VR(hi[0], hi[1], uu[2]) - VR(hi[0], hi[1], lu[2])) * fx) * fy) * fz;
+#undef VR
+    const float gs = S::kScaled ? (0.5f * 0.00003051944088f) : 0.5f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g[k] = (g[k] * (a.dim[k] / (float)a.N[k])) * gs;
+}
+
+constexpr int kMaxMarchSteps = 1 << 20;
+
+// One thread per pixel; 16x16 pixel tiles per block.  Block ids are remapped
+// so that the blocks an XCD receives (id mod 8) cover one contiguous band of
+// tiles, keeping the voxels neighbouring rays share in that XCD's L2.
+template <class S>
+__global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *__restrict__ st)
+{
+    const int nb = a.tiles_x * a.tiles_y;
+    const int per = a.blocks / 8;
+    const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (tile >= nb) return;
+    const int x = (tile % a.tiles_x) * 16 + (threadIdx.x & 15);
+    const int y = (tile / a.tiles_x) * 16 + (threadIdx.x >> 4);
+    if (x >= a.w || y >= a.h) return;
+
+    float view[16];
+    {
+        float iK[16], P[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
+        inverse_camera_matrix(a.cam, iK);
+        mul4(P, iK, view);  // m_pose * invK (gFusion.cpp:1298)
+    }
+    VolView<S> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1]};
+
+    const float o[3] = {view[12], view[13], view[14]};
+    float d[3];
+    rotate3(view, (float)x, (float)y, 1.0f, d);
+    float tmin[3], tmax[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float invR = 1.0f / d[k];
+        const float tbot = (-1.0f * invR) * o[k];
+        const float ttop = invR * (a.dim[k] - o[k]);
+        tmin[k] = gmin(ttop, tbot);
+        tmax[k] = gmax(ttop, tbot);
+    }
+    const float largest_tmin = gmax(gmax(tmin[0], tmin[1]), gmax(tmin[0], tmin[2]));
+    const float smallest_tmax = gmin(gmin(tmax[0], tmax[1]), gmin(tmax[0], tmax[2]));
+    const float tnear = gmax(largest_tmin, a.near_plane);
+    const float tfar = gmin(smallest_tmax, a.far_plane);
+
+    bool hit = false;
+    float hp[3], th = 0.0f;
+    if (tnear < tfar) {
+        float t = tnear, stepsize = a.large_step;
+        float f_t = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
+        float f_tt = 0.0f;
+        if (f_t >= 0.0f) {
+            int guard = 0;
+            for (; t < tfar && guard < kMaxMarchSteps; t += stepsize, ++guard) {
+                f_tt = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
+                if (f_tt < 0.0f) break;
+                if (f_tt < 0.8f) stepsize = a.step;
+                f_t = f_tt;
+            }
+            if (f_tt < 0.0f) {
+                t = t + stepsize * f_tt / (f_t - f_tt);
+                hp[0] = o[0] + d[0] * t;
+                hp[1] = o[1] + d[1] * t;
+                hp[2] = o[2] + d[2] * t;
+                th = t;
+                hit = true;
+            }
+        }
+    }
+    const int i = y * a.w + x;
+    if (hit && th > 0.0f) {
+        a.refV[i] = make_float4(hp[0], hp[1], hp[2], 1.0f);
+        float g[3];
+        gradient(V, a, hp, g);
+        const float len = sqrtf(dot3(g[0], g[1], g[2], g[0], g[1], g[2]));
+        if (len == 0.0f)
+            a.refN[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        else
+            a.refN[i] = make_float4(g[0] / len, g[1] / len, g[2] / len, 1.0f);
+    } else {
+        a.refV[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.refN[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_pyramid_u16(const PyramidArgs &a, const uint16_t *in, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pyramid<uint16_t>, dim3(a.blocks_total), dim3(256), 0, s, a, in);
+    return hipGetLastError();
+}
+hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pyramid<float>, dim3(a.blocks_total), dim3(256), 0, s, a, in);
+    return hipGetLastError();
+}
+hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_icp_iteration, dim3(a.h), dim3(kIcpThreads), 0, s, a, st);
+    return hipGetLastError();
+}
+hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_icp_finish, dim3(1), dim3(64), 0, s, st, frame);
+    return hipGetLastError();
+}
+hipError_t launch_state_set(DevState *st, const float *pose, int arm_reset, hipStream_t s)
+{
+    Mat16 m;
+    for (int i = 0; i < 16; ++i) m.m[i] = pose ? pose[i] : 0.0f;
+    hipLaunchKernelGGL(k_state_set, dim3(1), dim3(64), 0, s, st, m, pose ? 1 : 0, arm_reset);
+    return hipGetLastError();
+}
+
+static dim3 integrate_grid(const IntegrateArgs &a)
+{
+    return dim3((a.N[0] + 63) / 64, (a.N[1] + 3) / 4, (a.N[2] + a.zlen - 1) / a.zlen);
+}
+
+hipError_t launch_integrate(const IntegrateArgs &a, int storage, const DevState *st, hipStream_t s)
+{
+    const dim3 g = integrate_grid(a), b(64, 4);
+    switch (storage) {
+    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_integrate<StoreRG16I>, g, b, 0, s, a, st); break;
+    case KF_STORAGE_F32: hipLaunchKernelGGL(k_integrate<StoreF32>, g, b, 0, s, a, st); break;
+    case KF_STORAGE_F16: hipLaunchKernelGGL(k_integrate<StoreF16>, g, b, 0, s, a, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_count_integrate(const IntegrateArgs &a, const DevState *st, unsigned long long *counts,
+                                  hipStream_t s)
+{
+    const dim3 g((a.N[0] + 63) / 64, (a.N[1] + 3) / 4, 1), b(64, 4);
+    hipLaunchKernelGGL(k_count_integrate, g, b, 0, s, a, st, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s)
+{
+    const dim3 g(a.blocks), b(256);
+    switch (storage) {
+    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_raycast<StoreRG16I>, g, b, 0, s, a, st); break;
+    case KF_STORAGE_F32: hipLaunchKernelGGL(k_raycast<StoreF32>, g, b, 0, s, a, st); break;
+    case KF_STORAGE_F16: hipLaunchKernelGGL(k_raycast<StoreF16>, g, b, 0, s, a, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace kf

@@ -1,0 +1,329 @@
+"""Host-side mirror of the reference's `class gFusion` (include/gFusion.h:124-276).
+
+Method names, argument meaning and call order follow the reference so that a
+frame loop written against gFusion (kinect.cpp:654-695) reads the same here:
+
+    g = GFusion()
+    g.setConfig(cfg); g.setCameraParams((fx, fy, cx, cy)); g.setPose(init)
+    g.initTextures(); g.initVolume(); g.allocateBuffers()
+    g.Reset(init, False); g.raycast()
+    for depth in frames:
+        g.depthToVertex(depth); g.vertexToNormal()
+        tracked = g.Track()
+        if tracked or reset_window: g.integrate()
+        g.raycast()
+
+GL texture ids are replaced by device buffers of a kfusion session; the
+getters that returned texture ids return host copies instead.  All compute runs
+in the HIP library (libkfusion.so) -- there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as K
+
+
+class GFusionConfig:
+    """gFusionConfig (include/gFusion.h:42-82) with the reference defaults."""
+
+    def __init__(self):
+        self.volumeSize = (256, 256, 256)
+        self.volumeDimensions = (1.0, 1.0, 1.0)
+        self.mu = 0.05
+        self.maxWeight = 100.0
+        self.nearPlane = 0.1
+        self.farPlane = 3.0
+        self.dist_threshold = 0.05
+        self.normal_threshold = 0.8
+        self.iterations = [5, 5, 5]
+        self.track_threshold = 0.5
+        # not part of gFusionConfig: storage format and ICP combine rule
+        self.storage = "rg16i"
+        self.reduce_rule = K.RULE_LAST_ROW
+
+    def stepSize(self):
+        """include/gFusion.h:72-81"""
+        return np.float32(min(self.volumeDimensions)) / np.float32(max(self.volumeSize))
+
+
+def kinect_config(volume=512, dim=2.0, storage="f32", mu_voxels=3.0, iterations=(4, 5, 10),
+                  reduce_rule=K.RULE_LAST_ROW):
+    """The synthetic-sequence configuration: N^3 voxels over a `dim` cube,
+    truncation `mu_voxels` voxels, ICP iterations by level (0 = finest)."""
+    c = GFusionConfig()
+    n = volume if isinstance(volume, (tuple, list)) else (volume,) * 3
+    c.volumeSize = tuple(int(v) for v in n)
+    c.volumeDimensions = (float(dim),) * 3
+    c.mu = float(np.float32(dim) / np.float32(max(n)) * np.float32(mu_voxels))
+    c.iterations = list(iterations)
+    c.storage = storage
+    c.reduce_rule = reduce_rule
+    return c
+
+
+KINECT_V2_CAM = (365.5, 365.5, 255.5, 211.5)
+
+
+def to_kf_config(cfg, cam=KINECT_V2_CAM, width=512, height=424):
+    kc = K.KfConfig()
+    K.check(K.lib().kf_config_default(C.byref(kc)))
+    for i in range(3):
+        kc.volume_size[i] = int(cfg.volumeSize[i])
+        kc.volume_dim[i] = float(cfg.volumeDimensions[i])
+        kc.iterations[i] = int(cfg.iterations[i])
+    kc.mu = cfg.mu
+    kc.max_weight = cfg.maxWeight
+    kc.near_plane = cfg.nearPlane
+    kc.far_plane = cfg.farPlane
+    kc.dist_threshold = cfg.dist_threshold
+    kc.normal_threshold = cfg.normal_threshold
+    for i in range(4):
+        kc.cam[i] = float(cam[i])
+    kc.width = width
+    kc.height = height
+    kc.storage = K.STORAGE_NAMES[cfg.storage] if isinstance(cfg.storage, str) else int(cfg.storage)
+    kc.reduce_rule = int(cfg.reduce_rule)
+    return kc
+
+
+def translate(x, y, z):
+    m = np.eye(4, dtype=np.float32)
+    m[:3, 3] = (x, y, z)
+    return m
+
+
+def _pose_arg(pose):
+    """4x4 row-major numpy pose -> 16 column-major floats (glm layout)."""
+    p = np.ascontiguousarray(np.asarray(pose, dtype=np.float32).T).reshape(16)
+    return p, p.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class GFusion:
+    """gFusion over a kfusion session.  Poses are 4x4 numpy arrays (row-major,
+    camera -> world), converted to glm's column-major layout at the ABI."""
+
+    def __init__(self, stream=None, size=(512, 424)):
+        self.configuration = GFusionConfig()
+        self._cam = KINECT_V2_CAM
+        self._size = tuple(size)
+        self._pose = np.eye(4, dtype=np.float32)
+        self._stream = stream
+        self._s = None
+
+    # -- configuration (gFusion.h:180-195) ---------------------------------
+    def setConfig(self, config):
+        self.configuration = config
+
+    def setCameraParams(self, camPams, camPamsColor=None):
+        self._cam = tuple(float(v) for v in camPams)
+
+    def setPose(self, pose):
+        self._pose = np.asarray(pose, dtype=np.float32)
+        if self._s:
+            p, ptr = _pose_arg(self._pose)
+            K.check(K.lib().kf_set_pose(self._s, ptr))
+
+    def getPose(self):
+        if not self._s:
+            return self._pose.copy()
+        out = np.zeros(16, dtype=np.float32)
+        K.check(K.lib().kf_get_pose(self._s, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out.reshape(4, 4).T.copy()
+
+    def getTransPose(self):
+        """gFusion.h:267-274"""
+        return self.getPose()[:3, 3].copy()
+
+    # -- allocation (gFusion.cpp:200-330) ----------------------------------
+    def initTextures(self):
+        pass
+
+    def initVolume(self):
+        pass
+
+    def allocateBuffers(self):
+        """Creates the device session holding every buffer of the three init
+        calls (textures, volume, reduction buffers)."""
+        if self._s:
+            K.check(K.lib().kf_session_destroy(self._s))
+            self._s = None
+        kc = to_kf_config(self.configuration, self._cam, *self._size)
+        h = C.c_void_p()
+        K.check(K.lib().kf_session_create(C.byref(kc), self._stream, C.byref(h)))
+        self._s = h
+        self.setPose(self._pose)
+
+    @property
+    def session(self):
+        if not self._s:
+            raise K.KfError(K.KF_ERR_STATE, "allocateBuffers() has not been called")
+        return self._s
+
+    def close(self):
+        if self._s:
+            K.lib().kf_session_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reset (gFusion.cpp:226-287) ---------------------------------------
+    def Reset(self, pose, deleteFlag=False):
+        """Zero the volume, set the pose; the caller's raycast() follows as in
+        kinect.cpp:472-474 (kf_reset also raycasts and arms the reset window)."""
+        if deleteFlag:
+            self.allocateBuffers()
+        self._pose = np.asarray(pose, dtype=np.float32)
+        p, ptr = _pose_arg(self._pose)
+        K.check(K.lib().kf_reset(self.session, ptr))
+
+    def resetVolume(self):
+        K.check(K.lib().kf_reset_volume(self.session))
+
+    def resetPose(self, pose):
+        self.setPose(pose)
+
+    # -- per-frame calls ----------------------------------------------------
+    def depthToVertex(self, depthArray):
+        """gFusion.cpp:496: float mm (or uint16 mm) 512x424 depth; builds the
+        depth mip chain, vertex and normal maps of all 3 levels."""
+        d = np.ascontiguousarray(depthArray)
+        if d.dtype == np.uint16:
+            K.check(K.lib().kf_depth_to_vertex_u16(self.session, d.ctypes.data))
+        else:
+            d = np.ascontiguousarray(d, dtype=np.float32)
+            K.check(K.lib().kf_depth_to_vertex_f32(self.session, d.ctypes.data))
+
+    def vertexToNormal(self):
+        """gFusion.cpp:528: the normal maps are produced by the same launch as
+        the vertex maps in depthToVertex(); nothing is left to do here."""
+
+    def Track(self):
+        t = C.c_int(0)
+        K.check(K.lib().kf_track(self.session, C.byref(t)))
+        return bool(t.value)
+
+    def track(self, layer, first=True, save_pose=False):
+        """One ICP step at `layer` (track + reduce + solve + update fused)."""
+        K.check(K.lib().kf_icp_iteration(self.session, int(layer), int(first), int(save_pose)))
+
+    def reduce(self, layer):
+        """The reduction is fused into track(); kept for call compatibility."""
+
+    def getReduction(self):
+        """gFusion.cpp:1127-1168 on the last iteration's 8x32 partials:
+        returns (b[6], C[6x6], alignmentEnergy) under the session's rule."""
+        parts = self.readPartials()
+        row = parts[7] if self.configuration.reduce_rule == K.RULE_LAST_ROW else parts.sum(0)
+        b = row[1:7].copy()
+        Cm = np.zeros((6, 6), dtype=np.float32)
+        iu = np.triu_indices(6)
+        Cm[iu] = row[7:28]
+        Cm = Cm + np.triu(Cm, 1).T
+        ae = np.float32(np.sqrt(np.float32(row[0]) / np.float32(row[28])))
+        return b, Cm, ae
+
+    def readPartials(self):
+        out = np.zeros(256, dtype=np.float32)
+        ae = C.c_float()
+        K.check(K.lib().kf_read_reduction(self.session, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                          C.byref(ae)))
+        return out.reshape(8, 32)
+
+    def alignmentEnergy(self):
+        ae = C.c_float()
+        K.check(K.lib().kf_read_reduction(self.session, None, C.byref(ae)))
+        return ae.value
+
+    def integrate(self, gated=False):
+        K.check(K.lib().kf_integrate(self.session, K.INTEGRATE_GATED if gated else K.INTEGRATE_ALWAYS))
+
+    def raycast(self):
+        K.check(K.lib().kf_raycast(self.session))
+
+    def frame(self, depthArray):
+        """The frame block of kinect.cpp:654-695 without host synchronisation."""
+        d = np.ascontiguousarray(depthArray, dtype=np.float32)
+        K.check(K.lib().kf_frame_f32(self.session, d.ctypes.data))
+
+    def status(self):
+        t, it, c, r = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        K.check(K.lib().kf_read_status(self.session, C.byref(t), C.byref(it), C.byref(c), C.byref(r)))
+        return dict(tracked=t.value, iterations=it.value, counter=c.value, reset=r.value)
+
+    # -- read-back (gFusion.h:196-246) -------------------------------------
+    def _read_map(self, which, level=0):
+        w, h = self._size[0] >> level, self._size[1] >> level
+        comps = 1 if which == K.MAP_DEPTH else 4
+        out = np.zeros((h, w, comps), dtype=np.float32)
+        K.check(K.lib().kf_read_map(self.session, which, level, out.ctypes.data, out.nbytes))
+        return out if comps > 1 else out[..., 0]
+
+    def _write_map(self, which, arr, level=0):
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        K.check(K.lib().kf_write_map(self.session, which, level, a.ctypes.data, a.nbytes))
+
+    def getVerts(self):
+        """Reference (raycast) vertex map -- gFusion.h:210-215"""
+        return self._read_map(K.MAP_REF_VERTEX)
+
+    def getNorms(self):
+        """Reference (raycast) normal map -- gFusion.h:216-222"""
+        return self._read_map(K.MAP_REF_NORMAL)
+
+    def getDepthImage(self):
+        return self._read_map(K.MAP_DEPTH)
+
+    def getVertexMap(self, level=0):
+        return self._read_map(K.MAP_VERTEX, level)
+
+    def getNormalMap(self, level=0):
+        return self._read_map(K.MAP_NORMAL, level)
+
+    def setReferenceMaps(self, verts, norms):
+        self._write_map(K.MAP_REF_VERTEX, verts)
+        self._write_map(K.MAP_REF_NORMAL, norms)
+
+    def _plane_dtype(self):
+        s = self.configuration.storage
+        s = K.STORAGE_NAMES[s] if isinstance(s, str) else s
+        return {K.STORAGE_RG16I: np.int16, K.STORAGE_F32: np.float32, K.STORAGE_F16: np.float16}[s]
+
+    def getVolume(self):
+        """TSDF as (value, weight) planes of shape (Z, Y, X) in storage type."""
+        n = self.configuration.volumeSize
+        dt = self._plane_dtype()
+        v = np.zeros((n[2], n[1], n[0]), dtype=dt)
+        w = np.zeros((n[2], n[1], n[0]), dtype=dt)
+        K.check(K.lib().kf_read_tsdf(self.session, v.ctypes.data, w.ctypes.data))
+        return v, w
+
+    def setVolume(self, value, weight):
+        dt = self._plane_dtype()
+        v = np.ascontiguousarray(value, dtype=dt)
+        w = np.ascontiguousarray(weight, dtype=dt)
+        K.check(K.lib().kf_write_tsdf(self.session, v.ctypes.data, w.ctypes.data))
+
+    def setDebug(self, on=True):
+        K.check(K.lib().kf_set_debug(self.session, 1 if on else 0))
+
+    def readTrackPixels(self, level):
+        w, h = self._size[0] >> level, self._size[1] >> level
+        res = np.zeros(h * w, dtype=np.int32)
+        err = np.zeros(h * w, dtype=np.float32)
+        J = np.zeros((h * w, 6), dtype=np.float32)
+        K.check(K.lib().kf_read_track_pixels(self.session, level, res.ctypes.data, err.ctypes.data,
+                                             J.ctypes.data))
+        return res, err, J
+
+    def countIntegrateTraffic(self):
+        r, w = C.c_uint64(), C.c_uint64()
+        K.check(K.lib().kf_count_integrate_traffic(self.session, C.byref(r), C.byref(w)))
+        return r.value, w.value
+
+    def synchronize(self):
+        K.check(K.lib().kf_synchronize(self.session))

@@ -1,0 +1,324 @@
+"""CPU oracle (test infrastructure only): ctypes wrapper of kf_oracle.c.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+package, as the checker or the CPU baseline.  The product library never does.
+
+Arrays use the oracle's layouts: maps are (H, W, 4) in R (float32 for the
+FP32 oracle, float64 for the FP64 oracle); poses are 4x4 row-major numpy
+arrays converted to glm's column-major layout at the boundary; the TSDF is
+interleaved (value, weight) pairs, shape (Z, Y, X, 2).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+RG16I, F32X2, F16X2, F64X2 = 0, 1, 2, 3
+LAST_ROW, SUM_ALL = 0, 1
+_VOL_DTYPES = {RG16I: np.int16, F32X2: np.float32, F16X2: np.uint16, F64X2: np.float64}
+
+
+def build():
+    """Compiles libkforacle_f32.so / libkforacle_f64.so with the Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class Oracle:
+    """One oracle flavour: precision 'f32' (shader semantics) or 'f64'."""
+
+    def __init__(self, precision="f32"):
+        path = os.path.join(HERE, "libkforacle_%s.so" % precision)
+        if not os.path.exists(path):
+            build()
+        self.L = C.CDLL(path)
+        self.rs = self.L.kfo_real_size()
+        self.dt = np.float32 if self.rs == 4 else np.float64
+        self.R = C.c_float if self.rs == 4 else C.c_double
+        self._declare()
+
+    # -- ctypes plumbing ---------------------------------------------------
+    def _declare(self):
+        L, vp, i, R = self.L, C.c_void_p, C.c_int, self.R
+        sig = {
+            "kfo_f2h": ([C.c_float], C.c_uint16),
+            "kfo_h2f": ([C.c_uint16], C.c_float),
+            "kfo_camera_matrix": ([vp, vp], None),
+            "kfo_inverse_camera_matrix": ([vp, vp], None),
+            "kfo_mul4": ([vp, vp, vp], None),
+            "kfo_inverse4": ([vp, vp], None),
+            "kfo_mip_depth": ([vp, i, i, vp], None),
+            "kfo_depth_to_vertex": ([vp, i, i, vp, vp], None),
+            "kfo_vertex_to_normal": ([vp, i, i, vp], None),
+            "kfo_track_view": ([vp, vp, vp], None),
+            "kfo_track": ([vp, vp, i, i, vp, vp, i, i, vp, vp, R, R, vp, vp, vp], None),
+            "kfo_reduce": ([vp, vp, vp, i, i, vp], None),
+            "kfo_get_reduction": ([vp, i, vp, vp, vp], None),
+            "kfo_solve6": ([vp, vp, vp], None),
+            "kfo_pose_update": ([vp, vp], None),
+            "kfo_voxel_bytes": ([i], i),
+            "kfo_reset_volume": ([vp, i, vp], None),
+            "kfo_integrate": ([vp, i, vp, vp, vp, i, i, vp, vp, R, R], None),
+            "kfo_raycast_view": ([vp, vp, vp], None),
+            "kfo_raycast": ([vp, i, vp, vp, vp, vp, R, R, R, R, i, i, vp, vp], None),
+            "kfo_config_size": ([], i),
+            "kfo_session_create": ([vp], vp),
+            "kfo_session_destroy": ([vp], None),
+            "kfo_session_set_pose": ([vp, vp], None),
+            "kfo_session_get_pose": ([vp, vp], None),
+            "kfo_session_buffer": ([vp, i, i], vp),
+            "kfo_session_stats": ([vp, vp, vp, vp, vp, vp, vp], None),
+            "kfo_session_raycast": ([vp], None),
+            "kfo_session_reset": ([vp, vp], None),
+            "kfo_session_depth": ([vp, vp], None),
+            "kfo_session_track": ([vp], i),
+            "kfo_session_integrate": ([vp], None),
+            "kfo_session_frame": ([vp, vp], i),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+
+    def arr(self, a, dtype=None):
+        return np.ascontiguousarray(a, dtype=dtype or self.dt)
+
+    @staticmethod
+    def p(a):
+        return a.ctypes.data
+
+    def cm(self, M):
+        """row-major 4x4 -> column-major 16 (R)"""
+        return self.arr(np.asarray(M).T).reshape(16)
+
+    def rm(self, v):
+        return np.asarray(v).reshape(4, 4).T.copy()
+
+    # -- primitives --------------------------------------------------------
+    def f2h(self, x):
+        return self.L.kfo_f2h(float(x))
+
+    def h2f(self, h):
+        return self.L.kfo_h2f(int(h))
+
+    def camera_matrix(self, cam):
+        out = np.zeros(16, self.dt)
+        self.L.kfo_camera_matrix(self.p(self.arr(cam)), self.p(out))
+        return self.rm(out)
+
+    def inverse_camera_matrix(self, cam):
+        out = np.zeros(16, self.dt)
+        self.L.kfo_inverse_camera_matrix(self.p(self.arr(cam)), self.p(out))
+        return self.rm(out)
+
+    def mul4(self, A, B):
+        out = np.zeros(16, self.dt)
+        self.L.kfo_mul4(self.p(self.cm(A)), self.p(self.cm(B)), self.p(out))
+        return self.rm(out)
+
+    def inverse4(self, M):
+        out = np.zeros(16, self.dt)
+        self.L.kfo_inverse4(self.p(self.cm(M)), self.p(out))
+        return self.rm(out)
+
+    def mip_depth(self, d):
+        d = self.arr(d)
+        h, w = d.shape
+        out = np.zeros((h // 2, w // 2), self.dt)
+        self.L.kfo_mip_depth(self.p(d), w, h, self.p(out))
+        return out
+
+    def depth_to_vertex(self, d, cam_l):
+        d = self.arr(d)
+        h, w = d.shape
+        out = np.zeros((h, w, 4), self.dt)
+        self.L.kfo_depth_to_vertex(self.p(d), w, h, self.p(self.arr(cam_l)), self.p(out))
+        return out
+
+    def vertex_to_normal(self, v):
+        v = self.arr(v)
+        h, w = v.shape[:2]
+        out = np.zeros((h, w, 4), self.dt)
+        self.L.kfo_vertex_to_normal(self.p(v), w, h, self.p(out))
+        return out
+
+    def track_view(self, pose, cam):
+        out = np.zeros(16, self.dt)
+        self.L.kfo_track_view(self.p(self.cm(pose)), self.p(self.arr(cam)), self.p(out))
+        return self.rm(out)
+
+    def track(self, vert, norm, refV, refN, pose, cam, dist_th=0.05, normal_th=0.8):
+        vert, norm, refV, refN = (self.arr(a) for a in (vert, norm, refV, refN))
+        h, w = vert.shape[:2]
+        rh, rw = refV.shape[:2]
+        view = self.cm(self.track_view(pose, cam))
+        res = np.zeros(h * w, np.int32)
+        err = np.zeros(h * w, self.dt)
+        J = np.zeros((h * w, 6), self.dt)
+        self.L.kfo_track(self.p(vert), self.p(norm), w, h, self.p(refV), self.p(refN), rw, rh,
+                         self.p(self.cm(pose)), self.p(view), dist_th, normal_th,
+                         self.p(res), self.p(err), self.p(J))
+        return res, err, J
+
+    def reduce(self, res, err, J, w, h):
+        out = np.zeros(256, self.dt)
+        self.L.kfo_reduce(self.p(self.arr(res, np.int32)), self.p(self.arr(err)), self.p(self.arr(J)), w, h,
+                          self.p(out))
+        return out.reshape(8, 32)
+
+    def get_reduction(self, partials, rule=LAST_ROW):
+        b = np.zeros(6, self.dt)
+        Cm = np.zeros(36, self.dt)
+        ae = np.zeros(1, self.dt)
+        self.L.kfo_get_reduction(self.p(self.arr(partials).reshape(256)), rule, self.p(b), self.p(Cm), self.p(ae))
+        return b, Cm.reshape(6, 6), ae[0]
+
+    def solve6(self, Cm, b):
+        x = np.zeros(6, np.float64)
+        self.L.kfo_solve6(self.p(self.arr(Cm, np.float64).reshape(36)), self.p(self.arr(b, np.float64)), self.p(x))
+        return x
+
+    def pose_update(self, x, pose):
+        P = self.cm(pose)
+        self.L.kfo_pose_update(self.p(self.arr(x, np.float64)), self.p(P))
+        return self.rm(P)
+
+    def voxel_bytes(self, mode):
+        return self.L.kfo_voxel_bytes(mode)
+
+    def new_volume(self, N, mode):
+        return np.zeros((N[2], N[1], N[0], 2), _VOL_DTYPES[mode])
+
+    def integrate(self, vol, mode, dim, depth_mm, pose, cam, mu, max_weight=100.0):
+        N = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], np.int32)
+        d = self.arr(depth_mm)
+        self.L.kfo_integrate(self.p(vol), mode, self.p(N), self.p(self.arr(dim)), self.p(d), d.shape[1], d.shape[0],
+                             self.p(self.cm(pose)), self.p(self.arr(cam)), mu, max_weight)
+        return vol
+
+    def raycast(self, vol, mode, dim, pose, cam, near, far, step, large_step, w=512, h=424):
+        N = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], np.int32)
+        rv = np.zeros((h, w, 4), self.dt)
+        rn = np.zeros((h, w, 4), self.dt)
+        self.L.kfo_raycast(self.p(np.ascontiguousarray(vol)), mode, self.p(N), self.p(self.arr(dim)),
+                           self.p(self.cm(pose)), self.p(self.arr(cam)), near, far, step, large_step, w, h,
+                           self.p(rv), self.p(rn))
+        return rv, rn
+
+
+def _config_struct(R):
+    class Cfg(C.Structure):
+        _fields_ = [
+            ("vol", C.c_int * 3), ("dim", R * 3), ("mu", R), ("maxWeight", R), ("nearPlane", R),
+            ("farPlane", R), ("dist_threshold", R), ("normal_threshold", R), ("iterations", C.c_int * 3),
+            ("cam", R * 4), ("width", C.c_int), ("height", C.c_int), ("mode", C.c_int), ("rule", C.c_int),
+        ]
+    return Cfg
+
+
+class OracleSession:
+    """The oracle's gFusion state + the kinect.cpp frame block."""
+
+    def __init__(self, oracle, volume, dim, mu, mode, iterations=(4, 5, 10), cam=(365.5, 365.5, 255.5, 211.5),
+                 width=512, height=424, rule=LAST_ROW, max_weight=100.0, near=0.1, far=3.0, dist_th=0.05,
+                 normal_th=0.8):
+        self.o = oracle
+        Cfg = _config_struct(oracle.R)
+        assert C.sizeof(Cfg) == oracle.L.kfo_config_size()
+        c = Cfg()
+        n = volume if isinstance(volume, (tuple, list)) else (volume,) * 3
+        for k in range(3):
+            c.vol[k] = n[k]
+            c.dim[k] = dim
+            c.iterations[k] = iterations[k]
+        c.mu, c.maxWeight, c.nearPlane, c.farPlane = mu, max_weight, near, far
+        c.dist_threshold, c.normal_threshold = dist_th, normal_th
+        for k in range(4):
+            c.cam[k] = cam[k]
+        c.width, c.height, c.mode, c.rule = width, height, mode, rule
+        self.cfg = c
+        self.N = n
+        self.w, self.h = width, height
+        self.mode = mode
+        self.s = oracle.L.kfo_session_create(C.byref(c))
+
+    def __del__(self):
+        try:
+            self.o.L.kfo_session_destroy(self.s)
+        except Exception:
+            pass
+
+    def _buf(self, which, level, shape, dtype):
+        ptr = self.o.L.kfo_session_buffer(self.s, which, level)
+        n = int(np.prod(shape))
+        buf = (C.c_char * (n * np.dtype(dtype).itemsize)).from_address(ptr)
+        return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def reset(self, pose):
+        self.o.L.kfo_session_reset(self.s, self.o.p(self.o.cm(pose)))
+
+    def set_pose(self, pose):
+        self.o.L.kfo_session_set_pose(self.s, self.o.p(self.o.cm(pose)))
+
+    def pose(self):
+        out = np.zeros(16, self.o.dt)
+        self.o.L.kfo_session_get_pose(self.s, self.o.p(out))
+        return self.o.rm(out)
+
+    def depth(self, depth_mm):
+        d = np.ascontiguousarray(depth_mm, dtype=np.float32)
+        self.o.L.kfo_session_depth(self.s, d.ctypes.data)
+
+    def track(self):
+        return self.o.L.kfo_session_track(self.s)
+
+    def integrate(self):
+        self.o.L.kfo_session_integrate(self.s)
+
+    def raycast(self):
+        self.o.L.kfo_session_raycast(self.s)
+
+    def frame(self, depth_mm):
+        d = np.ascontiguousarray(depth_mm, dtype=np.float32)
+        return self.o.L.kfo_session_frame(self.s, d.ctypes.data)
+
+    def stats(self):
+        ae = np.zeros(1, self.o.dt)
+        ints = [C.c_int() for _ in range(5)]
+        self.o.L.kfo_session_stats(self.s, ae.ctypes.data, *[C.addressof(v) for v in ints])
+        t, it, ln, cnt, rst = (v.value for v in ints)
+        return dict(ae=float(ae[0]), tracked=t, iterations=it, log_n=ln, counter=cnt, reset=rst)
+
+    def log(self):
+        n = self.stats()["log_n"]
+        return self._buf(10, 0, (64, 8), np.float64)[:n].copy()
+
+    def map(self, which, level=0):
+        w, h = self.w >> level, self.h >> level
+        if which == 0:
+            return self._buf(0, level, (h, w), self.o.dt).copy()
+        return self._buf(which, level, (h, w, 4), self.o.dt).copy()
+
+    def set_map(self, which, arr, level=0):
+        w, h = self.w >> level, self.h >> level
+        dst = self._buf(which, level, (h, w, 4), self.o.dt)
+        dst[...] = arr
+
+    def volume(self):
+        n = self.N
+        return self._buf(5, 0, (n[2], n[1], n[0], 2), _VOL_DTYPES[self.mode]).copy()
+
+    def set_volume(self, vol):
+        n = self.N
+        dst = self._buf(5, 0, (n[2], n[1], n[0], 2), _VOL_DTYPES[self.mode])
+        dst[...] = vol
+
+    def partials(self):
+        return self._buf(9, 0, (8, 32), self.o.dt).copy()
+
+    def track_pixels(self, level):
+        w, h = self.w >> level, self.h >> level
+        return (self._buf(6, 0, (h * w,), np.int32).copy(), self._buf(7, 0, (h * w,), self.o.dt).copy(),
+                self._buf(8, 0, (h * w, 6), self.o.dt).copy())

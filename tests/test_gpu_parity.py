@@ -1,0 +1,330 @@
+"""Parity of the HIP path (through the C ABI) against the C oracle.
+
+Tolerances (stated by the north star for the full loop; tighter where the two
+sides evaluate identical FP32 expressions):
+  pyramid / per-pixel track / integrate (RG16I, F32) : bit-exact expected,
+      test allows <= 1e-6 relative and <= 0.01 % differing voxels;
+  ICP partial sums: relative 1e-4 of the term's magnitude (sum order differs);
+  pose per frame: <= 1e-4 rad and <= 0.1 mm;
+  TSDF: |delta| <= 1e-3 of truncation (value units), flips counted apart;
+  raycast vertex: <= 0.5 mm, hit/miss flips <= 0.1 % of pixels.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rigid, small_cam
+
+pytestmark = pytest.mark.gpu
+
+K = pytest.importorskip("oglfusionkinectv2_amd._lib")
+from oglfusionkinectv2_amd import GFusion, kinect_config, synth  # noqa: E402
+
+STORAGE_ORACLE = {"rg16i": oracle.RG16I, "f32": oracle.F32X2, "f16": oracle.F16X2}
+
+
+def make(volume=128, dim=2.0, storage="rg16i", size=(512, 424), cam=None, iterations=(4, 5, 10), rule=0):
+    g = GFusion(size=size)
+    cfg = kinect_config(volume, dim, storage, iterations=iterations, reduce_rule=rule)
+    g.setConfig(cfg)
+    if cam is not None:
+        g.setCameraParams(cam)
+    g.allocateBuffers()
+    return g, cfg
+
+
+def oracle_volume_planes(vol, storage):
+    """(Z,Y,X,2) oracle volume -> (value, weight) planes in the GPU storage dtype."""
+    if storage == "f16":
+        return vol[..., 0].view(np.float16), vol[..., 1].view(np.float16)
+    return vol[..., 0], vol[..., 1]
+
+
+# ---------------------------------------------------------------------------
+def test_pyramid_bit_exact(o32, seq20):
+    print("case: pyramid u16 512x424", flush=True)
+    depth, _ = seq20
+    g, cfg = make()
+    d = depth[3]
+    g.depthToVertex(d)
+    g.vertexToNormal()
+    s = oracle.OracleSession(o32, 128, 2.0, cfg.mu, oracle.RG16I)
+    s.depth(d.astype(np.float32))
+    assert np.array_equal(g.getDepthImage(), s.map(0, 0))
+    for l in range(3):
+        gv, gn = g.getVertexMap(l), g.getNormalMap(l)
+        ov, on = s.map(1, l), s.map(2, l)
+        np.testing.assert_array_equal(gv, ov)
+        # NaN normals (collinear neighbours) must agree too
+        np.testing.assert_array_equal(np.isnan(gn), np.isnan(on))
+        np.testing.assert_array_equal(np.nan_to_num(gn), np.nan_to_num(on))
+
+
+def test_pyramid_float_input_and_zero_border(o32):
+    print("case: pyramid f32 64x48", flush=True)
+    rng = np.random.default_rng(5)
+    w, h = 64, 48
+    d = rng.uniform(500, 3000, (h, w)).astype(np.float32)
+    d[rng.uniform(size=(h, w)) < 0.1] = 0
+    cam = small_cam(w, h)
+    g, cfg = make(volume=32, size=(w, h), cam=cam)
+    g.depthToVertex(d)
+    n0 = g.getNormalMap(0)
+    assert (n0[0, :, 0] == 2).all() and (n0[:, 0, 0] == 2).all()  # wrapped borders are invalid
+    s = oracle.OracleSession(o32, 32, 2.0, cfg.mu, oracle.RG16I, cam=cam, width=w, height=h)
+    s.depth(d)
+    for l in range(3):
+        np.testing.assert_array_equal(g.getVertexMap(l), s.map(1, l))
+        np.testing.assert_array_equal(np.nan_to_num(g.getNormalMap(l)), np.nan_to_num(s.map(2, l)))
+
+
+# ---------------------------------------------------------------------------
+def _icp_fixture(o32, seq20, storage="rg16i", frames=4):
+    """Oracle session integrated over the first frames, GPU session loaded with
+    the same volume, reference maps, input frame and a perturbed pose."""
+    depth, gt = seq20
+    g, cfg = make(storage=storage)
+    s = oracle.OracleSession(o32, 128, 2.0, cfg.mu, STORAGE_ORACLE[storage])
+    s.reset(gt[0])
+    for f in range(frames):
+        s.frame(depth[f].astype(np.float32))
+    vol = s.volume()
+    g.Reset(np.eye(4, dtype=np.float32))
+    g.setVolume(*oracle_volume_planes(vol, storage))
+    g.setReferenceMaps(s.map(3), s.map(4))
+    d = depth[frames].astype(np.float32)
+    g.depthToVertex(d)
+    s.depth(d)
+    return g, s, cfg
+
+
+def test_track_pixels_and_partials(o32, seq20):
+    print("case: icp iteration per level", flush=True)
+    g, s, cfg = _icp_fixture(o32, seq20)
+    pose = s.pose() @ rigid(0.004, -0.003, 0.002, [0.006, -0.004, 0.005])
+    g.setDebug(True)
+    for level in (2, 1, 0):
+        w, h = 512 >> level, 424 >> level
+        g.setPose(pose)
+        g.track(level, first=True, save_pose=True)
+        res, err, J = g.readTrackPixels(level)
+        ores, oerr, oJ = o32.track(s.map(1, level), s.map(2, level), s.map(3), s.map(4), pose, (365.5, 365.5, 255.5, 211.5))
+        np.testing.assert_array_equal(res, ores)
+        ok = res == 1
+        assert ok.sum() > 100
+        np.testing.assert_array_equal(err[ok], oerr[ok])
+        np.testing.assert_array_equal(J[ok], oJ[ok])
+        parts = g.readPartials()
+        oparts = o32.reduce(ores, oerr, oJ, w, h)
+        scale = np.abs(oparts).max(axis=0) + 1e-30
+        assert (np.abs(parts - oparts) / scale).max() < 1e-4
+        # the pose update of the last-row rule
+        b, C, ae = o32.get_reduction(oparts, oracle.LAST_ROW)
+        x = o32.solve6(C.astype(np.float64), b.astype(np.float64))
+        want = o32.pose_update(x, pose)
+        got = g.getPose()
+        assert np.abs(got[:3, 3] - want[:3, 3]).max() < 1e-4
+        assert np.abs(got[:3, :3] - want[:3, :3]).max() < 1e-4
+        assert abs(g.alignmentEnergy() - ae) <= 1e-4 * max(ae, 1e-6)
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+def test_track_full_matches_oracle(o32, seq20, rule):
+    print("case: Track() rule %d" % rule, flush=True)
+    depth, gt = seq20
+    g, cfg = make(rule=rule)
+    s = oracle.OracleSession(o32, 128, 2.0, cfg.mu, oracle.RG16I, rule=rule)
+    s.reset(gt[0])
+    for f in range(5):
+        s.frame(depth[f].astype(np.float32))
+    g.Reset(np.eye(4, dtype=np.float32))
+    g.setVolume(*oracle_volume_planes(s.volume(), "rg16i"))
+    g.setReferenceMaps(s.map(3), s.map(4))
+    g.setPose(s.pose())
+    d = depth[5].astype(np.float32)
+    g.depthToVertex(d)
+    s.depth(d)
+    t_gpu = g.Track()
+    t_cpu = s.track()
+    assert t_gpu == bool(t_cpu)
+    P, Q = g.getPose(), s.pose()
+    assert np.abs(P[:3, 3] - Q[:3, 3]).max() < 1e-4
+    dR = P[:3, :3] @ Q[:3, :3].T
+    ang = np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1))
+    assert ang < 1e-4
+
+
+def test_track_rejects_empty_and_far_frames(o32, seq20):
+    print("case: tracking failure paths", flush=True)
+    depth, gt = seq20
+    g, cfg = make()
+    g.Reset(gt[0])
+    # empty model: AE = 0 -> rejected, pose unchanged
+    g.depthToVertex(depth[1])
+    assert g.Track() is False
+    np.testing.assert_array_equal(g.getPose(), gt[0])
+    # all-zero depth frame
+    g.depthToVertex(np.zeros((424, 512), np.uint16))
+    assert g.Track() is False
+    np.testing.assert_array_equal(g.getPose(), gt[0])
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("storage", ["rg16i", "f32", "f16"])
+def test_integrate_matches_oracle(o32, seq20, storage):
+    print("case: integrate %s" % storage, flush=True)
+    depth, gt = seq20
+    g, cfg = make(storage=storage)
+    g.Reset(gt[0])
+    mode = STORAGE_ORACLE[storage]
+    vol = o32.new_volume((128, 128, 128), mode)
+    for f in range(6):
+        g.setPose(gt[f])
+        g.depthToVertex(depth[f])
+        g.integrate()
+        o32.integrate(vol, mode, (2.0, 2.0, 2.0), depth[f].astype(np.float32), gt[f], (365.5, 365.5, 255.5, 211.5),
+                      cfg.mu, 100.0)
+    v, w = g.getVolume()
+    ov, ow = oracle_volume_planes(vol, storage)
+    vf, ovf = v.astype(np.float64), ov.astype(np.float64)
+    if storage == "rg16i":
+        vf, ovf = vf / 32766.0, ovf / 32766.0
+    diff = np.abs(vf - ovf)
+    frac_exact = (diff == 0).mean()
+    print("exact voxel fraction", frac_exact)
+    assert (w.astype(np.float64) != ow.astype(np.float64)).mean() < 1e-4
+    assert (diff > 1e-3).mean() < 1e-4
+    assert frac_exact > 0.9999
+
+
+def test_integrate_zero_depth_is_noop():
+    print("case: integrate zero depth", flush=True)
+    g, cfg = make(volume=64)
+    g.Reset(np.eye(4, dtype=np.float32) + 0)
+    g.depthToVertex(np.zeros((424, 512), np.uint16))
+    g.integrate()
+    v, w = g.getVolume()
+    assert not v.any() and not w.any()
+
+
+# ---------------------------------------------------------------------------
+def _plane_volume(N, dim, z0, mu):
+    """TSDF of the plane z = z0 in normalised units (as integrate stores it)."""
+    vox = dim / N
+    zc = (np.arange(N) + 0.5) * vox
+    d = np.clip((z0 - zc) / mu, -0.1, 0.2)
+    v = np.broadcast_to(d[:, None, None], (N, N, N)).astype(np.float32)
+    return v
+
+
+@pytest.mark.parametrize("storage", ["rg16i", "f32", "f16"])
+def test_raycast_matches_oracle(o32, seq20, storage):
+    print("case: raycast %s" % storage, flush=True)
+    depth, gt = seq20
+    g, cfg = make(storage=storage)
+    s = oracle.OracleSession(o32, 128, 2.0, cfg.mu, STORAGE_ORACLE[storage])
+    s.reset(gt[0])
+    for f in range(6):
+        s.frame(depth[f].astype(np.float32))
+    g.Reset(np.eye(4, dtype=np.float32))
+    g.setVolume(*oracle_volume_planes(s.volume(), storage))
+    g.setPose(s.pose())
+    g.raycast()
+    s.raycast()
+    gv, gn = g.getVerts(), g.getNorms()
+    ov, on = s.map(3), s.map(4)
+    hit_g, hit_o = gv[..., 3] > 0, ov[..., 3] > 0
+    both = hit_g & hit_o
+    assert (hit_g != hit_o).mean() < 1e-3
+    assert both.mean() > 0.5
+    dv = np.abs(gv[both][:, :3] - ov[both][:, :3]).max()
+    print("max vertex diff (m)", dv, "exact", (gv[both] == ov[both]).all(axis=1).mean())
+    assert dv <= 0.5e-3
+    dn = np.abs(gn[both][:, :3] - on[both][:, :3]).max()
+    assert dn < 1e-3
+
+
+def test_raycast_analytic_plane():
+    print("case: raycast analytic plane", flush=True)
+    N, dim = 64, 2.0
+    g, cfg = make(volume=N, dim=dim, storage="f32")
+    v = _plane_volume(N, dim, 1.5, cfg.mu)
+    g.Reset(np.eye(4, dtype=np.float32))
+    g.setVolume(v, np.ones_like(v))
+    pose = np.eye(4, dtype=np.float32)
+    pose[:3, 3] = (1.0, 1.0, 0.0)
+    g.setPose(pose)
+    g.raycast()
+    rv, rn = g.getVerts(), g.getNorms()
+    hit = rv[..., 3] > 0
+    assert hit.mean() > 0.3
+    # surface within half a voxel step of the plane, normal = -z
+    assert np.abs(rv[hit][:, 2] - 1.5).max() < 0.5 * dim / N + 1e-4
+    assert np.abs(rn[hit][:, 2] + 1).max() < 1e-4
+
+
+def test_raycast_misses_outside_box():
+    print("case: raycast misses", flush=True)
+    g, cfg = make(volume=32, storage="f32")
+    g.Reset(np.eye(4, dtype=np.float32))
+    pose = np.eye(4, dtype=np.float32)
+    pose[:3, 3] = (5.0, 5.0, 5.0)  # volume behind the camera
+    g.setPose(pose)
+    g.raycast()
+    assert not g.getVerts().any() and not g.getNorms().any()
+
+
+# ---------------------------------------------------------------------------
+def test_frame_loop_matches_oracle(o32, seq20):
+    """Per-frame parity: before every frame the GPU session is loaded with the
+    oracle's full state (volume, reference maps, pose), then both run the same
+    frame; pose, tracking decision, frame flags, TSDF and raycast must agree."""
+    print("case: frame loop 128^3 x 12 frames", flush=True)
+    depth, gt = seq20
+    g, cfg = make()
+    s = oracle.OracleSession(o32, 128, 2.0, cfg.mu, oracle.RG16I)
+    g.Reset(gt[0])
+    s.reset(gt[0])
+    for f in range(12):
+        g.setVolume(*oracle_volume_planes(s.volume(), "rg16i"))
+        g.setReferenceMaps(s.map(3), s.map(4))
+        g.setPose(s.pose())
+        g.frame(depth[f].astype(np.float32))
+        s.frame(depth[f].astype(np.float32))
+        P, Q = g.getPose(), s.pose()
+        st, ost = g.status(), s.stats()
+        assert st["tracked"] == ost["tracked"], f
+        assert st["reset"] == ost["reset"] and st["counter"] == ost["counter"]
+        assert np.abs(P[:3, 3] - Q[:3, 3]).max() < 1e-4, f
+        dR = P[:3, :3] @ Q[:3, :3].T
+        assert np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1)) < 1e-4, f
+        v, w = g.getVolume()
+        ov = s.volume()
+        dv = np.abs(v.astype(np.int32) - ov[..., 0].astype(np.int32)) / 32766.0
+        assert (dv > 1e-3).mean() < 1e-3, f
+        gv, ovm = g.getVerts(), s.map(3)
+        both = (gv[..., 3] > 0) & (ovm[..., 3] > 0)
+        assert ((gv[..., 3] > 0) != (ovm[..., 3] > 0)).mean() < 1e-2, f
+        assert np.percentile(np.abs(gv[both][:, :3] - ovm[both][:, :3]).max(axis=1), 99) <= 0.5e-3, f
+
+
+def test_frame_loop_512_properties(seq20):
+    """512^3 FP32 (the benchmark workload): size-independent checks."""
+    print("case: frame loop 512^3 f32", flush=True)
+    depth, gt = seq20
+    g, cfg = make(volume=512, storage="f32")
+    g.Reset(gt[0])
+    for f in range(8):
+        g.frame(depth[f].astype(np.float32))
+    st = g.status()
+    assert st["tracked"] == 1 and st["counter"] == 8 and st["reset"] == 0
+    P = g.getPose()
+    assert np.linalg.norm(P[:3, 3] - gt[7][:3, 3]) < 0.05
+    v, w = g.getVolume()
+    assert w.max() <= cfg.maxWeight and w.min() >= 0
+    assert v.min() >= -0.1 and v.max() <= 0.2
+    assert ((w > 0) == (w >= 1)).all()
+    rv = g.getVerts()
+    hit = rv[..., 3] > 0
+    assert hit.mean() > 0.5
+    assert (rv[hit][:, :3] >= -1e-3).all() and (rv[hit][:, :3] <= 2.0 + 1e-3).all()
