@@ -97,6 +97,9 @@ class Oracle:
         return np.asarray(v).reshape(4, 4).T.copy()
 
     # -- primitives --------------------------------------------------------
+    # Every converted array is bound to a local name before the C call: a
+    # temporary whose address was taken could be freed (and its memory reused
+    # by the next conversion) before the call reads it.
     def f2h(self, x):
         return self.L.kfo_f2h(float(x))
 
@@ -104,23 +107,23 @@ class Oracle:
         return self.L.kfo_h2f(int(h))
 
     def camera_matrix(self, cam):
-        out = np.zeros(16, self.dt)
-        self.L.kfo_camera_matrix(self.p(self.arr(cam)), self.p(out))
+        out, c = np.zeros(16, self.dt), self.arr(cam)
+        self.L.kfo_camera_matrix(self.p(c), self.p(out))
         return self.rm(out)
 
     def inverse_camera_matrix(self, cam):
-        out = np.zeros(16, self.dt)
-        self.L.kfo_inverse_camera_matrix(self.p(self.arr(cam)), self.p(out))
+        out, c = np.zeros(16, self.dt), self.arr(cam)
+        self.L.kfo_inverse_camera_matrix(self.p(c), self.p(out))
         return self.rm(out)
 
     def mul4(self, A, B):
-        out = np.zeros(16, self.dt)
-        self.L.kfo_mul4(self.p(self.cm(A)), self.p(self.cm(B)), self.p(out))
+        out, a, b = np.zeros(16, self.dt), self.cm(A), self.cm(B)
+        self.L.kfo_mul4(self.p(a), self.p(b), self.p(out))
         return self.rm(out)
 
     def inverse4(self, M):
-        out = np.zeros(16, self.dt)
-        self.L.kfo_inverse4(self.p(self.cm(M)), self.p(out))
+        out, m = np.zeros(16, self.dt), self.cm(M)
+        self.L.kfo_inverse4(self.p(m), self.p(out))
         return self.rm(out)
 
     def mip_depth(self, d):
@@ -131,10 +134,10 @@ class Oracle:
         return out
 
     def depth_to_vertex(self, d, cam_l):
-        d = self.arr(d)
+        d, c = self.arr(d), self.arr(cam_l)
         h, w = d.shape
         out = np.zeros((h, w, 4), self.dt)
-        self.L.kfo_depth_to_vertex(self.p(d), w, h, self.p(self.arr(cam_l)), self.p(out))
+        self.L.kfo_depth_to_vertex(self.p(d), w, h, self.p(c), self.p(out))
         return out
 
     def vertex_to_normal(self, v):
@@ -145,8 +148,8 @@ class Oracle:
         return out
 
     def track_view(self, pose, cam):
-        out = np.zeros(16, self.dt)
-        self.L.kfo_track_view(self.p(self.cm(pose)), self.p(self.arr(cam)), self.p(out))
+        out, P, c = np.zeros(16, self.dt), self.cm(pose), self.arr(cam)
+        self.L.kfo_track_view(self.p(P), self.p(c), self.p(out))
         return self.rm(out)
 
     def track(self, vert, norm, refV, refN, pose, cam, dist_th=0.05, normal_th=0.8):
@@ -154,35 +157,38 @@ class Oracle:
         h, w = vert.shape[:2]
         rh, rw = refV.shape[:2]
         view = self.cm(self.track_view(pose, cam))
+        P = self.cm(pose)
         res = np.zeros(h * w, np.int32)
         err = np.zeros(h * w, self.dt)
         J = np.zeros((h * w, 6), self.dt)
         self.L.kfo_track(self.p(vert), self.p(norm), w, h, self.p(refV), self.p(refN), rw, rh,
-                         self.p(self.cm(pose)), self.p(view), dist_th, normal_th,
+                         self.p(P), self.p(view), dist_th, normal_th,
                          self.p(res), self.p(err), self.p(J))
         return res, err, J
 
     def reduce(self, res, err, J, w, h):
         out = np.zeros(256, self.dt)
-        self.L.kfo_reduce(self.p(self.arr(res, np.int32)), self.p(self.arr(err)), self.p(self.arr(J)), w, h,
-                          self.p(out))
+        r, e, j = self.arr(res, np.int32), self.arr(err), self.arr(J)
+        self.L.kfo_reduce(self.p(r), self.p(e), self.p(j), w, h, self.p(out))
         return out.reshape(8, 32)
 
     def get_reduction(self, partials, rule=LAST_ROW):
         b = np.zeros(6, self.dt)
         Cm = np.zeros(36, self.dt)
         ae = np.zeros(1, self.dt)
-        self.L.kfo_get_reduction(self.p(self.arr(partials).reshape(256)), rule, self.p(b), self.p(Cm), self.p(ae))
+        pp = self.arr(partials).reshape(256)
+        self.L.kfo_get_reduction(self.p(pp), rule, self.p(b), self.p(Cm), self.p(ae))
         return b, Cm.reshape(6, 6), ae[0]
 
     def solve6(self, Cm, b):
         x = np.zeros(6, np.float64)
-        self.L.kfo_solve6(self.p(self.arr(Cm, np.float64).reshape(36)), self.p(self.arr(b, np.float64)), self.p(x))
+        c, bb = self.arr(Cm, np.float64).reshape(36), self.arr(b, np.float64)
+        self.L.kfo_solve6(self.p(c), self.p(bb), self.p(x))
         return x
 
     def pose_update(self, x, pose):
-        P = self.cm(pose)
-        self.L.kfo_pose_update(self.p(self.arr(x, np.float64)), self.p(P))
+        P, xx = self.cm(pose), self.arr(x, np.float64)
+        self.L.kfo_pose_update(self.p(xx), self.p(P))
         return self.rm(P)
 
     def voxel_bytes(self, mode):
@@ -192,19 +198,20 @@ class Oracle:
         return np.zeros((N[2], N[1], N[0], 2), _VOL_DTYPES[mode])
 
     def integrate(self, vol, mode, dim, depth_mm, pose, cam, mu, max_weight=100.0):
+        assert vol.flags.c_contiguous and vol.dtype == _VOL_DTYPES[mode]
         N = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], np.int32)
-        d = self.arr(depth_mm)
-        self.L.kfo_integrate(self.p(vol), mode, self.p(N), self.p(self.arr(dim)), self.p(d), d.shape[1], d.shape[0],
-                             self.p(self.cm(pose)), self.p(self.arr(cam)), mu, max_weight)
+        d, dd, P, c = self.arr(depth_mm), self.arr(dim), self.cm(pose), self.arr(cam)
+        self.L.kfo_integrate(self.p(vol), mode, self.p(N), self.p(dd), self.p(d), d.shape[1], d.shape[0],
+                             self.p(P), self.p(c), mu, max_weight)
         return vol
 
     def raycast(self, vol, mode, dim, pose, cam, near, far, step, large_step, w=512, h=424):
         N = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], np.int32)
         rv = np.zeros((h, w, 4), self.dt)
         rn = np.zeros((h, w, 4), self.dt)
-        self.L.kfo_raycast(self.p(np.ascontiguousarray(vol)), mode, self.p(N), self.p(self.arr(dim)),
-                           self.p(self.cm(pose)), self.p(self.arr(cam)), near, far, step, large_step, w, h,
-                           self.p(rv), self.p(rn))
+        v, dd, P, c = np.ascontiguousarray(vol), self.arr(dim), self.cm(pose), self.arr(cam)
+        self.L.kfo_raycast(self.p(v), mode, self.p(N), self.p(dd), self.p(P), self.p(c), near, far, step,
+                           large_step, w, h, self.p(rv), self.p(rn))
         return rv, rn
 
 
@@ -257,10 +264,12 @@ class OracleSession:
         return np.frombuffer(buf, dtype=dtype).reshape(shape)
 
     def reset(self, pose):
-        self.o.L.kfo_session_reset(self.s, self.o.p(self.o.cm(pose)))
+        P = self.o.cm(pose)
+        self.o.L.kfo_session_reset(self.s, self.o.p(P))
 
     def set_pose(self, pose):
-        self.o.L.kfo_session_set_pose(self.s, self.o.p(self.o.cm(pose)))
+        P = self.o.cm(pose)
+        self.o.L.kfo_session_set_pose(self.s, self.o.p(P))
 
     def pose(self):
         out = np.zeros(16, self.o.dt)
