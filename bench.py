@@ -34,28 +34,57 @@ def parse():
     ap.add_argument("--volume", type=int, default=512)
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--cpu-frames", type=int, default=6)
+    ap.add_argument("--fast-math", type=int, default=0)
     return ap.parse_args()
 
 
-def make_session(volume, storage, stream):
+def make_session(volume, storage, stream, fast_math):
     from oglfusionkinectv2_amd import GFusion, kinect_config
+    from oglfusionkinectv2_amd import _lib as K
     g = GFusion(stream=stream)
     g.setConfig(kinect_config(volume, 2.0, storage))
     g.allocateBuffers()
+    g.setOption(K.OPT_FAST_MATH, 1 if fast_math else 0)
     return g
 
 
-def run_loop(torch, L, g, frames_dev, poses, warmup, steps, stream, time_kernels):
-    """Runs warmup + steps frames; returns (elapsed_s, per-frame ms list,
-    per-kernel mean ms dict)."""
+def run_frames(torch, L, g, frames_dev, poses, warmup, steps, stream):
+    """Headline pass: the product path (kf_frame_u16_dev, one pyramid launch +
+    one graph replay per frame).  Returns elapsed seconds and per-frame
+    latencies from events recorded between frames on the session stream."""
     import ctypes as C
     from oglfusionkinectv2_amd import _lib as K
     s = g.session
     fb = frames_dev[0].numel() * 2
     base = frames_dev.data_ptr()
     g.Reset(poses[0])
-    g.synchronize()
+    for i in range(warmup):
+        K.check(L.kf_frame_u16_dev(s, C.c_void_p(base + i * fb)))
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(steps):
+        K.check(L.kf_frame_u16_dev(s, C.c_void_p(base + (warmup + i) * fb)))
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
+    return elapsed, lat
+
+
+def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, stream, count_every=25):
+    """Per-kernel pass: the same frames issued stage by stage with HIP events
+    recorded on the session stream around each stage.  Every `count_every`
+    frames the integrate traffic of that frame is counted (a separate,
+    synchronising pass outside the bracketed stages)."""
+    import ctypes as C
+    from oglfusionkinectv2_amd import _lib as K
+    s = g.session
+    fb = frames_dev[0].numel() * 2
+    base = frames_dev.data_ptr()
+    g.Reset(poses[0])
     ev = torch.cuda.Event
 
     def frame(i, marks):
@@ -78,34 +107,34 @@ def run_loop(torch, L, g, frames_dev, poses, warmup, steps, stream, time_kernels
     for i in range(warmup):
         frame(i, None)
     torch.cuda.synchronize()
-    marks = [[ev(enable_timing=True) for _ in range(5)] for _ in range(steps)] if time_kernels else None
-    t0 = time.perf_counter()
+    marks = [[ev(enable_timing=True) for _ in range(5)] for _ in range(steps)]
+    counts = []
     for i in range(steps):
-        frame(warmup + i, marks[i] if marks else None)
+        frame(warmup + i, marks[i])
+        if i % count_every == count_every - 1:
+            counts.append(g.countIntegrateTraffic())
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    per_frame, kern = [], {}
-    if marks:
-        names = ["pyramid", "icp", "integrate", "raycast"]
-        acc = {n: [] for n in names}
-        for m in marks:
-            per_frame.append(m[0].elapsed_time(m[4]))
-            for k, n in enumerate(names):
-                acc[n].append(m[k].elapsed_time(m[k + 1]))
-        kern = {n: float(np.mean(v)) for n, v in acc.items()}
-    return elapsed, per_frame, kern
+    names = ["pyramid", "icp", "integrate", "raycast"]
+    acc = {n: [] for n in names}
+    for m in marks:
+        for k, n in enumerate(names):
+            acc[n].append(m[k].elapsed_time(m[k + 1]))
+    return {n: float(np.mean(v)) for n, v in acc.items()}, counts
 
 
-def cpu_baseline(depth, poses, volume, frames):
-    """The FP32 C oracle on the same workload, `frames` frames after reset."""
+def cpu_baseline(depth, poses, volume, frames, warm=2):
+    """The FP32 C oracle on the same workload: `warm` untimed frames after
+    reset, then `frames` timed frames."""
     import oracle
     o = oracle.Oracle("f32")
     vox = np.float32(2.0) / np.float32(volume)
     mu = float(vox * np.float32(3.0))
     s = oracle.OracleSession(o, volume, 2.0, mu, oracle.F32X2)
     s.reset(poses[0])
+    for f in range(warm):
+        s.frame(depth[f].astype(np.float32))
     t0 = time.perf_counter()
-    for f in range(frames):
+    for f in range(warm, warm + frames):
         s.frame(depth[f].astype(np.float32))
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -132,25 +161,30 @@ def main():
     torch.cuda.synchronize()
 
     # --- FP32 (headline) ---------------------------------------------------
-    g = make_session(args.volume, "f32", stream.cuda_stream)
-    elapsed, per_frame, kern = run_loop(torch, L, g, frames_dev, poses, args.warmup, args.steps, stream, True)
+    g = make_session(args.volume, "f32", stream.cuda_stream, args.fast_math)
+    elapsed, lat = run_frames(torch, L, g, frames_dev, poses, args.warmup, args.steps, stream)
     fps = args.steps / elapsed
     st = g.status()
     est = g.getPose()
     ate_gt = float(np.linalg.norm(est[:3, 3] - poses[nframes - 1][:3, 3]))
-    # integrate traffic under the reference semantics, counted on the last frame
-    vr, vw = g.countIntegrateTraffic()
+    kern, counts = run_kernels(torch, L, g, frames_dev, poses, args.warmup, args.steps, stream)
+    # integrate traffic under the reference semantics (sampled frames): each
+    # read voxel moves value + weight in and out, each written-only voxel out;
+    # plus the depth image in metres
     bv = 8
+    vr = float(np.mean([c[0] for c in counts])) if counts else 0.0
+    vw = float(np.mean([c[1] for c in counts])) if counts else 0.0
     int_bytes = (vr + vw) * bv + 424 * 512 * 4
     dom = max(kern, key=kern.get)
     g.close()
 
     # --- FP16 --------------------------------------------------------------
-    fps16, kern16 = None, {}
+    fps16, kern16, lat16 = None, {}, None
     if not args.no_fp16:
-        g16 = make_session(args.volume, "f16", stream.cuda_stream)
-        e16, pf16, kern16 = run_loop(torch, L, g16, frames_dev, poses, args.warmup, args.steps, stream, True)
+        g16 = make_session(args.volume, "f16", stream.cuda_stream, args.fast_math)
+        e16, lat16 = run_frames(torch, L, g16, frames_dev, poses, args.warmup, args.steps, stream)
         fps16 = args.steps / e16
+        kern16, _ = run_kernels(torch, L, g16, frames_dev, poses, args.warmup, args.steps, stream)
         g16.close()
 
     # --- CPU baseline --------------------------------------------------------
@@ -158,8 +192,8 @@ def main():
     if not args.no_cpu:
         cfps, threads, cdt = cpu_baseline(depth, poses, args.volume, args.cpu_frames)
         cpu = {"value": cfps, "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": "frames 0-%d of the same %d^3 FP32 sequence (after reset), C oracle, "
-                         "%d OpenMP threads, %.1f s" % (args.cpu_frames - 1, args.volume, threads, cdt)}
+               "sample": "frames 2-%d of the same %d^3 FP32 sequence (frames 0-1 untimed), C oracle, "
+                         "%d OpenMP threads, %.1f s" % (args.cpu_frames + 1, args.volume, threads, cdt)}
 
     int_ms = kern["integrate"]
     roof = {
@@ -173,7 +207,7 @@ def main():
         "algorithmic_bytes": int(int_bytes),
         "avg_ms": int_ms,
     }
-    pf = np.array(per_frame)
+    pf = np.array(lat)
     out = {
         "metric": "frames/s per GPU at 512^3 TSDF (FP32 TSDF+FP32 weight, single session)",
         "value": fps,
@@ -195,6 +229,9 @@ def main():
         "latency_ms": {"p50": float(np.percentile(pf, 50)), "p99": float(np.percentile(pf, 99))},
         "kernel_ms": kern,
         "kernel_ms_fp16": kern16,
+        "latency_ms_fp16": ({"p50": float(np.percentile(lat16, 50)), "p99": float(np.percentile(lat16, 99))}
+                            if lat16 else None),
+        "fast_math": args.fast_math,
         "dominant_kernel": dom,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -149,6 +149,25 @@ int kf_read_status(kf_session *s, int *tracked, int *iterations_run, int *counte
 int kf_set_debug(kf_session *s, int record_track_pixels);
 int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error, float *J);
 
+/* Tuning options (no counterpart in the reference; all keep its semantics).
+ *   KF_OPT_FAST_MATH      integrate uses hardware reciprocal / square root
+ *                         (about 1 ulp) instead of correctly rounded division;
+ *                         pixel indices stay exact.  Default 0.
+ *   KF_OPT_SKIP_EMPTY     raycast skips samples in 8^3 bricks that hold no
+ *                         negative value (exact: same hits).  Default 1.
+ *   KF_OPT_GRAPH          kf_frame_* replays a captured hipGraph.  Default 1.
+ *   KF_OPT_BRICK_REBUILD_PERIOD  frames between exact brick-flag rebuilds
+ *                         (flags are otherwise only set).  Default 16, 0 = never. */
+#define KF_OPT_FAST_MATH 1
+#define KF_OPT_SKIP_EMPTY 2
+#define KF_OPT_GRAPH 3
+#define KF_OPT_BRICK_REBUILD_PERIOD 4
+int kf_set_option(kf_session *s, int option, int value);
+int kf_get_option(kf_session *s, int option, int *value);
+/* Recompute the brick occupancy flags from the volume (done automatically by
+ * kf_reset / kf_write_tsdf and periodically by kf_frame_*). */
+int kf_rebuild_bricks(kf_session *s);
+
 /* Synchronise the session stream / get its stream (hipStream_t). */
 int kf_synchronize(kf_session *s);
 int kf_get_stream(kf_session *s, void **stream);

@@ -33,6 +33,11 @@ MAP_REF_NORMAL = 4
 INTEGRATE_ALWAYS = 0
 INTEGRATE_GATED = 1
 
+OPT_FAST_MATH = 1
+OPT_SKIP_EMPTY = 2
+OPT_GRAPH = 3
+OPT_BRICK_REBUILD_PERIOD = 4
+
 
 class KfConfig(C.Structure):
     """kf_config of include/kfusion.h (gFusionConfig + camera + depth size)."""
@@ -109,6 +114,9 @@ def lib():
         "kf_synchronize": [sess],
         "kf_get_stream": [sess, C.POINTER(C.c_void_p)],
         "kf_count_integrate_traffic": [sess, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+        "kf_set_option": [sess, i32, i32],
+        "kf_get_option": [sess, i32, C.POINTER(C.c_int)],
+        "kf_rebuild_bricks": [sess],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -87,6 +87,7 @@ struct kf_session {
     bool own_stream = false;
     int device = 0;
     float *depth0 = nullptr;
+    float *depth_m = nullptr;
     uint16_t *stage_u16 = nullptr;
     float *stage_f32 = nullptr;
     float4 *vert[3] = {nullptr, nullptr, nullptr};
@@ -96,6 +97,19 @@ struct kf_session {
     kf::DevState *st = nullptr;
     float *block_partials = nullptr;
     unsigned long long *counts = nullptr;
+    uint8_t *bflags = nullptr, *dflags = nullptr;
+    int NB[3] = {0, 0, 0};
+    size_t nbricks = 0;
+    // options (kf_set_option)
+    int fast_math = 0;
+    int skip_empty = 1;
+    int use_graph = 1;
+    int rebuild_period = 16;
+    // frame graph: ICP + gate + integrate + raycast, captured once
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_failed = 0;
+    long frames = 0;
     int debug = 0;
     int dbg_level = -1;
     int32_t *dbg_result = nullptr;
@@ -112,12 +126,23 @@ size_t session_bytes(const kf_config *c)
     const size_t p2 = (size_t)(c->width >> 2) * (c->height >> 2);
     const size_t vox = (size_t)c->volume_size[0] * c->volume_size[1] * c->volume_size[2];
     size_t b = 0;
-    b += p0 * 4 + p0 * 2 + p0 * 4;                 // depth0 + staging
+    b += p0 * 4 * 2 + p0 * 2 + p0 * 4;             // depth0, depth_m + staging
     b += 2 * 16 * (p0 + p1 + p2);                  // vertex + normal pyramids
     b += 2 * 16 * p0;                              // reference maps
     b += vox * 2 * voxel_plane_bytes(c->storage);  // TSDF value + weight planes
     b += sizeof(kf::DevState) + (size_t)c->height * 28 * 4 + 64;
+    const size_t nb = (size_t)((c->volume_size[0] + 7) / 8) * ((c->volume_size[1] + 7) / 8) *
+                      ((c->volume_size[2] + 7) / 8);
+    b += 2 * nb;                                   // brick flags + dilated flags
     return b;
+}
+
+void drop_graph(kf_session *s)
+{
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
 }
 
 void free_all(kf_session *s)
@@ -125,7 +150,8 @@ void free_all(kf_session *s)
     auto f = [](void *p) {
         if (p) (void)hipFree(p);
     };
-    f(s->depth0); f(s->stage_u16); f(s->stage_f32);
+    drop_graph(s);
+    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags);
     for (int l = 0; l < 3; ++l) {
         f(s->vert[l]);
         f(s->norm[l]);
@@ -153,6 +179,7 @@ kf::PyramidArgs pyramid_args(kf_session *s)
     }
     a.blocks_total = start;
     a.depth0 = s->depth0;
+    a.depth_m = s->depth_m;
     return a;
 }
 
@@ -167,7 +194,7 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated)
     for (int k = 0; k < 4; ++k) a.cam[k] = c.cam[k];
     a.mu = c.mu;
     a.max_weight = c.max_weight;
-    a.depth = s->depth0;
+    a.depth_m = s->depth_m;
     a.dw = c.width;
     a.dh = c.height;
     a.tsdf = s->tsdf;
@@ -179,6 +206,8 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated)
     zlen = (zlen + 7) / 8 * 8;
     a.zlen = std::max(zlen, 8);
     a.gated = gated;
+    a.bflags = s->bflags;
+    for (int k = 0; k < 3; ++k) a.NB[k] = s->NB[k];
     return a;
 }
 
@@ -206,7 +235,33 @@ kf::RaycastArgs raycast_args(kf_session *s)
     a.blocks = (a.tiles_x * a.tiles_y + 7) / 8 * 8;
     a.refV = s->refV;
     a.refN = s->refN;
+    a.dflags = s->dflags;
+    for (int k = 0; k < 3; ++k) a.NB[k] = s->NB[k];
+    a.skip = s->skip_empty;
     return a;
+}
+
+kf::BrickArgs brick_args(kf_session *s)
+{
+    kf::BrickArgs a{};
+    for (int k = 0; k < 3; ++k) {
+        a.N[k] = s->cfg.volume_size[k];
+        a.NB[k] = s->NB[k];
+    }
+    a.tsdf = s->tsdf;
+    a.flags = s->bflags;
+    a.dflags = s->dflags;
+    return a;
+}
+
+// exact brick flags from the whole value plane, then the dilated copy
+int rebuild_bricks(kf_session *s)
+{
+    kf::BrickArgs a = brick_args(s);
+    hipError_t e = kf::launch_brick_rebuild(a, s->cfg.storage, s->stream);
+    if (e == hipSuccess) e = kf::launch_brick_dilate(a, s->stream);
+    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("brick rebuild: ") + hipGetErrorString(e));
+    return KF_OK;
 }
 
 int run_pyramid_from_stage(kf_session *s, const void *dev, int is_u16)
@@ -271,9 +326,63 @@ int run_track(kf_session *s, int frame)
 int run_integrate(kf_session *s, int gated)
 {
     kf::IntegrateArgs a = integrate_args(s, gated);
-    hipError_t e = kf::launch_integrate(a, s->cfg.storage, s->st, s->stream);
+    hipError_t e = kf::launch_integrate(a, s->cfg.storage, s->fast_math, s->st, s->stream);
+    if (e == hipSuccess) e = kf::launch_brick_dilate(brick_args(s), s->stream);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("integrate launch: ") + hipGetErrorString(e));
     return KF_OK;
+}
+
+int run_raycast(kf_session *s);
+
+// Everything of the frame block after the pyramid: ICP with the frame gate,
+// gated integrate (+ dilated brick flags), raycast.  Replayed from a hipGraph
+// captured on first use (the launches and their arguments never change for a
+// session); brick flags are rebuilt exactly every rebuild_period frames.
+int run_frame_direct(kf_session *s)
+{
+    int rc = run_track(s, 1);
+    if (!rc) rc = run_integrate(s, 1);
+    if (!rc) rc = run_raycast(s);
+    return rc;
+}
+
+int run_frame_body(kf_session *s)
+{
+    int rc = KF_OK;
+    if (s->use_graph && !s->graph_failed && !s->debug) {
+        if (!s->graph_exec) {
+            hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                rc = run_frame_direct(s);
+                hipGraph_t g = nullptr;
+                hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+                if (rc == KF_OK && e2 == hipSuccess && g) {
+                    s->graph = g;
+                    if (hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+                        drop_graph(s);
+                        s->graph_failed = 1;
+                    }
+                } else {
+                    if (g) (void)hipGraphDestroy(g);
+                    s->graph_failed = 1;
+                }
+            } else {
+                s->graph_failed = 1;
+            }
+            (void)hipGetLastError();
+        }
+        if (s->graph_exec) {
+            KF_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+        } else {
+            rc = run_frame_direct(s);
+        }
+    } else {
+        rc = run_frame_direct(s);
+    }
+    if (rc) return rc;
+    s->frames++;
+    if (s->rebuild_period > 0 && s->frames % s->rebuild_period == 0) rc = rebuild_bricks(s);
+    return rc;
 }
 
 int run_raycast(kf_session *s)
@@ -372,7 +481,11 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
     s->voxels = (size_t)cfg->volume_size[0] * cfg->volume_size[1] * cfg->volume_size[2];
     const size_t vb = s->voxels * voxel_plane_bytes(cfg->storage);
     auto alloc = [&](void **p, size_t n) { return hipMalloc(p, n) == hipSuccess; };
-    bool ok = alloc((void **)&s->depth0, p0 * 4) && alloc((void **)&s->stage_u16, p0 * 2) &&
+    for (int k = 0; k < 3; ++k) s->NB[k] = (cfg->volume_size[k] + 7) / 8;
+    s->nbricks = (size_t)s->NB[0] * s->NB[1] * s->NB[2];
+    bool ok = alloc((void **)&s->depth0, p0 * 4) && alloc((void **)&s->depth_m, p0 * 4) &&
+              alloc((void **)&s->bflags, s->nbricks) && alloc((void **)&s->dflags, s->nbricks) &&
+              alloc((void **)&s->stage_u16, p0 * 2) &&
               alloc((void **)&s->stage_f32, p0 * 4) && alloc((void **)&s->refV, p0 * 16) &&
               alloc((void **)&s->refN, p0 * 16) && alloc(&s->tsdf, vb) && alloc(&s->weight, vb) &&
               alloc((void **)&s->st, sizeof(kf::DevState)) &&
@@ -389,6 +502,9 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
              hipMemsetAsync(s->refV, 0, p0 * 16, s->stream) == hipSuccess &&
              hipMemsetAsync(s->refN, 0, p0 * 16, s->stream) == hipSuccess &&
              hipMemsetAsync(s->depth0, 0, p0 * 4, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->depth_m, 0, p0 * 4, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->bflags, 0, s->nbricks, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->dflags, 0, s->nbricks, s->stream) == hipSuccess &&
              hipMemsetAsync(s->st, 0, sizeof(kf::DevState), s->stream) == hipSuccess;
     for (int l = 0; l < 3 && z; ++l) {
         const size_t n = (size_t)(cfg->width >> l) * (cfg->height >> l);
@@ -425,6 +541,8 @@ int kf_reset_volume(kf_session *s)
     const size_t vb = s->voxels * voxel_plane_bytes(s->cfg.storage);
     KF_HIP(hipMemsetAsync(s->tsdf, 0, vb, s->stream));
     KF_HIP(hipMemsetAsync(s->weight, 0, vb, s->stream));
+    KF_HIP(hipMemsetAsync(s->bflags, 0, s->nbricks, s->stream));
+    KF_HIP(hipMemsetAsync(s->dflags, 0, s->nbricks, s->stream));
     return KF_OK;
 }
 
@@ -529,19 +647,49 @@ int kf_frame_u16_dev(kf_session *s, const uint16_t *depth_mm_dev)
 {
     if (!s || !depth_mm_dev) return fail(KF_ERR_INVALID, "NULL argument");
     int rc = run_pyramid_from_stage(s, depth_mm_dev, 1);
-    if (!rc) rc = run_track(s, 1);
-    if (!rc) rc = run_integrate(s, 1);
-    if (!rc) rc = run_raycast(s);
-    return rc;
+    return rc ? rc : run_frame_body(s);
 }
 
 int kf_frame_f32(kf_session *s, const float *depth_mm)
 {
     int rc = kf_depth_to_vertex_f32(s, depth_mm);
-    if (!rc) rc = run_track(s, 1);
-    if (!rc) rc = run_integrate(s, 1);
-    if (!rc) rc = run_raycast(s);
-    return rc;
+    return rc ? rc : run_frame_body(s);
+}
+
+int kf_set_option(kf_session *s, int option, int value)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    switch (option) {
+    case KF_OPT_FAST_MATH: s->fast_math = value ? 1 : 0; break;
+    case KF_OPT_SKIP_EMPTY: s->skip_empty = value ? 1 : 0; break;
+    case KF_OPT_GRAPH: s->use_graph = value ? 1 : 0; break;
+    case KF_OPT_BRICK_REBUILD_PERIOD:
+        if (value < 0) return fail(KF_ERR_INVALID, "rebuild period must be >= 0");
+        s->rebuild_period = value;
+        break;
+    default: return fail(KF_ERR_INVALID, "unknown option");
+    }
+    drop_graph(s);  // captured launches carry the old arguments
+    return KF_OK;
+}
+
+int kf_get_option(kf_session *s, int option, int *value)
+{
+    if (!s || !value) return fail(KF_ERR_INVALID, "NULL argument");
+    switch (option) {
+    case KF_OPT_FAST_MATH: *value = s->fast_math; break;
+    case KF_OPT_SKIP_EMPTY: *value = s->skip_empty; break;
+    case KF_OPT_GRAPH: *value = s->use_graph && !s->graph_failed; break;
+    case KF_OPT_BRICK_REBUILD_PERIOD: *value = s->rebuild_period; break;
+    default: return fail(KF_ERR_INVALID, "unknown option");
+    }
+    return KF_OK;
+}
+
+int kf_rebuild_bricks(kf_session *s)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    return rebuild_bricks(s);
 }
 
 int kf_read_map(kf_session *s, int which, int level, void *host, size_t bytes)
@@ -586,6 +734,8 @@ int kf_write_tsdf(kf_session *s, const void *value, const void *weight)
     const size_t vb = s->voxels * voxel_plane_bytes(s->cfg.storage);
     if (value) KF_HIP(hipMemcpyAsync(s->tsdf, value, vb, hipMemcpyHostToDevice, s->stream));
     if (weight) KF_HIP(hipMemcpyAsync(s->weight, weight, vb, hipMemcpyHostToDevice, s->stream));
+    int rc = rebuild_bricks(s);
+    if (rc) return rc;
     KF_HIP(hipStreamSynchronize(s->stream));
     return KF_OK;
 }
@@ -626,6 +776,7 @@ int kf_set_debug(kf_session *s, int record)
         KF_HIP(hipMemsetAsync(s->dbg_J, 0, p0 * 24, s->stream));
     }
     s->debug = record ? 1 : 0;
+    drop_graph(s);
     return KF_OK;
 }
 

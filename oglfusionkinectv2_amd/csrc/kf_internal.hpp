@@ -16,7 +16,8 @@ struct PyramidArgs {
     float cam[4];           // level-0 fx, fy, cx, cy
     int block_start[3];     // first block of each level in the 1-D grid
     int blocks_total;
-    float *depth0;          // level-0 depth (float mm), read by integrate
+    float *depth0;          // level-0 depth (float mm)
+    float *depth_m;         // level-0 depth / 1000 (metres), read by integrate
     float4 *vert[3];
     float4 *norm[3];
 };
@@ -39,11 +40,19 @@ struct IntegrateArgs {
     float dim[3];
     float cam[4];
     float mu, max_weight;
-    const float *depth;  // level-0 depth, float mm
+    const float *depth_m; // level-0 depth / 1000 (metres)
     int dw, dh;
-    void *tsdf, *weight; // planes
-    int zlen;            // z voxels per segment (blockIdx.z)
+    void *tsdf, *weight;  // planes
+    int zlen;             // z voxels per segment (blockIdx.z), multiple of 8
     int gated;
+    uint8_t *bflags;      // 8^3 brick occupancy flags (set-only)
+    int NB[3];            // brick grid
+};
+
+struct BrickArgs {
+    int N[3], NB[3];
+    const void *tsdf;
+    uint8_t *flags, *dflags;
 };
 
 struct RaycastArgs {
@@ -55,6 +64,9 @@ struct RaycastArgs {
     int w, h;
     int tiles_x, tiles_y, blocks;
     float4 *refV, *refN;
+    const uint8_t *dflags;  // dilated brick flags
+    int NB[3];
+    int skip;               // empty-space skipping on / off
 };
 
 hipError_t launch_pyramid_u16(const PyramidArgs &a, const uint16_t *in, hipStream_t s);
@@ -62,7 +74,9 @@ hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t
 hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s);
 hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s);
 hipError_t launch_state_set(DevState *st, const float *pose, int arm_reset, hipStream_t s);
-hipError_t launch_integrate(const IntegrateArgs &a, int storage, const DevState *st, hipStream_t s);
+hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s);
+hipError_t launch_brick_rebuild(const BrickArgs &a, int storage, hipStream_t s);
+hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s);
 hipError_t launch_count_integrate(const IntegrateArgs &a, const DevState *st, unsigned long long *counts,
                                   hipStream_t s);
 hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s);

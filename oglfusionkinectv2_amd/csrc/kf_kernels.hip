@@ -31,6 +31,8 @@ struct StoreRG16I {  // integrate.cs:37-50 (RG16I texture)
     __device__ static float raw(V v) { return (float)v; }
     __device__ static V from_value(float t) { return (V)(int)(t * 32766.0f); }
     __device__ static V from_weight(float w) { return (V)(int)w; }
+    // value that can end a ray (< 0) or reach the 0.8 large-step threshold
+    __device__ static bool interesting(V v) { return v < 0 || v >= (V)25886; }  // 0.79 * 32766
 };
 struct StoreF32 {
     using V = float;
@@ -40,6 +42,7 @@ struct StoreF32 {
     __device__ static float raw(V v) { return v; }
     __device__ static V from_value(float t) { return t; }
     __device__ static V from_weight(float w) { return w; }
+    __device__ static bool interesting(V v) { return v < 0.0f || v >= 0.79f; }
 };
 struct StoreF16 {
     using V = _Float16;
@@ -49,6 +52,7 @@ struct StoreF16 {
     __device__ static float raw(V v) { return (float)v; }
     __device__ static V from_value(float t) { return (V)t; }
     __device__ static V from_weight(float w) { return (V)w; }
+    __device__ static bool interesting(V v) { return (float)v < 0.0f || (float)v >= 0.79f; }
 };
 
 // ---------------------------------------------------------------------------
@@ -111,7 +115,10 @@ __global__ __launch_bounds__(256) void k_pyramid(PyramidArgs a, const DT *__rest
     inverse_camera_matrix(cam, iK);
 
     const float dc = depth_at(in, a.w, l, x, y);
-    if (l == 0) a.depth0[idx] = dc;
+    if (l == 0) {
+        a.depth0[idx] = dc;
+        a.depth_m[idx] = dc / 1000.0f;  // integrate.cs:74, once per pixel instead of per voxel
+    }
     float v[3];
     vertex_of(dc, iK, x, y, v);
     a.vert[l][idx] = make_float4(v[0], v[1], v[2], 0.0f);
@@ -498,7 +505,7 @@ __global__ void k_state_set(DevState *st, Mat16 pose, int set_pose, int arm_rese
 // ---------------------------------------------------------------------------
 // Integrate
 // ---------------------------------------------------------------------------
-constexpr int kIntU = 8;  // z voxels per pipelined chunk
+constexpr int kIntU = 8;  // z voxels per pipelined chunk (= brick depth)
 
 struct ColumnSetup {
     float pos[3], cx[3], dl[3], cd[3];
@@ -520,13 +527,43 @@ __device__ __forceinline__ void column_setup(const IntegrateArgs &a, const float
     rotate3(K, c.dl[0], c.dl[1], c.dl[2], c.cd);
 }
 
+__device__ __forceinline__ void column_step(ColumnSetup &c)
+{
+    c.pos[0] += c.dl[0]; c.pos[1] += c.dl[1]; c.pos[2] += c.dl[2];
+    c.cx[0] += c.cd[0]; c.cx[1] += c.cd[1]; c.cx[2] += c.cd[2];
+}
+
+// Projected pixel coordinate num/den + 0.5 (integrate.cs:67).  FAST uses the
+// hardware reciprocal and falls back to the correctly rounded quotient when
+// the result lies within a few ulps of an integer, so the pixel index and the
+// image-bounds decision are always those of the exact expression.
+template <bool FAST>
+__device__ __forceinline__ float proj_coord(float num, float den, float rden)
+{
+    if (!FAST) return num / den + 0.5f;
+    float p = num * rden + 0.5f;
+    const float ap = fabsf(p);
+    if (ap < 8192.0f) {
+        const float fr = p - floorf(p);
+        const float thr = fmaxf(ap * 6e-7f, 1e-30f);
+        if (fr < thr || fr > 1.0f - thr) p = num / den + 0.5f;
+    }
+    return p;
+}
+
 // One thread per (x, y) z-column, 64 consecutive x per wavefront (coalesced
 // plane accesses).  blockIdx.z splits the column into segments; a segment
 // replays the reference's incremental stepping up to its start (ALU only) so
 // every voxel sees exactly the positions of integrate.cs:62.  Each chunk of
 // kIntU voxels first resolves projection + depth (loads batched), then issues
-// the voxel loads it needs, then the updates and stores.
-template <class S>
+// the voxel loads it needs, then the updates and stores.  Every chunk that
+// stores a negative value marks its 8^3 brick in the occupancy flags that the
+// raycast uses to skip empty space (flags are only ever set here).
+//
+// FAST = false evaluates every division and square root correctly rounded, as
+// the reference; FAST = true uses hardware reciprocal / square root (about 1
+// ulp) for the signed distance and the weighted average.
+template <class S, bool FAST>
 __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevState *__restrict__ st)
 {
     if (a.gated && !st->integrate_now) return;
@@ -538,15 +575,16 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevSta
     column_setup(a, st->pose, x, y, c);
     const int z0 = blockIdx.z * a.zlen;
     const int z1 = min(z0 + a.zlen, a.N[2]);
-    for (int z = 0; z < z0; ++z) {
-        c.pos[0] += c.dl[0]; c.pos[1] += c.dl[1]; c.pos[2] += c.dl[2];
-        c.cx[0] += c.cd[0]; c.cx[1] += c.cd[1]; c.cx[2] += c.cd[2];
-    }
+    for (int z = 0; z < z0; ++z) column_step(c);
     typename S::V *__restrict__ tv = (typename S::V *)a.tsdf;
     typename S::V *__restrict__ wv = (typename S::V *)a.weight;
     const size_t plane = (size_t)a.N[0] * a.N[1];
     const size_t col = (size_t)y * a.N[0] + x;
     const float dmax_x = (float)(a.dw - 1), dmax_y = (float)(a.dh - 1);
+    const float inv_mu = 1.0f / a.mu;
+    uint8_t *__restrict__ bflags = a.bflags;
+    const size_t brow = (size_t)(y >> 3) * a.NB[0] + (x >> 3);
+    const size_t bplane = (size_t)a.NB[0] * a.NB[1];
 
     for (int zb = z0; zb < z1; zb += kIntU) {
         int pix[kIntU];
@@ -555,29 +593,35 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevSta
         for (int k = 0; k < kIntU; ++k) {
             bool ok = (zb + k) < z1;
             ok = ok && !(c.pos[2] < 0.0001f);
-            const float px = c.cx[0] / c.cx[2] + 0.5f;
-            const float py = c.cx[1] / c.cx[2] + 0.5f;
+            const float r = FAST ? __builtin_amdgcn_rcpf(c.cx[2]) : 0.0f;
+            const float px = proj_coord<FAST>(c.cx[0], c.cx[2], r);
+            const float py = proj_coord<FAST>(c.cx[1], c.cx[2], r);
             ok = ok && !(px < 0.0f || px > dmax_x || py < 0.0f || py > dmax_y);
             const int ix = (int)(ok ? px : 0.0f), iy = (int)(ok ? py : 0.0f);
             pix[k] = ok ? iy * a.dw + ix : -1;
             cz[k] = c.cx[2];
-            const float qx = c.pos[0] / c.pos[2], qy = c.pos[1] / c.pos[2];
-            fac[k] = sqrtf((1.0f + qx * qx) + qy * qy);
-            c.pos[0] += c.dl[0]; c.pos[1] += c.dl[1]; c.pos[2] += c.dl[2];
-            c.cx[0] += c.cd[0]; c.cx[1] += c.cd[1]; c.cx[2] += c.cd[2];
+            if (FAST) {
+                // pos.z == cameraX.z exactly (row 2 of K is (0, 0, 1, 0))
+                const float qx = c.pos[0] * r, qy = c.pos[1] * r;
+                fac[k] = __builtin_amdgcn_sqrtf((1.0f + qx * qx) + qy * qy);
+            } else {
+                const float qx = c.pos[0] / c.pos[2], qy = c.pos[1] / c.pos[2];
+                fac[k] = sqrtf((1.0f + qx * qx) + qy * qy);
+            }
+            column_step(c);
         }
         float dep[kIntU];
 #pragma unroll
-        for (int k = 0; k < kIntU; ++k) dep[k] = a.depth[pix[k] < 0 ? 0 : pix[k]];
+        for (int k = 0; k < kIntU; ++k) dep[k] = a.depth_m[pix[k] < 0 ? 0 : pix[k]];
         int kind[kIntU];
         float sdf[kIntU];
 #pragma unroll
         for (int k = 0; k < kIntU; ++k) {
-            const float d = dep[k] / 1000.0f;
+            const float d = dep[k];  // depth / 1000 (integrate.cs:74), precomputed per pixel
             const float diff = (d - cz[k]) * fac[k];
             const bool live = pix[k] >= 0 && !(d == 0.0f);
             kind[k] = !live ? 0 : (diff > -a.mu ? 2 : 1);
-            sdf[k] = gmin(1.0f, diff / a.mu);
+            sdf[k] = gmin(1.0f, FAST ? diff * inv_mu : diff / a.mu);
         }
         typename S::V ot[kIntU], ow[kIntU];
 #pragma unroll
@@ -588,6 +632,7 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevSta
                 ow[k] = wv[vi];
             }
         }
+        bool neg = false;
 #pragma unroll
         for (int k = 0; k < kIntU; ++k) {
             if (kind[k] == 0) continue;
@@ -595,15 +640,19 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevSta
             float t = 0.0f, w = 0.0f;
             if (kind[k] == 2) {
                 const float t0 = S::value(ot[k]), w0 = S::weight(ow[k]);
-                const float wd = (w0 * t0 + sdf[k]) / (w0 + 1.0f);
+                const float wd = FAST ? (w0 * t0 + sdf[k]) * __builtin_amdgcn_rcpf(w0 + 1.0f)
+                                      : (w0 * t0 + sdf[k]) / (w0 + 1.0f);
                 if (wd < 0.2f) {
                     t = gmin(gmax(wd, -0.1f), 0.2f);
                     w = gmin(w0 + 1.0f, a.max_weight);
                 }
             }
-            tv[vi] = S::from_value(t);
+            const typename S::V nt = S::from_value(t);
+            tv[vi] = nt;
             wv[vi] = S::from_weight(w);
+            neg = neg || S::interesting(nt);
         }
+        if (neg) bflags[(size_t)(zb >> 3) * bplane + brow] = 1;
     }
 }
 
@@ -625,7 +674,7 @@ __global__ __launch_bounds__(256) void k_count_integrate(IntegrateArgs a, const 
         const float py = c.cx[1] / c.cx[2] + 0.5f;
         ok = ok && !(px < 0.0f || px > dmax_x || py < 0.0f || py > dmax_y);
         if (ok) {
-            const float d = a.depth[(int)py * a.dw + (int)px] / 1000.0f;
+            const float d = a.depth_m[(int)py * a.dw + (int)px];
             if (!(d == 0.0f)) {
                 const float qx = c.pos[0] / c.pos[2], qy = c.pos[1] / c.pos[2];
                 const float diff = (d - c.cx[2]) * sqrtf((1.0f + qx * qx) + qy * qy);
@@ -633,11 +682,50 @@ __global__ __launch_bounds__(256) void k_count_integrate(IntegrateArgs a, const 
                 if (diff > -a.mu) nr += 1;
             }
         }
-        c.pos[0] += c.dl[0]; c.pos[1] += c.dl[1]; c.pos[2] += c.dl[2];
-        c.cx[0] += c.cd[0]; c.cx[1] += c.cd[1]; c.cx[2] += c.cd[2];
+        column_step(c);
     }
     atomicAdd(&counts[0], nr);
     atomicAdd(&counts[1], nw);
+}
+
+// Exact brick occupancy: a brick is flagged when any of its voxels holds a
+// value that can stop a ray (negative) or reach the 0.8 step switch.  One
+// 8x8 block per brick, each thread a z-run of 8 voxels.
+template <class S>
+__global__ __launch_bounds__(64) void k_brick_rebuild(BrickArgs a)
+{
+    const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    const int x = bx * 8 + (threadIdx.x & 7), y = by * 8 + (threadIdx.x >> 3);
+    bool any = false;
+    if (x < a.N[0] && y < a.N[1]) {
+        const typename S::V *v = (const typename S::V *)a.tsdf;
+        const size_t plane = (size_t)a.N[0] * a.N[1];
+        for (int k = 0; k < 8; ++k) {
+            const int z = bz * 8 + k;
+            if (z < a.N[2]) any = any || S::interesting(v[(size_t)z * plane + (size_t)y * a.N[0] + x]);
+        }
+    }
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) a.flags[((size_t)bz * a.NB[1] + by) * a.NB[0] + bx] = any ? 1 : 0;
+}
+
+// dflags(b) = OR of flags over b + {0,1}^3: a trilinear sample whose lower
+// corner lies in brick b reads only voxels of those 8 bricks.
+__global__ __launch_bounds__(256) void k_brick_dilate(BrickArgs a)
+{
+    const int n = a.NB[0] * a.NB[1] * a.NB[2];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int bx = i % a.NB[0], by = (i / a.NB[0]) % a.NB[1], bz = i / (a.NB[0] * a.NB[1]);
+    uint8_t any = 0;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int X = bx + dx, Y = by + dy, Z = bz + dz;
+                if (X < a.NB[0] && Y < a.NB[1] && Z < a.NB[2])
+                    any |= a.flags[((size_t)Z * a.NB[1] + Y) * a.NB[0] + X];
+            }
+    a.dflags[i] = any ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -661,28 +749,50 @@ __device__ __forceinline__ int floor_index(float fl)
     return (fl > -2147483648.0f && fl < 2147483648.0f) ? (int)fl : INT_MIN;
 }
 
-// interpVol: raycast.cs:69-82
-template <class S>
-__device__ __forceinline__ float interp_vol(const VolView<S> &V, const RaycastArgs &a, float p0, float p1, float p2)
-{
-    const float s[3] = {(p0 * (float)a.N[0] / a.dim[0]) - 0.5f, (p1 * (float)a.N[1] / a.dim[1]) - 0.5f,
-                        (p2 * (float)a.N[2] / a.dim[2]) - 0.5f};
+// interpVol: raycast.cs:69-82, split into the index setup (shared with the
+// empty-space test) and the trilinear evaluation.
+struct Sample {
     int lo[3], hi[3];
-    float f[3];
+    float f[3], s[3];
+    bool valid;
+};
+
+__device__ __forceinline__ void sample_setup(const RaycastArgs &a, float p0, float p1, float p2, Sample &sm)
+{
+    sm.s[0] = (p0 * (float)a.N[0] / a.dim[0]) - 0.5f;
+    sm.s[1] = (p1 * (float)a.N[1] / a.dim[1]) - 0.5f;
+    sm.s[2] = (p2 * (float)a.N[2] / a.dim[2]) - 0.5f;
+    sm.valid = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float fl = floorf(s[k]);
+        const float fl = floorf(sm.s[k]);
         const int b = floor_index(fl);
-        f[k] = s[k] - fl;
-        lo[k] = (b == INT_MIN) ? b : max(b, 0);
-        hi[k] = (b == INT_MIN) ? b : min(b + 1, a.N[k] - 1);
+        sm.f[k] = sm.s[k] - fl;
+        sm.valid = sm.valid && (b != INT_MIN);
+        sm.lo[k] = (b == INT_MIN) ? b : max(b, 0);
+        sm.hi[k] = (b == INT_MIN) ? b : min(b + 1, a.N[k] - 1);
     }
+}
+
+template <class S>
+__device__ __forceinline__ float sample_eval(const VolView<S> &V, const Sample &sm)
+{
+    const int *lo = sm.lo, *hi = sm.hi;
+    const float *f = sm.f;
     const float r =
         ((V.raw(lo[0], lo[1], lo[2]) * (1 - f[0]) + V.raw(hi[0], lo[1], lo[2]) * f[0]) * (1 - f[1]) +
          (V.raw(lo[0], hi[1], lo[2]) * (1 - f[0]) + V.raw(hi[0], hi[1], lo[2]) * f[0]) * f[1]) * (1 - f[2]) +
         ((V.raw(lo[0], lo[1], hi[2]) * (1 - f[0]) + V.raw(hi[0], lo[1], hi[2]) * f[0]) * (1 - f[1]) +
          (V.raw(lo[0], hi[1], hi[2]) * (1 - f[0]) + V.raw(hi[0], hi[1], hi[2]) * f[0]) * f[1]) * f[2];
     return S::kScaled ? r * 0.00003051944088f : r;
+}
+
+template <class S>
+__device__ __forceinline__ float interp_vol(const VolView<S> &V, const RaycastArgs &a, float p0, float p1, float p2)
+{
+    Sample sm;
+    sample_setup(a, p0, p1, p2, sm);
+    return sample_eval(V, sm);
 }
 
 // getGradient: raycast.cs:229-276
@@ -789,14 +899,65 @@ __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *
         float f_t = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
         float f_tt = 0.0f;
         if (f_t >= 0.0f) {
+            // Empty-space skipping.  A sample whose lower trilinear corner lies
+            // in a brick with a clear dilated flag reads only voxels in
+            // [0, 0.79): its value is >= 0 and < 0.8, so it can neither end
+            // the march nor change the step.  Such samples are not evaluated,
+            // but t still advances by the same float additions, so every
+            // evaluated sample sits at exactly the reference's t.
+            const float ds[3] = {d[0] * (float)a.N[0] / a.dim[0], d[1] * (float)a.N[1] / a.dim[1],
+                                 d[2] * (float)a.N[2] / a.dim[2]};
+            float t_prev = t, t_safe = -1.0f;
+            bool have_prev = true, cur_empty = false;
+            int cur_brick = -1;
             int guard = 0;
-            for (; t < tfar && guard < kMaxMarchSteps; t += stepsize, ++guard) {
-                f_tt = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
+            for (; t < tfar && guard < kMaxMarchSteps; ++guard) {
+                const bool can_skip = a.skip && stepsize == a.step;
+                if (can_skip && t < t_safe) {
+                    t_prev = t;
+                    have_prev = false;
+                    t += stepsize;
+                    continue;
+                }
+                Sample sm;
+                sample_setup(a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t, sm);
+                if (can_skip && sm.valid) {
+                    const int b0 = sm.lo[0] >> 3, b1 = sm.lo[1] >> 3, b2 = sm.lo[2] >> 3;
+                    const int bid = (b2 * a.NB[1] + b1) * a.NB[0] + b0;
+                    if (bid != cur_brick) {
+                        cur_brick = bid;
+                        cur_empty = a.dflags[bid] == 0;
+                    }
+                    if (cur_empty) {
+                        // t below which the lower corner stays in this brick
+                        // (0.01-voxel margin covers the rounding of p and s)
+                        float ts = INFINITY;
+                        const int bb[3] = {b0, b1, b2};
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            if (ds[k] > 0.0f) {
+                                ts = fminf(ts, t + ((float)(8 * bb[k] + 8) - 0.01f - sm.s[k]) / ds[k]);
+                            } else if (ds[k] < 0.0f && bb[k] > 0) {
+                                ts = fminf(ts, t + ((float)(8 * bb[k]) + 0.01f - sm.s[k]) / ds[k]);
+                            }
+                        }
+                        t_safe = ts;
+                        t_prev = t;
+                        have_prev = false;
+                        t += stepsize;
+                        continue;
+                    }
+                }
+                f_tt = sample_eval(V, sm);
                 if (f_tt < 0.0f) break;
                 if (f_tt < 0.8f) stepsize = a.step;
                 f_t = f_tt;
+                have_prev = true;
+                t_prev = t;
+                t += stepsize;
             }
             if (f_tt < 0.0f) {
+                if (!have_prev) f_t = interp_vol(V, a, o[0] + d[0] * t_prev, o[1] + d[1] * t_prev, o[2] + d[2] * t_prev);
                 t = t + stepsize * f_tt / (f_t - f_tt);
                 hp[0] = o[0] + d[0] * t;
                 hp[1] = o[1] + d[1] * t;
@@ -858,15 +1019,38 @@ static dim3 integrate_grid(const IntegrateArgs &a)
     return dim3((a.N[0] + 63) / 64, (a.N[1] + 3) / 4, (a.N[2] + a.zlen - 1) / a.zlen);
 }
 
-hipError_t launch_integrate(const IntegrateArgs &a, int storage, const DevState *st, hipStream_t s)
+hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s)
 {
     const dim3 g = integrate_grid(a), b(64, 4);
+#define KF_INT(ST)                                                                     \
+    if (fast) hipLaunchKernelGGL((k_integrate<ST, true>), g, b, 0, s, a, st);         \
+    else hipLaunchKernelGGL((k_integrate<ST, false>), g, b, 0, s, a, st);
     switch (storage) {
-    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_integrate<StoreRG16I>, g, b, 0, s, a, st); break;
-    case KF_STORAGE_F32: hipLaunchKernelGGL(k_integrate<StoreF32>, g, b, 0, s, a, st); break;
-    case KF_STORAGE_F16: hipLaunchKernelGGL(k_integrate<StoreF16>, g, b, 0, s, a, st); break;
+    case KF_STORAGE_RG16I: KF_INT(StoreRG16I); break;
+    case KF_STORAGE_F32: KF_INT(StoreF32); break;
+    case KF_STORAGE_F16: KF_INT(StoreF16); break;
     default: return hipErrorInvalidValue;
     }
+#undef KF_INT
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_rebuild(const BrickArgs &a, int storage, hipStream_t s)
+{
+    const dim3 g(a.NB[0], a.NB[1], a.NB[2]), b(64);
+    switch (storage) {
+    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_brick_rebuild<StoreRG16I>, g, b, 0, s, a); break;
+    case KF_STORAGE_F32: hipLaunchKernelGGL(k_brick_rebuild<StoreF32>, g, b, 0, s, a); break;
+    case KF_STORAGE_F16: hipLaunchKernelGGL(k_brick_rebuild<StoreF16>, g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s)
+{
+    const int n = a.NB[0] * a.NB[1] * a.NB[2];
+    hipLaunchKernelGGL(k_brick_dilate, dim3((n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -325,5 +325,17 @@ class GFusion:
         K.check(K.lib().kf_count_integrate_traffic(self.session, C.byref(r), C.byref(w)))
         return r.value, w.value
 
+    def setOption(self, option, value):
+        """kf_set_option: K.OPT_FAST_MATH, OPT_SKIP_EMPTY, OPT_GRAPH, OPT_BRICK_REBUILD_PERIOD."""
+        K.check(K.lib().kf_set_option(self.session, int(option), int(value)))
+
+    def getOption(self, option):
+        v = C.c_int()
+        K.check(K.lib().kf_get_option(self.session, int(option), C.byref(v)))
+        return v.value
+
+    def rebuildBricks(self):
+        K.check(K.lib().kf_rebuild_bricks(self.session))
+
     def synchronize(self):
         K.check(K.lib().kf_synchronize(self.session))

@@ -328,3 +328,98 @@ def test_frame_loop_512_properties(seq20):
     hit = rv[..., 3] > 0
     assert hit.mean() > 0.5
     assert (rv[hit][:, :3] >= -1e-3).all() and (rv[hit][:, :3] <= 2.0 + 1e-3).all()
+
+
+# ---------------------------------------------------------------------------
+# Tuning options keep the reference semantics
+# ---------------------------------------------------------------------------
+def _oracle_state(o32, seq20, storage, frames=6, volume=128):
+    depth, gt = seq20
+    vox = np.float32(2.0) / np.float32(volume)
+    s = oracle.OracleSession(o32, volume, 2.0, float(vox * np.float32(3)), STORAGE_ORACLE[storage])
+    s.reset(gt[0])
+    for f in range(frames):
+        s.frame(depth[f].astype(np.float32))
+    return s
+
+
+@pytest.mark.parametrize("storage", ["rg16i", "f32", "f16"])
+def test_raycast_skip_is_exact(o32, seq20, storage):
+    """Empty-space skipping must return the same bits as the full march."""
+    print("case: raycast skip vs full %s" % storage, flush=True)
+    s = _oracle_state(o32, seq20, storage)
+    g, cfg = make(storage=storage)
+    g.Reset(np.eye(4, dtype=np.float32))
+    g.setVolume(*oracle_volume_planes(s.volume(), storage))  # rebuilds the brick flags
+    g.setPose(s.pose())
+    g.setOption(K.OPT_SKIP_EMPTY, 0)
+    g.raycast()
+    full_v, full_n = g.getVerts(), g.getNorms()
+    g.setOption(K.OPT_SKIP_EMPTY, 1)
+    g.raycast()
+    np.testing.assert_array_equal(g.getVerts(), full_v)
+    np.testing.assert_array_equal(g.getNorms(), full_n)
+    assert (full_v[..., 3] > 0).mean() > 0.5
+
+
+def test_brick_flags_cover_every_negative_voxel(seq20):
+    """Set-only flags maintained by integrate are a superset of the exact ones."""
+    print("case: brick flags superset", flush=True)
+    depth, gt = seq20
+    g, cfg = make(volume=128, storage="f32")
+    g.Reset(gt[0])
+    g.setOption(K.OPT_BRICK_REBUILD_PERIOD, 0)
+    g.setOption(K.OPT_SKIP_EMPTY, 0)
+    for f in range(8):
+        g.frame(depth[f].astype(np.float32))
+    g.setOption(K.OPT_SKIP_EMPTY, 1)
+    g.raycast()
+    a_v = g.getVerts()  # with the set-only flags
+    g.setOption(K.OPT_SKIP_EMPTY, 0)
+    g.raycast()
+    np.testing.assert_array_equal(a_v, g.getVerts())
+
+
+def test_fast_math_integrate_within_tolerance(o32, seq20):
+    print("case: fast-math integrate", flush=True)
+    depth, gt = seq20
+    g, cfg = make(storage="f32")
+    g.Reset(gt[0])
+    g.setOption(K.OPT_FAST_MATH, 1)
+    vol = o32.new_volume((128, 128, 128), oracle.F32X2)
+    for f in range(6):
+        g.setPose(gt[f])
+        g.depthToVertex(depth[f])
+        g.integrate()
+        o32.integrate(vol, oracle.F32X2, (2.0,) * 3, depth[f].astype(np.float32), gt[f],
+                      (365.5, 365.5, 255.5, 211.5), cfg.mu, 100.0)
+    v, w = g.getVolume()
+    dv = np.abs(v.astype(np.float64) - vol[..., 0])
+    flips = (w != vol[..., 1])
+    print("fast-math: max |dv| %.3g, >1e-3: %.3g, weight flips %.3g" % (dv[~flips].max(), (dv > 1e-3).mean(),
+                                                                         flips.mean()))
+    assert flips.mean() < 1e-4
+    assert (dv[~flips] > 1e-3).mean() < 1e-5
+    assert np.percentile(dv[~flips], 99.9) < 1e-5
+
+
+def test_graph_replay_matches_direct_launches(seq20):
+    """The captured frame graph, replayed, gives the same state as direct
+    launches (several replays with host reads in between)."""
+    print("case: graph vs direct", flush=True)
+    depth, gt = seq20
+    runs = []
+    for use_graph in (0, 1):
+        g, cfg = make(storage="f32")
+        g.setOption(K.OPT_GRAPH, use_graph)
+        g.Reset(gt[0])
+        poses = []
+        for f in range(8):
+            g.frame(depth[f].astype(np.float32))
+            poses.append(g.getPose())
+        assert g.getOption(K.OPT_GRAPH) == use_graph
+        runs.append((np.array(poses), g.getVolume(), g.getVerts(), g.status()))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1][0], runs[1][1][0])
+    np.testing.assert_array_equal(runs[0][2], runs[1][2])
+    assert runs[0][3] == runs[1][3]
