@@ -162,6 +162,12 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
 #define KF_OPT_SKIP_EMPTY 2
 #define KF_OPT_GRAPH 3
 #define KF_OPT_BRICK_REBUILD_PERIOD 4
+/*   KF_OPT_RAYCAST_XCD_GROUP  raycast tiles per XCD cluster: block b runs on
+ *                         XCD b % 8 and receives clusters of this many
+ *                         consecutive 16x16 tiles.  Default 1 (interleaved). */
+#define KF_OPT_RAYCAST_XCD_GROUP 5
+/*   KF_OPT_ICP_ROWS_PER_BLOCK rows of one row group per ICP block.  Default 6. */
+#define KF_OPT_ICP_ROWS_PER_BLOCK 6
 int kf_set_option(kf_session *s, int option, int value);
 int kf_get_option(kf_session *s, int option, int *value);
 /* Recompute the brick occupancy flags from the volume (done automatically by
@@ -175,6 +181,11 @@ int kf_get_stream(kf_session *s, void **stream);
 /* Algorithmic traffic of integrate for the current depth frame and pose:
  * voxels read and voxels written under the reference semantics (no timing). */
 int kf_count_integrate_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *voxels_written);
+
+/* Runs raycast() once more with counters: samples evaluated, samples skipped
+ * as empty space, empty cells entered, rays that hit, the largest number of
+ * samples one ray evaluated, and the largest evaluated + cells of one ray. */
+int kf_raycast_stats(kf_session *s, uint64_t stats[6]);
 
 const char *kf_last_error(void);
 const char *kf_version(void);

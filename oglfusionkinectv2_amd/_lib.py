@@ -37,6 +37,8 @@ OPT_FAST_MATH = 1
 OPT_SKIP_EMPTY = 2
 OPT_GRAPH = 3
 OPT_BRICK_REBUILD_PERIOD = 4
+OPT_RAYCAST_XCD_GROUP = 5
+OPT_ICP_ROWS_PER_BLOCK = 6
 
 
 class KfConfig(C.Structure):
@@ -117,6 +119,7 @@ def lib():
         "kf_set_option": [sess, i32, i32],
         "kf_get_option": [sess, i32, C.POINTER(C.c_int)],
         "kf_rebuild_bricks": [sess],
+        "kf_raycast_stats": [sess, C.POINTER(C.c_uint64)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -76,6 +76,9 @@ int validate(const kf_config *c)
     const float maxxyz = (float)std::max(std::max(c->volume_size[0], c->volume_size[1]), c->volume_size[2]);
     if (!(minxyz / maxxyz >= 1e-5f)) return fail(KF_ERR_INVALID, "voxel step below 1e-5 m");
     if (!(0.75f * c->mu >= 1e-5f)) return fail(KF_ERR_INVALID, "0.75*mu below 1e-5 m");
+    // every t += step of the ray march must change t up to far_plane
+    if (!(minxyz / maxxyz >= c->far_plane * 2e-6f) || !(0.75f * c->mu >= c->far_plane * 2e-6f))
+        return fail(KF_ERR_INVALID, "ray step too small for far_plane");
     return KF_OK;
 }
 
@@ -97,14 +100,16 @@ struct kf_session {
     kf::DevState *st = nullptr;
     float *block_partials = nullptr;
     unsigned long long *counts = nullptr;
-    uint8_t *bflags = nullptr, *dflags = nullptr;
-    int NB[3] = {0, 0, 0};
-    size_t nbricks = 0;
+    uint8_t *bflags = nullptr, *dflags = nullptr, *cflags = nullptr;
+    int NB[3] = {0, 0, 0}, NC[3] = {0, 0, 0};
+    size_t nbricks = 0, ncoarse = 0;
     // options (kf_set_option)
     int fast_math = 0;
     int skip_empty = 1;
     int use_graph = 1;
     int rebuild_period = 16;
+    int xcd_group = 1;
+    int icp_rows_per_block = 6;
     // frame graph: ICP + gate + integrate + raycast, captured once
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -130,10 +135,10 @@ size_t session_bytes(const kf_config *c)
     b += 2 * 16 * (p0 + p1 + p2);                  // vertex + normal pyramids
     b += 2 * 16 * p0;                              // reference maps
     b += vox * 2 * voxel_plane_bytes(c->storage);  // TSDF value + weight planes
-    b += sizeof(kf::DevState) + (size_t)c->height * 28 * 4 + 64;
+    b += sizeof(kf::DevState) + (size_t)(c->height + 8) * 32 * 4 + 64;
     const size_t nb = (size_t)((c->volume_size[0] + 7) / 8) * ((c->volume_size[1] + 7) / 8) *
                       ((c->volume_size[2] + 7) / 8);
-    b += 2 * nb;                                   // brick flags + dilated flags
+    b += 3 * nb;                                   // brick flags, dilated, coarse (bound)
     return b;
 }
 
@@ -151,7 +156,7 @@ void free_all(kf_session *s)
         if (p) (void)hipFree(p);
     };
     drop_graph(s);
-    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags);
+    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags); f(s->cflags);
     for (int l = 0; l < 3; ++l) {
         f(s->vert[l]);
         f(s->norm[l]);
@@ -199,9 +204,11 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated)
     a.dh = c.height;
     a.tsdf = s->tsdf;
     a.weight = s->weight;
-    // enough z segments that the launch holds >= 256K columns-segments
+    // enough z segments that the launch holds ~1M column segments (16 waves
+    // per SIMD to hide the dependent depth -> voxel load latency)
     const long cols = (long)c.volume_size[0] * c.volume_size[1];
-    int segs = (int)std::min<long>(16, std::max<long>(1, (262144 + cols - 1) / cols));
+    int segs = (int)std::min<long>(std::max(1, c.volume_size[2] / 16),
+                                   std::max<long>(1, (1048576 + cols - 1) / cols));
     int zlen = (c.volume_size[2] + segs - 1) / segs;
     zlen = (zlen + 7) / 8 * 8;
     a.zlen = std::max(zlen, 8);
@@ -232,12 +239,25 @@ kf::RaycastArgs raycast_args(kf_session *s)
     a.h = c.height;
     a.tiles_x = (c.width + 15) / 16;
     a.tiles_y = (c.height + 15) / 16;
-    a.blocks = (a.tiles_x * a.tiles_y + 7) / 8 * 8;
+    a.xcd_group = std::max(1, s->xcd_group);
+    a.blocks = (a.tiles_x * a.tiles_y + 8 * a.xcd_group - 1) / (8 * a.xcd_group) * (8 * a.xcd_group);
+    a.off32 = s->voxels < (size_t(1) << 32) ? 1 : 0;
     a.refV = s->refV;
     a.refN = s->refN;
     a.dflags = s->dflags;
-    for (int k = 0; k < 3; ++k) a.NB[k] = s->NB[k];
+    a.cflags = s->cflags;
+    for (int k = 0; k < 3; ++k) {
+        a.NB[k] = s->NB[k];
+        a.NC[k] = s->NC[k];
+    }
     a.skip = s->skip_empty;
+    a.dim_pow2 = 1;
+    for (int k = 0; k < 3; ++k) {
+        int e;
+        const float m = std::frexp(c.volume_dim[k], &e);
+        if (m != 0.5f) a.dim_pow2 = 0;
+        a.inv_dim[k] = 1.0f / c.volume_dim[k];
+    }
     return a;
 }
 
@@ -247,10 +267,12 @@ kf::BrickArgs brick_args(kf_session *s)
     for (int k = 0; k < 3; ++k) {
         a.N[k] = s->cfg.volume_size[k];
         a.NB[k] = s->NB[k];
+        a.NC[k] = s->NC[k];
     }
     a.tsdf = s->tsdf;
     a.flags = s->bflags;
     a.dflags = s->dflags;
+    a.cflags = s->cflags;
     return a;
 }
 
@@ -291,6 +313,9 @@ int run_icp(kf_session *s, int level, int first, int save_pose)
     a.save_pose = save_pose;
     a.rule = s->cfg.reduce_rule;
     a.block_partials = s->block_partials;
+    // blocks per row group: about kIcpRowsPerBlock rows of the group per block
+    const int rows_per_group = (a.h + 7) / 8;
+    a.rows_step = std::max(1, (rows_per_group + s->icp_rows_per_block - 1) / s->icp_rows_per_block);
     if (s->debug) {
         a.dbg_result = s->dbg_result;
         a.dbg_error = s->dbg_error;
@@ -481,16 +506,21 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
     s->voxels = (size_t)cfg->volume_size[0] * cfg->volume_size[1] * cfg->volume_size[2];
     const size_t vb = s->voxels * voxel_plane_bytes(cfg->storage);
     auto alloc = [&](void **p, size_t n) { return hipMalloc(p, n) == hipSuccess; };
-    for (int k = 0; k < 3; ++k) s->NB[k] = (cfg->volume_size[k] + 7) / 8;
+    for (int k = 0; k < 3; ++k) {
+        s->NB[k] = (cfg->volume_size[k] + 7) / 8;
+        s->NC[k] = (s->NB[k] + 3) / 4;
+    }
     s->nbricks = (size_t)s->NB[0] * s->NB[1] * s->NB[2];
+    s->ncoarse = (size_t)s->NC[0] * s->NC[1] * s->NC[2];
     bool ok = alloc((void **)&s->depth0, p0 * 4) && alloc((void **)&s->depth_m, p0 * 4) &&
               alloc((void **)&s->bflags, s->nbricks) && alloc((void **)&s->dflags, s->nbricks) &&
+              alloc((void **)&s->cflags, s->ncoarse) &&
               alloc((void **)&s->stage_u16, p0 * 2) &&
               alloc((void **)&s->stage_f32, p0 * 4) && alloc((void **)&s->refV, p0 * 16) &&
               alloc((void **)&s->refN, p0 * 16) && alloc(&s->tsdf, vb) && alloc(&s->weight, vb) &&
               alloc((void **)&s->st, sizeof(kf::DevState)) &&
-              alloc((void **)&s->block_partials, (size_t)cfg->height * 28 * 4) &&
-              alloc((void **)&s->counts, 2 * sizeof(unsigned long long));
+              alloc((void **)&s->block_partials, (size_t)(cfg->height + 8) * 32 * 4) &&
+              alloc((void **)&s->counts, 8 * sizeof(unsigned long long));
     for (int l = 0; l < 3 && ok; ++l) {
         const size_t n = (size_t)(cfg->width >> l) * (cfg->height >> l);
         ok = alloc((void **)&s->vert[l], n * 16) && alloc((void **)&s->norm[l], n * 16);
@@ -505,6 +535,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
              hipMemsetAsync(s->depth_m, 0, p0 * 4, s->stream) == hipSuccess &&
              hipMemsetAsync(s->bflags, 0, s->nbricks, s->stream) == hipSuccess &&
              hipMemsetAsync(s->dflags, 0, s->nbricks, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->cflags, 0, s->ncoarse, s->stream) == hipSuccess &&
              hipMemsetAsync(s->st, 0, sizeof(kf::DevState), s->stream) == hipSuccess;
     for (int l = 0; l < 3 && z; ++l) {
         const size_t n = (size_t)(cfg->width >> l) * (cfg->height >> l);
@@ -543,6 +574,7 @@ int kf_reset_volume(kf_session *s)
     KF_HIP(hipMemsetAsync(s->weight, 0, vb, s->stream));
     KF_HIP(hipMemsetAsync(s->bflags, 0, s->nbricks, s->stream));
     KF_HIP(hipMemsetAsync(s->dflags, 0, s->nbricks, s->stream));
+    KF_HIP(hipMemsetAsync(s->cflags, 0, s->ncoarse, s->stream));
     return KF_OK;
 }
 
@@ -667,6 +699,14 @@ int kf_set_option(kf_session *s, int option, int value)
         if (value < 0) return fail(KF_ERR_INVALID, "rebuild period must be >= 0");
         s->rebuild_period = value;
         break;
+    case KF_OPT_RAYCAST_XCD_GROUP:
+        if (value < 1 || value > 4096) return fail(KF_ERR_INVALID, "xcd group must be in [1, 4096]");
+        s->xcd_group = value;
+        break;
+    case KF_OPT_ICP_ROWS_PER_BLOCK:
+        if (value < 1 || value > 1024) return fail(KF_ERR_INVALID, "rows per block must be in [1, 1024]");
+        s->icp_rows_per_block = value;
+        break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     drop_graph(s);  // captured launches carry the old arguments
@@ -681,6 +721,8 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_SKIP_EMPTY: *value = s->skip_empty; break;
     case KF_OPT_GRAPH: *value = s->use_graph && !s->graph_failed; break;
     case KF_OPT_BRICK_REBUILD_PERIOD: *value = s->rebuild_period; break;
+    case KF_OPT_RAYCAST_XCD_GROUP: *value = s->xcd_group; break;
+    case KF_OPT_ICP_ROWS_PER_BLOCK: *value = s->icp_rows_per_block; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     return KF_OK;
@@ -804,6 +846,20 @@ int kf_get_stream(kf_session *s, void **stream)
 {
     if (!s || !stream) return fail(KF_ERR_INVALID, "NULL argument");
     *stream = (void *)s->stream;
+    return KF_OK;
+}
+
+int kf_raycast_stats(kf_session *s, uint64_t stats[6])
+{
+    if (!s || !stats) return fail(KF_ERR_INVALID, "NULL argument");
+    KF_HIP(hipMemsetAsync(s->counts, 0, 6 * sizeof(unsigned long long), s->stream));
+    kf::RaycastArgs a = raycast_args(s);
+    a.stats = s->counts;
+    KF_HIP(kf::launch_raycast(a, s->cfg.storage, s->st, s->stream));
+    unsigned long long h[6];
+    KF_HIP(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < 6; ++k) stats[k] = h[k];
     return KF_OK;
 }
 

@@ -30,7 +30,8 @@ struct IcpArgs {
     float cam[4];
     float dist_th, normal_th;
     int first, save_pose, rule;
-    float *block_partials;      // [h][28]
+    float *block_partials;      // [8 * rows_step][32] (28 used)
+    int rows_step;              // blocks per row group
     int32_t *dbg_result;        // optional per-pixel output
     float *dbg_error, *dbg_J;
 };
@@ -50,9 +51,9 @@ struct IntegrateArgs {
 };
 
 struct BrickArgs {
-    int N[3], NB[3];
+    int N[3], NB[3], NC[3];
     const void *tsdf;
-    uint8_t *flags, *dflags;
+    uint8_t *flags, *dflags, *cflags;
 };
 
 struct RaycastArgs {
@@ -65,8 +66,14 @@ struct RaycastArgs {
     int tiles_x, tiles_y, blocks;
     float4 *refV, *refN;
     const uint8_t *dflags;  // dilated brick flags
-    int NB[3];
+    const uint8_t *cflags;  // 32^3 coarse cells (OR of dflags)
+    int NB[3], NC[3];
     int skip;               // empty-space skipping on / off
+    unsigned long long *stats;  // optional: evaluated, skipped, cells skipped, hits, max eval, max iters
+    int dim_pow2;               // all volume_dim are powers of two
+    float inv_dim[3];
+    int off32;                  // voxel count < 2^32 (32-bit offsets)
+    int xcd_group;              // tiles per XCD cluster (block -> tile mapping)
 };
 
 hipError_t launch_pyramid_u16(const PyramidArgs &a, const uint16_t *in, hipStream_t s);

@@ -15,6 +15,8 @@
 #include <float.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
 
@@ -154,6 +156,53 @@ __global__ __launch_bounds__(256) void k_pyramid(PyramidArgs a, const DT *__rest
 // ---------------------------------------------------------------------------
 // ICP: fused track + reduce + solve + update
 // ---------------------------------------------------------------------------
+
+// LDL^T solve of the symmetric 6x6 system in FP64, fully unrolled so that it
+// stays in registers.  Returns false when a pivot is not clearly positive
+// (singular or indefinite system), leaving the general solve to solve6.
+__device__ __forceinline__ bool ldlt6(const double (&A)[36], const double (&b)[6], double (&x)[6])
+{
+    double L[6][6], D[6], y[6];
+    double dmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) dmax = fmax(dmax, fabs(A[i * 6 + i]));
+    if (!(dmax > 0.0)) return false;
+    const double tol = dmax * 1e-12;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double d = A[j * 6 + j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k] * D[k];
+        ok = ok && (d > tol);
+        D[j] = d;
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+            double s = A[i * 6 + j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k] * D[k];
+            L[i][j] = s / d;
+        }
+    }
+    if (!ok) return false;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double s = b[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+        y[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) y[i] /= D[i];
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; ++k) s -= L[k][i] * x[k];
+        x[i] = s;
+    }
+    return true;
+}
 
 // Minimum-norm least-squares solve of the symmetric 6x6 system (the contract
 // of Eigen::JacobiSVD::solve at gFusion.cpp:696-697): LDL^T in FP64 for the
@@ -318,6 +367,7 @@ __device__ __forceinline__ int track_pixel(const IcpArgs &a, const float *T, con
 
 constexpr int kIcpThreads = 256;
 constexpr int kTerms = 28;  // e^2, e*J[6], upper-triangle J J^T [21]
+constexpr int kPartStride = 32;  // floats per block-partial row (16-byte aligned rows)
 
 // One block per image row; the block's row group is y mod 8 (the work group
 // of reduce.cs that owns the row).  Per-thread FP32 products, DPP wave sums,
@@ -328,11 +378,14 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevSta
     __shared__ float wsum[kIcpThreads / 64][kTerms];
     __shared__ int last;
     __shared__ double gsum[8][kTerms];
+    __shared__ double wgsum[kIcpThreads / 64][8][kTerms];
     __shared__ double solve_ws[kSolveWs + 36 + 6 + 6];
 
     if (!a.first && st->level_done) return;  // early exit of this level (uniform)
 
-    const int y = blockIdx.x;
+    // block b serves row group g = b % 8 (the reduce.cs work group that owns
+    // rows y = g mod 8) and takes every rows_step-th row of that group
+    const int g = blockIdx.x & 7, jb = blockIdx.x >> 3;
     float T[16], view[16];
     {
         float K[16], ip[16];
@@ -347,6 +400,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevSta
 #pragma unroll
     for (int k = 0; k < kTerms; ++k) acc[k] = 0.0f;
 
+    for (int y = g + 8 * jb; y < a.h; y += 8 * a.rows_step)
     for (int x = threadIdx.x; x < a.w; x += kIcpThreads) {
         float e = 0.0f, J[6] = {0, 0, 0, 0, 0, 0};
         const int r = track_pixel(a, T, view, x, y, e, J);
@@ -381,80 +435,122 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevSta
     if (threadIdx.x < kTerms) {
         const int k = threadIdx.x;
         const float s = (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]);
-        a.block_partials[y * kTerms + k] = s;
+        a.block_partials[blockIdx.x * kPartStride + k] = s;
     }
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
+        // one agent-scope release per block: the barrier above orders the
+        // block's partial stores before it
+        __threadfence();
         const unsigned t = atomicAdd(&st->ticket, 1u);
-        last = (t == (unsigned)(a.h - 1));
+        last = (t == gridDim.x - 1);
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
+    __threadfence();  // acquire: every block's partials are visible
 
-    // second pass: FP64 sums of the block partials per row group
-    if (threadIdx.x < 8 * kTerms) {
-        const int g = threadIdx.x / kTerms, k = threadIdx.x % kTerms;
-        double s = 0.0;
-        for (int row = g; row < a.h; row += 8)
-            s += (double)__hip_atomic_load(&a.block_partials[row * kTerms + k], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-        gsum[g][k] = s;
+    // second pass in FP64.  Thread t sums block partials t, t + 256, ... (all
+    // of row group t % 8), lanes of the same group are combined with xor
+    // shuffles (8, 16, 32), then the four waves through LDS.  Fixed order.
+    {
+        double ps[kTerms];
+#pragma unroll
+        for (int k = 0; k < kTerms; ++k) ps[k] = 0.0;
+        for (int pb = threadIdx.x; pb < (int)gridDim.x; pb += kIcpThreads) {
+            const float4 *rp = reinterpret_cast<const float4 *>(a.block_partials + pb * kPartStride);
+            float4 q[kTerms / 4];
+#pragma unroll
+            for (int m = 0; m < kTerms / 4; ++m) q[m] = rp[m];
+#pragma unroll
+            for (int m = 0; m < kTerms / 4; ++m) {
+                ps[4 * m + 0] += (double)q[m].x;
+                ps[4 * m + 1] += (double)q[m].y;
+                ps[4 * m + 2] += (double)q[m].z;
+                ps[4 * m + 3] += (double)q[m].w;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kTerms; ++k) {
+            ps[k] += __shfl_xor(ps[k], 8, 64);
+            ps[k] += __shfl_xor(ps[k], 16, 64);
+            ps[k] += __shfl_xor(ps[k], 32, 64);
+        }
+        if (lane < 8) {
+#pragma unroll
+            for (int k = 0; k < kTerms; ++k) wgsum[wave][lane][k] = ps[k];
+        }
     }
     __syncthreads();
+    if (threadIdx.x < 8 * kTerms) {
+        const int g = threadIdx.x / kTerms, k = threadIdx.x % kTerms;
+        gsum[g][k] = (wgsum[0][g][k] + wgsum[1][g][k]) + (wgsum[2][g][k] + wgsum[3][g][k]);
+    }
+    __syncthreads();
+    // exposed 8x32 partials in the reduce.cs layout (one slot per thread);
+    // reduce.cs counts every pixel of the group's rows once (:47-62, :101)
+    {
+        const int g = threadIdx.x >> 5, k = threadIdx.x & 31;
+        const int rows = (a.h > g) ? (a.h - g + 7) / 8 : 0;
+        float val = 0.0f;
+        if (k < kTerms) val = (float)gsum[g][k];
+        else if (k == kTerms) val = (float)((double)rows * (double)a.w);
+        st->partials[threadIdx.x] = val;
+    }
     if (threadIdx.x != 0) return;
 
-    // counts: reduce.cs counts every pixel of the group's rows once (:47-62, :101)
-    double cnt[8];
-    for (int g = 0; g < 8; ++g) {
-        const int rows = (a.h > g) ? (a.h - g + 7) / 8 : 0;
-        cnt[g] = (double)rows * (double)a.w;
-    }
-    for (int g = 0; g < 8; ++g) {
-        for (int k = 0; k < kTerms; ++k) st->partials[g * 32 + k] = (float)gsum[g][k];
-        st->partials[g * 32 + 28] = (float)cnt[g];
-        st->partials[g * 32 + 29] = 0.0f;
-        st->partials[g * 32 + 30] = 0.0f;
-        st->partials[g * 32 + 31] = 0.0f;
-    }
-    double *v = solve_ws + kSolveWs;  // 28 combined terms (LDS)
-    double *C = v;                     // reused after b is extracted: 36
-    double *b = solve_ws + kSolveWs + 36;
-    double *x = b + 6;
-    double count;
+    // the rest runs on one lane with everything in registers (the grid is
+    // smaller than the chip, so the register budget costs no occupancy)
+    double v[kTerms], count = 0.0;
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) v[k] = 0.0;
     if (a.rule == 1) {  // sum all rows
-        count = 0.0;
-        for (int k = 0; k < kTerms; ++k) v[k] = 0.0;
+#pragma unroll
         for (int g = 0; g < 8; ++g) {
+#pragma unroll
             for (int k = 0; k < kTerms; ++k) v[k] += gsum[g][k];
-            count += cnt[g];
+            count += (double)((a.h > g) ? (a.h - g + 7) / 8 : 0) * (double)a.w;
         }
     } else {  // reference: rows 1..7 assigned over row 0, only row 7 survives
+#pragma unroll
         for (int k = 0; k < kTerms; ++k) v[k] = gsum[7][k];
-        count = cnt[7];
+        count = (double)((a.h > 7) ? (a.h - 7 + 7) / 8 : 0) * (double)a.w;
     }
     const float ae = sqrtf((float)v[0] / (float)count);  // gFusion.cpp:1167
+    double b[6], C[36], x[6];
+#pragma unroll
     for (int k = 0; k < 6; ++k) b[k] = v[1 + k];
     {
-        double jtj[21];
-        for (int k = 0; k < 21; ++k) jtj[k] = v[7 + k];
-        int q = 0;
+        int q = 7;
+#pragma unroll
         for (int p = 0; p < 6; ++p)
+#pragma unroll
             for (int s2 = p; s2 < 6; ++s2) {
-                C[p * 6 + s2] = jtj[q];
-                C[s2 * 6 + p] = jtj[q];
+                C[p * 6 + s2] = v[q];
+                C[s2 * 6 + p] = v[q];
                 ++q;
             }
     }
-    solve6(C, b, x, solve_ws);
-    if (a.save_pose)
-        for (int i = 0; i < 16; ++i) st->old_pose[i] = st->pose[i];
+    if (!ldlt6(C, b, x)) {
+        // singular / indefinite: minimum-norm solve in LDS
+        double *Cl = solve_ws + kSolveWs, *bl = Cl + 36, *xl = bl + 6;
+        for (int k = 0; k < 36; ++k) Cl[k] = C[k];
+        for (int k = 0; k < 6; ++k) bl[k] = b[k];
+        solve6(Cl, bl, xl, solve_ws);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) x[k] = xl[k];
+    }
     float P[16];
+#pragma unroll
     for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
+    if (a.save_pose) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) st->old_pose[i] = P[i];
+    }
     pose_update(x, P);
+#pragma unroll
     for (int i = 0; i < 16; ++i) st->pose[i] = P[i];
     double nrm = 0.0;
+#pragma unroll
     for (int k = 0; k < 6; ++k) {
         nrm += x[k] * x[k];
         st->x[k] = x[k];
@@ -575,7 +671,17 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevSta
     column_setup(a, st->pose, x, y, c);
     const int z0 = blockIdx.z * a.zlen;
     const int z1 = min(z0 + a.zlen, a.N[2]);
-    for (int z = 0; z < z0; ++z) column_step(c);
+    if (FAST) {
+        // segment start evaluated directly (a few ulps from the stepped value)
+        const float zf = (float)z0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            c.pos[k] = fmaf(zf, c.dl[k], c.pos[k]);
+            c.cx[k] = fmaf(zf, c.cd[k], c.cx[k]);
+        }
+    } else {
+        for (int z = 0; z < z0; ++z) column_step(c);  // the reference's stepping, replayed
+    }
     typename S::V *__restrict__ tv = (typename S::V *)a.tsdf;
     typename S::V *__restrict__ wv = (typename S::V *)a.weight;
     const size_t plane = (size_t)a.N[0] * a.N[1];
@@ -656,6 +762,103 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevSta
     }
 }
 
+// Reduced-arithmetic integrate (KF_OPT_FAST_MATH).  Same update rule as
+// k_integrate, with the per-voxel arithmetic cut to about a third: one
+// hardware reciprocal of the camera depth per voxel, the projection folded to
+// px = fx * (x / z) + cx + 0.5 (row 2 of K is (0, 0, 1, 0), so cameraX.z is
+// pos.z), hardware square root, and reciprocals of mu and (w + 1).  Positions
+// are stepped from the segment start as in the reference.  Results agree with
+// the correctly rounded evaluation to a few ulps; voxels on a pixel edge or a
+// branch threshold can take the other side (counted as flips by the tests).
+template <class S, bool OFF32>
+__global__ __launch_bounds__(256) void k_integrate_fast(IntegrateArgs a, const DevState *__restrict__ st)
+{
+    using Idx = typename std::conditional<OFF32, uint32_t, size_t>::type;
+    if (a.gated && !st->integrate_now) return;
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= a.N[0] || y >= a.N[1]) return;
+
+    ColumnSetup c;
+    column_setup(a, st->pose, x, y, c);
+    const int z0 = blockIdx.z * a.zlen;
+    const int z1 = min(z0 + a.zlen, a.N[2]);
+    float p0 = fmaf((float)z0, c.dl[0], c.pos[0]);
+    float p1 = fmaf((float)z0, c.dl[1], c.pos[1]);
+    float p2 = fmaf((float)z0, c.dl[2], c.pos[2]);
+    const float fx = a.cam[0], fy = a.cam[1], cxh = a.cam[2] + 0.5f, cyh = a.cam[3] + 0.5f;
+    const float dmax_x = (float)(a.dw - 1), dmax_y = (float)(a.dh - 1);
+    const float inv_mu = 1.0f / a.mu, neg_mu = -a.mu;
+    typename S::V *__restrict__ tv = (typename S::V *)a.tsdf;
+    typename S::V *__restrict__ wv = (typename S::V *)a.weight;
+    const Idx plane = (Idx)a.N[0] * (Idx)a.N[1];
+    const Idx col = (Idx)y * (Idx)a.N[0] + (Idx)x;
+    const size_t brow = (size_t)(y >> 3) * a.NB[0] + (x >> 3);
+    const size_t bplane = (size_t)a.NB[0] * a.NB[1];
+
+    for (int zb = z0; zb < z1; zb += kIntU) {
+        int pix[kIntU];
+        float pz[kIntU], fac[kIntU];
+#pragma unroll
+        for (int k = 0; k < kIntU; ++k) {
+            const float r = __builtin_amdgcn_rcpf(p2);
+            const float qx = p0 * r, qy = p1 * r;
+            const float px = fmaf(fx, qx, cxh), py = fmaf(fy, qy, cyh);
+            // bitwise & keeps the predicate branch-free
+            const bool ok = ((zb + k) < z1) & !(p2 < 0.0001f) & (px >= 0.0f) & (px <= dmax_x) & (py >= 0.0f) &
+                            (py <= dmax_y);
+            const int ipx = (int)fminf(fmaxf(px, 0.0f), dmax_x), ipy = (int)fminf(fmaxf(py, 0.0f), dmax_y);
+            pix[k] = ok ? ipy * a.dw + ipx : -1;
+            pz[k] = p2;
+            fac[k] = __builtin_amdgcn_sqrtf(fmaf(qx, qx, fmaf(qy, qy, 1.0f)));
+            p0 += c.dl[0];
+            p1 += c.dl[1];
+            p2 += c.dl[2];
+        }
+        float dep[kIntU];
+#pragma unroll
+        for (int k = 0; k < kIntU; ++k) dep[k] = a.depth_m[pix[k] < 0 ? 0 : pix[k]];
+        int kind[kIntU];
+        float sdf[kIntU];
+#pragma unroll
+        for (int k = 0; k < kIntU; ++k) {
+            const float diff = (dep[k] - pz[k]) * fac[k];
+            const bool live = (pix[k] >= 0) & (dep[k] != 0.0f);
+            kind[k] = !live ? 0 : (diff > neg_mu ? 2 : 1);
+            sdf[k] = fminf(1.0f, diff * inv_mu);
+        }
+        typename S::V ot[kIntU], ow[kIntU];
+#pragma unroll
+        for (int k = 0; k < kIntU; ++k) {
+            if (kind[k] == 2) {
+                const Idx vi = col + (Idx)(zb + k) * plane;
+                ot[k] = tv[vi];
+                ow[k] = wv[vi];
+            }
+        }
+        bool neg = false;
+#pragma unroll
+        for (int k = 0; k < kIntU; ++k) {
+            if (kind[k] == 0) continue;
+            const Idx vi = col + (Idx)(zb + k) * plane;
+            float t = 0.0f, w = 0.0f;
+            if (kind[k] == 2) {
+                const float t0 = S::value(ot[k]), w0 = S::weight(ow[k]);
+                const float wd = fmaf(w0, t0, sdf[k]) * __builtin_amdgcn_rcpf(w0 + 1.0f);
+                if (wd < 0.2f) {
+                    t = fminf(fmaxf(wd, -0.1f), 0.2f);
+                    w = fminf(w0 + 1.0f, a.max_weight);
+                }
+            }
+            const typename S::V nt = S::from_value(t);
+            tv[vi] = nt;
+            wv[vi] = S::from_weight(w);
+            neg = neg || S::interesting(nt);
+        }
+        if (neg) a.bflags[(size_t)(zb >> 3) * bplane + brow] = 1;
+    }
+}
+
 // Same walk as k_integrate without memory traffic on the volume: counts the
 // voxels the reference semantics read (diff > -mu) and write (any live voxel).
 __global__ __launch_bounds__(256) void k_count_integrate(IntegrateArgs a, const DevState *__restrict__ st,
@@ -709,6 +912,21 @@ __global__ __launch_bounds__(64) void k_brick_rebuild(BrickArgs a)
     if (threadIdx.x == 0) a.flags[((size_t)bz * a.NB[1] + by) * a.NB[0] + bx] = any ? 1 : 0;
 }
 
+// cflags(c) = OR of dflags over the 4^3 bricks of the 32^3 cell c.
+__global__ __launch_bounds__(256) void k_brick_coarse(BrickArgs a)
+{
+    const int n = a.NC[0] * a.NC[1] * a.NC[2];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int cx = i % a.NC[0], cy = (i / a.NC[0]) % a.NC[1], cz = i / (a.NC[0] * a.NC[1]);
+    uint8_t any = 0;
+    for (int z = cz * 4; z < min(cz * 4 + 4, a.NB[2]); ++z)
+        for (int y = cy * 4; y < min(cy * 4 + 4, a.NB[1]); ++y)
+            for (int x = cx * 4; x < min(cx * 4 + 4, a.NB[0]); ++x)
+                any |= a.dflags[((size_t)z * a.NB[1] + y) * a.NB[0] + x];
+    a.cflags[i] = any ? 1 : 0;
+}
+
 // dflags(b) = OR of flags over b + {0,1}^3: a trilinear sample whose lower
 // corner lies in brick b reads only voxels of those 8 bricks.
 __global__ __launch_bounds__(256) void k_brick_dilate(BrickArgs a)
@@ -736,6 +954,8 @@ struct VolView {
     const typename S::V *__restrict__ v;
     int N0, N1, N2;
     size_t plane;
+    uint32_t plane32;
+    int off32;  // every voxel index fits in 32 bits
     __device__ __forceinline__ float raw(int x, int y, int z) const
     {
         if ((unsigned)x >= (unsigned)N0 || (unsigned)y >= (unsigned)N1 || (unsigned)z >= (unsigned)N2) return 0.0f;
@@ -759,9 +979,17 @@ struct Sample {
 
 __device__ __forceinline__ void sample_setup(const RaycastArgs &a, float p0, float p1, float p2, Sample &sm)
 {
-    sm.s[0] = (p0 * (float)a.N[0] / a.dim[0]) - 0.5f;
-    sm.s[1] = (p1 * (float)a.N[1] / a.dim[1]) - 0.5f;
-    sm.s[2] = (p2 * (float)a.N[2] / a.dim[2]) - 0.5f;
+    // p * N / dim; when every dim is a power of two the division is an exact
+    // scaling, done as a multiply by 1/dim (same result bit for bit)
+    if (a.dim_pow2) {
+        sm.s[0] = ((p0 * (float)a.N[0]) * a.inv_dim[0]) - 0.5f;
+        sm.s[1] = ((p1 * (float)a.N[1]) * a.inv_dim[1]) - 0.5f;
+        sm.s[2] = ((p2 * (float)a.N[2]) * a.inv_dim[2]) - 0.5f;
+    } else {
+        sm.s[0] = (p0 * (float)a.N[0] / a.dim[0]) - 0.5f;
+        sm.s[1] = (p1 * (float)a.N[1] / a.dim[1]) - 0.5f;
+        sm.s[2] = (p2 * (float)a.N[2] / a.dim[2]) - 0.5f;
+    }
     sm.valid = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -779,11 +1007,38 @@ __device__ __forceinline__ float sample_eval(const VolView<S> &V, const Sample &
 {
     const int *lo = sm.lo, *hi = sm.hi;
     const float *f = sm.f;
-    const float r =
-        ((V.raw(lo[0], lo[1], lo[2]) * (1 - f[0]) + V.raw(hi[0], lo[1], lo[2]) * f[0]) * (1 - f[1]) +
-         (V.raw(lo[0], hi[1], lo[2]) * (1 - f[0]) + V.raw(hi[0], hi[1], lo[2]) * f[0]) * f[1]) * (1 - f[2]) +
-        ((V.raw(lo[0], lo[1], hi[2]) * (1 - f[0]) + V.raw(hi[0], lo[1], hi[2]) * f[0]) * (1 - f[1]) +
-         (V.raw(lo[0], hi[1], hi[2]) * (1 - f[0]) + V.raw(hi[0], hi[1], hi[2]) * f[0]) * f[1]) * f[2];
+    float c000, c100, c010, c110, c001, c101, c011, c111;
+    const bool inside = (unsigned)lo[0] < (unsigned)V.N0 && (unsigned)hi[0] < (unsigned)V.N0 &&
+                        (unsigned)lo[1] < (unsigned)V.N1 && (unsigned)hi[1] < (unsigned)V.N1 &&
+                        (unsigned)lo[2] < (unsigned)V.N2 && (unsigned)hi[2] < (unsigned)V.N2;
+    if (V.off32 && inside) {
+        // all 8 corners in the volume: 32-bit element offsets from one base
+        const uint32_t b = ((uint32_t)lo[2] * V.plane32 + (uint32_t)lo[1] * (uint32_t)V.N0) + (uint32_t)lo[0];
+        const uint32_t dx = (uint32_t)(hi[0] - lo[0]);
+        const uint32_t dy = (uint32_t)(hi[1] - lo[1]) * (uint32_t)V.N0;
+        const uint32_t dz = (uint32_t)(hi[2] - lo[2]) * V.plane32;
+        c000 = S::raw(V.v[b]);
+        c100 = S::raw(V.v[b + dx]);
+        c010 = S::raw(V.v[b + dy]);
+        c110 = S::raw(V.v[b + dy + dx]);
+        c001 = S::raw(V.v[b + dz]);
+        c101 = S::raw(V.v[b + dz + dx]);
+        c011 = S::raw(V.v[b + dz + dy]);
+        c111 = S::raw(V.v[b + dz + dy + dx]);
+    } else {  // corners may be out of bounds: they read 0 (image load semantics)
+        c000 = V.raw(lo[0], lo[1], lo[2]);
+        c100 = V.raw(hi[0], lo[1], lo[2]);
+        c010 = V.raw(lo[0], hi[1], lo[2]);
+        c110 = V.raw(hi[0], hi[1], lo[2]);
+        c001 = V.raw(lo[0], lo[1], hi[2]);
+        c101 = V.raw(hi[0], lo[1], hi[2]);
+        c011 = V.raw(lo[0], hi[1], hi[2]);
+        c111 = V.raw(hi[0], hi[1], hi[2]);
+    }
+    const float r = ((c000 * (1 - f[0]) + c100 * f[0]) * (1 - f[1]) + (c010 * (1 - f[0]) + c110 * f[0]) * f[1]) *
+                        (1 - f[2]) +
+                    ((c001 * (1 - f[0]) + c101 * f[0]) * (1 - f[1]) + (c011 * (1 - f[0]) + c111 * f[0]) * f[1]) *
+                        f[2];
     return S::kScaled ? r * 0.00003051944088f : r;
 }
 
@@ -850,6 +1105,7 @@ __device__ void gradient(const VolView<S> &V, const RaycastArgs &a, const float 
 }
 
 constexpr int kMaxMarchSteps = 1 << 20;
+constexpr int kBatch = 4;  // fine-step samples evaluated per march iteration
 
 // One thread per pixel; 16x16 pixel tiles per block.  Block ids are remapped
 // so that the blocks an XCD receives (id mod 8) cover one contiguous band of
@@ -857,9 +1113,12 @@ constexpr int kMaxMarchSteps = 1 << 20;
 template <class S>
 __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *__restrict__ st)
 {
+    // block b runs on XCD b % 8; XCD x receives the clusters of xcd_group
+    // consecutive tiles x, x + 8, x + 16, ... (1 = fully interleaved)
     const int nb = a.tiles_x * a.tiles_y;
-    const int per = a.blocks / 8;
-    const int tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int G = a.xcd_group;
+    const int j = blockIdx.x / 8;
+    const int tile = ((blockIdx.x % 8) + 8 * (j / G)) * G + (j % G);
     if (tile >= nb) return;
     const int x = (tile % a.tiles_x) * 16 + (threadIdx.x & 15);
     const int y = (tile / a.tiles_x) * 16 + (threadIdx.x >> 4);
@@ -873,7 +1132,8 @@ __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *
         inverse_camera_matrix(a.cam, iK);
         mul4(P, iK, view);  // m_pose * invK (gFusion.cpp:1298)
     }
-    VolView<S> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1]};
+    VolView<S> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
+                 (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32};
 
     const float o[3] = {view[12], view[13], view[14]};
     float d[3];
@@ -894,6 +1154,7 @@ __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *
 
     bool hit = false;
     float hp[3], th = 0.0f;
+    unsigned n_eval = 0, n_skip = 0, n_brick = 0;  // statistics mode only
     if (tnear < tfar) {
         float t = tnear, stepsize = a.large_step;
         float f_t = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
@@ -907,54 +1168,93 @@ __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *
             // evaluated sample sits at exactly the reference's t.
             const float ds[3] = {d[0] * (float)a.N[0] / a.dim[0], d[1] * (float)a.N[1] / a.dim[1],
                                  d[2] * (float)a.N[2] / a.dim[2]};
-            float t_prev = t, t_safe = -1.0f;
-            bool have_prev = true, cur_empty = false;
-            int cur_brick = -1;
+            float t_prev = t;
+            bool have_prev = true;
+            n_eval++;
             int guard = 0;
             for (; t < tfar && guard < kMaxMarchSteps; ++guard) {
                 const bool can_skip = a.skip && stepsize == a.step;
-                if (can_skip && t < t_safe) {
-                    t_prev = t;
-                    have_prev = false;
-                    t += stepsize;
-                    continue;
-                }
                 Sample sm;
                 sample_setup(a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t, sm);
                 if (can_skip && sm.valid) {
+                    // two-level occupancy: 32^3 coarse cells, then 8^3 bricks
+                    // (both flags loaded together)
                     const int b0 = sm.lo[0] >> 3, b1 = sm.lo[1] >> 3, b2 = sm.lo[2] >> 3;
-                    const int bid = (b2 * a.NB[1] + b1) * a.NB[0] + b0;
-                    if (bid != cur_brick) {
-                        cur_brick = bid;
-                        cur_empty = a.dflags[bid] == 0;
-                    }
-                    if (cur_empty) {
-                        // t below which the lower corner stays in this brick
+                    const uint8_t cf = a.cflags[((b2 >> 2) * a.NC[1] + (b1 >> 2)) * a.NC[0] + (b0 >> 2)];
+                    const uint8_t ff = a.dflags[(b2 * a.NB[1] + b1) * a.NB[0] + b0];
+                    const int shift = cf == 0 ? 5 : (ff == 0 ? 3 : 0);
+                    if (shift) {
+                        // t below which the lower corner stays in the empty cell
                         // (0.01-voxel margin covers the rounding of p and s)
+                        const float span = (float)(1 << shift);
                         float ts = INFINITY;
-                        const int bb[3] = {b0, b1, b2};
 #pragma unroll
                         for (int k = 0; k < 3; ++k) {
-                            if (ds[k] > 0.0f) {
-                                ts = fminf(ts, t + ((float)(8 * bb[k] + 8) - 0.01f - sm.s[k]) / ds[k]);
-                            } else if (ds[k] < 0.0f && bb[k] > 0) {
-                                ts = fminf(ts, t + ((float)(8 * bb[k]) + 0.01f - sm.s[k]) / ds[k]);
-                            }
+                            const float base = (float)((sm.lo[k] >> shift) << shift);
+                            if (ds[k] > 0.0f)
+                                ts = fminf(ts, t + (base + span - 0.01f - sm.s[k]) / ds[k]);
+                            else if (ds[k] < 0.0f && base > 0.0f)
+                                ts = fminf(ts, t + (base + 0.01f - sm.s[k]) / ds[k]);
                         }
-                        t_safe = ts;
-                        t_prev = t;
+                        // this sample and every later one below ts: same float steps as the reference
+                        do {
+                            t_prev = t;
+                            t += stepsize;
+                            n_skip++;
+                        } while (t < ts && t < tfar && ++guard < kMaxMarchSteps);
                         have_prev = false;
-                        t += stepsize;
+                        n_brick++;
                         continue;
                     }
                 }
-                f_tt = sample_eval(V, sm);
-                if (f_tt < 0.0f) break;
-                if (f_tt < 0.8f) stepsize = a.step;
-                f_t = f_tt;
-                have_prev = true;
-                t_prev = t;
-                t += stepsize;
+                if (stepsize != a.step) {
+                    // still on the large step: one sample at a time (raycast.cs:185-212)
+                    f_tt = sample_eval(V, sm);
+                    n_eval++;
+                    if (f_tt < 0.0f) break;
+                    if (f_tt < 0.8f) stepsize = a.step;
+                    f_t = f_tt;
+                    have_prev = true;
+                    t_prev = t;
+                    t += stepsize;
+                    continue;
+                }
+                // fine step: evaluate kBatch consecutive samples at once (their
+                // loads are independent), then consume them in march order.
+                float tj[kBatch], fv[kBatch];
+                tj[0] = t;
+                fv[0] = sample_eval(V, sm);
+#pragma unroll
+                for (int j = 1; j < kBatch; ++j) {
+                    tj[j] = tj[j - 1] + stepsize;
+                    Sample sj;
+                    sample_setup(a, o[0] + d[0] * tj[j], o[1] + d[1] * tj[j], o[2] + d[2] * tj[j], sj);
+                    fv[j] = sample_eval(V, sj);
+                }
+                bool stop = false;
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) {
+                    if (!stop) {
+                        if (!(tj[j] < tfar)) {  // the march ends before this sample
+                            t = tj[j];
+                            stop = true;
+                        } else {
+                            f_tt = fv[j];
+                            n_eval++;
+                            if (f_tt < 0.0f) {
+                                t = tj[j];
+                                stop = true;
+                            } else {
+                                f_t = f_tt;  // f_tt < 0.8 keeps stepsize = step
+                                have_prev = true;
+                                t_prev = tj[j];
+                            }
+                        }
+                    }
+                }
+                if (stop) break;
+                t = tj[kBatch - 1] + stepsize;
+                guard += kBatch - 1;
             }
             if (f_tt < 0.0f) {
                 if (!have_prev) f_t = interp_vol(V, a, o[0] + d[0] * t_prev, o[1] + d[1] * t_prev, o[2] + d[2] * t_prev);
@@ -981,6 +1281,14 @@ __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *
         a.refV[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         a.refN[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    if (a.stats) {
+        atomicAdd(&a.stats[0], (unsigned long long)n_eval);
+        atomicAdd(&a.stats[1], (unsigned long long)n_skip);
+        atomicAdd(&a.stats[2], (unsigned long long)n_brick);
+        atomicAdd(&a.stats[3], (unsigned long long)(hit && th > 0.0f));
+        atomicMax(&a.stats[4], (unsigned long long)n_eval);
+        atomicMax(&a.stats[5], (unsigned long long)(n_eval + n_brick));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -998,7 +1306,9 @@ hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t
 }
 hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_icp_iteration, dim3(a.h), dim3(kIcpThreads), 0, s, a, st);
+    // 8 row groups x rows_step blocks; each block takes every rows_step-th row
+    // of its group (fewer, fuller blocks: one release + one ticket each)
+    hipLaunchKernelGGL(k_icp_iteration, dim3(8 * a.rows_step), dim3(kIcpThreads), 0, s, a, st);
     return hipGetLastError();
 }
 hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s)
@@ -1022,8 +1332,10 @@ static dim3 integrate_grid(const IntegrateArgs &a)
 hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s)
 {
     const dim3 g = integrate_grid(a), b(64, 4);
+    const bool off32 = (size_t)a.N[0] * a.N[1] * a.N[2] < (size_t(1) << 32);
 #define KF_INT(ST)                                                                     \
-    if (fast) hipLaunchKernelGGL((k_integrate<ST, true>), g, b, 0, s, a, st);         \
+    if (fast && off32) hipLaunchKernelGGL((k_integrate_fast<ST, true>), g, b, 0, s, a, st); \
+    else if (fast) hipLaunchKernelGGL((k_integrate_fast<ST, false>), g, b, 0, s, a, st);  \
     else hipLaunchKernelGGL((k_integrate<ST, false>), g, b, 0, s, a, st);
     switch (storage) {
     case KF_STORAGE_RG16I: KF_INT(StoreRG16I); break;
@@ -1051,6 +1363,8 @@ hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s)
 {
     const int n = a.NB[0] * a.NB[1] * a.NB[2];
     hipLaunchKernelGGL(k_brick_dilate, dim3((n + 255) / 256), dim3(256), 0, s, a);
+    const int nc = a.NC[0] * a.NC[1] * a.NC[2];
+    hipLaunchKernelGGL(k_brick_coarse, dim3((nc + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -334,6 +334,11 @@ class GFusion:
         K.check(K.lib().kf_get_option(self.session, int(option), C.byref(v)))
         return v.value
 
+    def raycastStats(self):
+        st = (C.c_uint64 * 6)()
+        K.check(K.lib().kf_raycast_stats(self.session, st))
+        return dict(evaluated=st[0], skipped=st[1], bricks=st[2], hits=st[3], max_eval=st[4], max_iter=st[5])
+
     def rebuildBricks(self):
         K.check(K.lib().kf_rebuild_bricks(self.session))
 

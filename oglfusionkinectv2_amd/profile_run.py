@@ -16,12 +16,23 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
+def _stream_of(g):
+    from oglfusionkinectv2_amd import _lib as K
+    h = C.c_void_p()
+    K.check(K.lib().kf_get_stream(g.session, C.byref(h)))
+    return h.value
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--volume", type=int, default=512)
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--frames", type=int, default=40)
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--stats", type=int, default=0, help="print raycast / integrate statistics every N frames")
+    ap.add_argument("--fast-math", type=int, default=0)
+    ap.add_argument("--xcd-group", type=int, default=1)
+    ap.add_argument("--time", type=int, default=0, help="time raycast per frame with events and print the mean")
     args = ap.parse_args()
 
     import torch
@@ -34,12 +45,53 @@ def main():
     g = GFusion()
     g.setConfig(kinect_config(args.volume, 2.0, args.storage))
     g.allocateBuffers()
+    g.setOption(K.OPT_FAST_MATH, args.fast_math)
+    g.setOption(K.OPT_RAYCAST_XCD_GROUP, args.xcd_group)
     g.Reset(poses[0])
     fb = depth[0].nbytes
     for i in range(args.frames):
         K.check(L.kf_frame_u16_dev(g.session, C.c_void_p(dev.data_ptr() + i * fb)))
+        if args.stats and i % args.stats == args.stats - 1:
+            st = g.raycastStats()
+            vr, vw = g.countIntegrateTraffic()
+            err = np.linalg.norm(g.getPose()[:3, 3] - poses[i][:3, 3])
+            print("frame %d: raycast eval %.1f skip %.1f cells %.1f per ray (max eval %d, max eval+cells %d), "
+                  "hits %.3f; integrate read %d written %d; err %.3f m"
+                  % (i, st["evaluated"] / 217088, st["skipped"] / 217088, st["bricks"] / 217088, st["max_eval"],
+                     st["max_iter"], st["hits"] / 217088, vr, vw, err), flush=True)
     g.synchronize()
     print("frames done:", args.frames, "status", g.status(), flush=True)
+    if args.time:
+        for grp in (1, 2, 4, 8, 16, 27):
+            g.setOption(K.OPT_RAYCAST_XCD_GROUP, grp)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            K.check(L.kf_raycast(g.session))
+            g.synchronize()
+            stream = torch.cuda.ExternalStream(_stream_of(g))
+            e0.record(stream)
+            for _ in range(args.time):
+                K.check(L.kf_raycast(g.session))
+            e1.record(stream)
+            torch.cuda.synchronize()
+            g.synchronize()
+            print("raycast xcd_group %d: %.1f us" % (grp, e0.elapsed_time(e1) * 1000 / args.time), flush=True)
+        g.setOption(K.OPT_RAYCAST_XCD_GROUP, args.xcd_group)
+        pose = g.getPose()
+        for rpb in (1, 2, 4, 6, 8, 14, 27):
+            g.setOption(K.OPT_ICP_ROWS_PER_BLOCK, rpb)
+            g.setPose(pose)
+            K.check(L.kf_track_frame(g.session))
+            g.synchronize()
+            stream = torch.cuda.ExternalStream(_stream_of(g))
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.time):
+                g.setPose(pose)
+                K.check(L.kf_track_frame(g.session))
+            e1.record(stream)
+            torch.cuda.synchronize()
+            print("icp rows_per_block %d: %.1f us per Track (%d iterations)"
+                  % (rpb, e0.elapsed_time(e1) * 1000 / args.time, g.status()["iterations"]), flush=True)
     n = args.copy_mib * 1024 * 1024 // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
