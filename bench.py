@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=6)
-    ap.add_argument("--fast-math", type=int, default=0)
+    ap.add_argument("--fast-math", type=int, default=1)
     return ap.parse_args()
 
 
@@ -144,8 +144,8 @@ def cpu_baseline(depth, poses, volume, frames, warm=2):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (sessions shard "
-              "across GPUs with no communication, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: this benchmark runs on one GPU; --gpus %d is not supported (sessions are "
+              "independent and shard across GPUs with no communication)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
     from oglfusionkinectv2_amd import _lib as K

@@ -151,8 +151,9 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
 
 /* Tuning options (no counterpart in the reference; all keep its semantics).
  *   KF_OPT_FAST_MATH      integrate uses hardware reciprocal / square root
- *                         (about 1 ulp) instead of correctly rounded division;
- *                         pixel indices stay exact.  Default 0.
+ *                         (about 1 ulp; GLSL itself allows 2.5 ulp division)
+ *                         instead of correctly rounded division.  Default 1;
+ *                         0 reproduces the CPU oracle bit for bit.
  *   KF_OPT_SKIP_EMPTY     raycast skips samples in 8^3 bricks that hold no
  *                         negative value (exact: same hits).  Default 1.
  *   KF_OPT_GRAPH          kf_frame_* replays a captured hipGraph.  Default 1.
@@ -164,9 +165,9 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
 #define KF_OPT_BRICK_REBUILD_PERIOD 4
 /*   KF_OPT_RAYCAST_XCD_GROUP  raycast tiles per XCD cluster: block b runs on
  *                         XCD b % 8 and receives clusters of this many
- *                         consecutive 16x16 tiles.  Default 1 (interleaved). */
+ *                         consecutive 16x16 tiles.  Default 4. */
 #define KF_OPT_RAYCAST_XCD_GROUP 5
-/*   KF_OPT_ICP_ROWS_PER_BLOCK rows of one row group per ICP block.  Default 6. */
+/*   KF_OPT_ICP_ROWS_PER_BLOCK rows of one row group per ICP block.  Default 2. */
 #define KF_OPT_ICP_ROWS_PER_BLOCK 6
 int kf_set_option(kf_session *s, int option, int value);
 int kf_get_option(kf_session *s, int option, int *value);

@@ -104,12 +104,12 @@ struct kf_session {
     int NB[3] = {0, 0, 0}, NC[3] = {0, 0, 0};
     size_t nbricks = 0, ncoarse = 0;
     // options (kf_set_option)
-    int fast_math = 0;
+    int fast_math = 1;
     int skip_empty = 1;
     int use_graph = 1;
     int rebuild_period = 16;
-    int xcd_group = 1;
-    int icp_rows_per_block = 6;
+    int xcd_group = 4;
+    int icp_rows_per_block = 2;
     // frame graph: ICP + gate + integrate + raycast, captured once
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -892,24 +892,35 @@ __global__ __launch_bounds__(256) void k_count_integrate(IntegrateArgs a, const 
 }
 
 // Exact brick occupancy: a brick is flagged when any of its voxels holds a
-// value that can stop a ray (negative) or reach the 0.8 step switch.  One
-// 8x8 block per brick, each thread a z-run of 8 voxels.
+// value that can stop a ray (negative) or reach the 0.8 step switch.  A block
+// covers 64 x 8 columns (8 bricks along x) and one brick layer in z; each
+// wavefront reads one row of 64 consecutive voxels per z (coalesced), a
+// ballot splits it into the 8 bricks, and LDS ORs the 8 rows.
 template <class S>
-__global__ __launch_bounds__(64) void k_brick_rebuild(BrickArgs a)
+__global__ __launch_bounds__(512) void k_brick_rebuild(BrickArgs a)
 {
-    const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
-    const int x = bx * 8 + (threadIdx.x & 7), y = by * 8 + (threadIdx.x >> 3);
-    bool any = false;
+    __shared__ unsigned int any[8];
+    if (threadIdx.x < 8) any[threadIdx.x] = 0u;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lane, y = blockIdx.y * 8 + row, bz = blockIdx.z;
+    bool hit = false;
     if (x < a.N[0] && y < a.N[1]) {
         const typename S::V *v = (const typename S::V *)a.tsdf;
         const size_t plane = (size_t)a.N[0] * a.N[1];
+        const size_t base = (size_t)y * a.N[0] + x;
+#pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int z = bz * 8 + k;
-            if (z < a.N[2]) any = any || S::interesting(v[(size_t)z * plane + (size_t)y * a.N[0] + x]);
+            if (z < a.N[2]) hit = hit || S::interesting(v[(size_t)z * plane + base]);
         }
     }
-    any = __syncthreads_or(any);
-    if (threadIdx.x == 0) a.flags[((size_t)bz * a.NB[1] + by) * a.NB[0] + bx] = any ? 1 : 0;
+    const unsigned long long m = __ballot(hit);
+    if (lane < 8 && ((m >> (8 * lane)) & 0xFFull)) atomicOr(&any[lane], 1u);
+    __syncthreads();
+    const int bx = blockIdx.x * 8 + (int)threadIdx.x;
+    if (threadIdx.x < 8 && bx < a.NB[0])
+        a.flags[((size_t)bz * a.NB[1] + blockIdx.y) * a.NB[0] + bx] = any[threadIdx.x] ? 1 : 0;
 }
 
 // cflags(c) = OR of dflags over the 4^3 bricks of the 32^3 cell c.
@@ -1349,7 +1360,7 @@ hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const
 
 hipError_t launch_brick_rebuild(const BrickArgs &a, int storage, hipStream_t s)
 {
-    const dim3 g(a.NB[0], a.NB[1], a.NB[2]), b(64);
+    const dim3 g((a.N[0] + 63) / 64, a.NB[1], a.NB[2]), b(512);
     switch (storage) {
     case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_brick_rebuild<StoreRG16I>, g, b, 0, s, a); break;
     case KF_STORAGE_F32: hipLaunchKernelGGL(k_brick_rebuild<StoreF32>, g, b, 0, s, a); break;

@@ -30,8 +30,8 @@ def main():
     ap.add_argument("--frames", type=int, default=40)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--stats", type=int, default=0, help="print raycast / integrate statistics every N frames")
-    ap.add_argument("--fast-math", type=int, default=0)
-    ap.add_argument("--xcd-group", type=int, default=1)
+    ap.add_argument("--fast-math", type=int, default=1)
+    ap.add_argument("--xcd-group", type=int, default=4)
     ap.add_argument("--time", type=int, default=0, help="time raycast per frame with events and print the mean")
     args = ap.parse_args()
 

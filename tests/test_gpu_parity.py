@@ -1,6 +1,7 @@
 """Parity of the HIP path (through the C ABI) against the C oracle.
 
-Tolerances (stated by the north star for the full loop; tighter where the two
+Tolerances (for the full loop: pose 1e-4 rad / 0.1 mm per frame, TSDF 1e-3 of truncation,
+raycast vertex 0.5 mm; tighter where the two
 sides evaluate identical FP32 expressions):
   pyramid / per-pixel track / integrate (RG16I, F32) : bit-exact expected,
       test allows <= 1e-6 relative and <= 0.01 % differing voxels;
@@ -23,13 +24,16 @@ from oglfusionkinectv2_amd import GFusion, kinect_config, synth  # noqa: E402
 STORAGE_ORACLE = {"rg16i": oracle.RG16I, "f32": oracle.F32X2, "f16": oracle.F16X2}
 
 
-def make(volume=128, dim=2.0, storage="rg16i", size=(512, 424), cam=None, iterations=(4, 5, 10), rule=0):
+def make(volume=128, dim=2.0, storage="rg16i", size=(512, 424), cam=None, iterations=(4, 5, 10), rule=0, fast=0):
+    """fast=0: correctly rounded integrate arithmetic (bit-exact with the
+    oracle); fast=1: the library default (hardware reciprocal / sqrt)."""
     g = GFusion(size=size)
     cfg = kinect_config(volume, dim, storage, iterations=iterations, reduce_rule=rule)
     g.setConfig(cfg)
     if cam is not None:
         g.setCameraParams(cam)
     g.allocateBuffers()
+    g.setOption(K.OPT_FAST_MATH, fast)
     return g, cfg
 
 
@@ -171,10 +175,11 @@ def test_track_rejects_empty_and_far_frames(o32, seq20):
 
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("storage", ["rg16i", "f32", "f16"])
-def test_integrate_matches_oracle(o32, seq20, storage):
-    print("case: integrate %s" % storage, flush=True)
+@pytest.mark.parametrize("fast", [0, 1])
+def test_integrate_matches_oracle(o32, seq20, storage, fast):
+    print("case: integrate %s fast=%d" % (storage, fast), flush=True)
     depth, gt = seq20
-    g, cfg = make(storage=storage)
+    g, cfg = make(storage=storage, fast=fast)
     g.Reset(gt[0])
     mode = STORAGE_ORACLE[storage]
     vol = o32.new_volume((128, 128, 128), mode)
@@ -191,10 +196,11 @@ def test_integrate_matches_oracle(o32, seq20, storage):
         vf, ovf = vf / 32766.0, ovf / 32766.0
     diff = np.abs(vf - ovf)
     frac_exact = (diff == 0).mean()
-    print("exact voxel fraction", frac_exact)
-    assert (w.astype(np.float64) != ow.astype(np.float64)).mean() < 1e-4
-    assert (diff > 1e-3).mean() < 1e-4
-    assert frac_exact > 0.9999
+    flips = (w.astype(np.float64) != ow.astype(np.float64)) | (diff > 1e-3)
+    print("exact voxel fraction", frac_exact, "flips", flips.mean())
+    assert flips.mean() < 1e-4  # pixel / branch decisions taken the other way
+    if not fast:
+        assert frac_exact > 0.9999  # same expressions, correctly rounded: same bits
 
 
 def test_integrate_zero_depth_is_noop():
@@ -275,13 +281,14 @@ def test_raycast_misses_outside_box():
 
 
 # ---------------------------------------------------------------------------
-def test_frame_loop_matches_oracle(o32, seq20):
+@pytest.mark.parametrize("fast", [0, 1])
+def test_frame_loop_matches_oracle(o32, seq20, fast):
     """Per-frame parity: before every frame the GPU session is loaded with the
     oracle's full state (volume, reference maps, pose), then both run the same
     frame; pose, tracking decision, frame flags, TSDF and raycast must agree."""
-    print("case: frame loop 128^3 x 12 frames", flush=True)
+    print("case: frame loop 128^3 x 12 frames fast=%d" % fast, flush=True)
     depth, gt = seq20
-    g, cfg = make()
+    g, cfg = make(fast=fast)
     s = oracle.OracleSession(o32, 128, 2.0, cfg.mu, oracle.RG16I)
     g.Reset(gt[0])
     s.reset(gt[0])
