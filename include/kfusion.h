@@ -188,6 +188,24 @@ int kf_count_integrate_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *v
  * samples one ray evaluated, and the largest evaluated + cells of one ray. */
 int kf_raycast_stats(kf_session *s, uint64_t stats[6]);
 
+/* ---- multi-session batches -------------------------------------------------
+ * No counterpart in the reference (one session per process there).  A batch
+ * runs the frame block of many sessions with one launch per kernel type:
+ * sessions share depth size, ICP schedule, storage type and options; volume
+ * sizes may differ.  Each session's results are the same bits as running it
+ * alone with kf_frame_u16_dev.  The batch synchronises the sessions' own
+ * streams when created / re-filled and then queues on its own stream; call
+ * kf_batch_synchronize before using a member session alone again. */
+typedef struct kf_batch kf_batch;
+int kf_batch_create(kf_session *const *sessions, int n, void *stream, kf_batch **out); /* n <= 64 */
+int kf_batch_set_sessions(kf_batch *b, kf_session *const *sessions, int n);
+int kf_batch_destroy(kf_batch *b);
+int kf_batch_synchronize(kf_batch *b);
+int kf_batch_get_stream(kf_batch *b, void **stream);
+/* One frame for every session i with active[i] != 0 (active NULL: all), its
+ * 512x424 uint16 mm depth at depth_mm_dev[i] (device memory). */
+int kf_batch_frame_u16_dev(kf_batch *b, const uint16_t *const *depth_mm_dev, const int *active);
+
 const char *kf_last_error(void);
 const char *kf_version(void);
 

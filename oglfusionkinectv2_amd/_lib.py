@@ -129,17 +129,18 @@ def lib():
     L.kf_last_error.restype = C.c_char_p
     L.kf_version.argtypes = []
     L.kf_version.restype = C.c_char_p
-    _optional = {
+    batch = {
         "kf_batch_create": [C.POINTER(C.c_void_p), i32, vp, C.POINTER(C.c_void_p)],
+        "kf_batch_set_sessions": [vp, C.POINTER(C.c_void_p), i32],
         "kf_batch_destroy": [vp],
-        "kf_batch_frame_u16_dev": [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_int), i32],
-        "kf_batch_capture": [vp, i32],
+        "kf_batch_synchronize": [vp],
+        "kf_batch_get_stream": [vp, C.POINTER(C.c_void_p)],
+        "kf_batch_frame_u16_dev": [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_int)],
     }
-    for name, args in _optional.items():
-        if hasattr(L, name):
-            fn = getattr(L, name)
-            fn.argtypes = args
-            fn.restype = C.c_int
+    for name, args in batch.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
     _lib = L
     return L
 

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
@@ -295,7 +296,7 @@ int run_pyramid_from_stage(kf_session *s, const void *dev, int is_u16)
     return KF_OK;
 }
 
-int run_icp(kf_session *s, int level, int first, int save_pose)
+kf::IcpArgs icp_args(kf_session *s, int level, int first, int save_pose)
 {
     kf::IcpArgs a{};
     a.vert = s->vert[level];
@@ -316,6 +317,12 @@ int run_icp(kf_session *s, int level, int first, int save_pose)
     // blocks per row group: about kIcpRowsPerBlock rows of the group per block
     const int rows_per_group = (a.h + 7) / 8;
     a.rows_step = std::max(1, (rows_per_group + s->icp_rows_per_block - 1) / s->icp_rows_per_block);
+    return a;
+}
+
+int run_icp(kf_session *s, int level, int first, int save_pose)
+{
+    kf::IcpArgs a = icp_args(s, level, first, save_pose);
     if (s->debug) {
         a.dbg_result = s->dbg_result;
         a.dbg_error = s->dbg_error;
@@ -874,6 +881,197 @@ int kf_count_integrate_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *v
     KF_HIP(hipStreamSynchronize(s->stream));
     if (voxels_read) *voxels_read = h[0];
     if (voxels_written) *voxels_written = h[1];
+    return KF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Multi-session batches
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct kf_batch {
+    std::vector<kf_session *> sessions;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    kf::BatchSession *d_sess = nullptr;
+    int *d_prefix = nullptr;  // int, dilate, coarse prefixes: 3 x (kMaxBatch + 1)
+    int n = 0;
+    int storage = 0, fast = 0, off32 = 1;
+    int pyr_blocks = 0, ray_blocks = 0, nblk[3] = {0, 0, 0};
+    int total_int = 0, total_bricks = 0, total_coarse = 0;
+    std::vector<long> frames;
+};
+
+namespace {
+
+int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
+{
+    if (n < 1 || n > kf::kMaxBatch) return fail(KF_ERR_INVALID, "a batch holds 1..64 sessions");
+    for (int i = 0; i < n; ++i)
+        if (!sessions[i]) return fail(KF_ERR_INVALID, "NULL session in batch");
+    const kf_session *s0 = sessions[0];
+    int iters = 0;
+    for (int l = 0; l < 3; ++l) iters += s0->cfg.iterations[l];
+    if (iters == 0) return fail(KF_ERR_INVALID, "batched sessions need at least one ICP iteration");
+    for (int i = 0; i < n; ++i) {
+        const kf_session *si = sessions[i];
+        if (si->cfg.width != s0->cfg.width || si->cfg.height != s0->cfg.height || si->cfg.storage != s0->cfg.storage ||
+            si->fast_math != s0->fast_math || si->icp_rows_per_block != s0->icp_rows_per_block ||
+            si->xcd_group != s0->xcd_group || si->device != s0->device ||
+            std::memcmp(si->cfg.iterations, s0->cfg.iterations, sizeof(s0->cfg.iterations)) != 0)
+            return fail(KF_ERR_INVALID, "batched sessions must share depth size, ICP schedule, storage and options");
+        for (int j = 0; j < i; ++j)
+            if (sessions[j] == si) return fail(KF_ERR_INVALID, "a session appears twice in the batch");
+    }
+    // finish whatever the sessions queued on their own streams
+    for (int i = 0; i < n; ++i) KF_HIP(hipStreamSynchronize(sessions[i]->stream));
+    KF_HIP(hipStreamSynchronize(b->stream));  // tables may be read by queued launches
+
+    std::vector<kf::BatchSession> h(n);
+    std::vector<int> pre(3 * (kf::kMaxBatch + 1), 0);
+    int *ip = pre.data(), *dp = ip + kf::kMaxBatch + 1, *cp = dp + kf::kMaxBatch + 1;
+    b->off32 = 1;
+    for (int i = 0; i < n; ++i) {
+        kf_session *si = sessions[i];
+        kf::BatchSession &bs = h[i];
+        bs.pyr = pyramid_args(si);
+        for (int l = 0; l < 3; ++l) bs.icp[l] = icp_args(si, l, 0, 0);
+        bs.integ = integrate_args(si, 1);
+        bs.ray = raycast_args(si);
+        bs.brick = brick_args(si);
+        bs.st = si->st;
+        kf::integrate_grid_dims(bs.integ, bs.int_grid);
+        ip[i + 1] = ip[i] + bs.int_grid[0] * bs.int_grid[1] * bs.int_grid[2];
+        dp[i + 1] = dp[i] + (int)si->nbricks;
+        cp[i + 1] = cp[i] + (int)si->ncoarse;
+        if (si->voxels >= (size_t(1) << 32)) b->off32 = 0;
+    }
+    KF_HIP(hipMemcpyAsync(b->d_sess, h.data(), sizeof(kf::BatchSession) * n, hipMemcpyHostToDevice, b->stream));
+    KF_HIP(hipMemcpyAsync(b->d_prefix, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice, b->stream));
+    KF_HIP(hipStreamSynchronize(b->stream));
+    b->sessions.assign(sessions, sessions + n);
+    b->frames.assign(n, 0);
+    for (int i = 0; i < n; ++i) b->frames[i] = sessions[i]->frames;
+    b->n = n;
+    b->storage = s0->cfg.storage;
+    b->fast = s0->fast_math;
+    b->pyr_blocks = h[0].pyr.blocks_total;
+    b->ray_blocks = h[0].ray.blocks;
+    for (int l = 0; l < 3; ++l) b->nblk[l] = 8 * h[0].icp[l].rows_step;
+    b->total_int = ip[n];
+    b->total_bricks = dp[n];
+    b->total_coarse = cp[n];
+    return KF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kf_batch_create(kf_session *const *sessions, int n, void *stream, kf_batch **out)
+{
+    if (!out) return fail(KF_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    kf_batch *b = new (std::nothrow) kf_batch();
+    if (!b) return fail(KF_ERR_OOM, "host allocation failed");
+    auto cleanup = [&](int code) {
+        if (b->d_sess) (void)hipFree(b->d_sess);
+        if (b->d_prefix) (void)hipFree(b->d_prefix);
+        if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return code;
+    };
+    if (stream) {
+        b->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(fail(KF_ERR_HIP, "hipStreamCreate failed"));
+        b->own_stream = true;
+    }
+    if (hipMalloc((void **)&b->d_sess, sizeof(kf::BatchSession) * kf::kMaxBatch) != hipSuccess ||
+        hipMalloc((void **)&b->d_prefix, sizeof(int) * 3 * (kf::kMaxBatch + 1)) != hipSuccess)
+        return cleanup(fail(KF_ERR_OOM, "batch tables"));
+    int rc = batch_fill(b, sessions, n);
+    if (rc) return cleanup(rc);
+    *out = b;
+    return KF_OK;
+}
+
+int kf_batch_set_sessions(kf_batch *b, kf_session *const *sessions, int n)
+{
+    if (!b) return fail(KF_ERR_INVALID, "batch is NULL");
+    return batch_fill(b, sessions, n);
+}
+
+int kf_batch_destroy(kf_batch *b)
+{
+    if (!b) return KF_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->d_sess) (void)hipFree(b->d_sess);
+    if (b->d_prefix) (void)hipFree(b->d_prefix);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return KF_OK;
+}
+
+int kf_batch_synchronize(kf_batch *b)
+{
+    if (!b) return fail(KF_ERR_INVALID, "batch is NULL");
+    KF_HIP(hipStreamSynchronize(b->stream));
+    return KF_OK;
+}
+
+int kf_batch_get_stream(kf_batch *b, void **stream)
+{
+    if (!b || !stream) return fail(KF_ERR_INVALID, "NULL argument");
+    *stream = (void *)b->stream;
+    return KF_OK;
+}
+
+int kf_batch_frame_u16_dev(kf_batch *b, const uint16_t *const *depth_mm_dev, const int *active)
+{
+    if (!b || !depth_mm_dev) return fail(KF_ERR_INVALID, "NULL argument");
+    kf::BatchTick t{};
+    t.n = b->n;
+    t.sess = b->d_sess;
+    t.int_prefix = b->d_prefix;
+    t.dil_prefix = b->d_prefix + kf::kMaxBatch + 1;
+    t.coa_prefix = b->d_prefix + 2 * (kf::kMaxBatch + 1);
+    int nact = 0;
+    for (int i = 0; i < b->n; ++i) {
+        const bool on = active ? active[i] != 0 : true;
+        if (!on) continue;
+        if (!depth_mm_dev[i]) return fail(KF_ERR_INVALID, "active session without depth");
+        t.active |= 1ull << i;
+        t.depth[i] = depth_mm_dev[i];
+        ++nact;
+    }
+    if (!nact) return KF_OK;
+    hipError_t e = kf::launch_batch_pyramid(t, b->pyr_blocks, b->stream);
+    const kf_config &c = b->sessions[0]->cfg;
+    bool first_any = true;
+    for (int level = 2; level >= 0 && e == hipSuccess; --level)
+        for (int it = 0; it < c.iterations[level] && e == hipSuccess; ++it) {
+            e = kf::launch_batch_icp(t, level, it == 0, first_any ? 1 : 0, b->nblk[level], b->stream);
+            first_any = false;
+        }
+    if (e == hipSuccess) e = kf::launch_batch_finish(t, b->stream);
+    if (e == hipSuccess) e = kf::launch_batch_integrate(t, b->storage, b->fast, b->off32, b->total_int, b->stream);
+    if (e == hipSuccess) e = kf::launch_batch_bricks(t, b->total_bricks, b->total_coarse, b->stream);
+    if (e == hipSuccess) e = kf::launch_batch_raycast(t, b->storage, b->ray_blocks, b->stream);
+    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+    // per-session exact brick rebuild on the same schedule as kf_frame_*
+    for (int i = 0; i < b->n; ++i) {
+        if (!((t.active >> i) & 1ull)) continue;
+        kf_session *si = b->sessions[i];
+        si->frames = ++b->frames[i];
+        if (si->rebuild_period > 0 && si->frames % si->rebuild_period == 0) {
+            kf::BrickArgs ba = brick_args(si);
+            e = kf::launch_brick_rebuild(ba, si->cfg.storage, b->stream);
+            if (e == hipSuccess) e = kf::launch_brick_dilate(ba, b->stream);
+            if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("brick rebuild: ") + hipGetErrorString(e));
+        }
+    }
     return KF_OK;
 }
 

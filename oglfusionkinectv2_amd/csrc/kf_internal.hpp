@@ -76,6 +76,40 @@ struct RaycastArgs {
     int xcd_group;              // tiles per XCD cluster (block -> tile mapping)
 };
 
+// ---------------------------------------------------------------------------
+// Multi-session batches: one launch per kernel type for all active sessions.
+// ---------------------------------------------------------------------------
+constexpr int kMaxBatch = 64;
+
+// everything one session's kernels need, resident in device memory
+struct BatchSession {
+    PyramidArgs pyr;
+    IcpArgs icp[3];
+    IntegrateArgs integ;
+    RaycastArgs ray;
+    BrickArgs brick;
+    DevState *st;
+    int int_grid[3];  // integrate grid of this session
+};
+
+// per-launch arguments (by value): depth inputs and the active mask
+struct BatchTick {
+    const uint16_t *depth[kMaxBatch];
+    unsigned long long active;
+    int n;
+    const BatchSession *sess;                      // n entries
+    const int *int_prefix, *dil_prefix, *coa_prefix;  // n + 1 entries each
+};
+
+void integrate_grid_dims(const IntegrateArgs &a, int g[3]);
+hipError_t launch_batch_pyramid(const BatchTick &t, int blocks_per_session, hipStream_t s);
+hipError_t launch_batch_icp(const BatchTick &t, int level, int first, int save_pose, int nblk, hipStream_t s);
+hipError_t launch_batch_finish(const BatchTick &t, hipStream_t s);
+hipError_t launch_batch_integrate(const BatchTick &t, int storage, int fast, int off32, int total_blocks,
+                                  hipStream_t s);
+hipError_t launch_batch_bricks(const BatchTick &t, int total_bricks, int total_coarse, hipStream_t s);
+hipError_t launch_batch_raycast(const BatchTick &t, int storage, int blocks_per_session, hipStream_t s);
+
 hipError_t launch_pyramid_u16(const PyramidArgs &a, const uint16_t *in, hipStream_t s);
 hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t s);
 hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s);

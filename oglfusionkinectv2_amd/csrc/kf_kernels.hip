@@ -98,9 +98,8 @@ __device__ __forceinline__ void vertex_of(float d, const float iK[16], int x, in
 }
 
 template <typename DT>
-__global__ __launch_bounds__(256) void k_pyramid(PyramidArgs a, const DT *__restrict__ in)
+__device__ __forceinline__ void pyramid_body(const PyramidArgs &a, const DT *__restrict__ in, int b)
 {
-    int b = blockIdx.x;
     int l = 0;
     if (b >= a.block_start[2]) l = 2;
     else if (b >= a.block_start[1]) l = 1;
@@ -151,6 +150,12 @@ __global__ __launch_bounds__(256) void k_pyramid(PyramidArgs a, const DT *__rest
         n = make_float4(c0 / len, c1 / len, c2 / len, 1.0f);
     }
     a.norm[l][idx] = n;
+}
+
+template <typename DT>
+__global__ __launch_bounds__(256) void k_pyramid(PyramidArgs a, const DT *__restrict__ in)
+{
+    pyramid_body(a, in, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -373,7 +378,8 @@ constexpr int kPartStride = 32;  // floats per block-partial row (16-byte aligne
 // of reduce.cs that owns the row).  Per-thread FP32 products, DPP wave sums,
 // block partials in FP32, and the last block to finish sums the partials of
 // each group in FP64, applies the combine rule, solves and updates the pose.
-__global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevState *__restrict__ st)
+// blk / nblk: this block's index and the block count of the session's launch
+__device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict__ st, int blk, int nblk)
 {
     __shared__ float wsum[kIcpThreads / 64][kTerms];
     __shared__ int last;
@@ -385,7 +391,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevSta
 
     // block b serves row group g = b % 8 (the reduce.cs work group that owns
     // rows y = g mod 8) and takes every rows_step-th row of that group
-    const int g = blockIdx.x & 7, jb = blockIdx.x >> 3;
+    const int g = blk & 7, jb = blk >> 3;
     float T[16], view[16];
     {
         float K[16], ip[16];
@@ -435,7 +441,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevSta
     if (threadIdx.x < kTerms) {
         const int k = threadIdx.x;
         const float s = (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]);
-        a.block_partials[blockIdx.x * kPartStride + k] = s;
+        a.block_partials[blk * kPartStride + k] = s;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -443,7 +449,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevSta
         // block's partial stores before it
         __threadfence();
         const unsigned t = atomicAdd(&st->ticket, 1u);
-        last = (t == gridDim.x - 1);
+        last = (t == (unsigned)(nblk - 1));
     }
     __syncthreads();
     if (!last) return;
@@ -456,7 +462,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevSta
         double ps[kTerms];
 #pragma unroll
         for (int k = 0; k < kTerms; ++k) ps[k] = 0.0;
-        for (int pb = threadIdx.x; pb < (int)gridDim.x; pb += kIcpThreads) {
+        for (int pb = threadIdx.x; pb < nblk; pb += kIcpThreads) {
             const float4 *rp = reinterpret_cast<const float4 *>(a.block_partials + pb * kPartStride);
             float4 q[kTerms / 4];
 #pragma unroll
@@ -562,11 +568,15 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevSta
     st->ticket = 0u;
 }
 
+__global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevState *__restrict__ st)
+{
+    icp_body(a, st, blockIdx.x, gridDim.x);
+}
+
 // Track() accept / reject (gFusion.cpp:715-723); with `frame` the frame-loop
 // gate of kinect.cpp:677-691: integrate if tracked or within the reset window.
-__global__ void k_icp_finish(DevState *st, int frame)
+__device__ __forceinline__ void icp_finish_body(DevState *st, int frame)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const float ae = st->ae;
     int tracked;
     if (ae > 0.5f || ae == 0.0f) {
@@ -582,6 +592,12 @@ __global__ void k_icp_finish(DevState *st, int frame)
         if (go && st->counter > 2) st->reset = 0;
         st->counter += 1;
     }
+}
+
+__global__ void k_icp_finish(DevState *st, int frame)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    icp_finish_body(st, frame);
 }
 
 __global__ void k_state_set(DevState *st, Mat16 pose, int set_pose, int arm_reset)
@@ -660,16 +676,17 @@ __device__ __forceinline__ float proj_coord(float num, float den, float rden)
 // the reference; FAST = true uses hardware reciprocal / square root (about 1
 // ulp) for the signed distance and the weighted average.
 template <class S, bool FAST>
-__global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevState *__restrict__ st)
+__device__ __forceinline__ void integrate_body(const IntegrateArgs &a, const DevState *__restrict__ st, int bx, int by,
+                                               int bz)
 {
     if (a.gated && !st->integrate_now) return;
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * 4 + threadIdx.y;
+    const int x = bx * 64 + threadIdx.x;
+    const int y = by * 4 + threadIdx.y;
     if (x >= a.N[0] || y >= a.N[1]) return;
 
     ColumnSetup c;
     column_setup(a, st->pose, x, y, c);
-    const int z0 = blockIdx.z * a.zlen;
+    const int z0 = bz * a.zlen;
     const int z1 = min(z0 + a.zlen, a.N[2]);
     if (FAST) {
         // segment start evaluated directly (a few ulps from the stepped value)
@@ -762,6 +779,12 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevSta
     }
 }
 
+template <class S, bool FAST>
+__global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevState *__restrict__ st)
+{
+    integrate_body<S, FAST>(a, st, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 // Reduced-arithmetic integrate (KF_OPT_FAST_MATH).  Same update rule as
 // k_integrate, with the per-voxel arithmetic cut to about a third: one
 // hardware reciprocal of the camera depth per voxel, the projection folded to
@@ -771,17 +794,18 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevSta
 // the correctly rounded evaluation to a few ulps; voxels on a pixel edge or a
 // branch threshold can take the other side (counted as flips by the tests).
 template <class S, bool OFF32>
-__global__ __launch_bounds__(256) void k_integrate_fast(IntegrateArgs a, const DevState *__restrict__ st)
+__device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, const DevState *__restrict__ st, int bx,
+                                                    int by, int bz)
 {
     using Idx = typename std::conditional<OFF32, uint32_t, size_t>::type;
     if (a.gated && !st->integrate_now) return;
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * 4 + threadIdx.y;
+    const int x = bx * 64 + threadIdx.x;
+    const int y = by * 4 + threadIdx.y;
     if (x >= a.N[0] || y >= a.N[1]) return;
 
     ColumnSetup c;
     column_setup(a, st->pose, x, y, c);
-    const int z0 = blockIdx.z * a.zlen;
+    const int z0 = bz * a.zlen;
     const int z1 = min(z0 + a.zlen, a.N[2]);
     float p0 = fmaf((float)z0, c.dl[0], c.pos[0]);
     float p1 = fmaf((float)z0, c.dl[1], c.pos[1]);
@@ -859,6 +883,12 @@ __global__ __launch_bounds__(256) void k_integrate_fast(IntegrateArgs a, const D
     }
 }
 
+template <class S, bool OFF32>
+__global__ __launch_bounds__(256) void k_integrate_fast(IntegrateArgs a, const DevState *__restrict__ st)
+{
+    integrate_fast_body<S, OFF32>(a, st, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 // Same walk as k_integrate without memory traffic on the volume: counts the
 // voxels the reference semantics read (diff > -mu) and write (any live voxel).
 __global__ __launch_bounds__(256) void k_count_integrate(IntegrateArgs a, const DevState *__restrict__ st,
@@ -924,10 +954,9 @@ __global__ __launch_bounds__(512) void k_brick_rebuild(BrickArgs a)
 }
 
 // cflags(c) = OR of dflags over the 4^3 bricks of the 32^3 cell c.
-__global__ __launch_bounds__(256) void k_brick_coarse(BrickArgs a)
+__device__ __forceinline__ void brick_coarse_body(const BrickArgs &a, int i)
 {
     const int n = a.NC[0] * a.NC[1] * a.NC[2];
-    const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int cx = i % a.NC[0], cy = (i / a.NC[0]) % a.NC[1], cz = i / (a.NC[0] * a.NC[1]);
     uint8_t any = 0;
@@ -938,12 +967,16 @@ __global__ __launch_bounds__(256) void k_brick_coarse(BrickArgs a)
     a.cflags[i] = any ? 1 : 0;
 }
 
+__global__ __launch_bounds__(256) void k_brick_coarse(BrickArgs a)
+{
+    brick_coarse_body(a, blockIdx.x * 256 + threadIdx.x);
+}
+
 // dflags(b) = OR of flags over b + {0,1}^3: a trilinear sample whose lower
 // corner lies in brick b reads only voxels of those 8 bricks.
-__global__ __launch_bounds__(256) void k_brick_dilate(BrickArgs a)
+__device__ __forceinline__ void brick_dilate_body(const BrickArgs &a, int i)
 {
     const int n = a.NB[0] * a.NB[1] * a.NB[2];
-    const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int bx = i % a.NB[0], by = (i / a.NB[0]) % a.NB[1], bz = i / (a.NB[0] * a.NB[1]);
     uint8_t any = 0;
@@ -955,6 +988,11 @@ __global__ __launch_bounds__(256) void k_brick_dilate(BrickArgs a)
                     any |= a.flags[((size_t)Z * a.NB[1] + Y) * a.NB[0] + X];
             }
     a.dflags[i] = any ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_brick_dilate(BrickArgs a)
+{
+    brick_dilate_body(a, blockIdx.x * 256 + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1122,14 +1160,14 @@ constexpr int kBatch = 4;  // fine-step samples evaluated per march iteration
 // so that the blocks an XCD receives (id mod 8) cover one contiguous band of
 // tiles, keeping the voxels neighbouring rays share in that XCD's L2.
 template <class S>
-__global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *__restrict__ st)
+__device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevState *__restrict__ st, int blk)
 {
     // block b runs on XCD b % 8; XCD x receives the clusters of xcd_group
     // consecutive tiles x, x + 8, x + 16, ... (1 = fully interleaved)
     const int nb = a.tiles_x * a.tiles_y;
     const int G = a.xcd_group;
-    const int j = blockIdx.x / 8;
-    const int tile = ((blockIdx.x % 8) + 8 * (j / G)) * G + (j % G);
+    const int j = blk / 8;
+    const int tile = ((blk % 8) + 8 * (j / G)) * G + (j % G);
     if (tile >= nb) return;
     const int x = (tile % a.tiles_x) * 16 + (threadIdx.x & 15);
     const int y = (tile / a.tiles_x) * 16 + (threadIdx.x >> 4);
@@ -1302,6 +1340,101 @@ __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *
     }
 }
 
+template <class S>
+__global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *__restrict__ st)
+{
+    raycast_body<S>(a, st, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched kernels: block -> (session, local block).  Sessions of a batch share
+// the image size, ICP schedule, storage type and arithmetic mode; volume sizes
+// may differ (integrate / brick launches map blocks through prefix tables).
+// Blocks of sessions without a frame in this launch return at once.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool batch_active(const BatchTick &t, int i) { return (t.active >> i) & 1ull; }
+
+// largest i with prefix[i] <= b
+__device__ __forceinline__ int batch_find(const int *prefix, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_batch_pyramid(BatchTick t, int per)
+{
+    const int i = blockIdx.x / per;
+    if (!batch_active(t, i)) return;
+    const PyramidArgs a = t.sess[i].pyr;
+    pyramid_body<uint16_t>(a, t.depth[i], blockIdx.x % per);
+}
+
+__global__ __launch_bounds__(kIcpThreads) void k_batch_icp(BatchTick t, int level, int first, int save_pose, int nblk)
+{
+    const int i = blockIdx.x / nblk;
+    if (!batch_active(t, i)) return;
+    IcpArgs a = t.sess[i].icp[level];
+    a.first = first;
+    a.save_pose = save_pose;
+    icp_body(a, t.sess[i].st, blockIdx.x % nblk, nblk);
+}
+
+__global__ void k_batch_finish(BatchTick t)
+{
+    const int i = blockIdx.x;
+    if (threadIdx.x != 0 || !batch_active(t, i)) return;
+    icp_finish_body(t.sess[i].st, 1);
+}
+
+template <class S, bool FAST, bool OFF32>
+__global__ __launch_bounds__(256) void k_batch_integrate(BatchTick t)
+{
+    const int i = batch_find(t.int_prefix, t.n, blockIdx.x);
+    if (!batch_active(t, i)) return;
+    const IntegrateArgs a = t.sess[i].integ;
+    const int gx = t.sess[i].int_grid[0], gy = t.sess[i].int_grid[1];
+    const int local = blockIdx.x - t.int_prefix[i];
+    const int bx = local % gx, by = (local / gx) % gy, bz = local / (gx * gy);
+    if (FAST) integrate_fast_body<S, OFF32>(a, t.sess[i].st, bx, by, bz);
+    else integrate_body<S, false>(a, t.sess[i].st, bx, by, bz);
+}
+
+__global__ __launch_bounds__(256) void k_batch_dilate(BatchTick t)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= t.dil_prefix[t.n]) return;
+    const int i = batch_find(t.dil_prefix, t.n, b);
+    if (!batch_active(t, i)) return;
+    const BrickArgs a = t.sess[i].brick;
+    brick_dilate_body(a, b - t.dil_prefix[i]);
+}
+
+__global__ __launch_bounds__(256) void k_batch_coarse(BatchTick t)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= t.coa_prefix[t.n]) return;
+    const int i = batch_find(t.coa_prefix, t.n, b);
+    if (!batch_active(t, i)) return;
+    const BrickArgs a = t.sess[i].brick;
+    brick_coarse_body(a, b - t.coa_prefix[i]);
+}
+
+// blocks_per_session is a multiple of 8, so a block's XCD (global id mod 8)
+// equals its session-local id mod 8 and the per-session tile clustering holds
+template <class S>
+__global__ __launch_bounds__(256) void k_batch_raycast(BatchTick t, int per)
+{
+    const int i = blockIdx.x / per;
+    if (!batch_active(t, i)) return;
+    const RaycastArgs a = t.sess[i].ray;
+    raycast_body<S>(a, t.sess[i].st, blockIdx.x % per);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -1338,6 +1471,68 @@ hipError_t launch_state_set(DevState *st, const float *pose, int arm_reset, hipS
 static dim3 integrate_grid(const IntegrateArgs &a)
 {
     return dim3((a.N[0] + 63) / 64, (a.N[1] + 3) / 4, (a.N[2] + a.zlen - 1) / a.zlen);
+}
+
+void integrate_grid_dims(const IntegrateArgs &a, int g[3])
+{
+    const dim3 d = integrate_grid(a);
+    g[0] = (int)d.x;
+    g[1] = (int)d.y;
+    g[2] = (int)d.z;
+}
+
+hipError_t launch_batch_pyramid(const BatchTick &t, int per, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_pyramid, dim3(t.n * per), dim3(256), 0, s, t, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_icp(const BatchTick &t, int level, int first, int save_pose, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_icp, dim3(t.n * nblk), dim3(kIcpThreads), 0, s, t, level, first, save_pose, nblk);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_finish(const BatchTick &t, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_finish, dim3(t.n), dim3(64), 0, s, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_integrate(const BatchTick &t, int storage, int fast, int off32, int total, hipStream_t s)
+{
+    const dim3 g(total), b(64, 4);
+#define KF_BINT(ST)                                                                             \
+    if (fast && off32) hipLaunchKernelGGL((k_batch_integrate<ST, true, true>), g, b, 0, s, t);  \
+    else if (fast) hipLaunchKernelGGL((k_batch_integrate<ST, true, false>), g, b, 0, s, t);     \
+    else hipLaunchKernelGGL((k_batch_integrate<ST, false, false>), g, b, 0, s, t);
+    switch (storage) {
+    case KF_STORAGE_RG16I: KF_BINT(StoreRG16I); break;
+    case KF_STORAGE_F32: KF_BINT(StoreF32); break;
+    case KF_STORAGE_F16: KF_BINT(StoreF16); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef KF_BINT
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_bricks(const BatchTick &t, int total_bricks, int total_coarse, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_dilate, dim3((total_bricks + 255) / 256), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(k_batch_coarse, dim3((total_coarse + 255) / 256), dim3(256), 0, s, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_raycast(const BatchTick &t, int storage, int per, hipStream_t s)
+{
+    const dim3 g(t.n * per), b(256);
+    switch (storage) {
+    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_batch_raycast<StoreRG16I>, g, b, 0, s, t, per); break;
+    case KF_STORAGE_F32: hipLaunchKernelGGL(k_batch_raycast<StoreF32>, g, b, 0, s, t, per); break;
+    case KF_STORAGE_F16: hipLaunchKernelGGL(k_batch_raycast<StoreF16>, g, b, 0, s, t, per); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s)
