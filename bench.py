@@ -9,8 +9,11 @@ are resident in HBM before the timed region.
 
 Also reported: the same loop with the FP16 TSDF (4 B/voxel), per-frame
 p50/p99 latency, the dominant kernel's roofline figures from HIP events
-recorded on the session stream around it inside the timed region, and the CPU
-oracle timed on a bounded sample of the same workload.
+recorded on the session stream around it inside the timed region, the CPU
+oracle timed on a bounded sample of the same workload, and the multi-session
+FP16 legs (saturated 16 x 512^3 batch, the uneven 16-session mix, 16 x 512^3
+served at 30 Hz, Poisson arrivals under a 200 GB budget) under
+"multi_session".
 """
 import argparse
 import json
@@ -36,6 +39,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=6)
     ap.add_argument("--fast-math", type=int, default=1)
+    ap.add_argument("--no-serve", action="store_true", help="skip the batched and served legs")
+    ap.add_argument("--batch-ticks", type=int, default=100)
+    ap.add_argument("--serve-seconds", type=float, default=8.0,
+                    help="length of the fixed 16 x 512^3 at 30 Hz served run")
+    ap.add_argument("--poisson-seconds", type=float, default=40.0,
+                    help="how much of the Poisson arrival process (64 sessions at 1/s) to serve")
     return ap.parse_args()
 
 
@@ -141,6 +150,34 @@ def cpu_baseline(depth, poses, volume, frames, warm=2):
     return frames / dt, threads, dt
 
 
+def serving_legs(torch, args, fps32):
+    """Multi-session legs, all FP16 TSDF:
+    - saturated: 16 x 512^3 sessions with a frame due every tick (offline
+      throughput of the batched path), compared with the FP32 single session;
+    - batched: the uneven mix 4 x 128^3, 6 x 256^3, 4 x 384^3, 2 x 512^3, each
+      session its own seeded scene, every session due every tick;
+    - served: 16 x 512^3 sessions at 30 Hz, first frames spread over one
+      period, real-time request loop with continuous batching;
+    - poisson: sessions arriving at 1/s (64 in all), sizes {256..768}^3, 15 or
+      30 Hz, 200-900 frames each, 200 GB HBM budget; the first
+      `poisson_seconds` of the process are served."""
+    from oglfusionkinectv2_amd import serve
+    out = {}
+    sat = serve.saturate(torch, [512] * 16, args.batch_ticks, sequences=16,
+                         pool_frames=args.batch_ticks + 10)
+    sat["vs_fp32_single_session"] = sat["frames_per_s"] / fps32
+    out["saturated_16x512_f16"] = sat
+    mix = [128] * 4 + [256] * 6 + [384] * 4 + [512] * 2
+    out["batched_mix_f16"] = serve.saturate(torch, mix, args.batch_ticks, sequences=len(mix),
+                                            pool_frames=args.batch_ticks + 10)
+    out["served_16x512_30hz_f16"] = serve.serve(torch, serve.fixed_sessions(16, 512, 30.0, args.serve_seconds),
+                                                args.serve_seconds, sequences=16)
+    if args.poisson_seconds > 0:
+        out["served_poisson_f16"] = serve.serve(torch, serve.poisson_sessions(), args.poisson_seconds,
+                                                sequences=16)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -186,6 +223,11 @@ def main():
         fps16 = args.steps / e16
         kern16, _ = run_kernels(torch, L, g16, frames_dev, poses, args.warmup, args.steps, stream)
         g16.close()
+
+    # --- multi-session -----------------------------------------------------
+    multi = None
+    if not args.no_serve:
+        multi = serving_legs(torch, args, fps)
 
     # --- CPU baseline --------------------------------------------------------
     cpu = None
@@ -235,6 +277,7 @@ def main():
         "dominant_kernel": dom,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "multi_session": multi,
         "tracked_last": st["tracked"],
         "final_translation_error_vs_gt_m": ate_gt,
         "generator_s": t_gen,

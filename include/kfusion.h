@@ -193,9 +193,13 @@ int kf_raycast_stats(kf_session *s, uint64_t stats[6]);
  * runs the frame block of many sessions with one launch per kernel type:
  * sessions share depth size, ICP schedule, storage type and options; volume
  * sizes may differ.  Each session's results are the same bits as running it
- * alone with kf_frame_u16_dev.  The batch synchronises the sessions' own
- * streams when created / re-filled and then queues on its own stream; call
- * kf_batch_synchronize before using a member session alone again. */
+ * alone with kf_frame_u16_dev.  Creating or re-filling a batch does not block
+ * the host: the batch stream waits for work already queued on the members'
+ * own streams, and the new tables reach the device in stream order, so frames
+ * queued before a membership change still run with the old members.  Call
+ * kf_batch_synchronize before using a member session alone again;
+ * kf_session_destroy waits for the whole device, so a session may be
+ * destroyed once it is no longer a member. */
 typedef struct kf_batch kf_batch;
 int kf_batch_create(kf_session *const *sessions, int n, void *stream, kf_batch **out); /* n <= 64 */
 int kf_batch_set_sessions(kf_batch *b, kf_session *const *sessions, int n);

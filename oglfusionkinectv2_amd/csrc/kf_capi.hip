@@ -560,7 +560,8 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
 int kf_session_destroy(kf_session *s)
 {
     if (!s) return KF_OK;
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    // batches may still hold queued launches that read this session on their own streams
+    (void)hipDeviceSynchronize();
     free_all(s);
     delete s;
     return KF_OK;
@@ -900,9 +901,32 @@ struct kf_batch {
     int pyr_blocks = 0, ray_blocks = 0, nblk[3] = {0, 0, 0};
     int total_int = 0, total_bricks = 0, total_coarse = 0;
     std::vector<long> frames;
+    // Membership changes upload new tables with stream-ordered copies from a
+    // ring of pinned staging slots, so set_sessions never blocks the host on
+    // queued frames: a slot is reused only after its previous copy completed.
+    static constexpr int kSlots = 4;
+    static constexpr size_t kSlotBytes = sizeof(kf::BatchSession) * kf::kMaxBatch + sizeof(int) * 3 * (kf::kMaxBatch + 1);
+    unsigned char *h_stage[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t slot_ev[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+    bool slot_pending[kSlots] = {false, false, false, false};
+    int next_slot = 0;
+    hipEvent_t join_ev = nullptr;  // orders the members' own streams before the batch stream
 };
 
 namespace {
+
+void release_batch(kf_batch *b)
+{
+    if (b->d_sess) (void)hipFree(b->d_sess);
+    if (b->d_prefix) (void)hipFree(b->d_prefix);
+    for (int k = 0; k < kf_batch::kSlots; ++k) {
+        if (b->h_stage[k]) (void)hipHostFree(b->h_stage[k]);
+        if (b->slot_ev[k]) (void)hipEventDestroy(b->slot_ev[k]);
+    }
+    if (b->join_ev) (void)hipEventDestroy(b->join_ev);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
 
 int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
 {
@@ -923,13 +947,19 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
         for (int j = 0; j < i; ++j)
             if (sessions[j] == si) return fail(KF_ERR_INVALID, "a session appears twice in the batch");
     }
-    // finish whatever the sessions queued on their own streams
-    for (int i = 0; i < n; ++i) KF_HIP(hipStreamSynchronize(sessions[i]->stream));
-    KF_HIP(hipStreamSynchronize(b->stream));  // tables may be read by queued launches
-
-    std::vector<kf::BatchSession> h(n);
-    std::vector<int> pre(3 * (kf::kMaxBatch + 1), 0);
-    int *ip = pre.data(), *dp = ip + kf::kMaxBatch + 1, *cp = dp + kf::kMaxBatch + 1;
+    // work the members queued on their own streams runs before the next batch launch
+    for (int i = 0; i < n; ++i) {
+        if (sessions[i]->stream == b->stream) continue;
+        KF_HIP(hipEventRecord(b->join_ev, sessions[i]->stream));
+        KF_HIP(hipStreamWaitEvent(b->stream, b->join_ev, 0));
+    }
+    const int slot = b->next_slot;
+    if (b->slot_pending[slot]) KF_HIP(hipEventSynchronize(b->slot_ev[slot]));
+    b->slot_pending[slot] = false;
+    kf::BatchSession *h = reinterpret_cast<kf::BatchSession *>(b->h_stage[slot]);
+    int *ip = reinterpret_cast<int *>(b->h_stage[slot] + sizeof(kf::BatchSession) * kf::kMaxBatch);
+    int *dp = ip + kf::kMaxBatch + 1, *cp = dp + kf::kMaxBatch + 1;
+    std::memset(ip, 0, sizeof(int) * 3 * (kf::kMaxBatch + 1));
     b->off32 = 1;
     for (int i = 0; i < n; ++i) {
         kf_session *si = sessions[i];
@@ -946,9 +976,12 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
         cp[i + 1] = cp[i] + (int)si->ncoarse;
         if (si->voxels >= (size_t(1) << 32)) b->off32 = 0;
     }
-    KF_HIP(hipMemcpyAsync(b->d_sess, h.data(), sizeof(kf::BatchSession) * n, hipMemcpyHostToDevice, b->stream));
-    KF_HIP(hipMemcpyAsync(b->d_prefix, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice, b->stream));
-    KF_HIP(hipStreamSynchronize(b->stream));
+    // stream order: launches queued earlier still read the previous tables
+    KF_HIP(hipMemcpyAsync(b->d_sess, h, sizeof(kf::BatchSession) * n, hipMemcpyHostToDevice, b->stream));
+    KF_HIP(hipMemcpyAsync(b->d_prefix, ip, sizeof(int) * 3 * (kf::kMaxBatch + 1), hipMemcpyHostToDevice, b->stream));
+    KF_HIP(hipEventRecord(b->slot_ev[slot], b->stream));
+    b->slot_pending[slot] = true;
+    b->next_slot = (slot + 1) % kf_batch::kSlots;
     b->sessions.assign(sessions, sessions + n);
     b->frames.assign(n, 0);
     for (int i = 0; i < n; ++i) b->frames[i] = sessions[i]->frames;
@@ -975,10 +1008,8 @@ int kf_batch_create(kf_session *const *sessions, int n, void *stream, kf_batch *
     kf_batch *b = new (std::nothrow) kf_batch();
     if (!b) return fail(KF_ERR_OOM, "host allocation failed");
     auto cleanup = [&](int code) {
-        if (b->d_sess) (void)hipFree(b->d_sess);
-        if (b->d_prefix) (void)hipFree(b->d_prefix);
-        if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
-        delete b;
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        release_batch(b);
         return code;
     };
     if (stream) {
@@ -991,6 +1022,12 @@ int kf_batch_create(kf_session *const *sessions, int n, void *stream, kf_batch *
     if (hipMalloc((void **)&b->d_sess, sizeof(kf::BatchSession) * kf::kMaxBatch) != hipSuccess ||
         hipMalloc((void **)&b->d_prefix, sizeof(int) * 3 * (kf::kMaxBatch + 1)) != hipSuccess)
         return cleanup(fail(KF_ERR_OOM, "batch tables"));
+    for (int k = 0; k < kf_batch::kSlots; ++k)
+        if (hipHostMalloc((void **)&b->h_stage[k], kf_batch::kSlotBytes, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&b->slot_ev[k], hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(KF_ERR_OOM, "batch staging"));
+    if (hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(KF_ERR_HIP, "hipEventCreate failed"));
     int rc = batch_fill(b, sessions, n);
     if (rc) return cleanup(rc);
     *out = b;
@@ -1007,10 +1044,7 @@ int kf_batch_destroy(kf_batch *b)
 {
     if (!b) return KF_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->d_sess) (void)hipFree(b->d_sess);
-    if (b->d_prefix) (void)hipFree(b->d_prefix);
-    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
-    delete b;
+    release_batch(b);
     return KF_OK;
 }
 
