@@ -158,7 +158,10 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         negative value (exact: same hits).  Default 1.
  *   KF_OPT_GRAPH          kf_frame_* replays a captured hipGraph.  Default 1.
  *   KF_OPT_BRICK_REBUILD_PERIOD  frames between exact brick-flag rebuilds
- *                         (flags are otherwise only set).  Default 16, 0 = never. */
+ *                         (flags are otherwise only set): the raycast's
+ *                         occupancy flags, and the zero flags with which
+ *                         integrate neither reads nor rewrites bricks that
+ *                         hold only (0, 0).  Default 16, 0 = never. */
 #define KF_OPT_FAST_MATH 1
 #define KF_OPT_SKIP_EMPTY 2
 #define KF_OPT_GRAPH 3
@@ -171,8 +174,8 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
 #define KF_OPT_ICP_ROWS_PER_BLOCK 6
 int kf_set_option(kf_session *s, int option, int value);
 int kf_get_option(kf_session *s, int option, int *value);
-/* Recompute the brick occupancy flags from the volume (done automatically by
- * kf_reset / kf_write_tsdf and periodically by kf_frame_*). */
+/* Recompute the brick occupancy and zero flags from the volume (done
+ * automatically by kf_reset / kf_write_tsdf and periodically by kf_frame_*). */
 int kf_rebuild_bricks(kf_session *s);
 
 /* Synchronise the session stream / get its stream (hipStream_t). */

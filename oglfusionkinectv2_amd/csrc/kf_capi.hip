@@ -101,7 +101,7 @@ struct kf_session {
     kf::DevState *st = nullptr;
     float *block_partials = nullptr;
     unsigned long long *counts = nullptr;
-    uint8_t *bflags = nullptr, *dflags = nullptr, *cflags = nullptr;
+    uint8_t *bflags = nullptr, *dflags = nullptr, *cflags = nullptr, *zflags = nullptr;
     int NB[3] = {0, 0, 0}, NC[3] = {0, 0, 0};
     size_t nbricks = 0, ncoarse = 0;
     // options (kf_set_option)
@@ -139,7 +139,7 @@ size_t session_bytes(const kf_config *c)
     b += sizeof(kf::DevState) + (size_t)(c->height + 8) * 32 * 4 + 64;
     const size_t nb = (size_t)((c->volume_size[0] + 7) / 8) * ((c->volume_size[1] + 7) / 8) *
                       ((c->volume_size[2] + 7) / 8);
-    b += 3 * nb;                                   // brick flags, dilated, coarse (bound)
+    b += 4 * nb;                                   // brick flags, dilated, zero, coarse (bound)
     return b;
 }
 
@@ -157,7 +157,7 @@ void free_all(kf_session *s)
         if (p) (void)hipFree(p);
     };
     drop_graph(s);
-    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags); f(s->cflags);
+    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags); f(s->cflags); f(s->zflags);
     for (int l = 0; l < 3; ++l) {
         f(s->vert[l]);
         f(s->norm[l]);
@@ -215,6 +215,7 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated)
     a.zlen = std::max(zlen, 8);
     a.gated = gated;
     a.bflags = s->bflags;
+    a.zflags = s->zflags;
     for (int k = 0; k < 3; ++k) a.NB[k] = s->NB[k];
     return a;
 }
@@ -271,13 +272,15 @@ kf::BrickArgs brick_args(kf_session *s)
         a.NC[k] = s->NC[k];
     }
     a.tsdf = s->tsdf;
+    a.weight = s->weight;
     a.flags = s->bflags;
     a.dflags = s->dflags;
     a.cflags = s->cflags;
+    a.zflags = s->zflags;
     return a;
 }
 
-// exact brick flags from the whole value plane, then the dilated copy
+// exact brick flags (occupancy and zero) from both planes, then the dilated copy
 int rebuild_bricks(kf_session *s)
 {
     kf::BrickArgs a = brick_args(s);
@@ -521,6 +524,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
     s->ncoarse = (size_t)s->NC[0] * s->NC[1] * s->NC[2];
     bool ok = alloc((void **)&s->depth0, p0 * 4) && alloc((void **)&s->depth_m, p0 * 4) &&
               alloc((void **)&s->bflags, s->nbricks) && alloc((void **)&s->dflags, s->nbricks) &&
+              alloc((void **)&s->zflags, s->nbricks) &&
               alloc((void **)&s->cflags, s->ncoarse) &&
               alloc((void **)&s->stage_u16, p0 * 2) &&
               alloc((void **)&s->stage_f32, p0 * 4) && alloc((void **)&s->refV, p0 * 16) &&
@@ -542,6 +546,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
              hipMemsetAsync(s->depth_m, 0, p0 * 4, s->stream) == hipSuccess &&
              hipMemsetAsync(s->bflags, 0, s->nbricks, s->stream) == hipSuccess &&
              hipMemsetAsync(s->dflags, 0, s->nbricks, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->zflags, 0, s->nbricks, s->stream) == hipSuccess &&
              hipMemsetAsync(s->cflags, 0, s->ncoarse, s->stream) == hipSuccess &&
              hipMemsetAsync(s->st, 0, sizeof(kf::DevState), s->stream) == hipSuccess;
     for (int l = 0; l < 3 && z; ++l) {
@@ -582,6 +587,7 @@ int kf_reset_volume(kf_session *s)
     KF_HIP(hipMemsetAsync(s->weight, 0, vb, s->stream));
     KF_HIP(hipMemsetAsync(s->bflags, 0, s->nbricks, s->stream));
     KF_HIP(hipMemsetAsync(s->dflags, 0, s->nbricks, s->stream));
+    KF_HIP(hipMemsetAsync(s->zflags, 0, s->nbricks, s->stream));
     KF_HIP(hipMemsetAsync(s->cflags, 0, s->ncoarse, s->stream));
     return KF_OK;
 }

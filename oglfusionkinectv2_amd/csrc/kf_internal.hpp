@@ -47,13 +47,14 @@ struct IntegrateArgs {
     int zlen;             // z voxels per segment (blockIdx.z), multiple of 8
     int gated;
     uint8_t *bflags;      // 8^3 brick occupancy flags (set-only)
+    uint8_t *zflags;      // 8^3 bricks that may hold a voxel other than (0, 0) (set-only)
     int NB[3];            // brick grid
 };
 
 struct BrickArgs {
     int N[3], NB[3], NC[3];
-    const void *tsdf;
-    uint8_t *flags, *dflags, *cflags;
+    const void *tsdf, *weight;
+    uint8_t *flags, *dflags, *cflags, *zflags;
 };
 
 struct RaycastArgs {

@@ -35,6 +35,7 @@ struct StoreRG16I {  // integrate.cs:37-50 (RG16I texture)
     __device__ static V from_weight(float w) { return (V)(int)w; }
     // value that can end a ray (< 0) or reach the 0.8 large-step threshold
     __device__ static bool interesting(V v) { return v < 0 || v >= (V)25886; }  // 0.79 * 32766
+    __device__ static bool zero_bits(V v) { return v == 0; }
 };
 struct StoreF32 {
     using V = float;
@@ -45,6 +46,7 @@ struct StoreF32 {
     __device__ static V from_value(float t) { return t; }
     __device__ static V from_weight(float w) { return w; }
     __device__ static bool interesting(V v) { return v < 0.0f || v >= 0.79f; }
+    __device__ static bool zero_bits(V v) { return __float_as_uint(v) == 0u; }
 };
 struct StoreF16 {
     using V = _Float16;
@@ -55,6 +57,7 @@ struct StoreF16 {
     __device__ static V from_value(float t) { return (V)t; }
     __device__ static V from_weight(float w) { return (V)w; }
     __device__ static bool interesting(V v) { return (float)v < 0.0f || (float)v >= 0.79f; }
+    __device__ static bool zero_bits(V v) { return __builtin_bit_cast(uint16_t, v) == 0; }
 };
 
 // ---------------------------------------------------------------------------
@@ -746,16 +749,22 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs &a, const Dev
             kind[k] = !live ? 0 : (diff > -a.mu ? 2 : 1);
             sdf[k] = gmin(1.0f, FAST ? diff * inv_mu : diff / a.mu);
         }
+        // a brick with a clear zero flag holds only (0, 0): its voxels need not
+        // be read, and writing (0, 0) to them changes nothing
+        const size_t bidx = (size_t)(zb >> 3) * bplane + brow;
+        const bool zf = a.zflags[bidx] != 0;
         typename S::V ot[kIntU], ow[kIntU];
 #pragma unroll
         for (int k = 0; k < kIntU; ++k) {
-            if (kind[k] == 2) {
+            ot[k] = S::from_value(0.0f);
+            ow[k] = S::from_weight(0.0f);
+            if (kind[k] == 2 && zf) {
                 const size_t vi = col + (size_t)(zb + k) * plane;
                 ot[k] = tv[vi];
                 ow[k] = wv[vi];
             }
         }
-        bool neg = false;
+        bool neg = false, nz = false;
 #pragma unroll
         for (int k = 0; k < kIntU; ++k) {
             if (kind[k] == 0) continue;
@@ -770,12 +779,17 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs &a, const Dev
                     w = gmin(w0 + 1.0f, a.max_weight);
                 }
             }
-            const typename S::V nt = S::from_value(t);
-            tv[vi] = nt;
-            wv[vi] = S::from_weight(w);
+            const typename S::V nt = S::from_value(t), nw = S::from_weight(w);
+            const bool zero = S::zero_bits(nt) && S::zero_bits(nw);
+            if (zf || !zero) {
+                tv[vi] = nt;
+                wv[vi] = nw;
+            }
             neg = neg || S::interesting(nt);
+            nz = nz || !zero;
         }
-        if (neg) bflags[(size_t)(zb >> 3) * bplane + brow] = 1;
+        if (neg) bflags[bidx] = 1;
+        if (nz && !zf) a.zflags[bidx] = 1;
     }
 }
 
@@ -821,6 +835,10 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
     const size_t bplane = (size_t)a.NB[0] * a.NB[1];
 
     for (int zb = z0; zb < z1; zb += kIntU) {
+        // a brick with a clear zero flag holds only (0, 0): its voxels need not
+        // be read, and writing (0, 0) to them changes nothing
+        const size_t bidx = (size_t)(zb >> 3) * bplane + brow;
+        const bool zf = a.zflags[bidx] != 0;
         int pix[kIntU];
         float pz[kIntU], fac[kIntU];
 #pragma unroll
@@ -854,13 +872,15 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
         typename S::V ot[kIntU], ow[kIntU];
 #pragma unroll
         for (int k = 0; k < kIntU; ++k) {
-            if (kind[k] == 2) {
+            ot[k] = S::from_value(0.0f);
+            ow[k] = S::from_weight(0.0f);
+            if (kind[k] == 2 && zf) {
                 const Idx vi = col + (Idx)(zb + k) * plane;
                 ot[k] = tv[vi];
                 ow[k] = wv[vi];
             }
         }
-        bool neg = false;
+        bool neg = false, nz = false;
 #pragma unroll
         for (int k = 0; k < kIntU; ++k) {
             if (kind[k] == 0) continue;
@@ -874,12 +894,17 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
                     w = fminf(w0 + 1.0f, a.max_weight);
                 }
             }
-            const typename S::V nt = S::from_value(t);
-            tv[vi] = nt;
-            wv[vi] = S::from_weight(w);
+            const typename S::V nt = S::from_value(t), nw = S::from_weight(w);
+            const bool zero = S::zero_bits(nt) && S::zero_bits(nw);
+            if (zf || !zero) {
+                tv[vi] = nt;
+                wv[vi] = nw;
+            }
             neg = neg || S::interesting(nt);
+            nz = nz || !zero;
         }
-        if (neg) a.bflags[(size_t)(zb >> 3) * bplane + brow] = 1;
+        if (neg) a.bflags[bidx] = 1;
+        if (nz && !zf) a.zflags[bidx] = 1;
     }
 }
 
@@ -921,36 +946,46 @@ __global__ __launch_bounds__(256) void k_count_integrate(IntegrateArgs a, const 
     atomicAdd(&counts[1], nw);
 }
 
-// Exact brick occupancy: a brick is flagged when any of its voxels holds a
-// value that can stop a ray (negative) or reach the 0.8 step switch.  A block
-// covers 64 x 8 columns (8 bricks along x) and one brick layer in z; each
-// wavefront reads one row of 64 consecutive voxels per z (coalesced), a
-// ballot splits it into the 8 bricks, and LDS ORs the 8 rows.
+// Exact brick flags: `flags` marks a brick any of whose voxels holds a value
+// that can stop a ray (negative) or reach the 0.8 step switch; `zflags` marks
+// a brick holding any voxel other than (0, 0).  A block covers 64 x 8 columns
+// (8 bricks along x) and one brick layer in z; each wavefront reads one row of
+// 64 consecutive voxels per z of both planes (coalesced), a ballot splits it
+// into the 8 bricks, and LDS ORs the 8 rows.
 template <class S>
 __global__ __launch_bounds__(512) void k_brick_rebuild(BrickArgs a)
 {
-    __shared__ unsigned int any[8];
-    if (threadIdx.x < 8) any[threadIdx.x] = 0u;
+    __shared__ unsigned int any[16];
+    if (threadIdx.x < 16) any[threadIdx.x] = 0u;
     __syncthreads();
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
     const int x = blockIdx.x * 64 + lane, y = blockIdx.y * 8 + row, bz = blockIdx.z;
-    bool hit = false;
+    bool hit = false, nz = false;
     if (x < a.N[0] && y < a.N[1]) {
         const typename S::V *v = (const typename S::V *)a.tsdf;
+        const typename S::V *w = (const typename S::V *)a.weight;
         const size_t plane = (size_t)a.N[0] * a.N[1];
         const size_t base = (size_t)y * a.N[0] + x;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int z = bz * 8 + k;
-            if (z < a.N[2]) hit = hit || S::interesting(v[(size_t)z * plane + base]);
+            if (z < a.N[2]) {
+                const typename S::V vv = v[(size_t)z * plane + base], ww = w[(size_t)z * plane + base];
+                hit = hit || S::interesting(vv);
+                nz = nz || !S::zero_bits(vv) || !S::zero_bits(ww);
+            }
         }
     }
-    const unsigned long long m = __ballot(hit);
+    const unsigned long long m = __ballot(hit), mz = __ballot(nz);
     if (lane < 8 && ((m >> (8 * lane)) & 0xFFull)) atomicOr(&any[lane], 1u);
+    if (lane < 8 && ((mz >> (8 * lane)) & 0xFFull)) atomicOr(&any[8 + lane], 1u);
     __syncthreads();
     const int bx = blockIdx.x * 8 + (int)threadIdx.x;
-    if (threadIdx.x < 8 && bx < a.NB[0])
-        a.flags[((size_t)bz * a.NB[1] + blockIdx.y) * a.NB[0] + bx] = any[threadIdx.x] ? 1 : 0;
+    if (threadIdx.x < 8 && bx < a.NB[0]) {
+        const size_t bi = ((size_t)bz * a.NB[1] + blockIdx.y) * a.NB[0] + bx;
+        a.flags[bi] = any[threadIdx.x] ? 1 : 0;
+        a.zflags[bi] = any[8 + threadIdx.x] ? 1 : 0;
+    }
 }
 
 // cflags(c) = OR of dflags over the 4^3 bricks of the 32^3 cell c.
