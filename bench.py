@@ -9,7 +9,7 @@ are resident in HBM before the timed region.
 
 Also reported: the same loop with the FP16 TSDF (4 B/voxel), per-frame
 p50/p99 latency, the dominant kernel's roofline figures from HIP events
-recorded on the session stream around it inside the timed region, the CPU
+carried by its dispatch packets on the session stream, the CPU
 oracle timed on a bounded sample of the same workload, and the multi-session
 FP16 legs (saturated 16 x 512^3 batch, the uneven 16-session mix, 16 x 512^3
 served at 30 Hz, Poisson arrivals under a 200 GB budget) under
@@ -83,51 +83,32 @@ def run_frames(torch, L, g, frames_dev, poses, warmup, steps, stream):
     return elapsed, lat
 
 
-def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, stream, count_every=25):
-    """Per-kernel pass: the same frames issued stage by stage with HIP events
-    recorded on the session stream around each stage.  Every `count_every`
-    frames the integrate traffic of that frame is counted (a separate,
-    synchronising pass outside the bracketed stages)."""
+def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, count_every=25):
+    """Per-kernel pass: the same frames with stage timing on (the pyramid,
+    integrate and raycast launches carry start / stop events in their own
+    dispatch packets, so each figure is that kernel's duration; ICP is its
+    first iteration's start to the finish kernel's end).  Every `count_every`
+    frames the integrate traffic of that frame is counted (a separate pass
+    after the frame)."""
     import ctypes as C
     from oglfusionkinectv2_amd import _lib as K
     s = g.session
     fb = frames_dev[0].numel() * 2
     base = frames_dev.data_ptr()
+    g.setOption(K.OPT_STAGE_TIMING, 1)
     g.Reset(poses[0])
-    ev = torch.cuda.Event
-
-    def frame(i, marks):
-        ptr = C.c_void_p(base + i * fb)
-        if marks is not None:
-            marks[0].record(stream)
-        K.check(L.kf_depth_to_vertex_u16_dev(s, ptr))
-        if marks is not None:
-            marks[1].record(stream)
-        K.check(L.kf_track_frame(s))
-        if marks is not None:
-            marks[2].record(stream)
-        K.check(L.kf_integrate(s, K.INTEGRATE_GATED))
-        if marks is not None:
-            marks[3].record(stream)
-        K.check(L.kf_raycast(s))
-        if marks is not None:
-            marks[4].record(stream)
-
     for i in range(warmup):
-        frame(i, None)
+        K.check(L.kf_frame_u16_dev(s, C.c_void_p(base + i * fb)))
     torch.cuda.synchronize()
-    marks = [[ev(enable_timing=True) for _ in range(5)] for _ in range(steps)]
+    acc = {n: [] for n in ("pyramid", "icp", "integrate", "raycast")}
     counts = []
     for i in range(steps):
-        frame(warmup + i, marks[i])
+        K.check(L.kf_frame_u16_dev(s, C.c_void_p(base + (warmup + i) * fb)))
+        for n, v in g.stageTimes().items():
+            acc[n].append(v)
         if i % count_every == count_every - 1:
             counts.append(g.countIntegrateTraffic())
-    torch.cuda.synchronize()
-    names = ["pyramid", "icp", "integrate", "raycast"]
-    acc = {n: [] for n in names}
-    for m in marks:
-        for k, n in enumerate(names):
-            acc[n].append(m[k].elapsed_time(m[k + 1]))
+    g.setOption(K.OPT_STAGE_TIMING, 0)
     return {n: float(np.mean(v)) for n, v in acc.items()}, counts
 
 
@@ -204,7 +185,7 @@ def main():
     st = g.status()
     est = g.getPose()
     ate_gt = float(np.linalg.norm(est[:3, 3] - poses[nframes - 1][:3, 3]))
-    kern, counts = run_kernels(torch, L, g, frames_dev, poses, args.warmup, args.steps, stream)
+    kern, counts = run_kernels(torch, L, g, frames_dev, poses, args.warmup, args.steps)
     # integrate traffic under the reference semantics (sampled frames): each
     # read voxel moves value + weight in and out, each written-only voxel out;
     # plus the depth image in metres
@@ -221,7 +202,7 @@ def main():
         g16 = make_session(args.volume, "f16", stream.cuda_stream, args.fast_math)
         e16, lat16 = run_frames(torch, L, g16, frames_dev, poses, args.warmup, args.steps, stream)
         fps16 = args.steps / e16
-        kern16, _ = run_kernels(torch, L, g16, frames_dev, poses, args.warmup, args.steps, stream)
+        kern16, _ = run_kernels(torch, L, g16, frames_dev, poses, args.warmup, args.steps)
         g16.close()
 
     # --- multi-session -----------------------------------------------------

@@ -172,8 +172,19 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
 #define KF_OPT_RAYCAST_XCD_GROUP 5
 /*   KF_OPT_ICP_ROWS_PER_BLOCK rows of one row group per ICP block.  Default 2. */
 #define KF_OPT_ICP_ROWS_PER_BLOCK 6
+/*   KF_OPT_STAGE_TIMING   1: frames run without the graph and the pyramid,
+ *                         ICP (first iteration .. finish), integrate and
+ *                         raycast launches record start / stop events in
+ *                         their own dispatch packets; kf_read_stage_times
+ *                         returns the last durations.  Default 0. */
+#define KF_OPT_STAGE_TIMING 7
 int kf_set_option(kf_session *s, int option, int value);
 int kf_get_option(kf_session *s, int option, int *value);
+/* Durations in ms of the last pyramid, ICP, integrate and raycast stages run
+ * with KF_OPT_STAGE_TIMING on (synchronises the session stream).  No
+ * counterpart in the reference (its GL timer queries are commented out,
+ * gFusion.cpp:729-736, 783-807). */
+int kf_read_stage_times(kf_session *s, float ms[4]);
 /* Recompute the brick occupancy and zero flags from the volume (done
  * automatically by kf_reset / kf_write_tsdf and periodically by kf_frame_*). */
 int kf_rebuild_bricks(kf_session *s);

@@ -39,6 +39,7 @@ OPT_GRAPH = 3
 OPT_BRICK_REBUILD_PERIOD = 4
 OPT_RAYCAST_XCD_GROUP = 5
 OPT_ICP_ROWS_PER_BLOCK = 6
+OPT_STAGE_TIMING = 7
 
 
 class KfConfig(C.Structure):
@@ -119,6 +120,7 @@ def lib():
         "kf_set_option": [sess, i32, i32],
         "kf_get_option": [sess, i32, C.POINTER(C.c_int)],
         "kf_rebuild_bricks": [sess],
+        "kf_read_stage_times": [sess, C.POINTER(C.c_float)],
         "kf_raycast_stats": [sess, C.POINTER(C.c_uint64)],
     }
     for name, args in sigs.items():

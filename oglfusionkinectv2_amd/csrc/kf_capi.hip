@@ -102,6 +102,11 @@ struct kf_session {
     float *block_partials = nullptr;
     unsigned long long *counts = nullptr;
     uint8_t *bflags = nullptr, *dflags = nullptr, *cflags = nullptr, *zflags = nullptr;
+    float2 *dtiles = nullptr;  // 8x8-pixel (min, max) of the level-0 depth in metres
+    // stage timing (KF_OPT_STAGE_TIMING): start / stop events of pyramid, ICP
+    // (first iteration .. finish), integrate, raycast, carried by the dispatch packets
+    int timing = 0;
+    hipEvent_t tev[8] = {};
     int NB[3] = {0, 0, 0}, NC[3] = {0, 0, 0};
     size_t nbricks = 0, ncoarse = 0;
     // options (kf_set_option)
@@ -137,6 +142,7 @@ size_t session_bytes(const kf_config *c)
     b += 2 * 16 * p0;                              // reference maps
     b += vox * 2 * voxel_plane_bytes(c->storage);  // TSDF value + weight planes
     b += sizeof(kf::DevState) + (size_t)(c->height + 8) * 32 * 4 + 64;
+    b += (size_t)((c->width + 7) / 8) * ((c->height + 7) / 8) * 8;  // depth tiles
     const size_t nb = (size_t)((c->volume_size[0] + 7) / 8) * ((c->volume_size[1] + 7) / 8) *
                       ((c->volume_size[2] + 7) / 8);
     b += 4 * nb;                                   // brick flags, dilated, zero, coarse (bound)
@@ -157,13 +163,15 @@ void free_all(kf_session *s)
         if (p) (void)hipFree(p);
     };
     drop_graph(s);
-    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags); f(s->cflags); f(s->zflags);
+    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags); f(s->cflags); f(s->zflags); f(s->dtiles);
     for (int l = 0; l < 3; ++l) {
         f(s->vert[l]);
         f(s->norm[l]);
     }
     f(s->refV); f(s->refN); f(s->tsdf); f(s->weight); f(s->st); f(s->block_partials); f(s->counts);
     f(s->dbg_result); f(s->dbg_error); f(s->dbg_J);
+    for (int i = 0; i < 8; ++i)
+        if (s->tev[i]) (void)hipEventDestroy(s->tev[i]);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -176,16 +184,20 @@ kf::PyramidArgs pyramid_args(kf_session *s)
     a.w = s->cfg.width;
     a.h = s->cfg.height;
     for (int k = 0; k < 4; ++k) a.cam[k] = s->cfg.cam[k];
+    a.tiles_x = (a.w + 7) / 8;
+    a.tiles_y = (a.h + 7) / 8;
     int start = 0;
     for (int l = 0; l < 3; ++l) {
         a.block_start[l] = start;
-        start += (level_w(s, l) * level_h(s, l) + 255) / 256;
+        // level 0: one wavefront per 8x8 tile, four per block
+        start += l == 0 ? (a.tiles_x * a.tiles_y + 3) / 4 : (level_w(s, l) * level_h(s, l) + 255) / 256;
         a.vert[l] = s->vert[l];
         a.norm[l] = s->norm[l];
     }
     a.blocks_total = start;
     a.depth0 = s->depth0;
     a.depth_m = s->depth_m;
+    a.dtiles = s->dtiles;
     return a;
 }
 
@@ -216,6 +228,9 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated)
     a.gated = gated;
     a.bflags = s->bflags;
     a.zflags = s->zflags;
+    a.dtiles = s->dtiles;
+    a.tiles_x = (c.width + 7) / 8;
+    a.tiles_y = (c.height + 7) / 8;
     for (int k = 0; k < 3; ++k) a.NB[k] = s->NB[k];
     return a;
 }
@@ -290,11 +305,13 @@ int rebuild_bricks(kf_session *s)
     return KF_OK;
 }
 
+hipEvent_t tev(kf_session *s, int i) { return s->timing ? s->tev[i] : nullptr; }
+
 int run_pyramid_from_stage(kf_session *s, const void *dev, int is_u16)
 {
     kf::PyramidArgs a = pyramid_args(s);
-    hipError_t e = is_u16 ? kf::launch_pyramid_u16(a, (const uint16_t *)dev, s->stream)
-                          : kf::launch_pyramid_f32(a, (const float *)dev, s->stream);
+    hipError_t e = is_u16 ? kf::launch_pyramid_u16(a, (const uint16_t *)dev, s->stream, tev(s, 0), tev(s, 1))
+                          : kf::launch_pyramid_f32(a, (const float *)dev, s->stream, tev(s, 0), tev(s, 1));
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("pyramid launch: ") + hipGetErrorString(e));
     return KF_OK;
 }
@@ -323,7 +340,7 @@ kf::IcpArgs icp_args(kf_session *s, int level, int first, int save_pose)
     return a;
 }
 
-int run_icp(kf_session *s, int level, int first, int save_pose)
+int run_icp(kf_session *s, int level, int first, int save_pose, hipEvent_t t0 = nullptr)
 {
     kf::IcpArgs a = icp_args(s, level, first, save_pose);
     if (s->debug) {
@@ -332,7 +349,7 @@ int run_icp(kf_session *s, int level, int first, int save_pose)
         a.dbg_J = s->dbg_J;
         s->dbg_level = level;
     }
-    hipError_t e = kf::launch_icp_iteration(a, s->st, s->stream);
+    hipError_t e = kf::launch_icp_iteration(a, s->st, s->stream, t0);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("icp launch: ") + hipGetErrorString(e));
     return KF_OK;
 }
@@ -344,7 +361,7 @@ int run_track(kf_session *s, int frame)
     bool first_any = true;
     for (int level = 2; level >= 0; --level)
         for (int it = 0; it < s->cfg.iterations[level]; ++it) {
-            int rc = run_icp(s, level, it == 0, first_any ? 1 : 0);
+            int rc = run_icp(s, level, it == 0, first_any ? 1 : 0, first_any ? tev(s, 2) : nullptr);
             if (rc) return rc;
             first_any = false;
         }
@@ -353,7 +370,7 @@ int run_track(kf_session *s, int frame)
         hipError_t e = kf::launch_state_set(s->st, nullptr, 0, s->stream);
         if (e != hipSuccess) return fail(KF_ERR_HIP, hipGetErrorString(e));
     }
-    hipError_t e = kf::launch_icp_finish(s->st, frame, s->stream);
+    hipError_t e = kf::launch_icp_finish(s->st, frame, s->stream, tev(s, 3));
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("finish launch: ") + hipGetErrorString(e));
     return KF_OK;
 }
@@ -361,7 +378,7 @@ int run_track(kf_session *s, int frame)
 int run_integrate(kf_session *s, int gated)
 {
     kf::IntegrateArgs a = integrate_args(s, gated);
-    hipError_t e = kf::launch_integrate(a, s->cfg.storage, s->fast_math, s->st, s->stream);
+    hipError_t e = kf::launch_integrate(a, s->cfg.storage, s->fast_math, s->st, s->stream, tev(s, 4), tev(s, 5));
     if (e == hipSuccess) e = kf::launch_brick_dilate(brick_args(s), s->stream);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("integrate launch: ") + hipGetErrorString(e));
     return KF_OK;
@@ -384,7 +401,7 @@ int run_frame_direct(kf_session *s)
 int run_frame_body(kf_session *s)
 {
     int rc = KF_OK;
-    if (s->use_graph && !s->graph_failed && !s->debug) {
+    if (s->use_graph && !s->graph_failed && !s->debug && !s->timing) {
         if (!s->graph_exec) {
             hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
             if (e == hipSuccess) {
@@ -423,7 +440,7 @@ int run_frame_body(kf_session *s)
 int run_raycast(kf_session *s)
 {
     kf::RaycastArgs a = raycast_args(s);
-    hipError_t e = kf::launch_raycast(a, s->cfg.storage, s->st, s->stream);
+    hipError_t e = kf::launch_raycast(a, s->cfg.storage, s->st, s->stream, tev(s, 6), tev(s, 7));
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("raycast launch: ") + hipGetErrorString(e));
     return KF_OK;
 }
@@ -525,6 +542,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
     bool ok = alloc((void **)&s->depth0, p0 * 4) && alloc((void **)&s->depth_m, p0 * 4) &&
               alloc((void **)&s->bflags, s->nbricks) && alloc((void **)&s->dflags, s->nbricks) &&
               alloc((void **)&s->zflags, s->nbricks) &&
+              alloc((void **)&s->dtiles, (size_t)((cfg->width + 7) / 8) * ((cfg->height + 7) / 8) * 8) &&
               alloc((void **)&s->cflags, s->ncoarse) &&
               alloc((void **)&s->stage_u16, p0 * 2) &&
               alloc((void **)&s->stage_f32, p0 * 4) && alloc((void **)&s->refV, p0 * 16) &&
@@ -547,6 +565,8 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
              hipMemsetAsync(s->bflags, 0, s->nbricks, s->stream) == hipSuccess &&
              hipMemsetAsync(s->dflags, 0, s->nbricks, s->stream) == hipSuccess &&
              hipMemsetAsync(s->zflags, 0, s->nbricks, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->dtiles, 0, (size_t)((cfg->width + 7) / 8) * ((cfg->height + 7) / 8) * 8,
+                            s->stream) == hipSuccess &&
              hipMemsetAsync(s->cflags, 0, s->ncoarse, s->stream) == hipSuccess &&
              hipMemsetAsync(s->st, 0, sizeof(kf::DevState), s->stream) == hipSuccess;
     for (int l = 0; l < 3 && z; ++l) {
@@ -709,6 +729,12 @@ int kf_set_option(kf_session *s, int option, int value)
     case KF_OPT_FAST_MATH: s->fast_math = value ? 1 : 0; break;
     case KF_OPT_SKIP_EMPTY: s->skip_empty = value ? 1 : 0; break;
     case KF_OPT_GRAPH: s->use_graph = value ? 1 : 0; break;
+    case KF_OPT_STAGE_TIMING:
+        if (value && !s->tev[0])
+            for (int i = 0; i < 8; ++i)
+                if (hipEventCreate(&s->tev[i]) != hipSuccess) return fail(KF_ERR_HIP, "hipEventCreate failed");
+        s->timing = value ? 1 : 0;
+        break;
     case KF_OPT_BRICK_REBUILD_PERIOD:
         if (value < 0) return fail(KF_ERR_INVALID, "rebuild period must be >= 0");
         s->rebuild_period = value;
@@ -734,10 +760,23 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_FAST_MATH: *value = s->fast_math; break;
     case KF_OPT_SKIP_EMPTY: *value = s->skip_empty; break;
     case KF_OPT_GRAPH: *value = s->use_graph && !s->graph_failed; break;
+    case KF_OPT_STAGE_TIMING: *value = s->timing; break;
     case KF_OPT_BRICK_REBUILD_PERIOD: *value = s->rebuild_period; break;
     case KF_OPT_RAYCAST_XCD_GROUP: *value = s->xcd_group; break;
     case KF_OPT_ICP_ROWS_PER_BLOCK: *value = s->icp_rows_per_block; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
+    }
+    return KF_OK;
+}
+
+int kf_read_stage_times(kf_session *s, float ms[4])
+{
+    if (!s || !ms) return fail(KF_ERR_INVALID, "NULL argument");
+    if (!s->timing) return fail(KF_ERR_STATE, "stage timing is off (KF_OPT_STAGE_TIMING)");
+    KF_HIP(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < 4; ++k) {
+        const hipError_t e = hipEventElapsedTime(&ms[k], s->tev[2 * k], s->tev[2 * k + 1]);
+        if (e != hipSuccess) return fail(KF_ERR_STATE, "stage " + std::to_string(k) + " has not run with timing on");
     }
     return KF_OK;
 }

@@ -20,6 +20,8 @@ struct PyramidArgs {
     float *depth_m;         // level-0 depth / 1000 (metres), read by integrate
     float4 *vert[3];
     float4 *norm[3];
+    float2 *dtiles;         // per 8x8 level-0 tile: (min, max) of its nonzero depth_m
+    int tiles_x, tiles_y;
 };
 
 struct IcpArgs {
@@ -49,6 +51,8 @@ struct IntegrateArgs {
     uint8_t *bflags;      // 8^3 brick occupancy flags (set-only)
     uint8_t *zflags;      // 8^3 bricks that may hold a voxel other than (0, 0) (set-only)
     int NB[3];            // brick grid
+    const float2 *dtiles; // (min, max) nonzero depth_m per 8x8 pixel tile
+    int tiles_x, tiles_y;
 };
 
 struct BrickArgs {
@@ -111,16 +115,22 @@ hipError_t launch_batch_integrate(const BatchTick &t, int storage, int fast, int
 hipError_t launch_batch_bricks(const BatchTick &t, int total_bricks, int total_coarse, hipStream_t s);
 hipError_t launch_batch_raycast(const BatchTick &t, int storage, int blocks_per_session, hipStream_t s);
 
-hipError_t launch_pyramid_u16(const PyramidArgs &a, const uint16_t *in, hipStream_t s);
-hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t s);
-hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s);
-hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s);
+// Optional t0 / t1: events recorded by the kernel's own dispatch packet at its
+// start / end (exact kernel duration, no separate marker packets).
+hipError_t launch_pyramid_u16(const PyramidArgs &a, const uint16_t *in, hipStream_t s, hipEvent_t t0 = nullptr,
+                              hipEvent_t t1 = nullptr);
+hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t s, hipEvent_t t0 = nullptr,
+                              hipEvent_t t1 = nullptr);
+hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, hipEvent_t t0 = nullptr);
+hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s, hipEvent_t t1 = nullptr);
 hipError_t launch_state_set(DevState *st, const float *pose, int arm_reset, hipStream_t s);
-hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s);
+hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s,
+                            hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 hipError_t launch_brick_rebuild(const BrickArgs &a, int storage, hipStream_t s);
 hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s);
 hipError_t launch_count_integrate(const IntegrateArgs &a, const DevState *st, unsigned long long *counts,
                                   hipStream_t s);
-hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s);
+hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s,
+                          hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 
 }  // namespace kf

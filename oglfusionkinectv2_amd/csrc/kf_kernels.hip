@@ -12,6 +12,7 @@
 //   k_raycast        ray march + gradient normals (gFusion.cpp:1289-1366, raycast.cs)
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
+#include <hip/hip_ext.h>
 #include <float.h>
 #include <stdint.h>
 
@@ -19,6 +20,7 @@
 
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
+#include "kf_march.hpp"
 
 namespace kf {
 
@@ -100,6 +102,23 @@ __device__ __forceinline__ void vertex_of(float d, const float iK[16], int x, in
     v[2] = s * r[2];
 }
 
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Level 0 runs one wavefront per 8x8 pixel tile and also records the tile's
+// (min, max) nonzero depth in metres, which lets integrate pass over z-column
+// chunks that no depth can reach; levels 1-2 run one thread per pixel.
 template <typename DT>
 __device__ __forceinline__ void pyramid_body(const PyramidArgs &a, const DT *__restrict__ in, int b)
 {
@@ -107,9 +126,31 @@ __device__ __forceinline__ void pyramid_body(const PyramidArgs &a, const DT *__r
     if (b >= a.block_start[2]) l = 2;
     else if (b >= a.block_start[1]) l = 1;
     const int w = a.w >> l, h = a.h >> l;
-    const int idx = (b - a.block_start[l]) * 256 + threadIdx.x;
-    if (idx >= w * h) return;
-    const int x = idx % w, y = idx / w;
+    int x, y;
+    float dc;
+    if (l == 0) {
+        const int tile = (b - a.block_start[0]) * 4 + (int)(threadIdx.x >> 6);
+        if (tile >= a.tiles_x * a.tiles_y) return;  // whole wavefront
+        const int lane = threadIdx.x & 63;
+        x = (tile % a.tiles_x) * 8 + (lane & 7);
+        y = (tile / a.tiles_x) * 8 + (lane >> 3);
+        const bool inside = x < w && y < h;
+        dc = inside ? depth_at(in, a.w, 0, x, y) : 0.0f;
+        const float dm = dc / 1000.0f;  // integrate.cs:74, once per pixel instead of per voxel
+        const bool valid = inside && dm != 0.0f && dm == dm;
+        const float mn = wave_min(valid ? dm : INFINITY), mx = wave_max(valid ? dm : -INFINITY);
+        if (lane == 0) a.dtiles[tile] = make_float2(mn, mx);
+        if (!inside) return;
+        a.depth0[y * w + x] = dc;
+        a.depth_m[y * w + x] = dm;
+    } else {
+        const int idx = (b - a.block_start[l]) * 256 + threadIdx.x;
+        if (idx >= w * h) return;
+        x = idx % w;
+        y = idx / w;
+        dc = depth_at(in, a.w, l, x, y);
+    }
+    const int idx = y * w + x;
 
     float cam[4];
     const float sc = (float)(1 << l);
@@ -118,11 +159,6 @@ __device__ __forceinline__ void pyramid_body(const PyramidArgs &a, const DT *__r
     float iK[16];
     inverse_camera_matrix(cam, iK);
 
-    const float dc = depth_at(in, a.w, l, x, y);
-    if (l == 0) {
-        a.depth0[idx] = dc;
-        a.depth_m[idx] = dc / 1000.0f;  // integrate.cs:74, once per pixel instead of per voxel
-    }
     float v[3];
     vertex_of(dc, iK, x, y, v);
     a.vert[l][idx] = make_float4(v[0], v[1], v[2], 0.0f);
@@ -799,12 +835,90 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevSta
     integrate_body<S, FAST>(a, st, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// The voxels z of a column are at camera-frame positions pos + z * dl; the
+// ones that can pass the image test of integrate.cs:64-70 satisfy five linear
+// inequalities in z (depth in front of the near limit, pixel x and y inside
+// the image).  Solving them in double with a 1-pixel and half-near-limit
+// margin gives a range [zs, ze) outside of which every voxel of the column
+// fails that test by more than the fast path's rounding, so only [zs, ze) is
+// walked; inside it each voxel is still tested exactly.  zs stays on the
+// segment's chunk grid (multiples of 8 from z0).
+__device__ __forceinline__ void column_frustum(const ColumnSetup &c, float fx, float fy, float cxh, float cyh,
+                                               float dmax_x, float dmax_y, int z0, int z1, int &zs, int &ze)
+{
+    const double P0 = c.pos[0], P1 = c.pos[1], P2 = c.pos[2];
+    const double D0 = c.dl[0], D1 = c.dl[1], D2 = c.dl[2];
+    const double m = 1.0;
+    double lo = -1e30, hi = 1e30;
+    auto clip = [&](double A, double B) {  // keep z with A + B z >= 0
+        if (B > 0.0)
+            lo = fmax(lo, -A / B);
+        else if (B < 0.0)
+            hi = fmin(hi, -A / B);
+        else if (A < 0.0)
+            hi = -1e30;
+    };
+    clip(P2 - 0.00005, D2);
+    clip((double)fx * P0 + ((double)cxh + m) * P2, (double)fx * D0 + ((double)cxh + m) * D2);
+    clip(-((double)fx * P0 + ((double)cxh - (double)dmax_x - m) * P2),
+         -((double)fx * D0 + ((double)cxh - (double)dmax_x - m) * D2));
+    clip((double)fy * P1 + ((double)cyh + m) * P2, (double)fy * D1 + ((double)cyh + m) * D2);
+    clip(-((double)fy * P1 + ((double)cyh - (double)dmax_y - m) * P2),
+         -((double)fy * D1 + ((double)cyh - (double)dmax_y - m) * D2));
+    lo = fmax(floor(lo) - 1.0, (double)z0);
+    hi = fmin(ceil(hi) + 2.0, (double)z1);
+    if (!(lo < hi)) {
+        zs = ze = z0;
+        return;
+    }
+    zs = z0 + (((int)lo - z0) / kIntU) * kIntU;
+    ze = (int)hi;
+}
+
+// In a brick holding only (0, 0) a voxel changes only if it is live with
+// sdf < 0.2, i.e. depth - pos.z in (-mu, 0.2 mu) (diff = (depth - pos.z) * fac
+// with fac >= 1; 1 % margins cover rounding).  The 8 voxels of a chunk lie on a
+// segment from p to p + 7 dl; its projection stays inside the pixel box of the
+// two end points (+-1 pixel), so if no nonzero depth of the 8x8 tiles covering
+// that box lies within reach of the chunk's pos.z range, every voxel of the
+// chunk keeps (0, 0) and the chunk needs no work.  Boxes wider than 2x2 tiles
+// and chunks near the camera plane are always processed.
+__device__ __forceinline__ bool chunk_unreachable(const IntegrateArgs &a, const ColumnSetup &c, float p0, float p1,
+                                                  float p2, float fx, float fy, float cxh, float cyh, float mu_lo,
+                                                  float mu_hi)
+{
+    const float q0 = fmaf(7.0f, c.dl[0], p0), q1 = fmaf(7.0f, c.dl[1], p1), q2 = fmaf(7.0f, c.dl[2], p2);
+    const float zlo = fminf(p2, q2), zhi = fmaxf(p2, q2);
+    if (!(zlo > 0.001f)) return false;
+    const float rp = __builtin_amdgcn_rcpf(p2), rq = __builtin_amdgcn_rcpf(q2);
+    const float xa = fmaf(fx, p0 * rp, cxh), xb = fmaf(fx, q0 * rq, cxh);
+    const float ya = fmaf(fy, p1 * rp, cyh), yb = fmaf(fy, q1 * rq, cyh);
+    const float W = (float)a.dw, H = (float)a.dh;
+    const float x0 = fminf(xa, xb) - 1.0f, x1 = fmaxf(xa, xb) + 1.0f;
+    const float y0 = fminf(ya, yb) - 1.0f, y1 = fmaxf(ya, yb) + 1.0f;
+    if (!(x0 == x0 && x1 == x1 && y0 == y0 && y1 == y1)) return false;
+    if (x1 < 0.0f || y1 < 0.0f || x0 >= W || y0 >= H) return true;  // whole box off the image
+    const int tx0 = (int)fmaxf(x0, 0.0f) >> 3, tx1 = (int)fminf(x1, W - 1.0f) >> 3;
+    const int ty0 = (int)fmaxf(y0, 0.0f) >> 3, ty1 = (int)fminf(y1, H - 1.0f) >> 3;
+    if (tx1 - tx0 > 1 || ty1 - ty0 > 1) return false;
+    float dmin = INFINITY, dmax = -INFINITY;
+    for (int ty = ty0; ty <= ty1; ++ty)
+        for (int tx = tx0; tx <= tx1; ++tx) {
+            const float2 t = a.dtiles[ty * a.tiles_x + tx];
+            dmin = fminf(dmin, t.x);
+            dmax = fmaxf(dmax, t.y);
+        }
+    const float m = 1e-4f;
+    return !(zhi + m > dmin - mu_lo && zlo - m < dmax + mu_hi);
+}
+
 // Reduced-arithmetic integrate (KF_OPT_FAST_MATH).  Same update rule as
 // k_integrate, with the per-voxel arithmetic cut to about a third: one
 // hardware reciprocal of the camera depth per voxel, the projection folded to
 // px = fx * (x / z) + cx + 0.5 (row 2 of K is (0, 0, 1, 0), so cameraX.z is
-// pos.z), hardware square root, and reciprocals of mu and (w + 1).  Positions
-// are stepped from the segment start as in the reference.  Results agree with
+// pos.z), hardware square root, and reciprocals of mu and (w + 1).  Only the
+// column's frustum range is walked (column_frustum); positions are stepped
+// from its first chunk as in the reference.  Results agree with
 // the correctly rounded evaluation to a few ulps; voxels on a pixel edge or a
 // branch threshold can take the other side (counted as flips by the tests).
 template <class S, bool OFF32>
@@ -819,13 +933,13 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
 
     ColumnSetup c;
     column_setup(a, st->pose, x, y, c);
-    const int z0 = bz * a.zlen;
-    const int z1 = min(z0 + a.zlen, a.N[2]);
-    float p0 = fmaf((float)z0, c.dl[0], c.pos[0]);
-    float p1 = fmaf((float)z0, c.dl[1], c.pos[1]);
-    float p2 = fmaf((float)z0, c.dl[2], c.pos[2]);
     const float fx = a.cam[0], fy = a.cam[1], cxh = a.cam[2] + 0.5f, cyh = a.cam[3] + 0.5f;
     const float dmax_x = (float)(a.dw - 1), dmax_y = (float)(a.dh - 1);
+    const int z0 = bz * a.zlen;
+    const int z1 = min(z0 + a.zlen, a.N[2]);
+    int zs, ze;
+    column_frustum(c, fx, fy, cxh, cyh, dmax_x, dmax_y, z0, z1, zs, ze);
+    const float mu_lo = 0.2f * a.mu * 1.01f, mu_hi = a.mu * 1.01f;
     const float inv_mu = 1.0f / a.mu, neg_mu = -a.mu;
     typename S::V *__restrict__ tv = (typename S::V *)a.tsdf;
     typename S::V *__restrict__ wv = (typename S::V *)a.weight;
@@ -834,11 +948,15 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
     const size_t brow = (size_t)(y >> 3) * a.NB[0] + (x >> 3);
     const size_t bplane = (size_t)a.NB[0] * a.NB[1];
 
-    for (int zb = z0; zb < z1; zb += kIntU) {
+    for (int zb = zs; zb < ze; zb += kIntU) {
+        float p0 = fmaf((float)zb, c.dl[0], c.pos[0]);
+        float p1 = fmaf((float)zb, c.dl[1], c.pos[1]);
+        float p2 = fmaf((float)zb, c.dl[2], c.pos[2]);
         // a brick with a clear zero flag holds only (0, 0): its voxels need not
         // be read, and writing (0, 0) to them changes nothing
         const size_t bidx = (size_t)(zb >> 3) * bplane + brow;
         const bool zf = a.zflags[bidx] != 0;
+        if (chunk_unreachable(a, c, p0, p1, p2, fx, fy, cxh, cyh, mu_lo, mu_hi) && !zf) continue;
         int pix[kIntU];
         float pz[kIntU], fac[kIntU];
 #pragma unroll
@@ -1281,11 +1399,9 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
                                 ts = fminf(ts, t + (base + 0.01f - sm.s[k]) / ds[k]);
                         }
                         // this sample and every later one below ts: same float steps as the reference
-                        do {
-                            t_prev = t;
-                            t += stepsize;
-                            n_skip++;
-                        } while (t < ts && t < tfar && ++guard < kMaxMarchSteps);
+                        const int ns = march_to(t, t_prev, stepsize, fminf(ts, tfar));
+                        n_skip += ns;
+                        guard += ns;
                         have_prev = false;
                         n_brick++;
                         continue;
@@ -1473,26 +1589,35 @@ __global__ __launch_bounds__(256) void k_batch_raycast(BatchTick t, int per)
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_pyramid_u16(const PyramidArgs &a, const uint16_t *in, hipStream_t s)
+// Plain launch, or with start / stop events carried by the dispatch packet.
+template <typename... KArgs, typename... Args>
+static void launch_timed(void (*k)(KArgs...), dim3 g, dim3 b, hipStream_t s, hipEvent_t t0, hipEvent_t t1,
+                         Args... args)
 {
-    hipLaunchKernelGGL(k_pyramid<uint16_t>, dim3(a.blocks_total), dim3(256), 0, s, a, in);
+    if (t0 || t1) hipExtLaunchKernelGGL(k, g, b, 0, s, t0, t1, 0, args...);
+    else hipLaunchKernelGGL(k, g, b, 0, s, args...);
+}
+
+hipError_t launch_pyramid_u16(const PyramidArgs &a, const uint16_t *in, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
+{
+    launch_timed(k_pyramid<uint16_t>, dim3(a.blocks_total), dim3(256), s, t0, t1, a, in);
     return hipGetLastError();
 }
-hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t s)
+hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
-    hipLaunchKernelGGL(k_pyramid<float>, dim3(a.blocks_total), dim3(256), 0, s, a, in);
+    launch_timed(k_pyramid<float>, dim3(a.blocks_total), dim3(256), s, t0, t1, a, in);
     return hipGetLastError();
 }
-hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s)
+hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, hipEvent_t t0)
 {
     // 8 row groups x rows_step blocks; each block takes every rows_step-th row
     // of its group (fewer, fuller blocks: one release + one ticket each)
-    hipLaunchKernelGGL(k_icp_iteration, dim3(8 * a.rows_step), dim3(kIcpThreads), 0, s, a, st);
+    launch_timed(k_icp_iteration, dim3(8 * a.rows_step), dim3(kIcpThreads), s, t0, nullptr, a, st);
     return hipGetLastError();
 }
-hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s)
+hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s, hipEvent_t t1)
 {
-    hipLaunchKernelGGL(k_icp_finish, dim3(1), dim3(64), 0, s, st, frame);
+    launch_timed(k_icp_finish, dim3(1), dim3(64), s, nullptr, t1, st, frame);
     return hipGetLastError();
 }
 hipError_t launch_state_set(DevState *st, const float *pose, int arm_reset, hipStream_t s)
@@ -1570,14 +1695,15 @@ hipError_t launch_batch_raycast(const BatchTick &t, int storage, int per, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s)
+hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s,
+                            hipEvent_t t0, hipEvent_t t1)
 {
     const dim3 g = integrate_grid(a), b(64, 4);
     const bool off32 = (size_t)a.N[0] * a.N[1] * a.N[2] < (size_t(1) << 32);
 #define KF_INT(ST)                                                                     \
-    if (fast && off32) hipLaunchKernelGGL((k_integrate_fast<ST, true>), g, b, 0, s, a, st); \
-    else if (fast) hipLaunchKernelGGL((k_integrate_fast<ST, false>), g, b, 0, s, a, st);  \
-    else hipLaunchKernelGGL((k_integrate<ST, false>), g, b, 0, s, a, st);
+    if (fast && off32) launch_timed(k_integrate_fast<ST, true>, g, b, s, t0, t1, a, st);  \
+    else if (fast) launch_timed(k_integrate_fast<ST, false>, g, b, s, t0, t1, a, st);     \
+    else launch_timed(k_integrate<ST, false>, g, b, s, t0, t1, a, st);
     switch (storage) {
     case KF_STORAGE_RG16I: KF_INT(StoreRG16I); break;
     case KF_STORAGE_F32: KF_INT(StoreF32); break;
@@ -1617,13 +1743,14 @@ hipError_t launch_count_integrate(const IntegrateArgs &a, const DevState *st, un
     return hipGetLastError();
 }
 
-hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s)
+hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s, hipEvent_t t0,
+                          hipEvent_t t1)
 {
     const dim3 g(a.blocks), b(256);
     switch (storage) {
-    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_raycast<StoreRG16I>, g, b, 0, s, a, st); break;
-    case KF_STORAGE_F32: hipLaunchKernelGGL(k_raycast<StoreF32>, g, b, 0, s, a, st); break;
-    case KF_STORAGE_F16: hipLaunchKernelGGL(k_raycast<StoreF16>, g, b, 0, s, a, st); break;
+    case KF_STORAGE_RG16I: launch_timed(k_raycast<StoreRG16I>, g, b, s, t0, t1, a, st); break;
+    case KF_STORAGE_F32: launch_timed(k_raycast<StoreF32>, g, b, s, t0, t1, a, st); break;
+    case KF_STORAGE_F16: launch_timed(k_raycast<StoreF16>, g, b, s, t0, t1, a, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

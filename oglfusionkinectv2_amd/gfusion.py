@@ -339,6 +339,13 @@ class GFusion:
         K.check(K.lib().kf_raycast_stats(self.session, st))
         return dict(evaluated=st[0], skipped=st[1], bricks=st[2], hits=st[3], max_eval=st[4], max_iter=st[5])
 
+    def stageTimes(self):
+        """Last pyramid / ICP / integrate / raycast durations in ms (needs
+        setOption(OPT_STAGE_TIMING, 1))."""
+        ms = (C.c_float * 4)()
+        K.check(K.lib().kf_read_stage_times(self.session, ms))
+        return dict(pyramid=ms[0], icp=ms[1], integrate=ms[2], raycast=ms[3])
+
     def rebuildBricks(self):
         K.check(K.lib().kf_rebuild_bricks(self.session))
 

@@ -432,3 +432,28 @@ def test_graph_replay_matches_direct_launches(seq20):
     np.testing.assert_array_equal(runs[0][1][0], runs[1][1][0])
     np.testing.assert_array_equal(runs[0][2], runs[1][2])
     assert runs[0][3] == runs[1][3]
+
+
+def test_stage_timing_matches_graph_frames(seq20):
+    """Frames run with stage timing (direct launches carrying dispatch-packet
+    events) leave the same state as graph replays, and report positive
+    durations for every stage."""
+    print("case: stage timing vs graph", flush=True)
+    depth, gt = seq20
+    runs = []
+    for timing in (0, 1):
+        g, cfg = make(storage="f16")
+        g.setOption(K.OPT_STAGE_TIMING, timing)
+        g.Reset(gt[0])
+        times = []
+        for f in range(6):
+            g.frame(depth[f].astype(np.float32))
+            if timing:
+                times.append(g.stageTimes())
+        assert g.getOption(K.OPT_STAGE_TIMING) == timing
+        runs.append((g.getPose(), g.getVolume(), g.getVerts()))
+        for t in times:
+            assert all(v > 0 for v in t.values()), t
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1][0].view(np.uint16), runs[1][1][0].view(np.uint16))
+    np.testing.assert_array_equal(runs[0][2], runs[1][2])
