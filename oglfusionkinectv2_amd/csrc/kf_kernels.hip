@@ -38,6 +38,8 @@ struct StoreRG16I {  // integrate.cs:37-50 (RG16I texture)
     // value that can end a ray (< 0) or reach the 0.8 large-step threshold
     __device__ static bool interesting(V v) { return v < 0 || v >= (V)25886; }  // 0.79 * 32766
     __device__ static bool zero_bits(V v) { return v == 0; }
+    __device__ static uint32_t bits(V v) { return (uint32_t)(uint16_t)v; }
+    __device__ static V from_bits(uint32_t b) { return (V)(uint16_t)b; }
 };
 struct StoreF32 {
     using V = float;
@@ -49,17 +51,25 @@ struct StoreF32 {
     __device__ static V from_weight(float w) { return w; }
     __device__ static bool interesting(V v) { return v < 0.0f || v >= 0.79f; }
     __device__ static bool zero_bits(V v) { return __float_as_uint(v) == 0u; }
+    __device__ static uint32_t bits(V v) { return __float_as_uint(v); }
+    __device__ static V from_bits(uint32_t b) { return __uint_as_float(b); }
 };
+// binary16 values held as their bit patterns: 16-bit loads then land in a
+// register of their own (no packing of two halves into one register, which
+// ties independent loads together)
 struct StoreF16 {
-    using V = _Float16;
+    using V = uint16_t;
     static constexpr bool kScaled = false;
-    __device__ static float value(V v) { return (float)v; }
-    __device__ static float weight(V v) { return (float)v; }
-    __device__ static float raw(V v) { return (float)v; }
-    __device__ static V from_value(float t) { return (V)t; }
-    __device__ static V from_weight(float w) { return (V)w; }
-    __device__ static bool interesting(V v) { return (float)v < 0.0f || (float)v >= 0.79f; }
-    __device__ static bool zero_bits(V v) { return __builtin_bit_cast(uint16_t, v) == 0; }
+    __device__ static float half(V v) { return (float)__builtin_bit_cast(_Float16, v); }
+    __device__ static float value(V v) { return half(v); }
+    __device__ static float weight(V v) { return half(v); }
+    __device__ static float raw(V v) { return half(v); }
+    __device__ static V from_value(float t) { return __builtin_bit_cast(uint16_t, (_Float16)t); }
+    __device__ static V from_weight(float w) { return __builtin_bit_cast(uint16_t, (_Float16)w); }
+    __device__ static bool interesting(V v) { return half(v) < 0.0f || half(v) >= 0.79f; }
+    __device__ static bool zero_bits(V v) { return v == 0; }
+    __device__ static uint32_t bits(V v) { return (uint32_t)v; }
+    __device__ static V from_bits(uint32_t b) { return (V)b; }
 };
 
 // ---------------------------------------------------------------------------
@@ -987,15 +997,18 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
             kind[k] = !live ? 0 : (diff > neg_mu ? 2 : 1);
             sdf[k] = fminf(1.0f, diff * inv_mu);
         }
-        typename S::V ot[kIntU], ow[kIntU];
+        // stored bits, zero-extended to 32 bits (no register packing of
+        // 16-bit values, so the conditional loads stay independent and the
+        // conversions wait for them only after all are issued)
+        uint32_t ot[kIntU], ow[kIntU];
 #pragma unroll
         for (int k = 0; k < kIntU; ++k) {
-            ot[k] = S::from_value(0.0f);
-            ow[k] = S::from_weight(0.0f);
+            ot[k] = 0u;
+            ow[k] = 0u;
             if (kind[k] == 2 && zf) {
                 const Idx vi = col + (Idx)(zb + k) * plane;
-                ot[k] = tv[vi];
-                ow[k] = wv[vi];
+                ot[k] = S::bits(tv[vi]);
+                ow[k] = S::bits(wv[vi]);
             }
         }
         bool neg = false, nz = false;
@@ -1005,7 +1018,7 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
             const Idx vi = col + (Idx)(zb + k) * plane;
             float t = 0.0f, w = 0.0f;
             if (kind[k] == 2) {
-                const float t0 = S::value(ot[k]), w0 = S::weight(ow[k]);
+                const float t0 = S::value(S::from_bits(ot[k])), w0 = S::weight(S::from_bits(ow[k]));
                 const float wd = fmaf(w0, t0, sdf[k]) * __builtin_amdgcn_rcpf(w0 + 1.0f);
                 if (wd < 0.2f) {
                     t = fminf(fmaxf(wd, -0.1f), 0.2f);
@@ -1307,7 +1320,10 @@ __device__ void gradient(const VolView<S> &V, const RaycastArgs &a, const float 
 }
 
 constexpr int kMaxMarchSteps = 1 << 20;
-constexpr int kBatch = 4;  // fine-step samples evaluated per march iteration
+#ifndef KF_RAYCAST_BATCH
+#define KF_RAYCAST_BATCH 4
+#endif
+constexpr int kBatch = KF_RAYCAST_BATCH;  // fine-step samples evaluated per march iteration
 
 // One thread per pixel; 16x16 pixel tiles per block.  Block ids are remapped
 // so that the blocks an XCD receives (id mod 8) cover one contiguous band of
