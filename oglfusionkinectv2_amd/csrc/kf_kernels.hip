@@ -356,12 +356,13 @@ __device__ void solve6(const double *Cin, const double *b, double *x, double *ws
 }
 
 // glm::eulerAngleXYZ(x3,x4,x5) in double, cast to float, translation x0..2,
-// then m_pose = delta * m_pose in float (gFusion.cpp:700-705)
-__device__ void pose_update(const double x[6], float pose[16])
+// then m_pose = delta * m_pose in float (gFusion.cpp:700-705); cs[k] / sn[k]
+// are cos / sin of x[3 + k] in double
+__device__ void pose_update_cs(const double x[6], const double cs[3], const double sn[3], float pose[16])
 {
-    const double ca = cos(x[3]), sa = sin(x[3]);
-    const double cb = cos(x[4]), sb = sin(x[4]);
-    const double cc = cos(x[5]), sc = sin(x[5]);
+    const double ca = cs[0], sa = sn[0];
+    const double cb = cs[1], sb = sn[1];
+    const double cc = cs[2], sc = sn[2];
     const double Rm[3][3] = {
         {cb * cc, -cb * sc, sb},
         {sa * sb * cc + ca * sc, -sa * sb * sc + ca * cc, -sa * cb},
@@ -379,34 +380,60 @@ __device__ void pose_update(const double x[6], float pose[16])
     mul4(d, pose, pose);
 }
 
-// track.cs:47-163 for one pixel.  Returns the result code; error/J valid for 1.
-__device__ __forceinline__ int track_pixel(const IcpArgs &a, const float *T, const float *view, int x, int y,
-                                           float &err, float J[6])
+
+// track.cs:47-163 for one pixel, split in two so that a thread can issue the
+// loads of several pixels before it waits for any: track_project takes the
+// pixel's normal / vertex and finds the reference-map texel (code -1 / -2
+// decided here, 0 = pending), track_finish takes that texel and returns the
+// result code; error / J are valid for 1.
+struct TrackPx {
+    int code;
+    int ri;  // reference texel, -1 when the projection is NaN (image load out of bounds)
+    float pv[3];
+    float4 n;
+};
+
+__device__ __forceinline__ void track_project(const IcpArgs &a, const float *T, const float *view, float4 n,
+                                              float4 v, TrackPx &t)
 {
-    const int i = y * a.w + x;
-    const float4 n = a.norm[i];
-    if (n.x == 2.0f) return -1;
-    const float4 v = a.vert[i];
+    t.n = n;
+    t.ri = -1;
+    if (n.x == 2.0f) {
+        t.code = -1;
+        return;
+    }
     const float v4[4] = {v.x, v.y, v.z, 1.0f};
     float pv[4], pp[4];
     matvec4(T, v4, pv);
     matvec4(view, pv, pp);
+    t.pv[0] = pv[0];
+    t.pv[1] = pv[1];
+    t.pv[2] = pv[2];
     const float px = pp[0] / pp[2] + 0.5f;
     const float py = pp[1] / pp[2] + 0.5f;
-    if (px < 0.0f || px > (float)(a.rw - 1) || py < 0.0f || py > (float)(a.rh - 1)) return -2;
-    float rn[3] = {0.0f, 0.0f, 0.0f}, rv[3] = {0.0f, 0.0f, 0.0f};
+    if (px < 0.0f || px > (float)(a.rw - 1) || py < 0.0f || py > (float)(a.rh - 1)) {
+        t.code = -2;
+        return;
+    }
     // NaN coordinates pass the bounds test; their image load is out of bounds (zero)
-    if (px == px && py == py) {
-        const int ri = (int)py * a.rw + (int)px;
-        const float4 N4 = a.refN[ri];
-        const float4 V4 = a.refV[ri];
+    if (px == px && py == py) t.ri = (int)py * a.rw + (int)px;
+    t.code = 0;
+}
+
+__device__ __forceinline__ int track_finish(const IcpArgs &a, const float *T, const TrackPx &t, float4 N4, float4 V4,
+                                            float &err, float J[6])
+{
+    if (t.code != 0) return t.code;
+    float rn[3] = {0.0f, 0.0f, 0.0f}, rv[3] = {0.0f, 0.0f, 0.0f};
+    if (t.ri >= 0) {
         rn[0] = N4.x; rn[1] = N4.y; rn[2] = N4.z;
         rv[0] = V4.x; rv[1] = V4.y; rv[2] = V4.z;
     }
     if (rn[0] == -2.0f) return -3;
+    const float *pv = t.pv;
     const float diff[3] = {rv[0] - pv[0], rv[1] - pv[1], rv[2] - pv[2]};
     float pn[3];
-    rotate3(T, n.x, n.y, n.z, pn);
+    rotate3(T, t.n.x, t.n.y, t.n.z, pn);
     if (sqrtf(dot3(diff[0], diff[1], diff[2], diff[0], diff[1], diff[2])) > a.dist_th) return -4;
     if (dot3(pn[0], pn[1], pn[2], rn[0], rn[1], rn[2]) < a.normal_th) return -5;
     err = dot3(rn[0], rn[1], rn[2], diff[0], diff[1], diff[2]);
@@ -420,63 +447,105 @@ __device__ __forceinline__ int track_pixel(const IcpArgs &a, const float *T, con
 }
 
 constexpr int kIcpThreads = 256;
+#ifndef KF_ICP_PIXELS
+#define KF_ICP_PIXELS 4
+#endif
+constexpr int kIcpPx = KF_ICP_PIXELS;  // pixels whose loads a thread issues together
 constexpr int kTerms = 28;  // e^2, e*J[6], upper-triangle J J^T [21]
 constexpr int kPartStride = 32;  // floats per block-partial row (16-byte aligned rows)
 
-// One block per image row; the block's row group is y mod 8 (the work group
-// of reduce.cs that owns the row).  Per-thread FP32 products, DPP wave sums,
-// block partials in FP32, and the last block to finish sums the partials of
-// each group in FP64, applies the combine rule, solves and updates the pose.
-// blk / nblk: this block's index and the block count of the session's launch
-__device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict__ st, int blk, int nblk)
+// ICP iteration pieces (single-session and batched kernels).
+//
+// Block b serves row group g = b % 8 (the reduce.cs work group that owns rows
+// y = g mod 8) and takes every rows_step-th row of that group.  Per-thread FP32
+// products, DPP wave sums, block partials in FP32; the partials of each group
+// are then summed in FP64, the combine rule applied, the 6x6 solved and the
+// pose updated.
+
+// getCameraMatrix(camPams) * inverse(m_pose) (gFusion.cpp:747)
+__device__ __forceinline__ void icp_view(const IcpArgs &a, const float T[16], float view[16])
 {
-    __shared__ float wsum[kIcpThreads / 64][kTerms];
-    __shared__ int last;
-    __shared__ double gsum[8][kTerms];
-    __shared__ double wgsum[kIcpThreads / 64][8][kTerms];
-    __shared__ double solve_ws[kSolveWs + 36 + 6 + 6];
+    float K[16], ip[16];
+    camera_matrix(a.cam, K);
+    inverse4(T, ip);
+    mul4(K, ip, view);
+}
 
-    if (!a.first && st->level_done) return;  // early exit of this level (uniform)
-
-    // block b serves row group g = b % 8 (the reduce.cs work group that owns
-    // rows y = g mod 8) and takes every rows_step-th row of that group
+// this block's 28 sums at pose T written to dst[0..27] (threads < 28); ends
+// with the block synchronised
+__device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float T[16], int blk, float *dst,
+                                                   float (*wsum)[kTerms])
+{
     const int g = blk & 7, jb = blk >> 3;
-    float T[16], view[16];
-    {
-        float K[16], ip[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) T[i] = st->pose[i];
-        camera_matrix(a.cam, K);
-        inverse4(T, ip);
-        mul4(K, ip, view);  // getCameraMatrix(camPams) * inverse(m_pose) (gFusion.cpp:747)
-    }
+    float view[16];
+    icp_view(a, T, view);
 
     float acc[kTerms];
 #pragma unroll
     for (int k = 0; k < kTerms; ++k) acc[k] = 0.0f;
 
-    for (int y = g + 8 * jb; y < a.h; y += 8 * a.rows_step)
-    for (int x = threadIdx.x; x < a.w; x += kIcpThreads) {
-        float e = 0.0f, J[6] = {0, 0, 0, 0, 0, 0};
-        const int r = track_pixel(a, T, view, x, y, e, J);
-        if (a.dbg_result) {
-            const int i = y * a.w + x;
-            a.dbg_result[i] = r;
-            if (r == 1) {
-                a.dbg_error[i] = e;
+    // the thread's pixels in row-major order (rows y = g + 8 jb + 8 rows_step r,
+    // columns x = threadIdx.x + kIcpThreads m), kIcpPx at a time: all their
+    // map loads, then all their reference gathers, then the sums in order
+    const int y0 = g + 8 * jb, ystep = 8 * a.rows_step;
+    const int per_row = (a.w + kIcpThreads - 1) / kIcpThreads;
+    const int nrows = y0 < a.h ? (a.h - 1 - y0) / ystep + 1 : 0;
+    const int npx = nrows * per_row;
+    for (int j0 = 0; j0 < npx; j0 += kIcpPx) {
+        int idx[kIcpPx];
+        TrackPx t[kIcpPx];
 #pragma unroll
-                for (int k = 0; k < 6; ++k) a.dbg_J[6 * i + k] = J[k];
+        for (int p = 0; p < kIcpPx; ++p) {
+            const int j = j0 + p, r = j / per_row, m = j - r * per_row;
+            const int x = threadIdx.x + kIcpThreads * m, y = y0 + ystep * r;
+            idx[p] = (j < npx && x < a.w) ? y * a.w + x : -1;
+        }
+        float4 nv[kIcpPx], vv[kIcpPx];
+#pragma unroll
+        for (int p = 0; p < kIcpPx; ++p) {
+            nv[p] = make_float4(2.0f, 0.0f, 0.0f, 0.0f);
+            vv[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (idx[p] >= 0) {
+                nv[p] = a.norm[idx[p]];
+                vv[p] = a.vert[idx[p]];
             }
         }
-        if (r == 1) {
-            acc[0] += e * e;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) acc[1 + k] += e * J[k];
-            int q = 7;
+        for (int p = 0; p < kIcpPx; ++p) track_project(a, T, view, nv[p], vv[p], t[p]);
+        float4 N4[kIcpPx], V4[kIcpPx];
 #pragma unroll
-            for (int p = 0; p < 6; ++p)
+        for (int p = 0; p < kIcpPx; ++p) {
+            N4[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            V4[p] = N4[p];
+            if (idx[p] >= 0 && t[p].code == 0 && t[p].ri >= 0) {
+                N4[p] = a.refN[t[p].ri];
+                V4[p] = a.refV[t[p].ri];
+            }
+        }
 #pragma unroll
-                for (int s = p; s < 6; ++s) acc[q++] += J[p] * J[s];
+        for (int p = 0; p < kIcpPx; ++p) {
+            if (idx[p] < 0) continue;
+            float e = 0.0f, J[6] = {0, 0, 0, 0, 0, 0};
+            const int r = track_finish(a, T, t[p], N4[p], V4[p], e, J);
+            if (a.dbg_result) {
+                const int i = idx[p];
+                a.dbg_result[i] = r;
+                if (r == 1) {
+                    a.dbg_error[i] = e;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) a.dbg_J[6 * i + k] = J[k];
+                }
+            }
+            if (r == 1) {
+                acc[0] += e * e;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc[1 + k] += e * J[k];
+                int q = 7;
+#pragma unroll
+                for (int p2 = 0; p2 < 6; ++p2)
+#pragma unroll
+                    for (int s = p2; s < 6; ++s) acc[q++] += J[p2] * J[s];
+            }
         }
     }
 
@@ -489,30 +558,25 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
     __syncthreads();
     if (threadIdx.x < kTerms) {
         const int k = threadIdx.x;
-        const float s = (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]);
-        a.block_partials[blk * kPartStride + k] = s;
+        dst[k] = (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // one agent-scope release per block: the barrier above orders the
-        // block's partial stores before it
-        __threadfence();
-        const unsigned t = atomicAdd(&st->ticket, 1u);
-        last = (t == (unsigned)(nblk - 1));
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();  // acquire: every block's partials are visible
+}
 
-    // second pass in FP64.  Thread t sums block partials t, t + 256, ... (all
-    // of row group t % 8), lanes of the same group are combined with xor
-    // shuffles (8, 16, 32), then the four waves through LDS.  Fixed order.
+// FP64 sums per row group of the partial rows 0..nblk-1 into gsum.  Thread t
+// sums block partials t, t + 256, ... (all of row group t % 8), lanes of the
+// same group are combined with xor shuffles (8, 16, 32), then the four waves
+// through LDS.  Fixed order.  Ends with the block synchronised.
+__device__ __forceinline__ void icp_group_sums(const float *partials, int nblk, double (*wgsum)[8][kTerms],
+                                               double (*gsum)[kTerms])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     {
         double ps[kTerms];
 #pragma unroll
         for (int k = 0; k < kTerms; ++k) ps[k] = 0.0;
         for (int pb = threadIdx.x; pb < nblk; pb += kIcpThreads) {
-            const float4 *rp = reinterpret_cast<const float4 *>(a.block_partials + pb * kPartStride);
+            const float4 *rp = reinterpret_cast<const float4 *>(partials + pb * kPartStride);
             float4 q[kTerms / 4];
 #pragma unroll
             for (int m = 0; m < kTerms / 4; ++m) q[m] = rp[m];
@@ -541,20 +605,26 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
         gsum[g][k] = (wgsum[0][g][k] + wgsum[1][g][k]) + (wgsum[2][g][k] + wgsum[3][g][k]);
     }
     __syncthreads();
-    // exposed 8x32 partials in the reduce.cs layout (one slot per thread);
-    // reduce.cs counts every pixel of the group's rows once (:47-62, :101)
-    {
-        const int g = threadIdx.x >> 5, k = threadIdx.x & 31;
-        const int rows = (a.h > g) ? (a.h - g + 7) / 8 : 0;
-        float val = 0.0f;
-        if (k < kTerms) val = (float)gsum[g][k];
-        else if (k == kTerms) val = (float)((double)rows * (double)a.w);
-        st->partials[threadIdx.x] = val;
-    }
-    if (threadIdx.x != 0) return;
+}
 
-    // the rest runs on one lane with everything in registers (the grid is
-    // smaller than the chip, so the register budget costs no occupancy)
+// exposed 8x32 partials in the reduce.cs layout (one slot per thread);
+// reduce.cs counts every pixel of the group's rows once (:47-62, :101)
+__device__ __forceinline__ void icp_expose(const IcpArgs &a, double (*gsum)[kTerms], float *out)
+{
+    const int g = threadIdx.x >> 5, k = threadIdx.x & 31;
+    const int rows = (a.h > g) ? (a.h - g + 7) / 8 : 0;
+    float val = 0.0f;
+    if (k < kTerms) val = (float)gsum[g][k];
+    else if (k == kTerms) val = (float)((double)rows * (double)a.w);
+    out[threadIdx.x] = val;
+}
+
+// Combine rule, AE (gFusion.cpp:1143-1167) and the 6x6 solve, on one lane with
+// everything in registers (the grid is smaller than the chip, so the register
+// budget costs no occupancy).  Returns |x|.
+__device__ __forceinline__ double icp_solve(const IcpArgs &a, double (*gsum)[kTerms], double *solve_ws,
+                                            double (&x)[6], float &ae)
+{
     double v[kTerms], count = 0.0;
 #pragma unroll
     for (int k = 0; k < kTerms; ++k) v[k] = 0.0;
@@ -570,8 +640,8 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
         for (int k = 0; k < kTerms; ++k) v[k] = gsum[7][k];
         count = (double)((a.h > 7) ? (a.h - 7 + 7) / 8 : 0) * (double)a.w;
     }
-    const float ae = sqrtf((float)v[0] / (float)count);  // gFusion.cpp:1167
-    double b[6], C[36], x[6];
+    ae = sqrtf((float)v[0] / (float)count);  // gFusion.cpp:1167
+    double b[6], C[36];
 #pragma unroll
     for (int k = 0; k < 6; ++k) b[k] = v[1 + k];
     {
@@ -594,6 +664,53 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
 #pragma unroll
         for (int k = 0; k < 6; ++k) x[k] = xl[k];
     }
+    double nrm = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) nrm += x[k] * x[k];
+    return sqrt(nrm);
+}
+
+// One ICP iteration per launch (track.cs + reduce.cs + getReduction + solve +
+// pose update); the last block to finish (ticket) does the FP64 part.
+// blk / nblk: this block's index and the block count of the session's launch
+__device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict__ st, int blk, int nblk)
+{
+    __shared__ float wsum[kIcpThreads / 64][kTerms];
+    __shared__ int last;
+    __shared__ double gsum[8][kTerms];
+    __shared__ double wgsum[kIcpThreads / 64][8][kTerms];
+    __shared__ double solve_ws[kSolveWs + 36 + 6 + 6];
+
+    if (!a.first && st->level_done) return;  // early exit of this level (uniform)
+    float T[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) T[i] = st->pose[i];
+    icp_block_partials(a, T, blk, a.block_partials + blk * kPartStride, wsum);
+    if (threadIdx.x == 0) {
+        // one agent-scope release per block: the barrier above orders the
+        // block's partial stores before it
+        __threadfence();
+        const unsigned t = atomicAdd(&st->ticket, 1u);
+        last = (t == (unsigned)(nblk - 1));
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // acquire: every block's partials are visible
+    icp_group_sums(a.block_partials, nblk, wgsum, gsum);
+    icp_expose(a, gsum, st->partials);
+    if (threadIdx.x >= 64) return;
+    // lane 0 solves; lanes 0-2 then evaluate the three angles' cos / sin in
+    // parallel (the same double routines, so the same bits)
+    const int lane = threadIdx.x;
+    double x[6] = {0, 0, 0, 0, 0, 0}, nrm = 0.0;
+    float ae = 0.0f;
+    if (lane == 0) nrm = icp_solve(a, gsum, solve_ws, x, ae);
+    const double a3 = __shfl(x[3], 0, 64), a4 = __shfl(x[4], 0, 64), a5 = __shfl(x[5], 0, 64);
+    const double my_ang = lane == 1 ? a4 : lane == 2 ? a5 : a3;
+    const double c1 = cos(my_ang), s1 = sin(my_ang);
+    const double cs[3] = {c1, __shfl(c1, 1, 64), __shfl(c1, 2, 64)};
+    const double sn[3] = {s1, __shfl(s1, 1, 64), __shfl(s1, 2, 64)};
+    if (lane != 0) return;
     float P[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
@@ -601,16 +718,11 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
 #pragma unroll
         for (int i = 0; i < 16; ++i) st->old_pose[i] = P[i];
     }
-    pose_update(x, P);
+    pose_update_cs(x, cs, sn, P);
 #pragma unroll
     for (int i = 0; i < 16; ++i) st->pose[i] = P[i];
-    double nrm = 0.0;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        nrm += x[k] * x[k];
-        st->x[k] = x[k];
-    }
-    nrm = sqrt(nrm);
+    for (int k = 0; k < 6; ++k) st->x[k] = x[k];
     st->ae = ae;
     st->level_done = (nrm < 1e-5 && nrm != 0.0) ? 1 : 0;
     st->iters_run = (a.save_pose ? 0 : st->iters_run) + 1;

@@ -457,3 +457,4 @@ def test_stage_timing_matches_graph_frames(seq20):
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     np.testing.assert_array_equal(runs[0][1][0].view(np.uint16), runs[1][1][0].view(np.uint16))
     np.testing.assert_array_equal(runs[0][2], runs[1][2])
+
