@@ -340,38 +340,46 @@ kf::IcpArgs icp_args(kf_session *s, int level, int first, int save_pose)
     return a;
 }
 
-int run_icp(kf_session *s, int level, int first, int save_pose, hipEvent_t t0 = nullptr)
+int run_icp(kf_session *s, int level, int first, int save_pose, hipEvent_t t0 = nullptr, int finish = 0,
+            int frame = 0, hipEvent_t t1 = nullptr)
 {
     kf::IcpArgs a = icp_args(s, level, first, save_pose);
+    a.finish = finish;
+    a.frame = frame;
     if (s->debug) {
         a.dbg_result = s->dbg_result;
         a.dbg_error = s->dbg_error;
         a.dbg_J = s->dbg_J;
         s->dbg_level = level;
     }
-    hipError_t e = kf::launch_icp_iteration(a, s->st, s->stream, t0);
+    hipError_t e = kf::launch_icp_iteration(a, s->st, s->stream, t0, t1);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("icp launch: ") + hipGetErrorString(e));
     return KF_OK;
 }
 
 // Track() loop (gFusion.cpp:661-709): levels coarse -> fine, iterations[level]
 // each; the early exit is a device flag, so every launch is queued up front.
+// The last launch also applies the accept / reject (and the frame gate).
 int run_track(kf_session *s, int frame)
 {
+    int last_level = -1;  // levels run 2 -> 0: the last one with iterations
+    for (int level = 0; level <= 2 && last_level < 0; ++level)
+        if (s->cfg.iterations[level] > 0) last_level = level;
     bool first_any = true;
     for (int level = 2; level >= 0; --level)
         for (int it = 0; it < s->cfg.iterations[level]; ++it) {
-            int rc = run_icp(s, level, it == 0, first_any ? 1 : 0, first_any ? tev(s, 2) : nullptr);
+            const int fin = level == last_level && it == s->cfg.iterations[level] - 1;
+            int rc = run_icp(s, level, it == 0, first_any ? 1 : 0, first_any ? tev(s, 2) : nullptr, fin, frame,
+                             fin ? tev(s, 3) : nullptr);
             if (rc) return rc;
             first_any = false;
         }
     if (first_any) {
         // no iteration configured: AE stays 0 -> rejected, pose unchanged
         hipError_t e = kf::launch_state_set(s->st, nullptr, 0, s->stream);
+        if (e == hipSuccess) e = kf::launch_icp_finish(s->st, frame, s->stream);
         if (e != hipSuccess) return fail(KF_ERR_HIP, hipGetErrorString(e));
     }
-    hipError_t e = kf::launch_icp_finish(s->st, frame, s->stream, tev(s, 3));
-    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("finish launch: ") + hipGetErrorString(e));
     return KF_OK;
 }
 
@@ -940,17 +948,17 @@ struct kf_batch {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     kf::BatchSession *d_sess = nullptr;
-    int *d_prefix = nullptr;  // int, dilate, coarse prefixes: 3 x (kMaxBatch + 1)
+    int *d_prefix = nullptr;  // integrate and coarse-cell block prefixes: 2 x (kMaxBatch + 1)
     int n = 0;
     int storage = 0, fast = 0, off32 = 1;
     int pyr_blocks = 0, ray_blocks = 0, nblk[3] = {0, 0, 0};
-    int total_int = 0, total_bricks = 0, total_coarse = 0;
+    int total_int = 0, total_coarse = 0;
     std::vector<long> frames;
     // Membership changes upload new tables with stream-ordered copies from a
     // ring of pinned staging slots, so set_sessions never blocks the host on
     // queued frames: a slot is reused only after its previous copy completed.
     static constexpr int kSlots = 4;
-    static constexpr size_t kSlotBytes = sizeof(kf::BatchSession) * kf::kMaxBatch + sizeof(int) * 3 * (kf::kMaxBatch + 1);
+    static constexpr size_t kSlotBytes = sizeof(kf::BatchSession) * kf::kMaxBatch + sizeof(int) * 2 * (kf::kMaxBatch + 1);
     unsigned char *h_stage[kSlots] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t slot_ev[kSlots] = {nullptr, nullptr, nullptr, nullptr};
     bool slot_pending[kSlots] = {false, false, false, false};
@@ -1003,8 +1011,8 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
     b->slot_pending[slot] = false;
     kf::BatchSession *h = reinterpret_cast<kf::BatchSession *>(b->h_stage[slot]);
     int *ip = reinterpret_cast<int *>(b->h_stage[slot] + sizeof(kf::BatchSession) * kf::kMaxBatch);
-    int *dp = ip + kf::kMaxBatch + 1, *cp = dp + kf::kMaxBatch + 1;
-    std::memset(ip, 0, sizeof(int) * 3 * (kf::kMaxBatch + 1));
+    int *cp = ip + kf::kMaxBatch + 1;
+    std::memset(ip, 0, sizeof(int) * 2 * (kf::kMaxBatch + 1));
     b->off32 = 1;
     for (int i = 0; i < n; ++i) {
         kf_session *si = sessions[i];
@@ -1017,13 +1025,12 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
         bs.st = si->st;
         kf::integrate_grid_dims(bs.integ, bs.int_grid);
         ip[i + 1] = ip[i] + bs.int_grid[0] * bs.int_grid[1] * bs.int_grid[2];
-        dp[i + 1] = dp[i] + (int)si->nbricks;
         cp[i + 1] = cp[i] + (int)si->ncoarse;
         if (si->voxels >= (size_t(1) << 32)) b->off32 = 0;
     }
     // stream order: launches queued earlier still read the previous tables
     KF_HIP(hipMemcpyAsync(b->d_sess, h, sizeof(kf::BatchSession) * n, hipMemcpyHostToDevice, b->stream));
-    KF_HIP(hipMemcpyAsync(b->d_prefix, ip, sizeof(int) * 3 * (kf::kMaxBatch + 1), hipMemcpyHostToDevice, b->stream));
+    KF_HIP(hipMemcpyAsync(b->d_prefix, ip, sizeof(int) * 2 * (kf::kMaxBatch + 1), hipMemcpyHostToDevice, b->stream));
     KF_HIP(hipEventRecord(b->slot_ev[slot], b->stream));
     b->slot_pending[slot] = true;
     b->next_slot = (slot + 1) % kf_batch::kSlots;
@@ -1037,7 +1044,6 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
     b->ray_blocks = h[0].ray.blocks;
     for (int l = 0; l < 3; ++l) b->nblk[l] = 8 * h[0].icp[l].rows_step;
     b->total_int = ip[n];
-    b->total_bricks = dp[n];
     b->total_coarse = cp[n];
     return KF_OK;
 }
@@ -1065,7 +1071,7 @@ int kf_batch_create(kf_session *const *sessions, int n, void *stream, kf_batch *
         b->own_stream = true;
     }
     if (hipMalloc((void **)&b->d_sess, sizeof(kf::BatchSession) * kf::kMaxBatch) != hipSuccess ||
-        hipMalloc((void **)&b->d_prefix, sizeof(int) * 3 * (kf::kMaxBatch + 1)) != hipSuccess)
+        hipMalloc((void **)&b->d_prefix, sizeof(int) * 2 * (kf::kMaxBatch + 1)) != hipSuccess)
         return cleanup(fail(KF_ERR_OOM, "batch tables"));
     for (int k = 0; k < kf_batch::kSlots; ++k)
         if (hipHostMalloc((void **)&b->h_stage[k], kf_batch::kSlotBytes, hipHostMallocDefault) != hipSuccess ||
@@ -1114,8 +1120,7 @@ int kf_batch_frame_u16_dev(kf_batch *b, const uint16_t *const *depth_mm_dev, con
     t.n = b->n;
     t.sess = b->d_sess;
     t.int_prefix = b->d_prefix;
-    t.dil_prefix = b->d_prefix + kf::kMaxBatch + 1;
-    t.coa_prefix = b->d_prefix + 2 * (kf::kMaxBatch + 1);
+    t.coa_prefix = b->d_prefix + (kf::kMaxBatch + 1);
     int nact = 0;
     for (int i = 0; i < b->n; ++i) {
         const bool on = active ? active[i] != 0 : true;
@@ -1128,15 +1133,18 @@ int kf_batch_frame_u16_dev(kf_batch *b, const uint16_t *const *depth_mm_dev, con
     if (!nact) return KF_OK;
     hipError_t e = kf::launch_batch_pyramid(t, b->pyr_blocks, b->stream);
     const kf_config &c = b->sessions[0]->cfg;
+    int last_level = -1;
+    for (int level = 0; level <= 2 && last_level < 0; ++level)
+        if (c.iterations[level] > 0) last_level = level;
     bool first_any = true;
     for (int level = 2; level >= 0 && e == hipSuccess; --level)
         for (int it = 0; it < c.iterations[level] && e == hipSuccess; ++it) {
-            e = kf::launch_batch_icp(t, level, it == 0, first_any ? 1 : 0, b->nblk[level], b->stream);
+            const int fin = level == last_level && it == c.iterations[level] - 1;
+            e = kf::launch_batch_icp(t, level, it == 0, first_any ? 1 : 0, fin, b->nblk[level], b->stream);
             first_any = false;
         }
-    if (e == hipSuccess) e = kf::launch_batch_finish(t, b->stream);
     if (e == hipSuccess) e = kf::launch_batch_integrate(t, b->storage, b->fast, b->off32, b->total_int, b->stream);
-    if (e == hipSuccess) e = kf::launch_batch_bricks(t, b->total_bricks, b->total_coarse, b->stream);
+    if (e == hipSuccess) e = kf::launch_batch_bricks(t, b->total_coarse, b->stream);
     if (e == hipSuccess) e = kf::launch_batch_raycast(t, b->storage, b->ray_blocks, b->stream);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
     // per-session exact brick rebuild on the same schedule as kf_frame_*

@@ -32,6 +32,7 @@ struct IcpArgs {
     float cam[4];
     float dist_th, normal_th;
     int first, save_pose, rule;
+    int finish, frame;          // last launch of Track(): accept / reject (+ frame gate) at its end
     float *block_partials;      // [8 * rows_step][32] (28 used)
     int rows_step;              // blocks per row group
     int32_t *dbg_result;        // optional per-pixel output
@@ -103,16 +104,16 @@ struct BatchTick {
     unsigned long long active;
     int n;
     const BatchSession *sess;                      // n entries
-    const int *int_prefix, *dil_prefix, *coa_prefix;  // n + 1 entries each
+    const int *int_prefix, *coa_prefix;  // n + 1 entries each
 };
 
 void integrate_grid_dims(const IntegrateArgs &a, int g[3]);
 hipError_t launch_batch_pyramid(const BatchTick &t, int blocks_per_session, hipStream_t s);
-hipError_t launch_batch_icp(const BatchTick &t, int level, int first, int save_pose, int nblk, hipStream_t s);
-hipError_t launch_batch_finish(const BatchTick &t, hipStream_t s);
+hipError_t launch_batch_icp(const BatchTick &t, int level, int first, int save_pose, int finish, int nblk,
+                            hipStream_t s);
 hipError_t launch_batch_integrate(const BatchTick &t, int storage, int fast, int off32, int total_blocks,
                                   hipStream_t s);
-hipError_t launch_batch_bricks(const BatchTick &t, int total_bricks, int total_coarse, hipStream_t s);
+hipError_t launch_batch_bricks(const BatchTick &t, int total_coarse, hipStream_t s);
 hipError_t launch_batch_raycast(const BatchTick &t, int storage, int blocks_per_session, hipStream_t s);
 
 // Optional t0 / t1: events recorded by the kernel's own dispatch packet at its
@@ -121,7 +122,8 @@ hipError_t launch_pyramid_u16(const PyramidArgs &a, const uint16_t *in, hipStrea
                               hipEvent_t t1 = nullptr);
 hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t s, hipEvent_t t0 = nullptr,
                               hipEvent_t t1 = nullptr);
-hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, hipEvent_t t0 = nullptr);
+hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, hipEvent_t t0 = nullptr,
+                                hipEvent_t t1 = nullptr);
 hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s, hipEvent_t t1 = nullptr);
 hipError_t launch_state_set(DevState *st, const float *pose, int arm_reset, hipStream_t s);
 hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s,

@@ -5,10 +5,13 @@
 //                    depthToVert.cs:33-72, vertToNorm.cs:9-57)
 //   k_icp_iteration  fused track + reduce + combine + FP64 solve + pose update
 //                    (gFusion.cpp:669-708, track.cs:47-163, reduce.cs:27-127,
-//                    gFusion.cpp:1127-1168)
-//   k_icp_finish     accept / reject of Track() + frame-loop gate
+//                    gFusion.cpp:1127-1168); the last launch of Track() also
+//                    does the accept / reject and frame-loop gate
 //                    (gFusion.cpp:715-723, kinect.cpp:677-691)
-//   k_integrate      z-column TSDF update   (gFusion.cpp:1234-1287, integrate.cs:52-169)
+//   k_icp_finish     that accept / reject alone (kf_icp_finish)
+//   k_integrate_fast / k_integrate  z-column TSDF update (gFusion.cpp:1234-1287,
+//                    integrate.cs:52-169) + brick occupancy / zero flags
+//   k_brick_dilate   dilated brick flags + 32^3 coarse cells for the raycast
 //   k_raycast        ray march + gradient normals (gFusion.cpp:1289-1366, raycast.cs)
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
@@ -670,6 +673,27 @@ __device__ __forceinline__ double icp_solve(const IcpArgs &a, double (*gsum)[kTe
     return sqrt(nrm);
 }
 
+// Track() accept / reject (gFusion.cpp:715-723); with `frame` the frame-loop
+// gate of kinect.cpp:677-691: integrate if tracked or within the reset window.
+__device__ __forceinline__ void icp_finish_body(DevState *st, int frame)
+{
+    const float ae = st->ae;
+    int tracked;
+    if (ae > 0.5f || ae == 0.0f) {
+        for (int i = 0; i < 16; ++i) st->pose[i] = st->old_pose[i];
+        tracked = 0;
+    } else {
+        tracked = 1;
+    }
+    st->tracked = tracked;
+    if (frame) {
+        const int go = tracked || st->reset;
+        st->integrate_now = go;
+        if (go && st->counter > 2) st->reset = 0;
+        st->counter += 1;
+    }
+}
+
 // One ICP iteration per launch (track.cs + reduce.cs + getReduction + solve +
 // pose update); the last block to finish (ticket) does the FP64 part.
 // blk / nblk: this block's index and the block count of the session's launch
@@ -681,7 +705,10 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
     __shared__ double wgsum[kIcpThreads / 64][8][kTerms];
     __shared__ double solve_ws[kSolveWs + 36 + 6 + 6];
 
-    if (!a.first && st->level_done) return;  // early exit of this level (uniform)
+    if (!a.first && st->level_done) {  // early exit of this level (uniform)
+        if (a.finish && blk == 0 && threadIdx.x == 0) icp_finish_body(st, a.frame);
+        return;
+    }
     float T[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) T[i] = st->pose[i];
@@ -727,32 +754,12 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
     st->level_done = (nrm < 1e-5 && nrm != 0.0) ? 1 : 0;
     st->iters_run = (a.save_pose ? 0 : st->iters_run) + 1;
     st->ticket = 0u;
+    if (a.finish) icp_finish_body(st, a.frame);
 }
 
 __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevState *__restrict__ st)
 {
     icp_body(a, st, blockIdx.x, gridDim.x);
-}
-
-// Track() accept / reject (gFusion.cpp:715-723); with `frame` the frame-loop
-// gate of kinect.cpp:677-691: integrate if tracked or within the reset window.
-__device__ __forceinline__ void icp_finish_body(DevState *st, int frame)
-{
-    const float ae = st->ae;
-    int tracked;
-    if (ae > 0.5f || ae == 0.0f) {
-        for (int i = 0; i < 16; ++i) st->pose[i] = st->old_pose[i];
-        tracked = 0;
-    } else {
-        tracked = 1;
-    }
-    st->tracked = tracked;
-    if (frame) {
-        const int go = tracked || st->reset;
-        st->integrate_now = go;
-        if (go && st->counter > 2) st->reset = 0;
-        st->counter += 1;
-    }
 }
 
 __global__ void k_icp_finish(DevState *st, int frame)
@@ -1231,41 +1238,48 @@ __global__ __launch_bounds__(512) void k_brick_rebuild(BrickArgs a)
     }
 }
 
-// cflags(c) = OR of dflags over the 4^3 bricks of the 32^3 cell c.
-__device__ __forceinline__ void brick_coarse_body(const BrickArgs &a, int i)
+// One thread per 32^3 coarse cell c (4^3 bricks): the dilated flag of each of
+// its bricks b, dflags(b) = OR of flags over b + {0,1}^3 (a trilinear sample
+// whose lower corner lies in brick b reads only voxels of those 8 bricks), and
+// cflags(c) = OR of those 64 dilated flags.  The 5x5x5 brick flags the cell
+// needs are kept as 25 five-bit row masks.
+__device__ __forceinline__ void brick_dilate_body(const BrickArgs &a, int i)
 {
     const int n = a.NC[0] * a.NC[1] * a.NC[2];
     if (i >= n) return;
     const int cx = i % a.NC[0], cy = (i / a.NC[0]) % a.NC[1], cz = i / (a.NC[0] * a.NC[1]);
-    uint8_t any = 0;
-    for (int z = cz * 4; z < min(cz * 4 + 4, a.NB[2]); ++z)
-        for (int y = cy * 4; y < min(cy * 4 + 4, a.NB[1]); ++y)
-            for (int x = cx * 4; x < min(cx * 4 + 4, a.NB[0]); ++x)
-                any |= a.dflags[((size_t)z * a.NB[1] + y) * a.NB[0] + x];
-    a.cflags[i] = any ? 1 : 0;
-}
-
-__global__ __launch_bounds__(256) void k_brick_coarse(BrickArgs a)
-{
-    brick_coarse_body(a, blockIdx.x * 256 + threadIdx.x);
-}
-
-// dflags(b) = OR of flags over b + {0,1}^3: a trilinear sample whose lower
-// corner lies in brick b reads only voxels of those 8 bricks.
-__device__ __forceinline__ void brick_dilate_body(const BrickArgs &a, int i)
-{
-    const int n = a.NB[0] * a.NB[1] * a.NB[2];
-    if (i >= n) return;
-    const int bx = i % a.NB[0], by = (i / a.NB[0]) % a.NB[1], bz = i / (a.NB[0] * a.NB[1]);
-    uint8_t any = 0;
-    for (int dz = 0; dz < 2; ++dz)
-        for (int dy = 0; dy < 2; ++dy)
-            for (int dx = 0; dx < 2; ++dx) {
-                const int X = bx + dx, Y = by + dy, Z = bz + dz;
-                if (X < a.NB[0] && Y < a.NB[1] && Z < a.NB[2])
-                    any |= a.flags[((size_t)Z * a.NB[1] + Y) * a.NB[0] + X];
+    const int bx0 = 4 * cx, by0 = 4 * cy, bz0 = 4 * cz;
+    uint32_t row[5][5];  // bit x: flag of brick (bx0 + x, by0 + y, bz0 + z)
+#pragma unroll
+    for (int z = 0; z < 5; ++z)
+#pragma unroll
+        for (int y = 0; y < 5; ++y) {
+            uint32_t m = 0;
+            const int Y = by0 + y, Z = bz0 + z;
+            if (Y < a.NB[1] && Z < a.NB[2]) {
+                const uint8_t *f = a.flags + ((size_t)Z * a.NB[1] + Y) * a.NB[0];
+#pragma unroll
+                for (int x = 0; x < 5; ++x)
+                    if (bx0 + x < a.NB[0] && f[bx0 + x]) m |= 1u << x;
             }
-    a.dflags[i] = any ? 1 : 0;
+            row[z][y] = m;
+        }
+    uint32_t any = 0;
+#pragma unroll
+    for (int z = 0; z < 4; ++z)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const uint32_t yz = row[z][y] | row[z][y + 1] | row[z + 1][y] | row[z + 1][y + 1];
+            const uint32_t d = (yz | (yz >> 1)) & 0xFu;  // bit x: OR over x, x + 1
+            const int Y = by0 + y, Z = bz0 + z;
+            if (Y >= a.NB[1] || Z >= a.NB[2]) continue;
+            uint8_t *o = a.dflags + ((size_t)Z * a.NB[1] + Y) * a.NB[0];
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+                if (bx0 + x < a.NB[0]) o[bx0 + x] = (uint8_t)((d >> x) & 1u);
+            any |= d;
+        }
+    a.cflags[i] = any ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_brick_dilate(BrickArgs a)
@@ -1653,21 +1667,17 @@ __global__ __launch_bounds__(256) void k_batch_pyramid(BatchTick t, int per)
     pyramid_body<uint16_t>(a, t.depth[i], blockIdx.x % per);
 }
 
-__global__ __launch_bounds__(kIcpThreads) void k_batch_icp(BatchTick t, int level, int first, int save_pose, int nblk)
+__global__ __launch_bounds__(kIcpThreads) void k_batch_icp(BatchTick t, int level, int first, int save_pose,
+                                                           int finish, int nblk)
 {
     const int i = blockIdx.x / nblk;
     if (!batch_active(t, i)) return;
     IcpArgs a = t.sess[i].icp[level];
     a.first = first;
     a.save_pose = save_pose;
+    a.finish = finish;
+    a.frame = 1;
     icp_body(a, t.sess[i].st, blockIdx.x % nblk, nblk);
-}
-
-__global__ void k_batch_finish(BatchTick t)
-{
-    const int i = blockIdx.x;
-    if (threadIdx.x != 0 || !batch_active(t, i)) return;
-    icp_finish_body(t.sess[i].st, 1);
 }
 
 template <class S, bool FAST, bool OFF32>
@@ -1686,21 +1696,11 @@ __global__ __launch_bounds__(256) void k_batch_integrate(BatchTick t)
 __global__ __launch_bounds__(256) void k_batch_dilate(BatchTick t)
 {
     const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= t.dil_prefix[t.n]) return;
-    const int i = batch_find(t.dil_prefix, t.n, b);
-    if (!batch_active(t, i)) return;
-    const BrickArgs a = t.sess[i].brick;
-    brick_dilate_body(a, b - t.dil_prefix[i]);
-}
-
-__global__ __launch_bounds__(256) void k_batch_coarse(BatchTick t)
-{
-    const int b = blockIdx.x * 256 + threadIdx.x;
     if (b >= t.coa_prefix[t.n]) return;
     const int i = batch_find(t.coa_prefix, t.n, b);
     if (!batch_active(t, i)) return;
     const BrickArgs a = t.sess[i].brick;
-    brick_coarse_body(a, b - t.coa_prefix[i]);
+    brick_dilate_body(a, b - t.coa_prefix[i]);
 }
 
 // blocks_per_session is a multiple of 8, so a block's XCD (global id mod 8)
@@ -1736,11 +1736,11 @@ hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t
     launch_timed(k_pyramid<float>, dim3(a.blocks_total), dim3(256), s, t0, t1, a, in);
     return hipGetLastError();
 }
-hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, hipEvent_t t0)
+hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
     // 8 row groups x rows_step blocks; each block takes every rows_step-th row
     // of its group (fewer, fuller blocks: one release + one ticket each)
-    launch_timed(k_icp_iteration, dim3(8 * a.rows_step), dim3(kIcpThreads), s, t0, nullptr, a, st);
+    launch_timed(k_icp_iteration, dim3(8 * a.rows_step), dim3(kIcpThreads), s, t0, t1, a, st);
     return hipGetLastError();
 }
 hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s, hipEvent_t t1)
@@ -1775,15 +1775,11 @@ hipError_t launch_batch_pyramid(const BatchTick &t, int per, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_batch_icp(const BatchTick &t, int level, int first, int save_pose, int nblk, hipStream_t s)
+hipError_t launch_batch_icp(const BatchTick &t, int level, int first, int save_pose, int finish, int nblk,
+                            hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_icp, dim3(t.n * nblk), dim3(kIcpThreads), 0, s, t, level, first, save_pose, nblk);
-    return hipGetLastError();
-}
-
-hipError_t launch_batch_finish(const BatchTick &t, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_batch_finish, dim3(t.n), dim3(64), 0, s, t);
+    hipLaunchKernelGGL(k_batch_icp, dim3(t.n * nblk), dim3(kIcpThreads), 0, s, t, level, first, save_pose, finish,
+                       nblk);
     return hipGetLastError();
 }
 
@@ -1804,10 +1800,9 @@ hipError_t launch_batch_integrate(const BatchTick &t, int storage, int fast, int
     return hipGetLastError();
 }
 
-hipError_t launch_batch_bricks(const BatchTick &t, int total_bricks, int total_coarse, hipStream_t s)
+hipError_t launch_batch_bricks(const BatchTick &t, int total_coarse, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_dilate, dim3((total_bricks + 255) / 256), dim3(256), 0, s, t);
-    hipLaunchKernelGGL(k_batch_coarse, dim3((total_coarse + 255) / 256), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(k_batch_dilate, dim3((total_coarse + 255) / 256), dim3(256), 0, s, t);
     return hipGetLastError();
 }
 
@@ -1856,10 +1851,8 @@ hipError_t launch_brick_rebuild(const BrickArgs &a, int storage, hipStream_t s)
 
 hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s)
 {
-    const int n = a.NB[0] * a.NB[1] * a.NB[2];
-    hipLaunchKernelGGL(k_brick_dilate, dim3((n + 255) / 256), dim3(256), 0, s, a);
     const int nc = a.NC[0] * a.NC[1] * a.NC[2];
-    hipLaunchKernelGGL(k_brick_coarse, dim3((nc + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_brick_dilate, dim3((nc + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
