@@ -107,9 +107,10 @@ int kf_depth_to_vertex_f32_dev(kf_session *s, const float *depth_mm_dev);
  * accept/reject decision (0 < AE <= 0.5); the pose is restored on reject. */
 int kf_track(kf_session *s, int *tracked);
 
-/* Track() as the frame loop runs it: the ICP loop, the accept/reject step,
- * then the frame-loop gate of kinect.cpp:677-691 (integrate if tracked or
- * within the reset window; frame counter) -- all on the device, no sync. */
+/* The tracking step as the frame loop runs it: Track() and / or TrackSDF()
+ * (KF_OPT_TRACK_MODE), then the frame-loop gate of kinect.cpp:677-691
+ * (integrate if tracked or within the reset window; frame counter) -- all on
+ * the device, no sync. */
 int kf_track_frame(kf_session *s);
 
 /* One ICP iteration at `level` (track(level) + reduce(level) + getReduction +
@@ -119,15 +120,49 @@ int kf_icp_iteration(kf_session *s, int level, int first, int save_pose);
 /* The accept/reject step of Track() after the last iteration (gFusion.cpp:715-723). */
 int kf_icp_finish(kf_session *s);
 
+/* TrackSDF() (gFusion.cpp:835-1038, trackSDF.cs, reduceSDF.cs): point-to-
+ * volume Gauss-Newton tracking of the level vertex maps against the TSDF,
+ * coarse to fine, starting from m_pose_eig (set by kf_reset to the reset
+ * pose's translation with identity rotation, and moved only by a successful
+ * TrackSDF).  One persistent launch, no host round trip.  On success the pose
+ * becomes exp(result) * m_pose_eig; otherwise it is unchanged.  If `tracked`
+ * is non-NULL the call synchronises and returns the result. */
+int kf_track_sdf(kf_session *s, int *tracked);
+
+/* recoverPose() (gFusion.cpp:1616-1653): for each pose-library entry in order,
+ * set the pose to it and run TrackSDF, stopping at the first success.  The
+ * library holds the poses of successful kf_track / frame-loop Track() calls
+ * that lie 0.2 m or more from every earlier entry (updatePoseFinder,
+ * gFusion.cpp:1570-1597), at most KF_POSE_LIBRARY_MAX of them; kf_reset
+ * clears it.  If `found` is non-NULL the call synchronises and returns it. */
+int kf_recover_pose(kf_session *s, int *found);
+#define KF_POSE_LIBRARY_MAX 256
+/* The library poses (16 floats each, column-major) into `poses` (room for
+ * `capacity` poses, may be NULL) and their number into `count`. */
+int kf_read_pose_library(kf_session *s, float *poses, int capacity, int *count);
+/* getCumTwist (gFusion.h:260-265): the sum of the twists of successful TrackSDF calls. */
+int kf_get_cum_twist(kf_session *s, float twist[6]);
+/* TrackSDF state (any pointer may be NULL): m_pose_eig, m_cumTwist, the 8x32
+ * partials of the last iteration (reduceSDF.cs layout) and its alignment
+ * energy, iterations run by the last TrackSDF / recoverPose launch, and a log
+ * of 64 rows x 10 doubles (level, iteration, AE, |step|, result[6]).  Returns
+ * KF_ERR_STATE if a launch was aborted because its grid barrier timed out. */
+int kf_read_sdf_state(kf_session *s, float pose_eig[16], float cum_twist[6], float partials[256], float *ae,
+                      int *iterations_run, double *log);
+/* Overwrites m_pose_eig (no counterpart in the reference, which sets it only
+ * in resetPose and TrackSDF; used to load a known tracking state). */
+int kf_write_pose_eig(kf_session *s, const float pose_eig[16]);
+
 /* integrate() (gFusion.cpp:1234-1287).  mode = KF_INTEGRATE_ALWAYS or _GATED. */
 int kf_integrate(kf_session *s, int mode);
 /* raycast() (gFusion.cpp:1289-1366): refreshes the reference vertex/normal maps. */
 int kf_raycast(kf_session *s);
 
-/* The frame block of kinect.cpp:654-695 with point-to-plane tracking and
- * integration on: pyramid, Track, integrate if tracked or within the reset
- * window, raycast.  No host synchronisation; the tracking result stays on the
- * device (kf_read_status). */
+/* The frame block of kinect.cpp:654-695 with integration on: pyramid, Track
+ * and / or TrackSDF (KF_OPT_TRACK_MODE), integrate if tracked or within the
+ * reset window, recoverPose if the tracking failed (KF_OPT_RECOVER), raycast.
+ * No host synchronisation; the tracking result stays on the device
+ * (kf_read_status). */
 int kf_frame_u16_dev(kf_session *s, const uint16_t *depth_mm_dev);
 int kf_frame_f32(kf_session *s, const float *depth_mm);
 
@@ -178,6 +213,16 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         their own dispatch packets; kf_read_stage_times
  *                         returns the last durations.  Default 0. */
 #define KF_OPT_STAGE_TIMING 7
+/*   KF_OPT_TRACK_MODE     tracking of the frame loop (kinect.h:143-144):
+ *                         KF_TRACK_POINT_TO_PLANE (Track, default),
+ *                         KF_TRACK_POINT_TO_VOLUME (TrackSDF) or both (Track
+ *                         then TrackSDF, whose result counts). */
+#define KF_OPT_TRACK_MODE 8
+#define KF_TRACK_POINT_TO_PLANE 1
+#define KF_TRACK_POINT_TO_VOLUME 2
+/*   KF_OPT_RECOVER        1: the frame loop calls recoverPose when its
+ *                         tracking failed (kinect.cpp:686-689).  Default 1. */
+#define KF_OPT_RECOVER 9
 int kf_set_option(kf_session *s, int option, int value);
 int kf_get_option(kf_session *s, int option, int *value);
 /* Durations in ms of the last pyramid, ICP, integrate and raycast stages run
@@ -215,6 +260,8 @@ int kf_raycast_stats(kf_session *s, uint64_t stats[6]);
  * kf_session_destroy waits for the whole device, so a session may be
  * destroyed once it is no longer a member. */
 typedef struct kf_batch kf_batch;
+/* Batched sessions track point-to-plane (KF_OPT_TRACK_MODE = 1) and a batch
+ * frame does not run recoverPose (as KF_OPT_RECOVER = 0). */
 int kf_batch_create(kf_session *const *sessions, int n, void *stream, kf_batch **out); /* n <= 64 */
 int kf_batch_set_sessions(kf_batch *b, kf_session *const *sessions, int n);
 int kf_batch_destroy(kf_batch *b);

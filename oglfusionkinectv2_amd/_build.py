@@ -35,11 +35,11 @@ def _run(cmd):
 
 
 def build_kfusion():
-    srcs = [os.path.join(CSRC, f) for f in ("kf_kernels.hip", "kf_capi.hip", "kf_batch.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("kf_kernels.hip", "kf_capi.hip", "kf_sdf.hip")]
     srcs = [s for s in srcs if os.path.exists(s)]
     out = os.path.join(PKG, "libkfusion.so")
     tmp = out + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-shared"] + srcs + ["-o", tmp])
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-Wl,--no-undefined"] + srcs + ["-o", tmp])
     os.replace(tmp, out)
     return out
 

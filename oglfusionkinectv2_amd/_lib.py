@@ -40,6 +40,13 @@ OPT_BRICK_REBUILD_PERIOD = 4
 OPT_RAYCAST_XCD_GROUP = 5
 OPT_ICP_ROWS_PER_BLOCK = 6
 OPT_STAGE_TIMING = 7
+OPT_TRACK_MODE = 8
+OPT_RECOVER = 9
+
+TRACK_POINT_TO_PLANE = 1
+TRACK_POINT_TO_VOLUME = 2
+
+POSE_LIBRARY_MAX = 256
 
 
 class KfConfig(C.Structure):
@@ -122,6 +129,12 @@ def lib():
         "kf_rebuild_bricks": [sess],
         "kf_read_stage_times": [sess, C.POINTER(C.c_float)],
         "kf_raycast_stats": [sess, C.POINTER(C.c_uint64)],
+        "kf_track_sdf": [sess, C.POINTER(C.c_int)],
+        "kf_recover_pose": [sess, C.POINTER(C.c_int)],
+        "kf_read_pose_library": [sess, f32p, i32, C.POINTER(C.c_int)],
+        "kf_get_cum_twist": [sess, f32p],
+        "kf_write_pose_eig": [sess, f32p],
+        "kf_read_sdf_state": [sess, f32p, f32p, f32p, f32p, C.POINTER(C.c_int), C.POINTER(C.c_double)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

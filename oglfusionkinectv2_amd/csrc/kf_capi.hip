@@ -126,6 +126,13 @@ struct kf_session {
     int32_t *dbg_result = nullptr;
     float *dbg_error = nullptr, *dbg_J = nullptr;
     size_t voxels = 0;
+    // point-to-volume tracking and pose recovery
+    float *library = nullptr;       // pose library [kPoseLibraryMax][16]
+    float *sdf_partials = nullptr;  // 2 x sdf_blocks rows of 32 floats
+    float *sdf_slots = nullptr;     // last-column records, (H0 + H1 + H2) x 8 floats
+    int sdf_blocks = 0;             // persistent grid: at most one block per CU, multiple of 8
+    int track_mode = 1;             // KF_TRACK_POINT_TO_PLANE / _VOLUME / both
+    int recover = 1;                // recoverPose() in the frame loop when tracking fails
 };
 
 namespace {
@@ -143,6 +150,8 @@ size_t session_bytes(const kf_config *c)
     b += vox * 2 * voxel_plane_bytes(c->storage);  // TSDF value + weight planes
     b += sizeof(kf::DevState) + (size_t)(c->height + 8) * 32 * 4 + 64;
     b += (size_t)((c->width + 7) / 8) * ((c->height + 7) / 8) * 8;  // depth tiles
+    b += (size_t)kf::kPoseLibraryMax * 64 + 2 * 256 * 32 * 4;       // pose library, TrackSDF partials
+    b += (size_t)(c->height + (c->height >> 1) + (c->height >> 2)) * 32;  // TrackSDF record slots
     const size_t nb = (size_t)((c->volume_size[0] + 7) / 8) * ((c->volume_size[1] + 7) / 8) *
                       ((c->volume_size[2] + 7) / 8);
     b += 4 * nb;                                   // brick flags, dilated, zero, coarse (bound)
@@ -170,6 +179,7 @@ void free_all(kf_session *s)
     }
     f(s->refV); f(s->refN); f(s->tsdf); f(s->weight); f(s->st); f(s->block_partials); f(s->counts);
     f(s->dbg_result); f(s->dbg_error); f(s->dbg_J);
+    f(s->library); f(s->sdf_partials); f(s->sdf_slots);
     for (int i = 0; i < 8; ++i)
         if (s->tev[i]) (void)hipEventDestroy(s->tev[i]);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
@@ -334,6 +344,8 @@ kf::IcpArgs icp_args(kf_session *s, int level, int first, int save_pose)
     a.save_pose = save_pose;
     a.rule = s->cfg.reduce_rule;
     a.block_partials = s->block_partials;
+    a.library = s->library;
+    a.lib_max = kf::kPoseLibraryMax;
     // blocks per row group: about kIcpRowsPerBlock rows of the group per block
     const int rows_per_group = (a.h + 7) / 8;
     a.rows_step = std::max(1, (rows_per_group + s->icp_rows_per_block - 1) / s->icp_rows_per_block);
@@ -377,10 +389,50 @@ int run_track(kf_session *s, int frame)
     if (first_any) {
         // no iteration configured: AE stays 0 -> rejected, pose unchanged
         hipError_t e = kf::launch_state_set(s->st, nullptr, 0, s->stream);
-        if (e == hipSuccess) e = kf::launch_icp_finish(s->st, frame, s->stream);
+        if (e == hipSuccess) e = kf::launch_icp_finish(s->st, frame, s->library, kf::kPoseLibraryMax, s->stream);
         if (e != hipSuccess) return fail(KF_ERR_HIP, hipGetErrorString(e));
     }
     return KF_OK;
+}
+
+// TrackSDF / recoverPose (gFusion.cpp:835-1038, 1616-1653): one persistent launch
+int run_sdf(kf_session *s, int mode, int frame)
+{
+    kf::SdfArgs a{};
+    int base = 0;
+    for (int l = 0; l < 3; ++l) {
+        a.vert[l] = s->vert[l];
+        a.w[l] = level_w(s, l);
+        a.h[l] = level_h(s, l);
+        a.iters[l] = s->cfg.iterations[l];
+        a.slot_base[l] = base;
+        base += a.h[l];
+    }
+    a.tsdf = s->tsdf;
+    for (int k = 0; k < 3; ++k) {
+        a.N[k] = s->cfg.volume_size[k];
+        a.dim[k] = s->cfg.volume_dim[k];
+    }
+    a.rule = s->cfg.reduce_rule;
+    a.mode = mode;
+    a.frame = frame;
+    a.partials = s->sdf_partials;
+    a.slots = s->sdf_slots;
+    a.library = s->library;
+    a.spin_limit = 1u << 22;
+    hipError_t e = kf::launch_sdf_track(a, s->cfg.storage, s->st, s->sdf_blocks, s->stream);
+    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("TrackSDF launch: ") + hipGetErrorString(e));
+    return KF_OK;
+}
+
+// The tracking step of the frame block (kinect.cpp:666-674): Track() and / or
+// TrackSDF(); the last one run applies the frame gate.
+int run_tracking(kf_session *s, int frame)
+{
+    int rc = KF_OK;
+    if (s->track_mode & KF_TRACK_POINT_TO_PLANE) rc = run_track(s, frame && !(s->track_mode & KF_TRACK_POINT_TO_VOLUME));
+    if (!rc && (s->track_mode & KF_TRACK_POINT_TO_VOLUME)) rc = run_sdf(s, kf::kSdfTrack, frame);
+    return rc;
 }
 
 int run_integrate(kf_session *s, int gated)
@@ -400,8 +452,11 @@ int run_raycast(kf_session *s);
 // session); brick flags are rebuilt exactly every rebuild_period frames.
 int run_frame_direct(kf_session *s)
 {
-    int rc = run_track(s, 1);
+    int rc = run_tracking(s, 1);
     if (!rc) rc = run_integrate(s, 1);
+    // if (!tracked) recoverPose() (kinect.cpp:686-689): a no-op launch unless
+    // the tracking failed with a non-empty pose library
+    if (!rc && s->recover) rc = run_sdf(s, kf::kSdfRecoverIfLost, 0);
     if (!rc) rc = run_raycast(s);
     return rc;
 }
@@ -557,7 +612,10 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
               alloc((void **)&s->refN, p0 * 16) && alloc(&s->tsdf, vb) && alloc(&s->weight, vb) &&
               alloc((void **)&s->st, sizeof(kf::DevState)) &&
               alloc((void **)&s->block_partials, (size_t)(cfg->height + 8) * 32 * 4) &&
-              alloc((void **)&s->counts, 8 * sizeof(unsigned long long));
+              alloc((void **)&s->counts, 8 * sizeof(unsigned long long)) &&
+              alloc((void **)&s->library, (size_t)kf::kPoseLibraryMax * 64) &&
+              alloc((void **)&s->sdf_partials, 2 * 256 * 32 * 4) &&
+              alloc((void **)&s->sdf_slots, (size_t)(cfg->height + (cfg->height >> 1) + (cfg->height >> 2)) * 32);
     for (int l = 0; l < 3 && ok; ++l) {
         const size_t n = (size_t)(cfg->width >> l) * (cfg->height >> l);
         ok = alloc((void **)&s->vert[l], n * 16) && alloc((void **)&s->norm[l], n * 16);
@@ -576,14 +634,24 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
              hipMemsetAsync(s->dtiles, 0, (size_t)((cfg->width + 7) / 8) * ((cfg->height + 7) / 8) * 8,
                             s->stream) == hipSuccess &&
              hipMemsetAsync(s->cflags, 0, s->ncoarse, s->stream) == hipSuccess &&
-             hipMemsetAsync(s->st, 0, sizeof(kf::DevState), s->stream) == hipSuccess;
+             hipMemsetAsync(s->st, 0, sizeof(kf::DevState), s->stream) == hipSuccess &&
+             hipMemsetAsync(s->library, 0, (size_t)kf::kPoseLibraryMax * 64, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->sdf_slots, 0, (size_t)(cfg->height + (cfg->height >> 1) + (cfg->height >> 2)) * 32,
+                            s->stream) == hipSuccess;
     for (int l = 0; l < 3 && z; ++l) {
         const size_t n = (size_t)(cfg->width >> l) * (cfg->height >> l);
         z = hipMemsetAsync(s->vert[l], 0, n * 16, s->stream) == hipSuccess &&
             hipMemsetAsync(s->norm[l], 0, n * 16, s->stream) == hipSuccess;
     }
+    {
+        // persistent TrackSDF grid: every block must be resident at once
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || cus < 8)
+            return cleanup(KF_ERR_HIP, "cannot query the compute unit count");
+        s->sdf_blocks = (cus < 256 ? cus : 256) & ~7;
+    }
     const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-    if (!z || kf::launch_state_set(s->st, I, 0, s->stream) != hipSuccess ||
+    if (!z || kf::launch_state_set(s->st, I, kf::kStateResetPose, s->stream) != hipSuccess ||
         hipStreamSynchronize(s->stream) != hipSuccess)
         return cleanup(KF_ERR_HIP, "session initialisation failed");
     *out = s;
@@ -632,7 +700,7 @@ int kf_reset(kf_session *s, const float pose[16])
     if (!s || !pose) return fail(KF_ERR_INVALID, "NULL argument");
     int rc = kf_reset_volume(s);
     if (rc) return rc;
-    KF_HIP(kf::launch_state_set(s->st, pose, 1, s->stream));
+    KF_HIP(kf::launch_state_set(s->st, pose, kf::kStateArmWindow | kf::kStateResetPose, s->stream));
     return run_raycast(s);
 }
 
@@ -682,7 +750,7 @@ int kf_icp_iteration(kf_session *s, int level, int first, int save_pose)
 int kf_icp_finish(kf_session *s)
 {
     if (!s) return fail(KF_ERR_INVALID, "session is NULL");
-    KF_HIP(kf::launch_icp_finish(s->st, 0, s->stream));
+    KF_HIP(kf::launch_icp_finish(s->st, 0, s->library, kf::kPoseLibraryMax, s->stream));
     return KF_OK;
 }
 
@@ -701,7 +769,78 @@ int kf_track(kf_session *s, int *tracked)
 int kf_track_frame(kf_session *s)
 {
     if (!s) return fail(KF_ERR_INVALID, "session is NULL");
-    return run_track(s, 1);
+    return run_tracking(s, 1);
+}
+
+int kf_track_sdf(kf_session *s, int *tracked)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    int rc = run_sdf(s, kf::kSdfTrack, 0);
+    if (rc || !tracked) return rc;
+    int v[2];
+    KF_HIP(hipMemcpyAsync(v, &s->st->sdf_tracked, 2 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    *tracked = v[0];
+    return KF_OK;
+}
+
+int kf_recover_pose(kf_session *s, int *found)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    int rc = run_sdf(s, kf::kSdfRecover, 0);
+    if (rc || !found) return rc;
+    KF_HIP(hipMemcpyAsync(found, &s->st->sdf_tracked, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+int kf_read_pose_library(kf_session *s, float *poses, int capacity, int *count)
+{
+    if (!s || !count) return fail(KF_ERR_INVALID, "NULL argument");
+    int n = 0;
+    KF_HIP(hipMemcpyAsync(&n, &s->st->lib_n, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    *count = n;
+    if (poses && capacity > 0) {
+        const int m = n < capacity ? n : capacity;
+        if (m > 0) {
+            KF_HIP(hipMemcpyAsync(poses, s->library, (size_t)m * 64, hipMemcpyDeviceToHost, s->stream));
+            KF_HIP(hipStreamSynchronize(s->stream));
+        }
+    }
+    return KF_OK;
+}
+
+int kf_read_sdf_state(kf_session *s, float pose_eig[16], float cum_twist[6], float partials[256], float *ae,
+                      int *iterations_run, double *log)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    kf::DevState h;
+    KF_HIP(hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    if (h.sdf_error) return fail(KF_ERR_STATE, "a TrackSDF grid barrier timed out");
+    if (pose_eig) std::memcpy(pose_eig, h.pose_eig, 64);
+    if (cum_twist)
+        for (int k = 0; k < 6; ++k) cum_twist[k] = (float)h.cum_twist[k];
+    if (partials) std::memcpy(partials, h.sdf_partials, 256 * 4);
+    if (ae) *ae = h.sdf_ae;
+    if (iterations_run) *iterations_run = h.sdf_iters;
+    if (log) std::memcpy(log, h.sdf_log, sizeof(h.sdf_log));
+    return KF_OK;
+}
+
+int kf_write_pose_eig(kf_session *s, const float pose_eig[16])
+{
+    if (!s || !pose_eig) return fail(KF_ERR_INVALID, "NULL argument");
+    KF_HIP(hipMemcpyAsync(s->st->pose_eig, pose_eig, 64, hipMemcpyHostToDevice, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+int kf_get_cum_twist(kf_session *s, float twist[6])
+{
+    if (!twist) return fail(KF_ERR_INVALID, "NULL argument");
+    return kf_read_sdf_state(s, nullptr, twist, nullptr, nullptr, nullptr, nullptr);
 }
 
 int kf_integrate(kf_session *s, int mode)
@@ -755,6 +894,11 @@ int kf_set_option(kf_session *s, int option, int value)
         if (value < 1 || value > 1024) return fail(KF_ERR_INVALID, "rows per block must be in [1, 1024]");
         s->icp_rows_per_block = value;
         break;
+    case KF_OPT_TRACK_MODE:
+        if (value < 1 || value > 3) return fail(KF_ERR_INVALID, "track mode must be 1, 2 or 3");
+        s->track_mode = value;
+        break;
+    case KF_OPT_RECOVER: s->recover = value ? 1 : 0; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     drop_graph(s);  // captured launches carry the old arguments
@@ -772,6 +916,8 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_BRICK_REBUILD_PERIOD: *value = s->rebuild_period; break;
     case KF_OPT_RAYCAST_XCD_GROUP: *value = s->xcd_group; break;
     case KF_OPT_ICP_ROWS_PER_BLOCK: *value = s->icp_rows_per_block; break;
+    case KF_OPT_TRACK_MODE: *value = s->track_mode; break;
+    case KF_OPT_RECOVER: *value = s->recover; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     return KF_OK;
@@ -997,6 +1143,8 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
             si->xcd_group != s0->xcd_group || si->device != s0->device ||
             std::memcmp(si->cfg.iterations, s0->cfg.iterations, sizeof(s0->cfg.iterations)) != 0)
             return fail(KF_ERR_INVALID, "batched sessions must share depth size, ICP schedule, storage and options");
+        if (si->track_mode != KF_TRACK_POINT_TO_PLANE)
+            return fail(KF_ERR_INVALID, "batched sessions track point-to-plane only (KF_OPT_TRACK_MODE = 1)");
         for (int j = 0; j < i; ++j)
             if (sessions[j] == si) return fail(KF_ERR_INVALID, "a session appears twice in the batch");
     }

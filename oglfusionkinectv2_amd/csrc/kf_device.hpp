@@ -31,6 +31,20 @@ struct DevState {
     int reset;            // reset window flag (kinect.cpp:472, 680-681)
     int iters_run;        // ICP iterations executed in the last Track()
     unsigned int ticket;  // last-block ticket of the fused track+reduce kernel
+    // point-to-volume tracking (gFusion.h:426, :445) and the pose library (:424)
+    float pose_eig[16];       // m_pose_eig, column-major
+    double cum_twist[6];      // m_cumTwist
+    float sdf_partials[256];  // 8x32 partial sums of the last TrackSDF iteration (reduceSDF.cs layout)
+    float sdf_ae;             // its alignment energy
+    int sdf_tracked;          // result of the last TrackSDF() / recoverPose()
+    int sdf_iters;            // TrackSDF iterations executed by the last launch
+    int lib_n;                // pose library entries
+    int lib_overflow;         // a pose was not added because the library was full
+    int sdf_error;            // a grid barrier of the persistent TrackSDF kernel timed out
+    unsigned int sdf_bar;     // grid barrier arrivals of the persistent TrackSDF kernel
+    unsigned int sdf_exit;    // its exit ticket (the last block clears the barrier)
+    int sdf_abort;            // set by a block whose barrier wait timed out
+    double sdf_log[64][10];   // per iteration: level, iteration, AE, |step|, result[6]
 };
 
 struct Mat16 {

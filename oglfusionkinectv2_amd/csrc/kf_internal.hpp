@@ -34,6 +34,8 @@ struct IcpArgs {
     int first, save_pose, rule;
     int finish, frame;          // last launch of Track(): accept / reject (+ frame gate) at its end
     float *block_partials;      // [8 * rows_step][32] (28 used)
+    float *library;             // pose library [lib_max][16] (updatePoseFinder on success)
+    int lib_max;
     int rows_step;              // blocks per row group
     int32_t *dbg_result;        // optional per-pixel output
     float *dbg_error, *dbg_J;
@@ -82,6 +84,31 @@ struct RaycastArgs {
     int xcd_group;              // tiles per XCD cluster (block -> tile mapping)
 };
 
+// Point-to-volume tracking (persistent k_sdf_track, one launch per TrackSDF /
+// recoverPose).
+constexpr int kSdfTrack = 0;          // TrackSDF() from m_pose_eig
+constexpr int kSdfRecover = 1;        // recoverPose(): TrackSDF from each library entry
+constexpr int kSdfRecoverIfLost = 2;  // recoverPose() only if the frame's tracking failed
+constexpr int kPoseLibraryMax = 256;  // pose library capacity (entries)
+
+struct SdfArgs {
+    const float4 *vert[3];  // level vertex maps (camera frame)
+    int w[3], h[3];         // level sizes
+    int iters[3];           // iterations per level, 0 = finest
+    const void *tsdf;       // value plane
+    int N[3];
+    float dim[3];
+    int rule;               // combine rule of getSDFReduction
+    int mode;               // kSdf*
+    int frame;              // kSdfTrack inside the frame loop: apply the frame gate
+    float *partials;        // 2 x nblk rows of 32 floats (double-buffered per iteration)
+    float *slots;           // records left at the last-column buffer indices: [H0 + H1 + H2][8]
+    int slot_base[3];       // first slot of each level
+    const float *library;   // pose library [kPoseLibraryMax][16]
+    unsigned spin_limit;    // grid-barrier wait bound (polls)
+};
+hipError_t launch_sdf_track(const SdfArgs &a, int storage, DevState *st, int nblk, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // Multi-session batches: one launch per kernel type for all active sessions.
 // ---------------------------------------------------------------------------
@@ -124,8 +151,11 @@ hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t
                               hipEvent_t t1 = nullptr);
 hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, hipEvent_t t0 = nullptr,
                                 hipEvent_t t1 = nullptr);
-hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s, hipEvent_t t1 = nullptr);
-hipError_t launch_state_set(DevState *st, const float *pose, int arm_reset, hipStream_t s);
+hipError_t launch_icp_finish(DevState *st, int frame, float *lib, int lib_max, hipStream_t s,
+                             hipEvent_t t1 = nullptr);
+// state flags of launch_state_set (kStateSetPose is implied by a non-NULL pose)
+constexpr int kStateSetPose = 1, kStateArmWindow = 2, kStateResetPose = 4;
+hipError_t launch_state_set(DevState *st, const float *pose, int flags, hipStream_t s);
 hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s,
                             hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 hipError_t launch_brick_rebuild(const BrickArgs &a, int storage, hipStream_t s);

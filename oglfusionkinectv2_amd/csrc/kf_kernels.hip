@@ -24,56 +24,9 @@
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
 #include "kf_march.hpp"
+#include "kf_store.hpp"
 
 namespace kf {
-
-// ---------------------------------------------------------------------------
-// TSDF storage traits.  The volume is two planes (value, weight), x fastest.
-// ---------------------------------------------------------------------------
-struct StoreRG16I {  // integrate.cs:37-50 (RG16I texture)
-    using V = int16_t;
-    static constexpr bool kScaled = true;
-    __device__ static float value(V v) { return (float)v * 0.00003051944088f; }
-    __device__ static float weight(V v) { return (float)v; }
-    __device__ static float raw(V v) { return (float)v; }
-    __device__ static V from_value(float t) { return (V)(int)(t * 32766.0f); }
-    __device__ static V from_weight(float w) { return (V)(int)w; }
-    // value that can end a ray (< 0) or reach the 0.8 large-step threshold
-    __device__ static bool interesting(V v) { return v < 0 || v >= (V)25886; }  // 0.79 * 32766
-    __device__ static bool zero_bits(V v) { return v == 0; }
-    __device__ static uint32_t bits(V v) { return (uint32_t)(uint16_t)v; }
-    __device__ static V from_bits(uint32_t b) { return (V)(uint16_t)b; }
-};
-struct StoreF32 {
-    using V = float;
-    static constexpr bool kScaled = false;
-    __device__ static float value(V v) { return v; }
-    __device__ static float weight(V v) { return v; }
-    __device__ static float raw(V v) { return v; }
-    __device__ static V from_value(float t) { return t; }
-    __device__ static V from_weight(float w) { return w; }
-    __device__ static bool interesting(V v) { return v < 0.0f || v >= 0.79f; }
-    __device__ static bool zero_bits(V v) { return __float_as_uint(v) == 0u; }
-    __device__ static uint32_t bits(V v) { return __float_as_uint(v); }
-    __device__ static V from_bits(uint32_t b) { return __uint_as_float(b); }
-};
-// binary16 values held as their bit patterns: 16-bit loads then land in a
-// register of their own (no packing of two halves into one register, which
-// ties independent loads together)
-struct StoreF16 {
-    using V = uint16_t;
-    static constexpr bool kScaled = false;
-    __device__ static float half(V v) { return (float)__builtin_bit_cast(_Float16, v); }
-    __device__ static float value(V v) { return half(v); }
-    __device__ static float weight(V v) { return half(v); }
-    __device__ static float raw(V v) { return half(v); }
-    __device__ static V from_value(float t) { return __builtin_bit_cast(uint16_t, (_Float16)t); }
-    __device__ static V from_weight(float w) { return __builtin_bit_cast(uint16_t, (_Float16)w); }
-    __device__ static bool interesting(V v) { return half(v) < 0.0f || half(v) >= 0.79f; }
-    __device__ static bool zero_bits(V v) { return v == 0; }
-    __device__ static uint32_t bits(V v) { return (uint32_t)v; }
-    __device__ static V from_bits(uint32_t b) { return (V)b; }
-};
 
 // ---------------------------------------------------------------------------
 // Pyramid
@@ -675,6 +628,7 @@ __device__ __forceinline__ double icp_solve(const IcpArgs &a, double (*gsum)[kTe
 
 // Track() accept / reject (gFusion.cpp:715-723); with `frame` the frame-loop
 // gate of kinect.cpp:677-691: integrate if tracked or within the reset window.
+// One lane.
 __device__ __forceinline__ void icp_finish_body(DevState *st, int frame)
 {
     const float ae = st->ae;
@@ -694,6 +648,48 @@ __device__ __forceinline__ void icp_finish_body(DevState *st, int frame)
     }
 }
 
+// The accept / reject above on lane 0, then, when tracked, updatePoseFinder
+// (gFusion.cpp:722, 1570-1597) on the whole wavefront: the pose joins the
+// library unless its translation lies within 0.2 m (glm::l2Norm) of an
+// entry's.  All 64 lanes of one wavefront call this.
+__device__ __forceinline__ void icp_finish_wave(DevState *st, int frame, float *lib, int lib_max)
+{
+    const int lane = threadIdx.x & 63;
+    int tracked = 0, n = 0;
+    float P[16];
+    if (lane == 0) {
+        icp_finish_body(st, frame);
+        tracked = st->tracked;
+        n = st->lib_n;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
+    }
+    tracked = __shfl(tracked, 0, 64);
+    if (!tracked || !lib) return;
+    n = __shfl(n, 0, 64);
+    // lane i < 16 keeps pose element i
+    float mine = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const float v = __shfl(lane == 0 ? P[i] : 0.0f, 0, 64);
+        if (lane == i) mine = v;
+    }
+    const float px = __shfl(mine, 12, 64), py = __shfl(mine, 13, 64), pz = __shfl(mine, 14, 64);
+    bool near = false;
+    for (int i = lane; i < n; i += 64) {
+        const float *e = lib + 16 * i;
+        const float d0 = px - e[12], d1 = py - e[13], d2 = pz - e[14];
+        near = near || (sqrtf(dot3(d0, d1, d2, d0, d1, d2)) < 0.2f);
+    }
+    if (__any(near)) return;
+    if (n < lib_max) {
+        if (lane < 16) lib[16 * n + lane] = mine;
+        if (lane == 0) st->lib_n = n + 1;
+    } else if (lane == 0) {
+        st->lib_overflow = 1;
+    }
+}
+
 // One ICP iteration per launch (track.cs + reduce.cs + getReduction + solve +
 // pose update); the last block to finish (ticket) does the FP64 part.
 // blk / nblk: this block's index and the block count of the session's launch
@@ -706,7 +702,7 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
     __shared__ double solve_ws[kSolveWs + 36 + 6 + 6];
 
     if (!a.first && st->level_done) {  // early exit of this level (uniform)
-        if (a.finish && blk == 0 && threadIdx.x == 0) icp_finish_body(st, a.frame);
+        if (a.finish && blk == 0 && threadIdx.x < 64) icp_finish_wave(st, a.frame, a.library, a.lib_max);
         return;
     }
     float T[16];
@@ -737,24 +733,25 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
     const double c1 = cos(my_ang), s1 = sin(my_ang);
     const double cs[3] = {c1, __shfl(c1, 1, 64), __shfl(c1, 2, 64)};
     const double sn[3] = {s1, __shfl(s1, 1, 64), __shfl(s1, 2, 64)};
-    if (lane != 0) return;
-    float P[16];
+    if (lane == 0) {
+        float P[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
-    if (a.save_pose) {
+        for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
+        if (a.save_pose) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) st->old_pose[i] = P[i];
+            for (int i = 0; i < 16; ++i) st->old_pose[i] = P[i];
+        }
+        pose_update_cs(x, cs, sn, P);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) st->pose[i] = P[i];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st->x[k] = x[k];
+        st->ae = ae;
+        st->level_done = (nrm < 1e-5 && nrm != 0.0) ? 1 : 0;
+        st->iters_run = (a.save_pose ? 0 : st->iters_run) + 1;
+        st->ticket = 0u;
     }
-    pose_update_cs(x, cs, sn, P);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) st->pose[i] = P[i];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) st->x[k] = x[k];
-    st->ae = ae;
-    st->level_done = (nrm < 1e-5 && nrm != 0.0) ? 1 : 0;
-    st->iters_run = (a.save_pose ? 0 : st->iters_run) + 1;
-    st->ticket = 0u;
-    if (a.finish) icp_finish_body(st, a.frame);
+    if (a.finish) icp_finish_wave(st, a.frame, a.library, a.lib_max);
 }
 
 __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevState *__restrict__ st)
@@ -762,21 +759,35 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevSta
     icp_body(a, st, blockIdx.x, gridDim.x);
 }
 
-__global__ void k_icp_finish(DevState *st, int frame)
+__global__ void k_icp_finish(DevState *st, int frame, float *lib, int lib_max)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    icp_finish_body(st, frame);
+    if (threadIdx.x >= 64 || blockIdx.x != 0) return;
+    icp_finish_wave(st, frame, lib, lib_max);
 }
 
-__global__ void k_state_set(DevState *st, Mat16 pose, int set_pose, int arm_reset)
+// flags: kStateSetPose, kStateArmWindow (the 4-frame integrate window of
+// kinect.cpp:146, 677-682), kStateResetPose (resetPose, gFusion.cpp:275-286:
+// m_pose_eig = identity rotation with the pose's translation, m_cumTwist = 0,
+// and the pose library cleared as Reset does at :238)
+__global__ void k_state_set(DevState *st, Mat16 pose, int flags)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (set_pose)
+    if (flags & kStateSetPose)
         for (int i = 0; i < 16; ++i) st->pose[i] = pose.m[i];
-    if (arm_reset) {
+    if (flags & kStateArmWindow) {
         st->counter = 0;
         st->reset = 1;
         st->integrate_now = 0;
+    }
+    if (flags & kStateResetPose) {
+        for (int i = 0; i < 16; ++i) st->pose_eig[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+        st->pose_eig[12] = pose.m[12];
+        st->pose_eig[13] = pose.m[13];
+        st->pose_eig[14] = pose.m[14];
+        for (int k = 0; k < 6; ++k) st->cum_twist[k] = 0.0;
+        st->lib_n = 0;
+        st->lib_overflow = 0;
+        st->sdf_tracked = 0;
     }
     st->ticket = 0u;
     st->level_done = 0;
@@ -1743,16 +1754,18 @@ hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, h
     launch_timed(k_icp_iteration, dim3(8 * a.rows_step), dim3(kIcpThreads), s, t0, t1, a, st);
     return hipGetLastError();
 }
-hipError_t launch_icp_finish(DevState *st, int frame, hipStream_t s, hipEvent_t t1)
+hipError_t launch_icp_finish(DevState *st, int frame, float *lib, int lib_max, hipStream_t s, hipEvent_t t1)
 {
-    launch_timed(k_icp_finish, dim3(1), dim3(64), s, nullptr, t1, st, frame);
+    launch_timed(k_icp_finish, dim3(1), dim3(64), s, nullptr, t1, st, frame, lib, lib_max);
     return hipGetLastError();
 }
-hipError_t launch_state_set(DevState *st, const float *pose, int arm_reset, hipStream_t s)
+hipError_t launch_state_set(DevState *st, const float *pose, int flags, hipStream_t s)
 {
     Mat16 m;
     for (int i = 0; i < 16; ++i) m.m[i] = pose ? pose[i] : 0.0f;
-    hipLaunchKernelGGL(k_state_set, dim3(1), dim3(64), 0, s, st, m, pose ? 1 : 0, arm_reset);
+    if (pose) flags |= kStateSetPose;
+    else flags &= ~(kStateSetPose | kStateResetPose);
+    hipLaunchKernelGGL(k_state_set, dim3(1), dim3(64), 0, s, st, m, flags);
     return hipGetLastError();
 }
 

@@ -207,6 +207,52 @@ class GFusion:
         K.check(K.lib().kf_track(self.session, C.byref(t)))
         return bool(t.value)
 
+    def TrackSDF(self):
+        """gFusion.cpp:835-1038: point-to-volume tracking from m_pose_eig."""
+        t = C.c_int(0)
+        K.check(K.lib().kf_track_sdf(self.session, C.byref(t)))
+        return bool(t.value)
+
+    def recoverPose(self):
+        """gFusion.cpp:1616-1653: TrackSDF from each pose-library entry."""
+        t = C.c_int(0)
+        K.check(K.lib().kf_recover_pose(self.session, C.byref(t)))
+        return bool(t.value)
+
+    def getPoseLibrary(self):
+        """The poses of poseLibrary (gFusion.h:424) as (n, 4, 4) row-major."""
+        n = C.c_int()
+        buf = np.zeros((K.POSE_LIBRARY_MAX, 16), dtype=np.float32)
+        K.check(K.lib().kf_read_pose_library(self.session, buf.ctypes.data_as(C.POINTER(C.c_float)),
+                                             K.POSE_LIBRARY_MAX, C.byref(n)))
+        return np.array([b.reshape(4, 4).T for b in buf[:n.value]]).reshape(n.value, 4, 4)
+
+    def getCumTwist(self):
+        """gFusion.h:260-265"""
+        out = np.zeros(6, dtype=np.float32)
+        K.check(K.lib().kf_get_cum_twist(self.session, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def setPoseEig(self, pose):
+        """Loads m_pose_eig (a known tracking state; the reference sets it only
+        in resetPose and TrackSDF)."""
+        p, ptr = _pose_arg(pose)
+        K.check(K.lib().kf_write_pose_eig(self.session, ptr))
+
+    def sdfState(self):
+        """m_pose_eig (row-major), cumulative twist, last TrackSDF partials /
+        AE / iteration count and the per-iteration log rows."""
+        pe = np.zeros(16, dtype=np.float32)
+        tw = np.zeros(6, dtype=np.float32)
+        parts = np.zeros(256, dtype=np.float32)
+        ae, iters = C.c_float(), C.c_int()
+        log = np.zeros((64, 10), dtype=np.float64)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        K.check(K.lib().kf_read_sdf_state(self.session, fp(pe), fp(tw), fp(parts), C.byref(ae), C.byref(iters),
+                                          log.ctypes.data_as(C.POINTER(C.c_double))))
+        return dict(pose_eig=pe.reshape(4, 4).T.copy(), cum_twist=tw, partials=parts.reshape(8, 32),
+                    ae=ae.value, iterations=iters.value, log=log[:min(iters.value, 64)].copy())
+
     def track(self, layer, first=True, save_pose=False):
         """One ICP step at `layer` (track + reduce + solve + update fused)."""
         K.check(K.lib().kf_icp_iteration(self.session, int(layer), int(first), int(save_pose)))

@@ -76,6 +76,17 @@ class Oracle:
             "kfo_session_track": ([vp], i),
             "kfo_session_integrate": ([vp], None),
             "kfo_session_frame": ([vp, vp], i),
+            "kfo_sdf_sample": ([vp, i, vp, R, R, R], R),
+            "kfo_track_sdf_pixel": ([vp, i, vp, vp, vp, vp, vp], None),
+            "kfo_track_sdf": ([vp, i, vp, vp, vp, i, i, vp, vp], None),
+            "kfo_reduce_sdf": ([vp, i, i, vp], None),
+            "kfo_ldlt_solve6": ([vp, vp, vp], None),
+            "kfo_twist_exp": ([vp, vp], None),
+            "kfo_sdf_step": ([vp, vp, i, vp, vp], C.c_double),
+            "kfo_session_track_sdf": ([vp], i),
+            "kfo_session_recover_pose": ([vp], i),
+            "kfo_session_set_mode": ([vp, i, i], None),
+            "kfo_session_lib_size": ([vp], i),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -214,6 +225,54 @@ class Oracle:
                            large_step, w, h, self.p(rv), self.p(rn))
         return rv, rn
 
+    # -- point-to-volume tracking --------------------------------------------
+    @staticmethod
+    def _N(vol):
+        return np.array([vol.shape[2], vol.shape[1], vol.shape[0]], np.int32)
+
+    def sdf_sample(self, vol, mode, loc):
+        v, N = np.ascontiguousarray(vol), self._N(vol)
+        return self.L.kfo_sdf_sample(self.p(v), mode, self.p(N), float(loc[0]), float(loc[1]), float(loc[2]))
+
+    def track_sdf_pixel(self, vol, mode, dim, T, v):
+        vv, N, dd, P, pt = np.ascontiguousarray(vol), self._N(vol), self.arr(dim), self.cm(T), self.arr(v[:3])
+        rec = np.zeros(8, self.dt)
+        self.L.kfo_track_sdf_pixel(self.p(vv), mode, self.p(N), self.p(dd), self.p(P), self.p(pt), self.p(rec))
+        return rec
+
+    def track_sdf(self, vol, mode, dim, vert, T, buf=None):
+        """trackSDF over one level: returns the (h*w, 8) record buffer (records
+        of the last column are left as they were in `buf`, zero if None)."""
+        vv, N, dd, P, vt = np.ascontiguousarray(vol), self._N(vol), self.arr(dim), self.cm(T), self.arr(vert)
+        h, w = vt.shape[:2]
+        out = np.zeros((h * w, 8), self.dt) if buf is None else self.arr(buf).reshape(h * w, 8).copy()
+        self.L.kfo_track_sdf(self.p(vv), mode, self.p(N), self.p(dd), self.p(vt), w, h, self.p(P), self.p(out))
+        return out
+
+    def reduce_sdf(self, buf, w, h):
+        b = self.arr(buf).reshape(-1)
+        out = np.zeros(256, self.dt)
+        self.L.kfo_reduce_sdf(self.p(b), w, h, self.p(out))
+        return out.reshape(8, 32)
+
+    def ldlt_solve6(self, A, b):
+        x = np.zeros(6, np.float64)
+        a, bb = self.arr(A, np.float64).reshape(36), self.arr(b, np.float64)
+        self.L.kfo_ldlt_solve6(self.p(a), self.p(bb), self.p(x))
+        return x
+
+    def twist_exp(self, xi):
+        out, x = np.zeros(16, self.dt), self.arr(xi, np.float64)
+        self.L.kfo_twist_exp(self.p(x), self.p(out))
+        return self.rm(out)
+
+    def sdf_step(self, b, Cm, iteration, result, result_prev):
+        """One TrackSDF Gauss-Newton step; returns (cnorm, result, result_prev)."""
+        bb, cc = self.arr(b), self.arr(Cm).reshape(36)
+        r, rp = self.arr(result, np.float64).copy(), self.arr(result_prev, np.float64).copy()
+        cn = self.L.kfo_sdf_step(self.p(bb), self.p(cc), int(iteration), self.p(r), self.p(rp))
+        return cn, r, rp
+
 
 def _config_struct(R):
     class Cfg(C.Structure):
@@ -292,6 +351,43 @@ class OracleSession:
     def frame(self, depth_mm):
         d = np.ascontiguousarray(depth_mm, dtype=np.float32)
         return self.o.L.kfo_session_frame(self.s, d.ctypes.data)
+
+    def set_mode(self, track_mode=1, recover=1):
+        """track_mode: 1 = Track (point-to-plane), 2 = TrackSDF, 3 = both;
+        recover: call recoverPose() on a frame whose tracking failed."""
+        self.o.L.kfo_session_set_mode(self.s, int(track_mode), int(recover))
+
+    def track_sdf(self):
+        return self.o.L.kfo_session_track_sdf(self.s)
+
+    def recover_pose(self):
+        return self.o.L.kfo_session_recover_pose(self.s)
+
+    def library(self):
+        n = self.o.L.kfo_session_lib_size(self.s)
+        lib = self._buf(14, 0, (256, 16), self.o.dt)[:n]
+        return np.array([self.o.rm(p) for p in lib]).reshape(n, 4, 4)
+
+    def pose_eig(self):
+        return self.o.rm(self._buf(15, 0, (16,), self.o.dt).copy())
+
+    def set_pose_eig(self, pose):
+        self._buf(15, 0, (16,), self.o.dt)[...] = self.o.cm(pose)
+
+    def cum_twist(self):
+        return self._buf(16, 0, (6,), np.float64).copy()
+
+    def sdf_buffer(self):
+        return self._buf(11, 0, (self.h * self.w, 8), self.o.dt).copy()
+
+    def set_sdf_buffer(self, buf):
+        self._buf(11, 0, (self.h * self.w, 8), self.o.dt)[...] = buf
+
+    def sdf_partials(self):
+        return self._buf(12, 0, (8, 32), self.o.dt).copy()
+
+    def sdf_log(self):
+        return self._buf(13, 0, (64, 10), np.float64).copy()
 
     def stats(self):
         ae = np.zeros(1, self.o.dt)

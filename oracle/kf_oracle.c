@@ -32,11 +32,13 @@ typedef double R;
 #define KC(x) x
 #define KSQRT sqrt
 #define KFLOOR floor
+#define KMODF modf
 #else
 typedef float R;
 #define KC(x) x##f
 #define KSQRT sqrtf
 #define KFLOOR floorf
+#define KMODF modff
 #endif
 
 #define KFO_EXPORT __attribute__((visibility("default")))
@@ -818,8 +820,264 @@ KFO_EXPORT void kfo_raycast(const void *vol, int mode, const int N[3], const R d
 }
 
 /* ------------------------------------------------------------------------- */
+/* Point-to-volume tracking: gFusion.cpp:835-1038 (TrackSDF), :1040-1084      */
+/* (trackSDF), :1086-1112 (reduceSDF), :1195-1232 (getSDFReduction),          */
+/* trackSDF.cs:186-220, 222-235, 296-306, 442-530, reduceSDF.cs:28-116.       */
+/* ------------------------------------------------------------------------- */
+
+/* imageLoad(volumeData, p).x * 0.00003051944088f (trackSDF.cs:195-205): the
+ * stored value in truncation units, zero out of bounds */
+static inline R sdf_voxel(const void *vol, int mode, const int N[3], int x, int y, int z)
+{
+    return vol_raw(vol, mode, N, x, y, z) * raw_scale(mode);
+}
+
+/* float(J) + 1.0 converted back to int (ivec3(I, J + 1.0, K)) */
+static inline int ip1(int v) { return v == INT32_MIN ? v : v + 1; }
+
+/* SDF(location), trackSDF.cs:186-220: modf splits each coordinate into an
+ * integer part (toward zero) and a signed fraction; corners (I..I+1, J..J+1,
+ * K..K+1) are blended z first, then y, then x. */
+KFO_EXPORT R kfo_sdf_sample(const void *vol, int mode, const int N[3], R lx, R ly, R lz)
+{
+    R i, j, k;
+    R x = KMODF(lx, &i), y = KMODF(ly, &j), z = KMODF(lz, &k);
+    int I = f2i(i), J = f2i(j), K = f2i(k);
+    R N0 = sdf_voxel(vol, mode, N, I, J, K);
+    R N1 = sdf_voxel(vol, mode, N, I, ip1(J), K);
+    R N2 = sdf_voxel(vol, mode, N, ip1(I), J, K);
+    R N3 = sdf_voxel(vol, mode, N, ip1(I), ip1(J), K);
+    R N4 = sdf_voxel(vol, mode, N, I, J, ip1(K));
+    R N5 = sdf_voxel(vol, mode, N, I, ip1(J), ip1(K));
+    R N6 = sdf_voxel(vol, mode, N, ip1(I), J, ip1(K));
+    R N7 = sdf_voxel(vol, mode, N, ip1(I), ip1(J), ip1(K));
+    R a0 = N0 * (KC(1.0) - z) + N4 * z;
+    R a1 = N1 * (KC(1.0) - z) + N5 * z;
+    R b0 = N2 * (KC(1.0) - z) + N6 * z;
+    R b1 = N3 * (KC(1.0) - z) + N7 * z;
+    return (a0 * (KC(1.0) - y) + a1 * y) * (KC(1.0) - x) + (b0 * (KC(1.0) - y) + b1 * y) * x;
+}
+
+/* trackSDF.cs:460-530 for one pixel: rec = {h, D, J[6]} */
+KFO_EXPORT void kfo_track_sdf_pixel(const void *vol, int mode, const int N[3], const R dim[3], const R T[16],
+                                    const R v[3], R rec[8])
+{
+    R cp[4] = {v[0], v[1], v[2], KC(1.0)};
+    R pv[4];
+    matvec4(T, cp, pv); /* projectedVertex = Ttrack * vec4(cameraPoint.xyz, 1) */
+    /* cvp (:474): x and y lose 0.5 twice (inside and outside the vec3), z once */
+    R c0 = ((pv[0] * (R)N[0] / dim[0]) - KC(0.5)) - KC(0.5);
+    R c1 = ((pv[1] * (R)N[1] / dim[1]) - KC(0.5)) - KC(0.5);
+    R c2 = (pv[2] * (R)N[2] / dim[2]) - KC(0.5);
+    R D = kfo_sdf_sample(vol, mode, N, c0, c1, c2);
+    /* SDFGradient (:223-235): central difference over +-1 voxel index, divided
+     * by 2 * (volDim.x / volSize.x) for every axis */
+    R den = KC(2.0) * (dim[0] / (R)N[0]);
+    R gx = (kfo_sdf_sample(vol, mode, N, c0 + KC(1.0), c1 + KC(0.0), c2 + KC(0.0)) -
+            kfo_sdf_sample(vol, mode, N, c0 - KC(1.0), c1 - KC(0.0), c2 - KC(0.0))) / den;
+    R gy = (kfo_sdf_sample(vol, mode, N, c0 + KC(0.0), c1 + KC(1.0), c2 + KC(0.0)) -
+            kfo_sdf_sample(vol, mode, N, c0 - KC(0.0), c1 - KC(1.0), c2 - KC(0.0))) / den;
+    R gz = (kfo_sdf_sample(vol, mode, N, c0 + KC(0.0), c1 + KC(0.0), c2 + KC(1.0)) -
+            kfo_sdf_sample(vol, mode, N, c0 - KC(0.0), c1 - KC(0.0), c2 - KC(1.0))) / den;
+    R absD = D < 0 ? -D : D;
+    for (int k = 0; k < 8; ++k) rec[k] = 0;
+    if (D < KC(1.0) && D > KC(-1.0)) {
+        /* getJ(dSDF_dx, dx_dxi) (:296-306) with the dx_dxi of :492-497 */
+        R px = pv[0], py = pv[1], pz = pv[2];
+        rec[0] = absD < KC(0.02) ? KC(1.0) : KC(0.02) / absD; /* huber (:522) */
+        rec[1] = D;
+        rec[2] = (gx * KC(0.0) + gy * (-pz)) + gz * py;
+        rec[3] = (gx * pz + gy * KC(0.0)) + gz * (-px);
+        rec[4] = (gx * (-py) + gy * px) + gz * KC(0.0);
+        rec[5] = (gx * KC(1.0) + gy * KC(0.0)) + gz * KC(0.0);
+        rec[6] = (gx * KC(0.0) + gy * KC(1.0)) + gz * KC(0.0);
+        rec[7] = (gx * KC(0.0) + gy * KC(0.0)) + gz * KC(1.0);
+    }
+}
+
+/* trackSDF.cs main over a (w x h) level: writes the 8-value record of every
+ * pixel with x < w - 1 (the shader's bounds test, :455) into buf[y*w + x];
+ * records of the last column keep whatever an earlier dispatch left there. */
+KFO_EXPORT void kfo_track_sdf(const void *vol, int mode, const int N[3], const R dim[3], const R *vert, int w,
+                              int h, const R T[16], R *buf)
+{
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w - 1; ++x) {
+            const R *v = vert + 4 * (y * w + x);
+            kfo_track_sdf_pixel(vol, mode, N, dim, T, v, buf + 8 * (y * w + x));
+        }
+}
+
+/* reduceSDF.cs:28-116, dispatched as 8 WGs x 112 threads (gFusion.cpp:1099):
+ * serial FP32 sums per thread in row order, D^2 added twice (:52, :60), the
+ * count [28] over every pixel of the rows, then the 112 rows added serially. */
+KFO_EXPORT void kfo_reduce_sdf(const R *buf, int w, int h, R out[256])
+{
+    R (*S)[112][32] = (R (*)[112][32])malloc(sizeof(R) * 8 * 112 * 32);
+#pragma omp parallel for schedule(static)
+    for (int gs = 0; gs < 8 * 112; ++gs) {
+        int g = gs / 112, s = gs % 112;
+        R sums[32];
+        for (int k = 0; k < 32; ++k) sums[k] = 0;
+        for (int y = g; y < h; y += 8)
+            for (int x = s; x < w; x += 112) {
+                const R *r = buf + 8 * (y * w + x);
+                R hh = r[0], D = r[1];
+                const R *J = r + 2;
+                sums[0] += D * D;
+                for (int i = 0; i < 6; ++i) sums[i + 1] += hh * J[i] * D;
+                sums[0] += D * D;
+                int q = 7;
+                for (int a = 0; a < 6; ++a)
+                    for (int b = a; b < 6; ++b) sums[q++] += hh * J[a] * J[b];
+                sums[28] += KC(1.0);
+            }
+        memcpy(S[g][s], sums, sizeof(sums));
+    }
+    for (int g = 0; g < 8; ++g)
+        for (int k = 0; k < 32; ++k) {
+            R acc = S[g][0][k];
+            for (int s = 1; s < 112; ++s) acc += S[g][s][k];
+            out[g * 32 + k] = acc;
+        }
+    free(S);
+}
+
+/* Eigen::LDLT<MatrixXd>(A).solve(b) (gFusion.cpp:918): in-place LDL^T of the
+ * lower triangle with symmetric pivoting on the largest remaining |diagonal|,
+ * then P^T L^-T D^+ L^-1 P b where D^+ zeroes pivots with |d| <= DBL_MIN
+ * (Eigen's LDLT::_solve_impl).  Inner products are summed in index order. */
+KFO_EXPORT void kfo_ldlt_solve6(const double Ain[36], const double b[6], double x[6])
+{
+    double m[6][6];
+    int tr[6];
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) m[i][j] = Ain[i * 6 + j];
+    for (int k = 0; k < 6; ++k) {
+        int big = k;
+        double bv = fabs(m[k][k]);
+        for (int i = k + 1; i < 6; ++i)
+            if (fabs(m[i][i]) > bv) { bv = fabs(m[i][i]); big = i; }
+        tr[k] = big;
+        if (big != k) {
+            for (int j = 0; j < k; ++j) { double t = m[k][j]; m[k][j] = m[big][j]; m[big][j] = t; }
+            for (int i = big + 1; i < 6; ++i) { double t = m[i][k]; m[i][k] = m[i][big]; m[i][big] = t; }
+            { double t = m[k][k]; m[k][k] = m[big][big]; m[big][big] = t; }
+            for (int i = k + 1; i < big; ++i) { double t = m[i][k]; m[i][k] = m[big][i]; m[big][i] = t; }
+        }
+        double temp[6];
+        if (k > 0) {
+            for (int j = 0; j < k; ++j) temp[j] = m[j][j] * m[k][j];
+            double s = 0.0;
+            for (int j = 0; j < k; ++j) s += m[k][j] * temp[j];
+            m[k][k] -= s;
+            for (int i = k + 1; i < 6; ++i) {
+                double t = 0.0;
+                for (int j = 0; j < k; ++j) t += m[i][j] * temp[j];
+                m[i][k] -= t;
+            }
+        }
+        double akk = m[k][k];
+        int valid = fabs(akk) > 0.0;
+        if (k == 0 && !valid) {
+            /* the whole diagonal is zero (or NaN): Eigen stops here with the
+             * identity permutation and the matrix left as it is; the solve
+             * below still runs on it */
+            for (int j = 0; j < 6; ++j) tr[j] = j;
+            break;
+        }
+        if (valid)
+            for (int i = k + 1; i < 6; ++i) m[i][k] /= akk;
+    }
+    double y[6];
+    for (int i = 0; i < 6; ++i) y[i] = b[i];
+    for (int k = 0; k < 6; ++k) { double t = y[k]; y[k] = y[tr[k]]; y[tr[k]] = t; }
+    for (int i = 0; i < 6; ++i) { /* unit lower solve */
+        double s = y[i];
+        for (int j = 0; j < i; ++j) s -= m[i][j] * y[j];
+        y[i] = s;
+    }
+    for (int i = 0; i < 6; ++i) y[i] = (fabs(m[i][i]) > DBL_MIN) ? y[i] / m[i][i] : 0.0;
+    for (int i = 5; i >= 0; --i) { /* unit upper (L^T) solve */
+        double s = y[i];
+        for (int j = i + 1; j < 6; ++j) s -= m[j][i] * y[j];
+        y[i] = s;
+    }
+    for (int k = 5; k >= 0; --k) { double t = y[k]; y[k] = y[tr[k]]; y[tr[k]] = t; }
+    for (int i = 0; i < 6; ++i) x[i] = y[i];
+}
+
+/* Twist(xi).exp() (gFusion.cpp:810-833, :866): the 4x4 twist matrix holds xi
+ * as R ([w]x = rows (0,-w2,w1), (w2,0,-w0), (-w1,w0,0), translation xi3..5);
+ * its matrix exponential in closed form (Rodrigues rotation, V = I + B[w]x +
+ * C[w]x^2), evaluated in double and rounded to R.  Column-major out. */
+KFO_EXPORT void kfo_twist_exp(const double xi[6], R out[16])
+{
+    double w0 = (double)(R)xi[0], w1 = (double)(R)xi[1], w2 = (double)(R)xi[2];
+    double v[3] = {(double)(R)xi[3], (double)(R)xi[4], (double)(R)xi[5]};
+    double th2 = w0 * w0 + w1 * w1 + w2 * w2;
+    double A, B, Cc;
+    if (th2 < 1e-8) {
+        A = 1.0 - th2 / 6.0 + th2 * th2 / 120.0;
+        B = 0.5 - th2 / 24.0 + th2 * th2 / 720.0;
+        Cc = 1.0 / 6.0 - th2 / 120.0 + th2 * th2 / 5040.0;
+    } else {
+        double th = sqrt(th2), s = sin(th), c = cos(th);
+        A = s / th;
+        B = (1.0 - c) / th2;
+        Cc = (th - s) / (th2 * th);
+    }
+    double W[3][3] = {{0.0, -w2, w1}, {w2, 0.0, -w0}, {-w1, w0, 0.0}};
+    double W2[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) W2[r][c] = W[r][0] * W[0][c] + W[r][1] * W[1][c] + W[r][2] * W[2][c];
+    double Rm[3][3], V[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double I = (r == c) ? 1.0 : 0.0;
+            Rm[r][c] = I + A * W[r][c] + B * W2[r][c];
+            V[r][c] = I + B * W[r][c] + Cc * W2[r][c];
+        }
+    for (int c = 0; c < 3; ++c) {
+        for (int r = 0; r < 3; ++r) out[c * 4 + r] = (R)Rm[r][c];
+        out[c * 4 + 3] = 0;
+    }
+    for (int r = 0; r < 3; ++r) out[12 + r] = (R)(V[r][0] * v[0] + V[r][1] * v[1] + V[r][2] * v[2]);
+    out[15] = KC(1.0);
+}
+
+/* One Gauss-Newton step of TrackSDF (gFusion.cpp:878-931): scale C and b by
+ * 1 / max(C), damp with 0.01 * iteration on the diagonal, result -= LDLT
+ * solve; returns |result - result_prev| and updates both. */
+KFO_EXPORT double kfo_sdf_step(const R b[6], const R C[36], int iteration, double result[6], double result_prev[6])
+{
+    double dC[36], db[6], dx[6];
+    double mx = (double)C[0];
+    for (int k = 0; k < 36; ++k) {
+        dC[k] = (double)C[k];
+        if (dC[k] > mx) mx = dC[k];
+    }
+    double scaling = 1.0 / mx;
+    for (int k = 0; k < 36; ++k) dC[k] *= scaling;
+    for (int k = 0; k < 6; ++k) db[k] = (double)b[k] * scaling;
+    for (int k = 0; k < 6; ++k) dC[k * 6 + k] += (double)iteration * 0.01;
+    kfo_ldlt_solve6(dC, db, dx);
+    double cn = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        result[k] = result[k] - dx[k];
+        double d = result[k] - result_prev[k];
+        cn += d * d;
+    }
+    for (int k = 0; k < 6; ++k) result_prev[k] = result[k];
+    return sqrt(cn);
+}
+
+/* ------------------------------------------------------------------------- */
 /* Session: gFusion host state + the frame block of kinect.cpp:654-695        */
 /* ------------------------------------------------------------------------- */
+#define KFO_LIB_MAX 256
+
 typedef struct {
     int vol[3];
     R dim[3];
@@ -847,6 +1105,18 @@ typedef struct {
     /* per-iteration log of the last Track(): level, AE, x[6] */
     int log_n;
     double log[64][8];
+    /* point-to-volume tracking state (gFusion.h:426, :445) */
+    R pose_eig[16];      /* m_pose_eig, column-major */
+    double cum_twist[6]; /* m_cumTwist */
+    R *sdf_buf;          /* m_bufferSDFReduction: 8 values per level-0 pixel */
+    R sdf_partials[256];
+    int sdf_log_n;       /* last TrackSDF(): level, iteration, AE, Cnorm, result[6] */
+    double sdf_log[64][10];
+    /* pose library (gFusion.h:424): poses only, the depth copies are never read */
+    int lib_n;
+    R lib[KFO_LIB_MAX][16];
+    int track_mode;      /* 1 = point-to-plane (Track), 2 = point-to-volume (TrackSDF), 3 = both */
+    int recover;         /* call recoverPose() when tracking fails (kinect.cpp:686-689) */
 } kfo_session;
 
 KFO_EXPORT int kfo_config_size(void) { return (int)sizeof(kfo_config); }
@@ -876,7 +1146,11 @@ KFO_EXPORT kfo_session *kfo_session_create(const kfo_config *cfg)
     s->error = (R *)calloc((size_t)w * h, sizeof(R));
     s->J = (R *)calloc(6 * (size_t)w * h, sizeof(R));
     s->vol = calloc((size_t)cfg->vol[0] * cfg->vol[1] * cfg->vol[2], (size_t)kfo_voxel_bytes(cfg->mode));
-    for (int i = 0; i < 16; ++i) s->pose[i] = (i % 5 == 0) ? KC(1.0) : KC(0.0);
+    /* glBufferData(..., NULL) at gFusion.cpp:349: contents taken as zero */
+    s->sdf_buf = (R *)calloc(8 * (size_t)w * h, sizeof(R));
+    for (int i = 0; i < 16; ++i) s->pose[i] = s->pose_eig[i] = (i % 5 == 0) ? KC(1.0) : KC(0.0);
+    s->track_mode = 1;
+    s->recover = 1;
     return s;
 }
 
@@ -887,6 +1161,7 @@ KFO_EXPORT void kfo_session_destroy(kfo_session *s)
         free(s->depth[l]); free(s->vert[l]); free(s->norm[l]);
     }
     free(s->refV); free(s->refN); free(s->result); free(s->error); free(s->J); free(s->vol);
+    free(s->sdf_buf);
     free(s);
 }
 
@@ -907,6 +1182,12 @@ KFO_EXPORT void *kfo_session_buffer(kfo_session *s, int which, int level)
     case 8: return s->J;
     case 9: return s->partials;
     case 10: return s->log;
+    case 11: return s->sdf_buf;
+    case 12: return s->sdf_partials;
+    case 13: return s->sdf_log;
+    case 14: return s->lib;
+    case 15: return s->pose_eig;
+    case 16: return s->cum_twist;
     }
     return NULL;
 }
@@ -931,10 +1212,27 @@ KFO_EXPORT void kfo_session_raycast(kfo_session *s)
 KFO_EXPORT void kfo_session_reset(kfo_session *s, const R pose[16])
 {
     kfo_reset_volume(s->vol, s->cfg.mode, s->cfg.vol);
+    s->lib_n = 0; /* poseLibrary.resize(0) */
+    /* resetPose (gFusion.cpp:275-286): m_pose_eig = identity rotation with the
+     * pose's translation; m_cumTwist = 0 */
     memcpy(s->pose, pose, 16 * sizeof(R));
+    for (int i = 0; i < 16; ++i) s->pose_eig[i] = (i % 5 == 0) ? KC(1.0) : KC(0.0);
+    s->pose_eig[12] = pose[12]; s->pose_eig[13] = pose[13]; s->pose_eig[14] = pose[14];
+    for (int k = 0; k < 6; ++k) s->cum_twist[k] = 0.0;
     kfo_session_raycast(s);
     s->counter = 0;
     s->reset = 1;
+}
+
+/* updatePoseFinder (gFusion.cpp:1570-1597): add the pose to the library when
+ * its translation is at least 0.2 m (glm::l2Norm) from every entry's. */
+static void update_pose_finder(kfo_session *s)
+{
+    for (int i = 0; i < s->lib_n; ++i) {
+        R d0 = s->pose[12] - s->lib[i][12], d1 = s->pose[13] - s->lib[i][13], d2 = s->pose[14] - s->lib[i][14];
+        if (KSQRT(dot3(d0, d1, d2, d0, d1, d2)) < KC(0.2)) return;
+    }
+    if (s->lib_n < KFO_LIB_MAX) memcpy(s->lib[s->lib_n++], s->pose, 16 * sizeof(R));
 }
 
 /* depthToVertex(float*) + vertexToNormal(): gFusion.cpp:496-547 */
@@ -996,11 +1294,89 @@ KFO_EXPORT int kfo_session_track(kfo_session *s)
         tracked = 0;
     } else {
         tracked = 1;
+        update_pose_finder(s);
     }
     s->last_ae = ae;
     s->last_tracked = tracked;
     return tracked;
 }
+
+/* TrackSDF(): gFusion.cpp:835-1038.  Starts from m_pose_eig (not m_pose);
+ * per level (coarse to fine) and iteration: camToWorld = exp(Twist(result)) *
+ * m_pose_eig_prev, trackSDF + reduceSDF + getSDFReduction, scaled and damped
+ * LDLT step; a converged iteration ((AE != 0 && AE < 1e-3) || |step| < 1e-3)
+ * marks tracked and ends only its level.  On success m_pose = m_pose_eig =
+ * exp(result) * m_pose_eig_prev and m_cumTwist += result; otherwise both are
+ * restored. */
+KFO_EXPORT int kfo_session_track_sdf(kfo_session *s)
+{
+    const kfo_config *c = &s->cfg;
+    R oldPose[16], prev[16];
+    memcpy(oldPose, s->pose, sizeof(oldPose));
+    memcpy(prev, s->pose_eig, sizeof(prev));
+    double result[6] = {0, 0, 0, 0, 0, 0}, result_prev[6] = {0, 0, 0, 0, 0, 0};
+    int tracked = 0;
+    s->sdf_log_n = 0;
+    memset(s->sdf_log, 0, sizeof(s->sdf_log));
+    for (int level = 2; level >= 0; --level) {
+        int w = c->width >> level, h = c->height >> level;
+        for (int it = 0; it < c->iterations[level]; ++it) {
+            R E[16], cam[16];
+            kfo_twist_exp(result, E);
+            kfo_mul4(E, prev, cam);
+            kfo_track_sdf(s->vol, c->mode, c->vol, c->dim, s->vert[level], w, h, cam, s->sdf_buf);
+            kfo_reduce_sdf(s->sdf_buf, w, h, s->sdf_partials);
+            R b[6], C[36], ae;
+            kfo_get_reduction(s->sdf_partials, c->rule, b, C, &ae);
+            double cn = kfo_sdf_step(b, C, it, result, result_prev);
+            if (s->sdf_log_n < 64) {
+                double *L = s->sdf_log[s->sdf_log_n++];
+                L[0] = level; L[1] = it; L[2] = (double)ae; L[3] = cn;
+                for (int k = 0; k < 6; ++k) L[4 + k] = result[k];
+            }
+            if (((double)ae != 0.0 && (double)ae < 1e-3) || cn < 1e-3) {
+                tracked = 1;
+                break;
+            }
+        }
+    }
+    double sum = 0.0;
+    for (int k = 0; k < 6; ++k) sum += result[k];
+    if (sum != sum)
+        for (int k = 0; k < 6; ++k) result[k] = 0.0;
+    if (tracked) {
+        R E[16];
+        kfo_twist_exp(result, E);
+        kfo_mul4(E, prev, s->pose_eig);
+        memcpy(s->pose, s->pose_eig, sizeof(s->pose));
+        s->pose[3] = s->pose[7] = s->pose[11] = 0;
+        s->pose[15] = KC(1.0);
+        for (int k = 0; k < 6; ++k) s->cum_twist[k] += result[k];
+    } else {
+        memcpy(s->pose_eig, prev, sizeof(prev));
+        memcpy(s->pose, oldPose, sizeof(oldPose));
+    }
+    return tracked;
+}
+
+/* recoverPose(): gFusion.cpp:1616-1653 -- for each library entry in order,
+ * m_pose = entry, then TrackSDF(); stop at the first success. */
+KFO_EXPORT int kfo_session_recover_pose(kfo_session *s)
+{
+    for (int i = 0; i < s->lib_n; ++i) {
+        memcpy(s->pose, s->lib[i], sizeof(s->pose));
+        if (kfo_session_track_sdf(s)) return 1;
+    }
+    return 0;
+}
+
+KFO_EXPORT void kfo_session_set_mode(kfo_session *s, int track_mode, int recover)
+{
+    s->track_mode = track_mode;
+    s->recover = recover;
+}
+
+KFO_EXPORT int kfo_session_lib_size(const kfo_session *s) { return s->lib_n; }
 
 /* integrate(): gFusion.cpp:1234-1287 */
 KFO_EXPORT void kfo_session_integrate(kfo_session *s)
@@ -1010,17 +1386,22 @@ KFO_EXPORT void kfo_session_integrate(kfo_session *s)
                   c->mu, c->maxWeight);
 }
 
-/* The frame block of kinect.cpp:654-695 with P2P tracking and integration on:
- * depth -> pyramid, Track, integrate if tracked or within the reset window
- * (frames 0-3 after a reset), raycast.  Returns tracked. */
+/* The frame block of kinect.cpp:654-695 with integration on: depth ->
+ * pyramid, Track and / or TrackSDF (track_mode; P2P alone by default,
+ * kinect.h:143-144), integrate if tracked or within the reset window (frames
+ * 0-3 after a reset), recoverPose if not tracked, raycast.  Returns tracked. */
 KFO_EXPORT int kfo_session_frame(kfo_session *s, const float *depth_mm)
 {
     kfo_session_depth(s, depth_mm);
-    int tracked = kfo_session_track(s);
+    int tracked = 0;
+    if (s->track_mode & 1) tracked = kfo_session_track(s);
+    if (s->track_mode & 2) tracked = kfo_session_track_sdf(s);
+    s->last_tracked = tracked;
     if (tracked || s->reset) {
         kfo_session_integrate(s);
         if (s->counter > 2) s->reset = 0;
     }
+    if (!tracked && s->recover) kfo_session_recover_pose(s);
     s->counter++;
     kfo_session_raycast(s);
     return tracked;
