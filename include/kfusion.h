@@ -247,6 +247,35 @@ int kf_count_integrate_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *v
  * samples one ray evaluated, and the largest evaluated + cells of one ray. */
 int kf_raycast_stats(kf_session *s, uint64_t stats[6]);
 
+/* ---- surface export (MCubes, mcubes.cpp / histoPyramids.cs) ----------------
+ * Marching cubes over the session's TSDF as "save stl" runs it
+ * (kinect.cpp:487-504: gridSize = volumeSize, iso level 0): cubes within 10
+ * voxels of the volume's border and cubes with no value > 0 among their first
+ * seven corners produce nothing (histoPyramids.cs:110-152); triangles are
+ * numbered in the order of the reference's histopyramid traversal (Morton
+ * order of the cubes, x then z then y fastest per level) and their vertices
+ * are (x, y, z, 0) in voxel-index units, interpolated along the cube edges
+ * (traverseHistoPyramids.cs:190-250).  The triangle table is built from the
+ * cube topology (ambiguous faces separate the below-iso corners), which gives
+ * every case the reference table's triangle count and surface polygon. */
+/* Classifies the volume and writes every triangle into a device buffer the
+ * session owns; returns the triangle count (synchronises). */
+int kf_mesh_generate(kf_session *s, float iso, uint64_t *triangles);
+/* Copies min(triangles, max_triangles) triangles, 12 floats each. */
+int kf_mesh_read(kf_session *s, float *vertices, uint64_t max_triangles);
+/* exportMesh (mcubes.cpp:429-509): binary STL of the last kf_mesh_generate,
+ * with the reference's layout (zero 80-byte header, a count field of four
+ * times the triangle count, normals (1, 1, 1), zero attribute words). */
+int kf_mesh_export_stl(kf_session *s, const char *path);
+/* exportPointCloud (mcubes.cpp:390-427): ASCII PLY of every vertex. */
+int kf_mesh_export_ply(kf_session *s, const char *path);
+/* The triangle table: tri[c * 16 + 3 k + v] edge of vertex v of triangle k of
+ * cube case c (255 after the last), ntri[c] triangles of case c.  Edge e joins
+ * corners (e & 3, (e + 1) & 3) on the bottom face (e < 4), the same on the top
+ * face (4 <= e < 8, corners + 4), corner e - 8 to e - 4 otherwise; corner k is
+ * at (x, y, z) = {(0,0,0),(1,0,0),(1,0,1),(0,0,1),(0,1,0),(1,1,0),(1,1,1),(0,1,1)}[k]. */
+int kf_mc_tables(uint8_t *tri, uint8_t *ntri);
+
 /* ---- multi-session batches -------------------------------------------------
  * No counterpart in the reference (one session per process there).  A batch
  * runs the frame block of many sessions with one launch per kernel type:

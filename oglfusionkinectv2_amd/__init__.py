@@ -3,5 +3,7 @@ KinectFusion loop: HIP kernels behind the C ABI of include/kfusion.h, with a
 host mirror of the reference's gFusion class."""
 from . import _lib
 from .gfusion import GFusion, GFusionConfig, kinect_config, translate, KINECT_V2_CAM
+from .mcubes import MCubes, MCubeConfig
 
-__all__ = ["GFusion", "GFusionConfig", "kinect_config", "translate", "KINECT_V2_CAM", "_lib"]
+__all__ = ["GFusion", "GFusionConfig", "kinect_config", "translate", "KINECT_V2_CAM", "MCubes", "MCubeConfig",
+           "_lib"]

@@ -134,12 +134,18 @@ def lib():
         "kf_read_pose_library": [sess, f32p, i32, C.POINTER(C.c_int)],
         "kf_get_cum_twist": [sess, f32p],
         "kf_write_pose_eig": [sess, f32p],
+        "kf_mesh_generate": [sess, C.c_float, C.POINTER(C.c_uint64)],
+        "kf_mesh_read": [sess, f32p, C.c_uint64],
+        "kf_mesh_export_stl": [sess, C.c_char_p],
+        "kf_mesh_export_ply": [sess, C.c_char_p],
         "kf_read_sdf_state": [sess, f32p, f32p, f32p, f32p, C.POINTER(C.c_int), C.POINTER(C.c_double)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    L.kf_mc_tables.argtypes = [vp, vp]
+    L.kf_mc_tables.restype = C.c_int
     L.kf_last_error.argtypes = []
     L.kf_last_error.restype = C.c_char_p
     L.kf_version.argtypes = []

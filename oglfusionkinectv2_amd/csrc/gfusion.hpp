@@ -174,6 +174,25 @@ public:
         check(kf_track(session(), &tracked));
         return tracked != 0;
     }
+    // TrackSDF (gFusion.cpp:835-1038) and recoverPose (:1616-1653)
+    bool TrackSDF()
+    {
+        int tracked = 0;
+        check(kf_track_sdf(session(), &tracked));
+        return tracked != 0;
+    }
+    bool recoverPose()
+    {
+        int found = 0;
+        check(kf_recover_pose(session(), &found));
+        return found != 0;
+    }
+    std::vector<float> getCumTwist()
+    {
+        std::vector<float> t(6);
+        check(kf_get_cum_twist(session(), t.data()));
+        return t;
+    }
     void track(int layer) { check(kf_icp_iteration(session(), layer, 1, 1)); }
     void reduce(int /*layer*/) {}  // fused into track()
     void getReduction(std::vector<float> &b, std::vector<float> &C, float &alignmentEnergy)
@@ -225,4 +244,54 @@ private:
     float cam_[4] = {365.5f, 365.5f, 255.5f, 211.5f};
     int width_ = 512, height_ = 424;
     Mat4 pose_;
+};
+
+// mCubeConfig (include/mcubes.h:25-43)
+struct mCubeConfig {
+    float gridSize[3] = {128, 128, 128};
+    unsigned numVoxels = 128 * 128 * 128;
+    float isoValue = 1650.0f;
+    unsigned maxVerts = 128 * 128 * 128;
+};
+
+// C++ drop-in for `class MCubes` (include/mcubes.h:46-170) over kf_mesh_*:
+// the volume is the gFusion session's TSDF.
+class MCubes {
+public:
+    void setVolumeTexture(gFusion &g) { g_ = &g; }
+    void setConfig(const mCubeConfig &c) { config_ = c; }
+    void init() {}
+    void setIsolevel(float iso) { iso_ = iso; }
+    void generateMarchingCubes()
+    {
+        uint64_t n = 0;
+        check(kf_mesh_generate(g_->session(), iso_, &n));
+        triangles_ = n;
+    }
+    // m_totalSum: the vertex count (mcubes.h:76-79)
+    int getNumberTriangles() const { return (int)(3 * triangles_); }
+    std::vector<float> getPosBuffer()
+    {
+        std::vector<float> v((size_t)triangles_ * 12);
+        if (triangles_) check(kf_mesh_read(g_->session(), v.data(), triangles_));
+        return v;
+    }
+    void exportMesh(const std::string &path = "data/meshes/marchingCubesBin.stl")
+    {
+        check(kf_mesh_export_stl(g_->session(), path.c_str()));
+    }
+    void exportPointCloud(const std::string &path = "data/meshes/marchingCubesBin.ply")
+    {
+        check(kf_mesh_export_ply(g_->session(), path.c_str()));
+    }
+
+private:
+    static void check(int rc)
+    {
+        if (rc != KF_OK) throw gfusion_error(rc, std::string("kfusion: ") + kf_last_error());
+    }
+    gFusion *g_ = nullptr;
+    mCubeConfig config_;
+    float iso_ = 0.0f;
+    uint64_t triangles_ = 0;
 };

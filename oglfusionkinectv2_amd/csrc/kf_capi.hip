@@ -21,6 +21,7 @@
 
 #include "kf_device.hpp"
 #include "kf_internal.hpp"
+#include "kf_session.hpp"
 
 namespace {
 
@@ -85,56 +86,6 @@ int validate(const kf_config *c)
 
 }  // namespace
 
-struct kf_session {
-    kf_config cfg;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    int device = 0;
-    float *depth0 = nullptr;
-    float *depth_m = nullptr;
-    uint16_t *stage_u16 = nullptr;
-    float *stage_f32 = nullptr;
-    float4 *vert[3] = {nullptr, nullptr, nullptr};
-    float4 *norm[3] = {nullptr, nullptr, nullptr};
-    float4 *refV = nullptr, *refN = nullptr;
-    void *tsdf = nullptr, *weight = nullptr;
-    kf::DevState *st = nullptr;
-    float *block_partials = nullptr;
-    unsigned long long *counts = nullptr;
-    uint8_t *bflags = nullptr, *dflags = nullptr, *cflags = nullptr, *zflags = nullptr;
-    float2 *dtiles = nullptr;  // 8x8-pixel (min, max) of the level-0 depth in metres
-    // stage timing (KF_OPT_STAGE_TIMING): start / stop events of pyramid, ICP
-    // (first iteration .. finish), integrate, raycast, carried by the dispatch packets
-    int timing = 0;
-    hipEvent_t tev[8] = {};
-    int NB[3] = {0, 0, 0}, NC[3] = {0, 0, 0};
-    size_t nbricks = 0, ncoarse = 0;
-    // options (kf_set_option)
-    int fast_math = 1;
-    int skip_empty = 1;
-    int use_graph = 1;
-    int rebuild_period = 16;
-    int xcd_group = 4;
-    int icp_rows_per_block = 2;
-    // frame graph: ICP + gate + integrate + raycast, captured once
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_failed = 0;
-    long frames = 0;
-    int debug = 0;
-    int dbg_level = -1;
-    int32_t *dbg_result = nullptr;
-    float *dbg_error = nullptr, *dbg_J = nullptr;
-    size_t voxels = 0;
-    // point-to-volume tracking and pose recovery
-    float *library = nullptr;       // pose library [kPoseLibraryMax][16]
-    float *sdf_partials = nullptr;  // 2 x sdf_blocks rows of 32 floats
-    float *sdf_slots = nullptr;     // last-column records, (H0 + H1 + H2) x 8 floats
-    int sdf_blocks = 0;             // persistent grid: at most one block per CU, multiple of 8
-    int track_mode = 1;             // KF_TRACK_POINT_TO_PLANE / _VOLUME / both
-    int recover = 1;                // recoverPose() in the frame loop when tracking fails
-};
-
 namespace {
 
 size_t session_bytes(const kf_config *c)
@@ -180,6 +131,7 @@ void free_all(kf_session *s)
     f(s->refV); f(s->refN); f(s->tsdf); f(s->weight); f(s->st); f(s->block_partials); f(s->counts);
     f(s->dbg_result); f(s->dbg_error); f(s->dbg_J);
     f(s->library); f(s->sdf_partials); f(s->sdf_slots);
+    f(s->mc_tab); f(s->mc_count); f(s->mc_offset); f(s->mc_part); f(s->mc_total); f(s->mc_verts);
     for (int i = 0; i < 8; ++i)
         if (s->tev[i]) (void)hipEventDestroy(s->tev[i]);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
@@ -529,6 +481,8 @@ int map_ptr(kf_session *s, int which, int level, void **ptr, size_t *bytes)
 }
 
 }  // namespace
+
+int kf_fail(int code, const char *msg) { return fail(code, msg); }
 
 extern "C" {
 

@@ -68,6 +68,7 @@ int main(int argc, char **argv)
                 if (counter > 2) reset = false;
             }
             if (!tracked && f > 0) ++untracked;
+            if (!tracked) gfusion.recoverPose();
             counter++;
             gfusion.raycast();
             std::vector<float> t = gfusion.getTransPose();

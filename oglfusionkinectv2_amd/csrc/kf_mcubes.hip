@@ -1,0 +1,548 @@
+// kf_mcubes.hip -- surface export of the TSDF for gfx950: the MCubes pass of
+// mcubes.cpp:206-390 (histoPyramids.cs classifyCubes + constructHPLevel,
+// traverseHistoPyramids.cs) and the files of exportMesh / exportPointCloud
+// (mcubes.cpp:390-509).
+//
+// The reference builds a histopyramid of per-cube triangle counts and walks
+// it from the top for every triangle index; at each level the eight children
+// are taken in the order x, then z, then y (cubeOffsets), so triangle t lands
+// in the cube whose Morton key (bits interleaved y, z, x per level) holds the
+// t-th triangle in key order.  Here the same order comes from a prefix sum:
+//   k_mc_count  one wavefront per 8^3 brick (a run of 512 consecutive Morton
+//               keys): classify its cubes, write the brick's triangle count;
+//               bricks whose dilated occupancy flag is clear hold no value
+//               below zero and are skipped exactly (iso <= 0)
+//   k_scan_*    exclusive prefix sum of the brick counts in Morton order
+//   k_mc_emit   one wavefront per non-empty brick: classify again, prefix
+//               over its cubes in Morton order, write every triangle's three
+//               vertices at its global index
+// so the vertex buffer is the reference's (pos[target * 3 + vertexNr]).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kf_device.hpp"
+#include "kf_session.hpp"
+#include "kf_store.hpp"
+
+namespace kf {
+namespace {
+
+constexpr int kMcWaves = 4;  // bricks per block (one wavefront each)
+
+struct McArgs {
+    const void *tsdf;
+    int N[3];
+    float iso;
+    int levels;              // Morton levels of the brick grid (PB = 1 << levels)
+    const uint8_t *dflags;   // dilated brick flags (NULL: classify every brick)
+    int NB[3];
+    const uint8_t *tri;      // [256][16] edge indices, 255 after the last triangle
+    const uint8_t *ntri;     // [256]
+    uint32_t *count;         // [PB^3] Morton order
+    const uint32_t *offset;  // [PB^3] exclusive prefix of count
+    float4 *verts;           // 3 per triangle
+    unsigned long long cap;  // triangles verts holds
+    unsigned long long nslots;
+};
+
+// corner k of the cube in classification-bit order, and edge k's end corners
+__constant__ int8_t kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1},
+                                     {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
+__constant__ int8_t kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                    {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+// (x, y, z) of a Morton key with `levels` digits; digit = x + 2 z + 4 y
+__device__ __forceinline__ void morton_decode(uint32_t m, int levels, int &x, int &y, int &z)
+{
+    x = y = z = 0;
+    for (int b = 0; b < levels; ++b) {
+        const uint32_t d = (m >> (3 * b)) & 7u;
+        x |= (int)(d & 1u) << b;
+        z |= (int)((d >> 1) & 1u) << b;
+        y |= (int)(d >> 2) << b;
+    }
+}
+
+template <class S>
+__device__ __forceinline__ float mc_field(const typename S::V *__restrict__ v, const McArgs &a, int x, int y, int z)
+{
+    if ((unsigned)x >= (unsigned)a.N[0] || (unsigned)y >= (unsigned)a.N[1] || (unsigned)z >= (unsigned)a.N[2])
+        return 0.0f;
+    return S::raw(v[((size_t)z * a.N[1] + y) * a.N[0] + x]);
+}
+
+// classifyCubes (histoPyramids.cs:104-170): cube index of the cube at voxel
+// (x, y, z), 0 for the 10-voxel border and for cubes without a value > 0 in
+// the first seven corners
+template <class S>
+__device__ __forceinline__ int mc_classify(const typename S::V *__restrict__ v, const McArgs &a, int x, int y, int z)
+{
+    if (x <= 10 || x >= a.N[0] - 10 || y <= 10 || y >= a.N[1] - 10 || z <= 10 || z >= a.N[2] - 10) return 0;
+    // field[k] at cubeOffsets[k] = (k & 1, k >> 2, (k >> 1) & 1) in (x, y, z)
+    float f[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) f[k] = mc_field<S>(v, a, x + (k & 1), y + (k >> 2), z + ((k >> 1) & 1));
+    bool pos = false;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) pos = pos || f[k] > 0.0f;
+    if (!pos) return -1;  // stored as (0, 0): no triangles
+    const float iso = a.iso;
+    return (int)(f[0] < iso) | (int)(f[1] < iso) << 1 | (int)(f[3] < iso) << 2 | (int)(f[2] < iso) << 3 |
+           (int)(f[4] < iso) << 4 | (int)(f[5] < iso) << 5 | (int)(f[7] < iso) << 6 | (int)(f[6] < iso) << 7;
+}
+
+__device__ __forceinline__ bool brick_live(const McArgs &a, int bx, int by, int bz)
+{
+    if (bx * 8 >= a.N[0] || by * 8 >= a.N[1] || bz * 8 >= a.N[2]) return false;
+    if (!a.dflags) return true;
+    return a.dflags[((size_t)bz * a.NB[1] + by) * a.NB[0] + bx] != 0;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_exclusive_u32(uint32_t v, int lane)
+{
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    return inc - v;
+}
+
+template <class S>
+__global__ __launch_bounds__(64 * kMcWaves) void k_mc_count(McArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long m = (unsigned long long)blockIdx.x * kMcWaves + (threadIdx.x >> 6);
+    if (m >= a.nslots) return;
+    int bx, by, bz;
+    morton_decode((uint32_t)m, a.levels, bx, by, bz);
+    uint32_t n = 0;
+    if (brick_live(a, bx, by, bz)) {
+        const typename S::V *__restrict__ v = (const typename S::V *)a.tsdf;
+        for (int j = 0; j < 8; ++j) {
+            int lx, ly, lz;
+            morton_decode((uint32_t)(8 * lane + j), 3, lx, ly, lz);
+            const int ci = mc_classify<S>(v, a, 8 * bx + lx, 8 * by + ly, 8 * bz + lz);
+            if (ci > 0) n += a.ntri[ci];
+        }
+        n = wave_sum_u32(n);
+    }
+    if (lane == 0) a.count[m] = n;
+}
+
+template <class S>
+__global__ __launch_bounds__(64 * kMcWaves) void k_mc_emit(McArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long m = (unsigned long long)blockIdx.x * kMcWaves + (threadIdx.x >> 6);
+    if (m >= a.nslots) return;
+    if (a.count[m] == 0) return;  // uniform over the wavefront
+    int bx, by, bz;
+    morton_decode((uint32_t)m, a.levels, bx, by, bz);
+    const typename S::V *__restrict__ v = (const typename S::V *)a.tsdf;
+    int ci[8];
+    uint32_t n = 0;
+    for (int j = 0; j < 8; ++j) {
+        int lx, ly, lz;
+        morton_decode((uint32_t)(8 * lane + j), 3, lx, ly, lz);
+        const int c = mc_classify<S>(v, a, 8 * bx + lx, 8 * by + ly, 8 * bz + lz);
+        ci[j] = c > 0 ? c : 0;
+        n += a.ntri[ci[j]];
+    }
+    unsigned long long t = (unsigned long long)a.offset[m] + wave_exclusive_u32(n, lane);
+    for (int j = 0; j < 8; ++j) {
+        if (!ci[j]) continue;
+        int lx, ly, lz;
+        morton_decode((uint32_t)(8 * lane + j), 3, lx, ly, lz);
+        const int cx = 8 * bx + lx, cy = 8 * by + ly, cz = 8 * bz + lz;
+        const uint8_t *tri = a.tri + 16 * ci[j];
+        const int nt = a.ntri[ci[j]];
+        for (int k = 0; k < nt; ++k, ++t) {
+            if (t >= a.cap) continue;
+            for (int q = 0; q < 3; ++q) {
+                // traverseHistoPyramids.cs:196-250
+                const int e = tri[3 * k + q];
+                const int c0 = kEdge[e][0], c1 = kEdge[e][1];
+                const int p0[3] = {cx + kCorner[c0][0], cy + kCorner[c0][1], cz + kCorner[c0][2]};
+                const int p1[3] = {cx + kCorner[c1][0], cy + kCorner[c1][1], cz + kCorner[c1][2]};
+                const float v0 = mc_field<S>(v, a, p0[0], p0[1], p0[2]);
+                float diff = (a.iso - v0) / (mc_field<S>(v, a, p1[0], p1[1], p1[2]) - v0);
+                if (diff > 1.0f || diff < 0.0f) diff = 0.5f;
+                float4 o;
+                o.x = (float)p0[0] * (1.0f - diff) + (float)p1[0] * diff;
+                o.y = (float)p0[1] * (1.0f - diff) + (float)p1[1] * diff;
+                o.z = (float)p0[2] * (1.0f - diff) + (float)p1[2] * diff;
+                o.w = 0.0f;
+                a.verts[3 * t + q] = o;
+            }
+        }
+    }
+}
+
+// Exclusive scan of n uint32 in place of `out` (from `in`): blocks of 1024,
+// block totals scanned by one block, then added back.  Grand total to *total.
+constexpr int kScanBlock = 1024;
+
+__global__ __launch_bounds__(256) void k_scan_blocks(const uint32_t *in, uint32_t *out, uint32_t *part,
+                                                     unsigned long long n)
+{
+    __shared__ uint32_t wsum[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long base = (unsigned long long)blockIdx.x * kScanBlock + 4 * threadIdx.x;
+    uint32_t v[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        s += v[k];
+    }
+    const uint32_t ex = wave_exclusive_u32(s, lane);
+    if (lane == 63) wsum[wave] = ex + s;
+    __syncthreads();
+    uint32_t wbase = 0;
+    for (int w = 0; w < wave; ++w) wbase += wsum[w];
+    uint32_t run = wbase + ex;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255) part[blockIdx.x] = run;
+}
+
+// one block: exclusive scan of the nb block totals (nb <= 256 * 16)
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t *part, int nb, unsigned long long *total)
+{
+    __shared__ uint32_t wsum[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int kPer = 16;
+    uint32_t v[kPer], s = 0;
+    const int base = kPer * threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        v[k] = base + k < nb ? part[base + k] : 0u;
+        s += v[k];
+    }
+    const uint32_t ex = wave_exclusive_u32(s, lane);
+    if (lane == 63) wsum[wave] = ex + s;
+    __syncthreads();
+    uint32_t wbase = 0;
+    for (int w = 0; w < wave; ++w) wbase += wsum[w];
+    uint32_t run = wbase + ex;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (base + k < nb) part[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255) *total = run;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t *out, const uint32_t *part, unsigned long long n)
+{
+    const unsigned long long base = (unsigned long long)blockIdx.x * kScanBlock + 4 * threadIdx.x;
+    const uint32_t add = part[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) out[base + k] += add;
+}
+
+// ---------------------------------------------------------------------------
+// Triangle table from the cube topology: the surface crosses every edge whose
+// corners lie on different sides of the iso level (bit set = below it); on
+// each face the crossings join in pairs, a face with four crossings
+// separating its two below-iso corners; the pieces close into polygons, each
+// oriented with its normal towards the above-iso side and split into a fan of
+// triangles from its lowest edge.  Per-case triangle counts are those of the
+// reference's table (nrOfTriangles, tables.h).
+// ---------------------------------------------------------------------------
+const int kHostCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}, {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
+const int kHostEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                              {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+const int kHostFace[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {3, 2, 6, 7}, {0, 3, 7, 4}, {1, 2, 6, 5}};
+
+int host_edge(int a, int b)
+{
+    for (int k = 0; k < 12; ++k)
+        if ((kHostEdge[k][0] == a && kHostEdge[k][1] == b) || (kHostEdge[k][0] == b && kHostEdge[k][1] == a)) return k;
+    return -1;
+}
+
+void build_tables(uint8_t tri[256][16], uint8_t ntri[256])
+{
+    for (int c = 0; c < 256; ++c) {
+        bool below[8];
+        for (int v = 0; v < 8; ++v) below[v] = (c >> v) & 1;
+        int link[12][2], deg[12] = {0};
+        for (const auto &f : kHostFace) {
+            int es[4];
+            bool cr[4];
+            int n = 0;
+            for (int i = 0; i < 4; ++i) {
+                es[i] = host_edge(f[i], f[(i + 1) % 4]);
+                cr[i] = below[f[i]] != below[f[(i + 1) % 4]];
+                n += cr[i];
+            }
+            auto join = [&](int p, int q) {
+                link[p][deg[p]++] = q;
+                link[q][deg[q]++] = p;
+            };
+            if (n == 2) {
+                int k[2], m = 0;
+                for (int i = 0; i < 4; ++i)
+                    if (cr[i]) k[m++] = es[i];
+                join(k[0], k[1]);
+            } else if (n == 4) {
+                if (below[f[0]]) {  // corners 0 and 2 are below: cut each off
+                    join(es[3], es[0]);
+                    join(es[1], es[2]);
+                } else {
+                    join(es[0], es[1]);
+                    join(es[2], es[3]);
+                }
+            }
+        }
+        for (int i = 0; i < 16; ++i) tri[c][i] = 255;
+        bool used[12] = {false};
+        int nt = 0;
+        for (int e0 = 0; e0 < 12; ++e0) {
+            if (!deg[e0] || used[e0]) continue;
+            int cyc[12], m = 0, prev = -1, cur = e0;
+            for (;;) {
+                cyc[m++] = cur;
+                used[cur] = true;
+                const int nx = link[cur][0] != prev ? link[cur][0] : link[cur][1];
+                if (nx == e0 || used[nx]) break;
+                prev = cur;
+                cur = nx;
+            }
+            double P[12][3], nrm[3] = {0, 0, 0};
+            for (int i = 0; i < m; ++i)
+                for (int k = 0; k < 3; ++k)
+                    P[i][k] = 0.5 * (kHostCorner[kHostEdge[cyc[i]][0]][k] + kHostCorner[kHostEdge[cyc[i]][1]][k]);
+            for (int i = 0; i < m; ++i) {
+                const double *p = P[i], *q = P[(i + 1) % m];
+                nrm[0] += (p[1] - q[1]) * (p[2] + q[2]);
+                nrm[1] += (p[2] - q[2]) * (p[0] + q[0]);
+                nrm[2] += (p[0] - q[0]) * (p[1] + q[1]);
+            }
+            double dir = 0;  // Newell normal against the below -> above edge directions
+            for (int i = 0; i < m; ++i) {
+                const int va = kHostEdge[cyc[i]][0], vb = kHostEdge[cyc[i]][1];
+                const int lo = below[va] ? va : vb, hi = below[va] ? vb : va;
+                for (int k = 0; k < 3; ++k) dir += nrm[k] * (kHostCorner[hi][k] - kHostCorner[lo][k]);
+            }
+            if (dir < 0)
+                for (int i = 1, j = m - 1; i < j; ++i, --j) std::swap(cyc[i], cyc[j]);
+            for (int i = 1; i + 1 < m; ++i) {
+                tri[c][3 * nt + 0] = (uint8_t)cyc[0];
+                tri[c][3 * nt + 1] = (uint8_t)cyc[i];
+                tri[c][3 * nt + 2] = (uint8_t)cyc[i + 1];
+                ++nt;
+            }
+        }
+        ntri[c] = (uint8_t)nt;
+    }
+}
+
+#define MC_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) return kf_fail(KF_ERR_HIP, (std::string(#call) + ": " + hipGetErrorString(e_)).c_str()); \
+    } while (0)
+
+int ensure_mesh_buffers(kf_session *s, unsigned long long slots)
+{
+    if (!s->mc_tab) {
+        static uint8_t tab[256 * 16 + 256];
+        static bool built = false;
+        if (!built) {
+            build_tables(reinterpret_cast<uint8_t(*)[16]>(tab), tab + 256 * 16);
+            built = true;
+        }
+        MC_HIP(hipMalloc((void **)&s->mc_tab, sizeof(tab)));
+        MC_HIP(hipMemcpyAsync(s->mc_tab, tab, sizeof(tab), hipMemcpyHostToDevice, s->stream));
+        MC_HIP(hipStreamSynchronize(s->stream));
+    }
+    if (s->mc_slots < slots) {
+        (void)hipFree(s->mc_count);
+        (void)hipFree(s->mc_offset);
+        (void)hipFree(s->mc_part);
+        s->mc_count = s->mc_offset = s->mc_part = nullptr;
+        s->mc_slots = 0;
+        const size_t nb = (slots + kScanBlock - 1) / kScanBlock;
+        MC_HIP(hipMalloc((void **)&s->mc_count, slots * 4));
+        MC_HIP(hipMalloc((void **)&s->mc_offset, slots * 4));
+        MC_HIP(hipMalloc((void **)&s->mc_part, nb * 4));
+        s->mc_slots = slots;
+    }
+    if (!s->mc_total) MC_HIP(hipMalloc((void **)&s->mc_total, sizeof(unsigned long long)));
+    return KF_OK;
+}
+
+template <class S>
+void launch_mc(const McArgs &a, bool emit, hipStream_t st)
+{
+    const unsigned blocks = (unsigned)((a.nslots + kMcWaves - 1) / kMcWaves);
+    if (emit) hipLaunchKernelGGL(k_mc_emit<S>, dim3(blocks), dim3(64 * kMcWaves), 0, st, a);
+    else hipLaunchKernelGGL(k_mc_count<S>, dim3(blocks), dim3(64 * kMcWaves), 0, st, a);
+}
+
+void launch_mc_storage(int storage, const McArgs &a, bool emit, hipStream_t st)
+{
+    switch (storage) {
+    case KF_STORAGE_RG16I: launch_mc<StoreRG16I>(a, emit, st); break;
+    case KF_STORAGE_F32: launch_mc<StoreF32>(a, emit, st); break;
+    default: launch_mc<StoreF16>(a, emit, st); break;
+    }
+}
+
+}  // namespace
+}  // namespace kf
+
+using kf::kScanBlock;
+
+extern "C" {
+
+int kf_mc_tables(uint8_t *tri, uint8_t *ntri)
+{
+    if (!tri || !ntri) return kf_fail(KF_ERR_INVALID, "NULL argument");
+    kf::build_tables(reinterpret_cast<uint8_t(*)[16]>(tri), ntri);
+    return KF_OK;
+}
+
+int kf_mesh_generate(kf_session *s, float iso, uint64_t *triangles)
+{
+    if (!s) return kf_fail(KF_ERR_INVALID, "session is NULL");
+    const int *N = s->cfg.volume_size;
+    int P = 8;
+    while (P < N[0] || P < N[1] || P < N[2]) P *= 2;
+    int levels = 0;
+    while ((8 << levels) < P) ++levels;
+    const unsigned long long slots = 1ull << (3 * levels);
+    const size_t nb = (size_t)((slots + kScanBlock - 1) / kScanBlock);
+    if (nb > 256 * 16) return kf_fail(KF_ERR_INVALID, "volume too large for the surface export scan");
+    int rc = kf::ensure_mesh_buffers(s, slots);
+    if (rc) return rc;
+    kf::McArgs a{};
+    a.tsdf = s->tsdf;
+    for (int k = 0; k < 3; ++k) {
+        a.N[k] = N[k];
+        a.NB[k] = s->NB[k];
+    }
+    a.iso = iso;
+    a.levels = levels;
+    // a brick whose dilated flag is clear holds no value below zero around it
+    a.dflags = (iso <= 0.0f) ? s->dflags : nullptr;
+    a.tri = s->mc_tab;
+    a.ntri = s->mc_tab + 256 * 16;
+    a.count = s->mc_count;
+    a.offset = s->mc_offset;
+    a.nslots = slots;
+    kf::launch_mc_storage(s->cfg.storage, a, false, s->stream);
+    MC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(kf::k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, s->stream, s->mc_count, s->mc_offset,
+                       s->mc_part, slots);
+    hipLaunchKernelGGL(kf::k_scan_top, dim3(1), dim3(256), 0, s->stream, s->mc_part, (int)nb, s->mc_total);
+    hipLaunchKernelGGL(kf::k_scan_add, dim3((unsigned)nb), dim3(256), 0, s->stream, s->mc_offset, s->mc_part, slots);
+    MC_HIP(hipGetLastError());
+    unsigned long long total = 0;
+    MC_HIP(hipMemcpyAsync(&total, s->mc_total, sizeof(total), hipMemcpyDeviceToHost, s->stream));
+    MC_HIP(hipStreamSynchronize(s->stream));
+    if (total > s->mc_cap) {
+        (void)hipFree(s->mc_verts);
+        s->mc_verts = nullptr;
+        s->mc_cap = 0;
+        const size_t cap = (size_t)total + total / 4 + 1024;
+        MC_HIP(hipMalloc((void **)&s->mc_verts, cap * 3 * sizeof(float4)));
+        s->mc_cap = cap;
+    }
+    a.verts = s->mc_verts;
+    a.cap = s->mc_cap;
+    if (total) {
+        kf::launch_mc_storage(s->cfg.storage, a, true, s->stream);
+        MC_HIP(hipGetLastError());
+    }
+    s->mc_triangles = total;
+    if (triangles) *triangles = total;
+    return KF_OK;
+}
+
+int kf_mesh_read(kf_session *s, float *vertices, uint64_t max_triangles)
+{
+    if (!s || !vertices) return kf_fail(KF_ERR_INVALID, "NULL argument");
+    const unsigned long long n = s->mc_triangles < max_triangles ? s->mc_triangles : max_triangles;
+    if (n) MC_HIP(hipMemcpyAsync(vertices, s->mc_verts, n * 3 * sizeof(float4), hipMemcpyDeviceToHost, s->stream));
+    MC_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
+// exportMesh (mcubes.cpp:429-509): binary STL, 80-byte header, a triangle
+// count of posData.size() / 3 (four times the triangles: posData holds four
+// floats per vertex), then per triangle the normal (1, 1, 1), three vertices
+// (x, y, z in voxel-index units) and a zero attribute word.
+int kf_mesh_export_stl(kf_session *s, const char *path)
+{
+    if (!s || !path) return kf_fail(KF_ERR_INVALID, "NULL argument");
+    const unsigned long long nt = s->mc_triangles;
+    std::vector<float> pos((size_t)nt * 12);
+    if (nt) {
+        int rc = kf_mesh_read(s, pos.data(), nt);
+        if (rc) return rc;
+    }
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return kf_fail(KF_ERR_INVALID, (std::string("cannot open ") + path).c_str());
+    char hdr[80];
+    std::memset(hdr, 0, sizeof(hdr));
+    const uint32_t count = (uint32_t)((nt * 12) / 3);
+    const uint16_t attr = 0;
+    const float one[3] = {1.0f, 1.0f, 1.0f};
+    bool ok = std::fwrite(hdr, 1, 80, f) == 80 && std::fwrite(&count, 4, 1, f) == 1;
+    std::vector<char> rec(50);
+    for (unsigned long long t = 0; t < nt && ok; ++t) {
+        const float *p = pos.data() + 12 * t;
+        std::memcpy(rec.data(), one, 12);
+        std::memcpy(rec.data() + 12, p + 0, 12);
+        std::memcpy(rec.data() + 24, p + 4, 12);
+        std::memcpy(rec.data() + 36, p + 8, 12);
+        std::memcpy(rec.data() + 48, &attr, 2);
+        ok = std::fwrite(rec.data(), 1, 50, f) == 50;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return kf_fail(KF_ERR_INVALID, (std::string("write failed: ") + path).c_str());
+    return KF_OK;
+}
+
+// exportPointCloud (mcubes.cpp:390-427): ASCII PLY of every vertex.
+int kf_mesh_export_ply(kf_session *s, const char *path)
+{
+    if (!s || !path) return kf_fail(KF_ERR_INVALID, "NULL argument");
+    const unsigned long long nt = s->mc_triangles;
+    std::vector<float> pos((size_t)nt * 12);
+    if (nt) {
+        int rc = kf_mesh_read(s, pos.data(), nt);
+        if (rc) return rc;
+    }
+    FILE *f = std::fopen(path, "w");
+    if (!f) return kf_fail(KF_ERR_INVALID, (std::string("cannot open ") + path).c_str());
+    bool ok = std::fprintf(f, "ply\nformat ascii 1.0\nelement vertex %llu\nproperty float x\nproperty float y\n"
+                              "property float z\nend_header\n", nt * 3) > 0;
+    for (unsigned long long v = 0; v < nt * 3 && ok; ++v)
+        ok = std::fprintf(f, "%g %g %g\n", pos[4 * v], pos[4 * v + 1], pos[4 * v + 2]) > 0;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return kf_fail(KF_ERR_INVALID, (std::string("write failed: ") + path).c_str());
+    return KF_OK;
+}
+
+}  // extern "C"

@@ -87,6 +87,9 @@ class Oracle:
             "kfo_session_recover_pose": ([vp], i),
             "kfo_session_set_mode": ([vp, i, i], None),
             "kfo_session_lib_size": ([vp], i),
+            "kfo_mc_tables": ([vp, vp], None),
+            "kfo_mc_classify": ([vp, i, vp, R, vp, vp], None),
+            "kfo_mc_extract": ([vp, i, i, R, vp, C.c_long], C.c_long),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -272,6 +275,31 @@ class Oracle:
         r, rp = self.arr(result, np.float64).copy(), self.arr(result_prev, np.float64).copy()
         cn = self.L.kfo_sdf_step(self.p(bb), self.p(cc), int(iteration), self.p(r), self.p(rp))
         return cn, r, rp
+
+    # -- surface export (histopyramid marching cubes) ---------------------
+    def mc_tables(self):
+        """(tri[256, 16] edge indices with 255 after the last triangle, ntri[256])"""
+        tri = np.zeros((256, 16), np.uint8)
+        nt = np.zeros(256, np.uint8)
+        self.L.kfo_mc_tables(self.p(tri), self.p(nt))
+        return tri, nt
+
+    def mc_classify(self, vol, mode, iso=0.0):
+        v, N = np.ascontiguousarray(vol), self._N(vol)
+        nt = np.zeros(vol.shape[:3], np.uint8)
+        ci = np.zeros(vol.shape[:3], np.uint8)
+        self.L.kfo_mc_classify(self.p(v), mode, self.p(N), float(iso), self.p(nt), self.p(ci))
+        return nt, ci
+
+    def mc_extract(self, vol, mode, iso=0.0):
+        """Triangle vertices (T, 3, 4) in voxel-index units, (x, y, z, 0)."""
+        v = np.ascontiguousarray(vol)
+        n = vol.shape[0]
+        assert vol.shape[:3] == (n, n, n) and n & (n - 1) == 0
+        total = self.L.kfo_mc_extract(self.p(v), mode, n, float(iso), None, 0)
+        out = np.zeros((max(total, 1), 3, 4), self.dt)
+        self.L.kfo_mc_extract(self.p(v), mode, n, float(iso), self.p(out), total)
+        return out[:total]
 
 
 def _config_struct(R):

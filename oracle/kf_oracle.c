@@ -1074,6 +1074,234 @@ KFO_EXPORT double kfo_sdf_step(const R b[6], const R C[36], int iteration, doubl
 }
 
 /* ------------------------------------------------------------------------- */
+/* Surface export: MCubes::histoPyramids (mcubes.cpp:206-390), exportMesh     */
+/* (:429-509), histoPyramids.cs (classifyCubes :104-170, constructHPLevel      */
+/* :42-90), traverseHistoPyramids.cs (scanHPLevel :52-109, traverseHPLevel     */
+/* :126-256).                                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* Cube corners in the order of the classification bits (histoPyramids.cs:
+ * 156-163 reads cubeOffsets 0,1,3,2,4,5,7,6 into bits 0..7) and the 12 edges
+ * as (corner, corner); edge k joins the corners listed, in that order. */
+static const int MC_CORNER[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1},
+                                    {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
+static const int MC_EDGE[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                   {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+static const int MC_FACE[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                  {3, 2, 6, 7}, {0, 3, 7, 4}, {1, 2, 6, 5}};
+
+static int mc_edge_of(int a, int b)
+{
+    for (int k = 0; k < 12; ++k)
+        if ((MC_EDGE[k][0] == a && MC_EDGE[k][1] == b) || (MC_EDGE[k][0] == b && MC_EDGE[k][1] == a)) return k;
+    return -1;
+}
+
+/* Triangle table built from the cube's topology: the surface crosses every
+ * edge whose corners lie on different sides of the iso level (bit set = below
+ * it); on each face the crossings are joined in pairs, a face with four
+ * crossings separating its two below-iso corners; the joined segments close
+ * into polygons, each oriented with its normal towards the above-iso side and
+ * split into a fan of triangles from its lowest edge.  tri[c][0..3n) holds the
+ * n triangles of case c as edge indices, 255 after them; ntri[c] = n. */
+KFO_EXPORT void kfo_mc_tables(uint8_t tri[256][16], uint8_t ntri[256])
+{
+    for (int c = 0; c < 256; ++c) {
+        int in[8];
+        for (int v = 0; v < 8; ++v) in[v] = (c >> v) & 1;
+        int nb[12][2], deg[12];
+        for (int e = 0; e < 12; ++e) deg[e] = 0;
+        for (int f = 0; f < 6; ++f) {
+            int es[4], cr[4], n = 0;
+            for (int i = 0; i < 4; ++i) {
+                int a = MC_FACE[f][i], b = MC_FACE[f][(i + 1) % 4];
+                es[i] = mc_edge_of(a, b);
+                cr[i] = in[a] != in[b];
+                n += cr[i];
+            }
+            int seg[2][2], ns = 0;
+            if (n == 2) {
+                int k = 0;
+                for (int i = 0; i < 4; ++i)
+                    if (cr[i]) seg[0][k++] = es[i];
+                ns = 1;
+            } else if (n == 4) {
+                if (in[MC_FACE[f][0]]) { /* corners 0 and 2 are below: cut them off */
+                    seg[0][0] = es[3]; seg[0][1] = es[0];
+                    seg[1][0] = es[1]; seg[1][1] = es[2];
+                } else {
+                    seg[0][0] = es[0]; seg[0][1] = es[1];
+                    seg[1][0] = es[2]; seg[1][1] = es[3];
+                }
+                ns = 2;
+            }
+            for (int k = 0; k < ns; ++k) {
+                int a = seg[k][0], b = seg[k][1];
+                nb[a][deg[a]++] = b;
+                nb[b][deg[b]++] = a;
+            }
+        }
+        for (int i = 0; i < 16; ++i) tri[c][i] = 255;
+        int used[12] = {0}, nt = 0;
+        for (int e0 = 0; e0 < 12; ++e0) {
+            if (!deg[e0] || used[e0]) continue;
+            int cyc[12], m = 0, prev = -1, cur = e0;
+            for (;;) {
+                cyc[m++] = cur;
+                used[cur] = 1;
+                int nx = nb[cur][0] != prev ? nb[cur][0] : nb[cur][1];
+                if (nx == e0 || used[nx]) break;
+                prev = cur;
+                cur = nx;
+            }
+            /* orientation: Newell normal of the edge midpoints against the
+             * below -> above directions of the polygon's edges */
+            double P[12][3], nrm[3] = {0, 0, 0};
+            for (int i = 0; i < m; ++i)
+                for (int k = 0; k < 3; ++k)
+                    P[i][k] = 0.5 * (MC_CORNER[MC_EDGE[cyc[i]][0]][k] + MC_CORNER[MC_EDGE[cyc[i]][1]][k]);
+            for (int i = 0; i < m; ++i) {
+                const double *a = P[i], *b = P[(i + 1) % m];
+                nrm[0] += (a[1] - b[1]) * (a[2] + b[2]);
+                nrm[1] += (a[2] - b[2]) * (a[0] + b[0]);
+                nrm[2] += (a[0] - b[0]) * (a[1] + b[1]);
+            }
+            double dir = 0;
+            for (int i = 0; i < m; ++i) {
+                int va = MC_EDGE[cyc[i]][0], vb = MC_EDGE[cyc[i]][1];
+                int lo = in[va] ? va : vb, hi = in[va] ? vb : va;
+                for (int k = 0; k < 3; ++k) dir += nrm[k] * (MC_CORNER[hi][k] - MC_CORNER[lo][k]);
+            }
+            if (dir < 0)
+                for (int i = 1, j = m - 1; i < j; ++i, --j) { int t = cyc[i]; cyc[i] = cyc[j]; cyc[j] = t; }
+            for (int i = 1; i + 1 < m; ++i) {
+                tri[c][3 * nt + 0] = (uint8_t)cyc[0];
+                tri[c][3 * nt + 1] = (uint8_t)cyc[i];
+                tri[c][3 * nt + 2] = (uint8_t)cyc[i + 1];
+                ++nt;
+            }
+        }
+        ntri[c] = (uint8_t)nt;
+    }
+}
+
+/* stored value at a voxel as histoPyramids.cs getVolumeData reads it for the
+ * RG16I volume (float of the int16), the value itself for the other modes */
+static inline R mc_field(const void *vol, int mode, const int N[3], int x, int y, int z)
+{
+    return vol_raw(vol, mode, N, x, y, z);
+}
+
+/* classifyCubes (histoPyramids.cs:104-170) for every voxel: (numTri, cubeIndex);
+ * the 10-voxel border keeps (0, 0) (the texture's initial contents). */
+KFO_EXPORT void kfo_mc_classify(const void *vol, int mode, const int N[3], R iso, uint8_t *ntri_out,
+                                uint8_t *cube_out)
+{
+    uint8_t tri[256][16], ntri[256];
+    kfo_mc_tables(tri, ntri);
+    static const int order[8] = {0, 1, 3, 2, 4, 5, 7, 6}; /* cubeOffsets index of bit k */
+    static const int OFF[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 0, 1}, {1, 0, 1},
+                                  {0, 1, 0}, {1, 1, 0}, {0, 1, 1}, {1, 1, 1}};
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N[2]; ++z)
+        for (int y = 0; y < N[1]; ++y)
+            for (int x = 0; x < N[0]; ++x) {
+                size_t i = ((size_t)z * N[1] + y) * N[0] + x;
+                ntri_out[i] = 0;
+                cube_out[i] = 0;
+                if (x <= 10 || x >= N[0] - 10 || y <= 10 || y >= N[1] - 10 || z <= 10 || z >= N[2] - 10)
+                    continue;
+                R field[8];
+                for (int k = 0; k < 8; ++k)
+                    field[k] = mc_field(vol, mode, N, x + OFF[k][0], y + OFF[k][1], z + OFF[k][2]);
+                int pos = 0;
+                for (int k = 0; k < 7; ++k) /* field[7] is not looked at (:143) */
+                    if (field[k] > 0) pos = 1;
+                if (!pos) continue;
+                int ci = 0;
+                for (int k = 0; k < 8; ++k) ci |= (field[order[k]] < iso) << k;
+                ntri_out[i] = ntri[ci];
+                cube_out[i] = (uint8_t)ci;
+            }
+}
+
+/* The whole histopyramid export for a cubic power-of-two volume: build the
+ * levels by 2x2x2 sums (constructHPLevel), read the total at the top, then
+ * for every triangle index walk the levels top-down (scanHPLevel) to its cube
+ * and emit the triangle's three vertices (traverseHPLevel :190-250) as
+ * (x, y, z, 0) in voxel-index units.  Returns the number of triangles; writes
+ * at most `cap` triangles (3 x 4 values each) into `pos` (may be NULL). */
+KFO_EXPORT long kfo_mc_extract(const void *vol, int mode, int n, R iso, R *pos, long cap)
+{
+    int N[3] = {n, n, n};
+    int L = 0;
+    while ((1 << L) < n) ++L;
+    size_t nv = (size_t)n * n * n;
+    uint8_t *nt0 = (uint8_t *)malloc(nv), *ci0 = (uint8_t *)malloc(nv);
+    kfo_mc_classify(vol, mode, N, iso, nt0, ci0);
+    uint32_t **lev = (uint32_t **)calloc((size_t)L + 1, sizeof(uint32_t *));
+    for (int l = 1; l <= L; ++l) {
+        int m = n >> l, pm = n >> (l - 1);
+        lev[l] = (uint32_t *)calloc((size_t)m * m * m, 4);
+        for (int z = 0; z < m; ++z)
+            for (int y = 0; y < m; ++y)
+                for (int x = 0; x < m; ++x) {
+                    uint32_t acc = 0;
+                    for (int k = 0; k < 8; ++k) {
+                        int X = 2 * x + (k & 1), Z = 2 * z + ((k >> 1) & 1), Y = 2 * y + (k >> 2);
+                        size_t j = ((size_t)Z * pm + Y) * pm + X;
+                        acc += l == 1 ? nt0[j] : lev[l - 1][j];
+                    }
+                    lev[l][((size_t)z * m + y) * m + x] = acc;
+                }
+    }
+    long total = L > 0 ? (long)lev[L][0] : (long)nt0[0];
+    uint8_t tri[256][16], ntri[256];
+    kfo_mc_tables(tri, ntri);
+    long lim = pos ? (total < cap ? total : cap) : 0;
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < lim; ++t) {
+        uint32_t target = (uint32_t)t;
+        uint32_t cx = 0, cy = 0, cz = 0, cw = 0;
+        for (int l = L - 1; l >= 0; --l) {
+            int m = n >> l;
+            uint32_t nb[8];
+            for (int k = 0; k < 8; ++k) {
+                uint32_t X = cx + (k & 1), Z = cz + ((k >> 1) & 1), Y = cy + (k >> 2);
+                size_t j = ((size_t)Z * m + Y) * m + X;
+                nb[k] = (X < (uint32_t)m && Y < (uint32_t)m && Z < (uint32_t)m) ? (l == 0 ? nt0[j] : lev[l][j]) : 0;
+            }
+            uint32_t acc = cw, child = 0, add = 0;
+            for (int k = 0; k < 7; ++k) {
+                acc += nb[k];
+                if (acc <= target) { child++; add += nb[k]; }
+            }
+            cx += child & 1; cz += (child >> 1) & 1; cy += child >> 2;
+            cx *= 2; cy *= 2; cz *= 2;
+            cw += add;
+        }
+        cx /= 2; cy /= 2; cz /= 2;
+        int ci = ci0[((size_t)cz * n + cy) * n + cx];
+        for (int v = 0; v < 3; ++v) {
+            int e = tri[ci][3 * (target - cw) + v];
+            const int *a = MC_CORNER[MC_EDGE[e][0]], *b = MC_CORNER[MC_EDGE[e][1]];
+            int p0[3] = {(int)cx + a[0], (int)cy + a[1], (int)cz + a[2]};
+            int p1[3] = {(int)cx + b[0], (int)cy + b[1], (int)cz + b[2]};
+            R v0 = mc_field(vol, mode, N, p0[0], p0[1], p0[2]);
+            R diff = (iso - v0) / (mc_field(vol, mode, N, p1[0], p1[1], p1[2]) - v0);
+            if (diff > 1 || diff < 0) diff = KC(0.5);
+            R *o = pos + 4 * (3 * t + v);
+            /* mix(p0, p1, diff) = p0 * (1 - diff) + p1 * diff */
+            for (int k = 0; k < 3; ++k) o[k] = (R)p0[k] * (KC(1.0) - diff) + (R)p1[k] * diff;
+            o[3] = 0;
+        }
+    }
+    for (int l = 1; l <= L; ++l) free(lev[l]);
+    free(lev); free(nt0); free(ci0);
+    return total;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Session: gFusion host state + the frame block of kinect.cpp:654-695        */
 /* ------------------------------------------------------------------------- */
 #define KFO_LIB_MAX 256
