@@ -47,6 +47,7 @@ extern "C" {
 #define KF_MAP_NORMAL 2     /* float4 normal (or (2,0,0,0)), level l */
 #define KF_MAP_REF_VERTEX 3 /* float4 raycast world vertex, level 0  */
 #define KF_MAP_REF_NORMAL 4 /* float4 raycast world normal, level 0  */
+#define KF_MAP_INTENSITY 5  /* float4 intensity projection, level 0 (kf_intensity_projection) */
 
 /* ---- integrate gating ------------------------------------------------------ */
 #define KF_INTEGRATE_ALWAYS 0 /* gFusion::integrate() */
@@ -153,6 +154,12 @@ int kf_read_sdf_state(kf_session *s, float pose_eig[16], float cum_twist[6], flo
  * in resetPose and TrackSDF; used to load a known tracking state). */
 int kf_write_pose_eig(kf_session *s, const float pose_eig[16]);
 
+/* intensityProjection() (gFusion.cpp:1368-1409, mip.cs): for every pixel the
+ * sum of the positive TSDF samples along its ray, one every 0.75 mu from the
+ * box / near-plane entry to min(box exit, far plane), as (sum, 0, 0, 1) into
+ * the KF_MAP_INTENSITY image (kf_read_map). */
+int kf_intensity_projection(kf_session *s);
+
 /* integrate() (gFusion.cpp:1234-1287).  mode = KF_INTEGRATE_ALWAYS or _GATED. */
 int kf_integrate(kf_session *s, int mode);
 /* raycast() (gFusion.cpp:1289-1366): refreshes the reference vertex/normal maps. */
@@ -246,6 +253,22 @@ int kf_count_integrate_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *v
  * as empty space, empty cells entered, rays that hit, the largest number of
  * samples one ray evaluated, and the largest evaluated + cells of one ray. */
 int kf_raycast_stats(kf_session *s, uint64_t stats[6]);
+
+/* ---- input adapters --------------------------------------------------------
+ * camPams of kinect.h (the depth camera's intrinsics and distortion). */
+typedef struct kf_cam_params {
+    float fx, fy, ppx, ppy;
+    float k1, k2, p1, p2, k3;
+} kf_cam_params;
+/* loadCalibration (kinect.cpp:78-161): an OpenCV-style XML file (the <data>
+ * of Camera_Matrix, 3x3 row-major, and of Distortion_Coefficients, 5 values)
+ * or a JSON file ("camera_matrix" 3x3, "dist_coeff" 5 values), chosen by the
+ * extension.  The reference's JSON branch always opens
+ * resources/infrared.json; here the file named is read.  KF_ERR_INVALID for
+ * another extension or a malformed file (the reference returns 0 / a tinyxml2
+ * error code).  The frame loop then sets the camera to (fx, fx, ppx, ppy)
+ * (kinect.cpp:176-177). */
+int kf_load_calibration(const char *path, kf_cam_params *out);
 
 /* ---- surface export (MCubes, mcubes.cpp / histoPyramids.cs) ----------------
  * Marching cubes over the session's TSDF as "save stl" runs it

@@ -35,7 +35,7 @@ def _run(cmd):
 
 
 def build_kfusion():
-    srcs = [os.path.join(CSRC, f) for f in ("kf_kernels.hip", "kf_capi.hip", "kf_sdf.hip", "kf_mcubes.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("kf_kernels.hip", "kf_capi.hip", "kf_sdf.hip", "kf_mcubes.hip", "kf_io.cpp")]
     srcs = [s for s in srcs if os.path.exists(s)]
     out = os.path.join(PKG, "libkfusion.so")
     tmp = out + ".tmp"

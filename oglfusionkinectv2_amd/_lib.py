@@ -29,6 +29,7 @@ MAP_VERTEX = 1
 MAP_NORMAL = 2
 MAP_REF_VERTEX = 3
 MAP_REF_NORMAL = 4
+MAP_INTENSITY = 5
 
 INTEGRATE_ALWAYS = 0
 INTEGRATE_GATED = 1
@@ -135,6 +136,7 @@ def lib():
         "kf_get_cum_twist": [sess, f32p],
         "kf_write_pose_eig": [sess, f32p],
         "kf_mesh_generate": [sess, C.c_float, C.POINTER(C.c_uint64)],
+        "kf_intensity_projection": [sess],
         "kf_mesh_read": [sess, f32p, C.c_uint64],
         "kf_mesh_export_stl": [sess, C.c_char_p],
         "kf_mesh_export_ply": [sess, C.c_char_p],
@@ -144,6 +146,8 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    L.kf_load_calibration.argtypes = [C.c_char_p, vp]
+    L.kf_load_calibration.restype = C.c_int
     L.kf_mc_tables.argtypes = [vp, vp]
     L.kf_mc_tables.restype = C.c_int
     L.kf_last_error.argtypes = []

@@ -213,6 +213,8 @@ public:
         for (int i = 0; i < 6; ++i)
             for (int j = i; j < 6; ++j, ++q) C[i * 6 + j] = C[j * 6 + i] = row[q];
     }
+    void intensityProjection() { check(kf_intensity_projection(session())); }
+    std::vector<float> getIntensityProjection() { return readMap(KF_MAP_INTENSITY, 0, 4); }
     void integrate() { check(kf_integrate(session(), KF_INTEGRATE_ALWAYS)); }
     void raycast() { check(kf_raycast(session())); }
     // The whole frame block without host synchronisation (device-side gate).
@@ -295,3 +297,17 @@ private:
     float iso_ = 0.0f;
     uint64_t triangles_ = 0;
 };
+
+// loadCalibration (kinect.cpp:78-161): 1 on success, 0 for an extension other
+// than .xml / .json; throws gfusion_error for a file it cannot read or parse.
+inline int loadCalibration(const std::string &calibFile, kf_cam_params &out)
+{
+    const auto ends = [&](const char *e) {
+        const size_t n = std::string(e).size();
+        return calibFile.size() >= n && calibFile.compare(calibFile.size() - n, n, e) == 0;
+    };
+    if (!ends(".xml") && !ends(".json")) return 0;
+    const int rc = kf_load_calibration(calibFile.c_str(), &out);
+    if (rc != KF_OK) throw gfusion_error(rc, std::string("kfusion: ") + kf_last_error());
+    return 1;
+}

@@ -131,7 +131,7 @@ void free_all(kf_session *s)
     f(s->refV); f(s->refN); f(s->tsdf); f(s->weight); f(s->st); f(s->block_partials); f(s->counts);
     f(s->dbg_result); f(s->dbg_error); f(s->dbg_J);
     f(s->library); f(s->sdf_partials); f(s->sdf_slots);
-    f(s->mc_tab); f(s->mc_count); f(s->mc_offset); f(s->mc_part); f(s->mc_total); f(s->mc_verts);
+    f(s->mc_tab); f(s->mc_count); f(s->mc_offset); f(s->mc_part); f(s->mc_total); f(s->mc_verts); f(s->mip);
     for (int i = 0; i < 8; ++i)
         if (s->tev[i]) (void)hipEventDestroy(s->tev[i]);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
@@ -476,6 +476,10 @@ int map_ptr(kf_session *s, int which, int level, void **ptr, size_t *bytes)
     case KF_MAP_REF_NORMAL:
         if (level != 0) return fail(KF_ERR_INVALID, "reference maps are level 0");
         *ptr = s->refN; *bytes = n * 16; return KF_OK;
+    case KF_MAP_INTENSITY:
+        if (level != 0) return fail(KF_ERR_INVALID, "the intensity projection is level 0");
+        if (!s->mip) return fail(KF_ERR_STATE, "kf_intensity_projection has not run");
+        *ptr = s->mip; *bytes = n * 16; return KF_OK;
     }
     return fail(KF_ERR_INVALID, "unknown map");
 }
@@ -795,6 +799,19 @@ int kf_get_cum_twist(kf_session *s, float twist[6])
 {
     if (!twist) return fail(KF_ERR_INVALID, "NULL argument");
     return kf_read_sdf_state(s, nullptr, twist, nullptr, nullptr, nullptr, nullptr);
+}
+
+int kf_intensity_projection(kf_session *s)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    if (!s->mip) {
+        const size_t p0 = (size_t)s->cfg.width * s->cfg.height;
+        if (hipMalloc((void **)&s->mip, p0 * 16) != hipSuccess) return fail(KF_ERR_OOM, "intensity image");
+    }
+    kf::RaycastArgs a = raycast_args(s);
+    hipError_t e = kf::launch_intensity_projection(a, s->cfg.storage, s->st, s->mip, s->stream);
+    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("intensity projection: ") + hipGetErrorString(e));
+    return KF_OK;
 }
 
 int kf_integrate(kf_session *s, int mode)

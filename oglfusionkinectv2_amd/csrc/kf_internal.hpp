@@ -162,6 +162,8 @@ hipError_t launch_brick_rebuild(const BrickArgs &a, int storage, hipStream_t s);
 hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s);
 hipError_t launch_count_integrate(const IntegrateArgs &a, const DevState *st, unsigned long long *counts,
                                   hipStream_t s);
+hipError_t launch_intensity_projection(const RaycastArgs &a, int storage, const DevState *st, float4 *out,
+                                       hipStream_t s);
 hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s,
                           hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 

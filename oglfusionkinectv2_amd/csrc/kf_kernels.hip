@@ -1877,6 +1877,65 @@ hipError_t launch_count_integrate(const IntegrateArgs &a, const DevState *st, un
     return hipGetLastError();
 }
 
+// intensityProjection (gFusion.cpp:1368-1409, mip.cs:33-107): per pixel, the
+// sum of the positive trilinear samples of the TSDF along the ray at t =
+// tnear, tnear + largeStep, ... < tfar; written as (sum, 0, 0, 1).
+template <class S>
+__global__ __launch_bounds__(256) void k_intensity_projection(RaycastArgs a, const DevState *__restrict__ st,
+                                                              float4 *__restrict__ out)
+{
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= a.w || y >= a.h) return;
+    float view[16];
+    {
+        float iK[16], P[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
+        inverse_camera_matrix(a.cam, iK);
+        mul4(P, iK, view);  // m_pose * invK (gFusion.cpp:1376)
+    }
+    VolView<S> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
+                 (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32};
+    const float o[3] = {view[12], view[13], view[14]};
+    float d[3];
+    rotate3(view, (float)x, (float)y, 1.0f, d);
+    float tmin[3], tmax[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float invR = 1.0f / d[k];
+        const float tbot = (-1.0f * invR) * o[k];
+        const float ttop = invR * (a.dim[k] - o[k]);
+        tmin[k] = gmin(ttop, tbot);
+        tmax[k] = gmax(ttop, tbot);
+    }
+    const float largest_tmin = gmax(gmax(tmin[0], tmin[1]), gmax(tmin[0], tmin[2]));
+    const float smallest_tmax = gmin(gmin(tmax[0], tmax[1]), gmin(tmax[0], tmax[2]));
+    const float tnear = gmax(largest_tmin, a.near_plane);
+    const float tfar = gmin(smallest_tmax, a.far_plane);
+    float sum = 0.0f;
+    if (tnear < tfar) {
+        int guard = 0;
+        for (float t = tnear; t < tfar && guard < kMaxMarchSteps; t += a.large_step, ++guard) {
+            const float f = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
+            if (f > 0.0f) sum += f;
+        }
+    }
+    out[(size_t)y * a.w + x] = make_float4(sum, 0.0f, 0.0f, 1.0f);
+}
+
+hipError_t launch_intensity_projection(const RaycastArgs &a, int storage, const DevState *st, float4 *out,
+                                       hipStream_t s)
+{
+    const dim3 g((a.w + 15) / 16, (a.h + 15) / 16), b(256);
+    switch (storage) {
+    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_intensity_projection<StoreRG16I>, g, b, 0, s, a, st, out); break;
+    case KF_STORAGE_F32: hipLaunchKernelGGL(k_intensity_projection<StoreF32>, g, b, 0, s, a, st, out); break;
+    case KF_STORAGE_F16: hipLaunchKernelGGL(k_intensity_projection<StoreF16>, g, b, 0, s, a, st, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s, hipEvent_t t0,
                           hipEvent_t t1)
 {

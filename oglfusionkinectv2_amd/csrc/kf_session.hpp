@@ -67,6 +67,7 @@ struct kf_session {
     float4 *mc_verts = nullptr;         // 3 (x, y, z, 0) vertices per triangle, voxel-index units
     size_t mc_cap = 0;                  // triangles mc_verts holds
     unsigned long long mc_triangles = 0;  // triangles of the last kf_mesh_generate
+    float4 *mip = nullptr;              // intensityProjection image (RGBA32F, W x H)
 };
 
 // sets the calling thread's error message (kf_last_error); returns code

@@ -285,6 +285,12 @@ class GFusion:
         K.check(K.lib().kf_read_reduction(self.session, None, C.byref(ae)))
         return ae.value
 
+    def intensityProjection(self):
+        """gFusion.cpp:1368-1409: returns the (H, W, 4) image (sum of the
+        positive TSDF samples per ray, 0, 0, 1)."""
+        K.check(K.lib().kf_intensity_projection(self.session))
+        return self._read_map(K.MAP_INTENSITY)
+
     def integrate(self, gated=False):
         K.check(K.lib().kf_integrate(self.session, K.INTEGRATE_GATED if gated else K.INTEGRATE_ALWAYS))
 

@@ -87,6 +87,7 @@ class Oracle:
             "kfo_session_recover_pose": ([vp], i),
             "kfo_session_set_mode": ([vp, i, i], None),
             "kfo_session_lib_size": ([vp], i),
+            "kfo_intensity_projection": ([vp, i, vp, vp, vp, vp, R, R, R, i, i, vp], None),
             "kfo_mc_tables": ([vp, vp], None),
             "kfo_mc_classify": ([vp, i, vp, R, vp, vp], None),
             "kfo_mc_extract": ([vp, i, i, R, vp, C.c_long], C.c_long),
@@ -275,6 +276,14 @@ class Oracle:
         r, rp = self.arr(result, np.float64).copy(), self.arr(result_prev, np.float64).copy()
         cn = self.L.kfo_sdf_step(self.p(bb), self.p(cc), int(iteration), self.p(r), self.p(rp))
         return cn, r, rp
+
+    def intensity_projection(self, vol, mode, dim, pose, cam, near, far, large_step, w=512, h=424):
+        N = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], np.int32)
+        out = np.zeros((h, w, 4), self.dt)
+        v, dd, P, c = np.ascontiguousarray(vol), self.arr(dim), self.cm(pose), self.arr(cam)
+        self.L.kfo_intensity_projection(self.p(v), mode, self.p(N), self.p(dd), self.p(P), self.p(c), near, far,
+                                        large_step, w, h, self.p(out))
+        return out
 
     # -- surface export (histopyramid marching cubes) ---------------------
     def mc_tables(self):

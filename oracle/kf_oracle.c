@@ -820,6 +820,46 @@ KFO_EXPORT void kfo_raycast(const void *vol, int mode, const int N[3], const R d
 }
 
 /* ------------------------------------------------------------------------- */
+/* intensityProjection: gFusion.cpp:1368-1409 + mip.cs:33-107                  */
+/* ------------------------------------------------------------------------- */
+KFO_EXPORT void kfo_intensity_projection(const void *vol, int mode, const int N[3], const R dim[3],
+                                         const R pose[16], const R cam[4], R nearPlane, R farPlane,
+                                         R largeStep, int w, int h, R *out)
+{
+    R view[16];
+    kfo_raycast_view(pose, cam, view);
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            R o[3] = {view[12], view[13], view[14]};
+            R d[3];
+            rotate3(view, (R)x, (R)y, KC(1.0), d);
+            R tmin[3], tmax[3];
+            for (int k = 0; k < 3; ++k) {
+                R invR = KC(1.0) / d[k];
+                R tbot = (KC(-1.0) * invR) * o[k];
+                R ttop = invR * (dim[k] - o[k]);
+                tmin[k] = gmin(ttop, tbot);
+                tmax[k] = gmax(ttop, tbot);
+            }
+            R largest_tmin = gmax(gmax(tmin[0], tmin[1]), gmax(tmin[0], tmin[2]));
+            R smallest_tmax = gmin(gmin(tmax[0], tmax[1]), gmin(tmax[0], tmax[2]));
+            R tnear = gmax(largest_tmin, nearPlane);
+            R tfar = gmin(smallest_tmax, farPlane);
+            R sum = 0, zsum = 0;
+            if (tnear < tfar) {
+                for (R t = tnear; t < tfar; t += largeStep) {
+                    R f = interp_vol(vol, mode, N, dim, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
+                    if (f > 0) sum += f;
+                    else if (f == 0) zsum += f;
+                }
+            }
+            R *px = out + 4 * ((size_t)y * w + x);
+            px[0] = sum; px[1] = 0; px[2] = zsum; px[3] = KC(1.0);
+        }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Point-to-volume tracking: gFusion.cpp:835-1038 (TrackSDF), :1040-1084      */
 /* (trackSDF), :1086-1112 (reduceSDF), :1195-1232 (getSDFReduction),          */
 /* trackSDF.cs:186-220, 222-235, 296-306, 442-530, reduceSDF.cs:28-116.       */

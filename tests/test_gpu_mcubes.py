@@ -150,3 +150,27 @@ def test_mesh_512_after_frames_is_consistent(seq20):
     assert P.min() >= 11 - 1e-4 and P.max() <= 512 - 9 + 1e-4
     near = np.abs(P - np.round(P)) < 1e-4
     assert np.all(near.sum(axis=2) >= 2)  # on an edge: two integer coordinates
+
+
+def test_intensity_projection_matches_oracle(o32, seq20):
+    """intensityProjection (mip.cs) on a fused 128^3 volume: the same FP32
+    sample expressions and additions on both sides."""
+    print("case: intensity projection 128^3", flush=True)
+    depth, gt = seq20
+    cfg = kinect_config(128, 2.0, "rg16i")
+    s = oracle.OracleSession(o32, 128, 2.0, cfg.mu, oracle.RG16I)
+    s.reset(gt[0])
+    for f in range(5):
+        s.frame(depth[f].astype(np.float32))
+    vol = s.volume()
+    g = GFusion()
+    g.setConfig(cfg)
+    g.allocateBuffers()
+    g.Reset(gt[0])
+    g.setVolume(*planes(vol, "rg16i"))
+    g.setPose(s.pose())
+    img = g.intensityProjection()
+    want = o32.intensity_projection(vol, oracle.RG16I, (2.0,) * 3, s.pose(), (365.5, 365.5, 255.5, 211.5),
+                                    0.1, 3.0, np.float32(0.75) * np.float32(cfg.mu))
+    assert (want[..., 0] > 0).mean() > 0.2
+    np.testing.assert_array_equal(img, want)
