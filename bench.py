@@ -112,6 +112,42 @@ def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, count_every=25):
     return {n: float(np.mean(v)) for n, v in acc.items()}, counts
 
 
+def row_timings(torch, L, g, frames_dev, nframes, reps=3):
+    """The paths beside the frame loop on the same 512^3 session after the
+    timed frames: TrackSDF (one persistent launch, from the session's
+    m_pose_eig), recoverPose's no-op launch when tracking succeeded (the
+    frame graph carries it), the surface export (marching cubes over the
+    whole TSDF) and intensityProjection.  Wall-clock over synchronised
+    calls, best of `reps`."""
+    import ctypes as C
+    from oglfusionkinectv2_amd import _lib as K
+    from oglfusionkinectv2_amd import MCubes
+    s = g.session
+    out = {}
+
+    def best(fn):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            K.check(L.kf_synchronize(s))
+            ts.append(time.perf_counter() - t0)
+        return 1e3 * min(ts), r
+
+    g.setPoseEig(g.getPose())
+    out["track_sdf_ms"], tracked = best(g.TrackSDF)
+    out["track_sdf_tracked"] = bool(tracked)
+    out["track_sdf_iterations"] = g.sdfState()["iterations"]
+    mc = MCubes()
+    mc.setVolumeTexture(g)
+    mc.setIsolevel(0.0)
+    out["mesh_export_ms"], ntri = best(mc.generateMarchingCubes)
+    out["mesh_triangles"] = int(ntri)
+    out["intensity_projection_ms"], _ = best(lambda: K.check(L.kf_intensity_projection(s)))
+    return out
+
+
 def cpu_baseline(depth, poses, volume, frames, warm=2):
     """The FP32 C oracle on the same workload: `warm` untimed frames after
     reset, then `frames` timed frames."""
@@ -194,6 +230,7 @@ def main():
     vw = float(np.mean([c[1] for c in counts])) if counts else 0.0
     int_bytes = (vr + vw) * bv + 424 * 512 * 4
     dom = max(kern, key=kern.get)
+    extras = row_timings(torch, L, g, frames_dev, args.warmup + args.steps)
     g.close()
 
     # --- FP16 --------------------------------------------------------------
@@ -258,6 +295,7 @@ def main():
         "dominant_kernel": dom,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "beside_the_loop": extras,
         "multi_session": multi,
         "tracked_last": st["tracked"],
         "final_translation_error_vs_gt_m": ate_gt,

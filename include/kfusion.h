@@ -220,6 +220,17 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         their own dispatch packets; kf_read_stage_times
  *                         returns the last durations.  Default 0. */
 #define KF_OPT_STAGE_TIMING 7
+/*   KF_OPT_ICP_ALL_GROUPS 1: every ICP iteration sums all 8 row groups of
+ *                         reduce.cs.  0 (default): under KF_RULE_LAST_ROW only
+ *                         row group 7, the one getReduction reads, is summed
+ *                         (same pose, AE and decisions); the other rows of
+ *                         kf_read_reduction's partials then hold zero sums.
+ *                         Per-pixel recording (kf_set_debug) sums all groups. */
+#define KF_OPT_ICP_ALL_GROUPS 10
+/*   KF_OPT_RAYCAST_LANES  lanes that march one ray together: 1 (default) or
+ *                         4 (the four evaluate 16 consecutive fine-step
+ *                         samples per round).  Same results either way. */
+#define KF_OPT_RAYCAST_LANES 11
 /*   KF_OPT_TRACK_MODE     tracking of the frame loop (kinect.h:143-144):
  *                         KF_TRACK_POINT_TO_PLANE (Track, default),
  *                         KF_TRACK_POINT_TO_VOLUME (TrackSDF) or both (Track

@@ -43,6 +43,8 @@ OPT_ICP_ROWS_PER_BLOCK = 6
 OPT_STAGE_TIMING = 7
 OPT_TRACK_MODE = 8
 OPT_RECOVER = 9
+OPT_ICP_ALL_GROUPS = 10
+OPT_RAYCAST_LANES = 11
 
 TRACK_POINT_TO_PLANE = 1
 TRACK_POINT_TO_VOLUME = 2

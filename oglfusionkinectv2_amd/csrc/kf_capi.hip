@@ -216,8 +216,10 @@ kf::RaycastArgs raycast_args(kf_session *s)
     a.tsdf = s->tsdf;
     a.w = c.width;
     a.h = c.height;
-    a.tiles_x = (c.width + 15) / 16;
-    a.tiles_y = (c.height + 15) / 16;
+    a.lanes = s->ray_lanes;
+    const int tile = a.lanes == 4 ? 8 : 16;  // 256 threads: 256 or 64 rays
+    a.tiles_x = (c.width + tile - 1) / tile;
+    a.tiles_y = (c.height + tile - 1) / tile;
     a.xcd_group = std::max(1, s->xcd_group);
     a.blocks = (a.tiles_x * a.tiles_y + 8 * a.xcd_group - 1) / (8 * a.xcd_group) * (8 * a.xcd_group);
     a.off32 = s->voxels < (size_t(1) << 32) ? 1 : 0;
@@ -301,6 +303,9 @@ kf::IcpArgs icp_args(kf_session *s, int level, int first, int save_pose)
     // blocks per row group: about kIcpRowsPerBlock rows of the group per block
     const int rows_per_group = (a.h + 7) / 8;
     a.rows_step = std::max(1, (rows_per_group + s->icp_rows_per_block - 1) / s->icp_rows_per_block);
+    // getReduction's last-row rule reads only row group 7 (gFusion.cpp:1143-1149):
+    // the other groups' sums are not needed unless they are to be inspected
+    a.group_only = (s->cfg.reduce_rule == KF_RULE_LAST_ROW && !s->icp_all_groups && !s->debug) ? 7 : -1;
     return a;
 }
 
@@ -865,6 +870,11 @@ int kf_set_option(kf_session *s, int option, int value)
         if (value < 1 || value > 1024) return fail(KF_ERR_INVALID, "rows per block must be in [1, 1024]");
         s->icp_rows_per_block = value;
         break;
+    case KF_OPT_ICP_ALL_GROUPS: s->icp_all_groups = value ? 1 : 0; break;
+    case KF_OPT_RAYCAST_LANES:
+        if (value != 1 && value != 4) return fail(KF_ERR_INVALID, "raycast lanes per ray must be 1 or 4");
+        s->ray_lanes = value;
+        break;
     case KF_OPT_TRACK_MODE:
         if (value < 1 || value > 3) return fail(KF_ERR_INVALID, "track mode must be 1, 2 or 3");
         s->track_mode = value;
@@ -888,6 +898,8 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_RAYCAST_XCD_GROUP: *value = s->xcd_group; break;
     case KF_OPT_ICP_ROWS_PER_BLOCK: *value = s->icp_rows_per_block; break;
     case KF_OPT_TRACK_MODE: *value = s->track_mode; break;
+    case KF_OPT_ICP_ALL_GROUPS: *value = s->icp_all_groups; break;
+    case KF_OPT_RAYCAST_LANES: *value = s->ray_lanes; break;
     case KF_OPT_RECOVER: *value = s->recover; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
@@ -1069,6 +1081,7 @@ struct kf_batch {
     int n = 0;
     int storage = 0, fast = 0, off32 = 1;
     int pyr_blocks = 0, ray_blocks = 0, nblk[3] = {0, 0, 0};
+    int ray_lanes = 1;
     int total_int = 0, total_coarse = 0;
     std::vector<long> frames;
     // Membership changes upload new tables with stream-ordered copies from a
@@ -1112,6 +1125,8 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
         if (si->cfg.width != s0->cfg.width || si->cfg.height != s0->cfg.height || si->cfg.storage != s0->cfg.storage ||
             si->fast_math != s0->fast_math || si->icp_rows_per_block != s0->icp_rows_per_block ||
             si->xcd_group != s0->xcd_group || si->device != s0->device ||
+            si->icp_all_groups != s0->icp_all_groups || si->cfg.reduce_rule != s0->cfg.reduce_rule ||
+            si->ray_lanes != s0->ray_lanes ||
             std::memcmp(si->cfg.iterations, s0->cfg.iterations, sizeof(s0->cfg.iterations)) != 0)
             return fail(KF_ERR_INVALID, "batched sessions must share depth size, ICP schedule, storage and options");
         if (si->track_mode != KF_TRACK_POINT_TO_PLANE)
@@ -1161,7 +1176,8 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
     b->fast = s0->fast_math;
     b->pyr_blocks = h[0].pyr.blocks_total;
     b->ray_blocks = h[0].ray.blocks;
-    for (int l = 0; l < 3; ++l) b->nblk[l] = 8 * h[0].icp[l].rows_step;
+    b->ray_lanes = h[0].ray.lanes;
+    for (int l = 0; l < 3; ++l) b->nblk[l] = (h[0].icp[l].group_only >= 0 ? 1 : 8) * h[0].icp[l].rows_step;
     b->total_int = ip[n];
     b->total_coarse = cp[n];
     return KF_OK;
@@ -1264,7 +1280,7 @@ int kf_batch_frame_u16_dev(kf_batch *b, const uint16_t *const *depth_mm_dev, con
         }
     if (e == hipSuccess) e = kf::launch_batch_integrate(t, b->storage, b->fast, b->off32, b->total_int, b->stream);
     if (e == hipSuccess) e = kf::launch_batch_bricks(t, b->total_coarse, b->stream);
-    if (e == hipSuccess) e = kf::launch_batch_raycast(t, b->storage, b->ray_blocks, b->stream);
+    if (e == hipSuccess) e = kf::launch_batch_raycast(t, b->storage, b->ray_lanes, b->ray_blocks, b->stream);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
     // per-session exact brick rebuild on the same schedule as kf_frame_*
     for (int i = 0; i < b->n; ++i) {

@@ -37,6 +37,7 @@ struct IcpArgs {
     float *library;             // pose library [lib_max][16] (updatePoseFinder on success)
     int lib_max;
     int rows_step;              // blocks per row group
+    int group_only;             // -1: all 8 row groups; g: only row group g (the one the combine rule reads)
     int32_t *dbg_result;        // optional per-pixel output
     float *dbg_error, *dbg_J;
 };
@@ -82,6 +83,7 @@ struct RaycastArgs {
     float inv_dim[3];
     int off32;                  // voxel count < 2^32 (32-bit offsets)
     int xcd_group;              // tiles per XCD cluster (block -> tile mapping)
+    int lanes;                  // lanes per ray: 1 (16x16-pixel tiles) or 4 (8x8-pixel tiles)
 };
 
 // Point-to-volume tracking (persistent k_sdf_track, one launch per TrackSDF /
@@ -141,7 +143,7 @@ hipError_t launch_batch_icp(const BatchTick &t, int level, int first, int save_p
 hipError_t launch_batch_integrate(const BatchTick &t, int storage, int fast, int off32, int total_blocks,
                                   hipStream_t s);
 hipError_t launch_batch_bricks(const BatchTick &t, int total_coarse, hipStream_t s);
-hipError_t launch_batch_raycast(const BatchTick &t, int storage, int blocks_per_session, hipStream_t s);
+hipError_t launch_batch_raycast(const BatchTick &t, int storage, int lanes, int blocks_per_session, hipStream_t s);
 
 // Optional t0 / t1: events recorded by the kernel's own dispatch packet at its
 // start / end (exact kernel duration, no separate marker packets).

@@ -428,11 +428,10 @@ __device__ __forceinline__ void icp_view(const IcpArgs &a, const float T[16], fl
 }
 
 // this block's 28 sums at pose T written to dst[0..27] (threads < 28); ends
-// with the block synchronised
-__device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float T[16], int blk, float *dst,
+// with the block synchronised.  g: row group, jb: block within the group
+__device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float T[16], int g, int jb, float *dst,
                                                    float (*wsum)[kTerms])
 {
-    const int g = blk & 7, jb = blk >> 3;
     float view[16];
     icp_view(a, T, view);
 
@@ -519,19 +518,22 @@ __device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float
     __syncthreads();
 }
 
-// FP64 sums per row group of the partial rows 0..nblk-1 into gsum.  Thread t
-// sums block partials t, t + 256, ... (all of row group t % 8), lanes of the
-// same group are combined with xor shuffles (8, 16, 32), then the four waves
-// through LDS.  Fixed order.  Ends with the block synchronised.
-__device__ __forceinline__ void icp_group_sums(const float *partials, int nblk, double (*wgsum)[8][kTerms],
-                                               double (*gsum)[kTerms])
+// FP64 sums per row group of the partial rows 0..nrows-1 into gsum (row r
+// holds a block of row group r % 8; with group_only >= 0 only that group's
+// rows were written and the other groups sum to zero).  Thread t sums partial
+// rows t, t + 256, ..., lanes of the same group are combined with xor
+// shuffles (8, 16, 32), then the four waves through LDS.  Fixed order.  Ends
+// with the block synchronised.
+__device__ __forceinline__ void icp_group_sums(const float *partials, int nrows, int group_only,
+                                               double (*wgsum)[8][kTerms], double (*gsum)[kTerms])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     {
         double ps[kTerms];
 #pragma unroll
         for (int k = 0; k < kTerms; ++k) ps[k] = 0.0;
-        for (int pb = threadIdx.x; pb < nblk; pb += kIcpThreads) {
+        const bool mine = group_only < 0 || (threadIdx.x & 7) == group_only;
+        for (int pb = threadIdx.x; mine && pb < nrows; pb += kIcpThreads) {
             const float4 *rp = reinterpret_cast<const float4 *>(partials + pb * kPartStride);
             float4 q[kTerms / 4];
 #pragma unroll
@@ -708,7 +710,11 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
     float T[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) T[i] = st->pose[i];
-    icp_block_partials(a, T, blk, a.block_partials + blk * kPartStride, wsum);
+    // all groups: block b serves group b % 8; one group g: block b serves g
+    // and writes partial row 8 b + g (the row that group's blocks would use)
+    const int g = a.group_only >= 0 ? a.group_only : (blk & 7);
+    const int jb = a.group_only >= 0 ? blk : (blk >> 3);
+    icp_block_partials(a, T, g, jb, a.block_partials + (8 * jb + g) * kPartStride, wsum);
     if (threadIdx.x == 0) {
         // one agent-scope release per block: the barrier above orders the
         // block's partial stores before it
@@ -719,7 +725,7 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
     __syncthreads();
     if (!last) return;
     __threadfence();  // acquire: every block's partials are visible
-    icp_group_sums(a.block_partials, nblk, wgsum, gsum);
+    icp_group_sums(a.block_partials, 8 * a.rows_step, a.group_only, wgsum, gsum);
     icp_expose(a, gsum, st->partials);
     if (threadIdx.x >= 64) return;
     // lane 0 solves; lanes 0-2 then evaluate the three angles' cos / sin in
@@ -1256,46 +1262,32 @@ __global__ __launch_bounds__(512) void k_brick_rebuild(BrickArgs a)
 // needs are kept as 25 five-bit row masks.
 __device__ __forceinline__ void brick_dilate_body(const BrickArgs &a, int i)
 {
+    // one wavefront per 32^3 coarse cell: lane l holds brick (4 cx + l % 4,
+    // 4 cy + (l / 4) % 4, 4 cz + l / 16); its dilated flag is the OR over the
+    // bricks b .. b + 1 in each axis (the trilinear corners of a sample in b)
     const int n = a.NC[0] * a.NC[1] * a.NC[2];
-    if (i >= n) return;
+    if (i >= n) return;  // the whole wavefront
+    const int lane = threadIdx.x & 63;
     const int cx = i % a.NC[0], cy = (i / a.NC[0]) % a.NC[1], cz = i / (a.NC[0] * a.NC[1]);
-    const int bx0 = 4 * cx, by0 = 4 * cy, bz0 = 4 * cz;
-    uint32_t row[5][5];  // bit x: flag of brick (bx0 + x, by0 + y, bz0 + z)
+    const int bx = 4 * cx + (lane & 3), by = 4 * cy + ((lane >> 2) & 3), bz = 4 * cz + (lane >> 4);
+    const bool in = bx < a.NB[0] && by < a.NB[1] && bz < a.NB[2];
+    uint32_t d = 0;
+    if (in) {
 #pragma unroll
-    for (int z = 0; z < 5; ++z)
-#pragma unroll
-        for (int y = 0; y < 5; ++y) {
-            uint32_t m = 0;
-            const int Y = by0 + y, Z = bz0 + z;
-            if (Y < a.NB[1] && Z < a.NB[2]) {
-                const uint8_t *f = a.flags + ((size_t)Z * a.NB[1] + Y) * a.NB[0];
-#pragma unroll
-                for (int x = 0; x < 5; ++x)
-                    if (bx0 + x < a.NB[0] && f[bx0 + x]) m |= 1u << x;
-            }
-            row[z][y] = m;
+        for (int k = 0; k < 8; ++k) {
+            const int X = bx + (k & 1), Y = by + ((k >> 1) & 1), Z = bz + (k >> 2);
+            if (X < a.NB[0] && Y < a.NB[1] && Z < a.NB[2])
+                d |= a.flags[((size_t)Z * a.NB[1] + Y) * a.NB[0] + X];
         }
-    uint32_t any = 0;
-#pragma unroll
-    for (int z = 0; z < 4; ++z)
-#pragma unroll
-        for (int y = 0; y < 4; ++y) {
-            const uint32_t yz = row[z][y] | row[z][y + 1] | row[z + 1][y] | row[z + 1][y + 1];
-            const uint32_t d = (yz | (yz >> 1)) & 0xFu;  // bit x: OR over x, x + 1
-            const int Y = by0 + y, Z = bz0 + z;
-            if (Y >= a.NB[1] || Z >= a.NB[2]) continue;
-            uint8_t *o = a.dflags + ((size_t)Z * a.NB[1] + Y) * a.NB[0];
-#pragma unroll
-            for (int x = 0; x < 4; ++x)
-                if (bx0 + x < a.NB[0]) o[bx0 + x] = (uint8_t)((d >> x) & 1u);
-            any |= d;
-        }
-    a.cflags[i] = any ? 1 : 0;
+        a.dflags[((size_t)bz * a.NB[1] + by) * a.NB[0] + bx] = d ? 1 : 0;
+    }
+    const bool any = __any(d != 0);
+    if (lane == 0) a.cflags[i] = any ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_brick_dilate(BrickArgs a)
 {
-    brick_dilate_body(a, blockIdx.x * 256 + threadIdx.x);
+    brick_dilate_body(a, blockIdx.x * 4 + (threadIdx.x >> 6));
 }
 
 // ---------------------------------------------------------------------------
@@ -1523,20 +1515,32 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
             // evaluated sample sits at exactly the reference's t.
             const float ds[3] = {d[0] * (float)a.N[0] / a.dim[0], d[1] * (float)a.N[1] / a.dim[1],
                                  d[2] * (float)a.N[2] / a.dim[2]};
+            // 1 / ds: the exit bound below is a skip limit with a 0.01-voxel
+            // margin, far wider than the ulp a reciprocal adds
+            const float rds[3] = {1.0f / ds[0], 1.0f / ds[1], 1.0f / ds[2]};
             float t_prev = t;
             bool have_prev = true;
             n_eval++;
             int guard = 0;
+            // flags of the 32^3 cell and 8^3 brick of the next sample, loaded
+            // together with the previous fine-step batch (have_flags)
+            uint8_t cf_n = 1, ff_n = 1;
+            bool have_flags = false;
             for (; t < tfar && guard < kMaxMarchSteps; ++guard) {
                 const bool can_skip = a.skip && stepsize == a.step;
                 Sample sm;
                 sample_setup(a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t, sm);
                 if (can_skip && sm.valid) {
                     // two-level occupancy: 32^3 coarse cells, then 8^3 bricks
-                    // (both flags loaded together)
-                    const int b0 = sm.lo[0] >> 3, b1 = sm.lo[1] >> 3, b2 = sm.lo[2] >> 3;
-                    const uint8_t cf = a.cflags[((b2 >> 2) * a.NC[1] + (b1 >> 2)) * a.NC[0] + (b0 >> 2)];
-                    const uint8_t ff = a.dflags[(b2 * a.NB[1] + b1) * a.NB[0] + b0];
+                    uint8_t cf = cf_n, ff = ff_n;
+                    if (!have_flags) {
+                        const int b0 = sm.lo[0] >> 3, b1 = sm.lo[1] >> 3, b2 = sm.lo[2] >> 3;
+                        cf = ff = 1;  // outside the brick grid: never skipped
+                        if (b0 < a.NB[0] && b1 < a.NB[1] && b2 < a.NB[2]) {
+                            cf = a.cflags[((b2 >> 2) * a.NC[1] + (b1 >> 2)) * a.NC[0] + (b0 >> 2)];
+                            ff = a.dflags[(b2 * a.NB[1] + b1) * a.NB[0] + b0];
+                        }
+                    }
                     const int shift = cf == 0 ? 5 : (ff == 0 ? 3 : 0);
                     if (shift) {
                         // t below which the lower corner stays in the empty cell
@@ -1547,19 +1551,21 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
                         for (int k = 0; k < 3; ++k) {
                             const float base = (float)((sm.lo[k] >> shift) << shift);
                             if (ds[k] > 0.0f)
-                                ts = fminf(ts, t + (base + span - 0.01f - sm.s[k]) / ds[k]);
+                                ts = fminf(ts, t + (base + span - 0.01f - sm.s[k]) * rds[k]);
                             else if (ds[k] < 0.0f && base > 0.0f)
-                                ts = fminf(ts, t + (base + 0.01f - sm.s[k]) / ds[k]);
+                                ts = fminf(ts, t + (base + 0.01f - sm.s[k]) * rds[k]);
                         }
                         // this sample and every later one below ts: same float steps as the reference
                         const int ns = march_to(t, t_prev, stepsize, fminf(ts, tfar));
                         n_skip += ns;
                         guard += ns;
                         have_prev = false;
+                        have_flags = false;
                         n_brick++;
                         continue;
                     }
                 }
+                have_flags = false;
                 if (stepsize != a.step) {
                     // still on the large step: one sample at a time (raycast.cs:185-212)
                     f_tt = sample_eval(V, sm);
@@ -1573,7 +1579,8 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
                     continue;
                 }
                 // fine step: evaluate kBatch consecutive samples at once (their
-                // loads are independent), then consume them in march order.
+                // loads are independent) and fetch the flags of the sample after
+                // them, then consume the batch in march order.
                 float tj[kBatch], fv[kBatch];
                 tj[0] = t;
                 fv[0] = sample_eval(V, sm);
@@ -1583,6 +1590,19 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
                     Sample sj;
                     sample_setup(a, o[0] + d[0] * tj[j], o[1] + d[1] * tj[j], o[2] + d[2] * tj[j], sj);
                     fv[j] = sample_eval(V, sj);
+                }
+                const float t_next = tj[kBatch - 1] + stepsize;
+                if (a.skip && t_next < tfar) {
+                    // the next sample lies in the box: its lower corner is a
+                    // voxel of the volume (checked all the same)
+                    Sample sn;
+                    sample_setup(a, o[0] + d[0] * t_next, o[1] + d[1] * t_next, o[2] + d[2] * t_next, sn);
+                    const int b0 = sn.lo[0] >> 3, b1 = sn.lo[1] >> 3, b2 = sn.lo[2] >> 3;
+                    if (sn.valid && b0 < a.NB[0] && b1 < a.NB[1] && b2 < a.NB[2]) {
+                        cf_n = a.cflags[((b2 >> 2) * a.NC[1] + (b1 >> 2)) * a.NC[0] + (b0 >> 2)];
+                        ff_n = a.dflags[(b2 * a.NB[1] + b1) * a.NB[0] + b0];
+                        have_flags = true;
+                    }
                 }
                 bool stop = false;
 #pragma unroll
@@ -1606,7 +1626,7 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
                     }
                 }
                 if (stop) break;
-                t = tj[kBatch - 1] + stepsize;
+                t = t_next;
                 guard += kBatch - 1;
             }
             if (f_tt < 0.0f) {
@@ -1644,10 +1664,227 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
     }
 }
 
+// The same march with four lanes per ray (8x8-pixel tiles, 64 rays per
+// block).  The four lanes hold identical march state and make identical
+// decisions (occupancy flags, fast-forwards, the large step); on the fine
+// step they evaluate the next 16 samples together, lane q taking samples q,
+// q + 4, q + 8, q + 12, each at the t the reference's additions give, and a
+// two-step shuffle finds the first sample (in march order) that ends the
+// march.  A ray's chain of dependent loads is four times shorter and the
+// grid has four times as many wavefronts to hide latency.
+constexpr int kCoopB = 4;                 // samples per lane per round
+constexpr int kCoopN = 4 * kCoopB;        // consecutive samples per round
+
+template <class S>
+__device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const DevState *__restrict__ st, int blk)
+{
+    const int nb = a.tiles_x * a.tiles_y;
+    const int G = a.xcd_group;
+    const int j = blk / 8;
+    const int tile = ((blk % 8) + 8 * (j / G)) * G + (j % G);
+    if (tile >= nb) return;
+    const int q = threadIdx.x & 3, r = threadIdx.x >> 2;
+    const int gbase = (int)(threadIdx.x & 63) & ~3;  // the group's first lane in the wavefront
+    const int x = (tile % a.tiles_x) * 8 + (r & 7);
+    const int y = (tile / a.tiles_x) * 8 + (r >> 3);
+    if (x >= a.w || y >= a.h) return;  // the whole group
+
+    float view[16];
+    {
+        float iK[16], P[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
+        inverse_camera_matrix(a.cam, iK);
+        mul4(P, iK, view);  // m_pose * invK (gFusion.cpp:1298)
+    }
+    VolView<S> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
+                 (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32};
+
+    const float o[3] = {view[12], view[13], view[14]};
+    float d[3];
+    rotate3(view, (float)x, (float)y, 1.0f, d);
+    float tmin[3], tmax[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float invR = 1.0f / d[k];
+        const float tbot = (-1.0f * invR) * o[k];
+        const float ttop = invR * (a.dim[k] - o[k]);
+        tmin[k] = gmin(ttop, tbot);
+        tmax[k] = gmax(ttop, tbot);
+    }
+    const float largest_tmin = gmax(gmax(tmin[0], tmin[1]), gmax(tmin[0], tmin[2]));
+    const float smallest_tmax = gmin(gmin(tmax[0], tmax[1]), gmin(tmax[0], tmax[2]));
+    const float tnear = gmax(largest_tmin, a.near_plane);
+    const float tfar = gmin(smallest_tmax, a.far_plane);
+
+    bool hit = false;
+    float hp[3], th = 0.0f;
+    unsigned n_eval = 0, n_skip = 0, n_brick = 0;  // statistics mode only
+    if (tnear < tfar) {
+        float t = tnear, stepsize = a.large_step;
+        float f_t = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
+        float f_tt = 0.0f;
+        if (f_t >= 0.0f) {
+            const float ds[3] = {d[0] * (float)a.N[0] / a.dim[0], d[1] * (float)a.N[1] / a.dim[1],
+                                 d[2] * (float)a.N[2] / a.dim[2]};
+            // 1 / ds: the exit bound below is a skip limit with a 0.01-voxel
+            // margin, far wider than the ulp a reciprocal adds
+            const float rds[3] = {1.0f / ds[0], 1.0f / ds[1], 1.0f / ds[2]};
+            float t_prev = t;
+            bool have_prev = true;
+            n_eval++;
+            int guard = 0;
+            for (; t < tfar && guard < kMaxMarchSteps; ++guard) {
+                const bool can_skip = a.skip && stepsize == a.step;
+                Sample sm;
+                sample_setup(a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t, sm);
+                if (can_skip && sm.valid) {
+                    // empty-space skipping exactly as the one-lane march
+                    const int b0 = sm.lo[0] >> 3, b1 = sm.lo[1] >> 3, b2 = sm.lo[2] >> 3;
+                    uint8_t cf = 1, ff = 1;  // outside the brick grid: never skipped
+                    if (b0 < a.NB[0] && b1 < a.NB[1] && b2 < a.NB[2]) {
+                        cf = a.cflags[((b2 >> 2) * a.NC[1] + (b1 >> 2)) * a.NC[0] + (b0 >> 2)];
+                        ff = a.dflags[(b2 * a.NB[1] + b1) * a.NB[0] + b0];
+                    }
+                    const int shift = cf == 0 ? 5 : (ff == 0 ? 3 : 0);
+                    if (shift) {
+                        const float span = (float)(1 << shift);
+                        float ts = INFINITY;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const float base = (float)((sm.lo[k] >> shift) << shift);
+                            if (ds[k] > 0.0f)
+                                ts = fminf(ts, t + (base + span - 0.01f - sm.s[k]) * rds[k]);
+                            else if (ds[k] < 0.0f && base > 0.0f)
+                                ts = fminf(ts, t + (base + 0.01f - sm.s[k]) * rds[k]);
+                        }
+                        const int ns = march_to(t, t_prev, stepsize, fminf(ts, tfar));
+                        n_skip += ns;
+                        guard += ns;
+                        have_prev = false;
+                        n_brick++;
+                        continue;
+                    }
+                }
+                if (stepsize != a.step) {
+                    // the large step: one sample (raycast.cs:185-212), the same on every lane
+                    f_tt = sample_eval(V, sm);
+                    n_eval++;
+                    if (f_tt < 0.0f) break;
+                    if (f_tt < 0.8f) stepsize = a.step;
+                    f_t = f_tt;
+                    have_prev = true;
+                    t_prev = t;
+                    t += stepsize;
+                    continue;
+                }
+                // fine step: this lane's samples q, q + 4, q + 8, q + 12 of the
+                // next 16, at the t of the reference's sequential additions
+                float tb[kCoopB], fv[kCoopB];
+                {
+                    float tc = t;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        if (k < q) tc += stepsize;
+                    tb[0] = tc;
+#pragma unroll
+                    for (int b = 1; b < kCoopB; ++b) {
+                        float u = tb[b - 1];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) u += stepsize;
+                        tb[b] = u;
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < kCoopB; ++b) {
+                    Sample sb;
+                    sample_setup(a, o[0] + d[0] * tb[b], o[1] + d[1] * tb[b], o[2] + d[2] * tb[b], sb);
+                    fv[b] = sample_eval(V, sb);
+                }
+                int mine = kCoopN;  // first sample of this lane that ends the march
+#pragma unroll
+                for (int b = kCoopB - 1; b >= 0; --b)
+                    if (!(tb[b] < tfar) || fv[b] < 0.0f) mine = 4 * b + q;
+                int m = min(mine, __shfl_xor(mine, 1, 64));
+                m = min(m, __shfl_xor(m, 2, 64));
+                if (m == kCoopN) {
+                    // all 16 taken; the last (lane 3, slot 3) becomes f_t
+                    f_t = __shfl(fv[kCoopB - 1], gbase + 3, 64);
+                    t_prev = __shfl(tb[kCoopB - 1], gbase + 3, 64);
+                    have_prev = true;
+                    t = t_prev + stepsize;
+                    n_eval += kCoopN;
+                    guard += kCoopN - 1;
+                    continue;
+                }
+                const int mb = m >> 2;
+                const float tsel = mb == 0 ? tb[0] : mb == 1 ? tb[1] : mb == 2 ? tb[2] : tb[3];
+                const float fsel = mb == 0 ? fv[0] : mb == 1 ? fv[1] : mb == 2 ? fv[2] : fv[3];
+                const float tm = __shfl(tsel, gbase + (m & 3), 64);
+                const float fm = __shfl(fsel, gbase + (m & 3), 64);
+                if (m > 0) {
+                    const int pb = (m - 1) >> 2;
+                    const float tps = pb == 0 ? tb[0] : pb == 1 ? tb[1] : pb == 2 ? tb[2] : tb[3];
+                    const float fps = pb == 0 ? fv[0] : pb == 1 ? fv[1] : pb == 2 ? fv[2] : fv[3];
+                    t_prev = __shfl(tps, gbase + ((m - 1) & 3), 64);
+                    f_t = __shfl(fps, gbase + ((m - 1) & 3), 64);
+                    have_prev = true;
+                }
+                t = tm;
+                if (!(tm < tfar)) {  // the march ends before sample m
+                    n_eval += m;
+                    break;
+                }
+                f_tt = fm;  // < 0: got it
+                n_eval += m + 1;
+                break;
+            }
+            if (f_tt < 0.0f) {
+                if (!have_prev) f_t = interp_vol(V, a, o[0] + d[0] * t_prev, o[1] + d[1] * t_prev, o[2] + d[2] * t_prev);
+                t = t + stepsize * f_tt / (f_t - f_tt);
+                hp[0] = o[0] + d[0] * t;
+                hp[1] = o[1] + d[1] * t;
+                hp[2] = o[2] + d[2] * t;
+                th = t;
+                hit = true;
+            }
+        }
+    }
+    if (q != 0) return;
+    const int i = y * a.w + x;
+    if (hit && th > 0.0f) {
+        a.refV[i] = make_float4(hp[0], hp[1], hp[2], 1.0f);
+        float g[3];
+        gradient(V, a, hp, g);
+        const float len = sqrtf(dot3(g[0], g[1], g[2], g[0], g[1], g[2]));
+        if (len == 0.0f)
+            a.refN[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        else
+            a.refN[i] = make_float4(g[0] / len, g[1] / len, g[2] / len, 1.0f);
+    } else {
+        a.refV[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.refN[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    if (a.stats) {
+        atomicAdd(&a.stats[0], (unsigned long long)n_eval);
+        atomicAdd(&a.stats[1], (unsigned long long)n_skip);
+        atomicAdd(&a.stats[2], (unsigned long long)n_brick);
+        atomicAdd(&a.stats[3], (unsigned long long)(hit && th > 0.0f));
+        atomicMax(&a.stats[4], (unsigned long long)n_eval);
+        atomicMax(&a.stats[5], (unsigned long long)(n_eval + n_brick));
+    }
+}
+
 template <class S>
 __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *__restrict__ st)
 {
     raycast_body<S>(a, st, blockIdx.x);
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_raycast_coop(RaycastArgs a, const DevState *__restrict__ st)
+{
+    raycast_coop_body<S>(a, st, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1706,7 +1943,7 @@ __global__ __launch_bounds__(256) void k_batch_integrate(BatchTick t)
 
 __global__ __launch_bounds__(256) void k_batch_dilate(BatchTick t)
 {
-    const int b = blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);  // one wavefront per coarse cell
     if (b >= t.coa_prefix[t.n]) return;
     const int i = batch_find(t.coa_prefix, t.n, b);
     if (!batch_active(t, i)) return;
@@ -1723,6 +1960,15 @@ __global__ __launch_bounds__(256) void k_batch_raycast(BatchTick t, int per)
     if (!batch_active(t, i)) return;
     const RaycastArgs a = t.sess[i].ray;
     raycast_body<S>(a, t.sess[i].st, blockIdx.x % per);
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_batch_raycast_coop(BatchTick t, int per)
+{
+    const int i = blockIdx.x / per;
+    if (!batch_active(t, i)) return;
+    const RaycastArgs a = t.sess[i].ray;
+    raycast_coop_body<S>(a, t.sess[i].st, blockIdx.x % per);
 }
 
 // ---------------------------------------------------------------------------
@@ -1751,7 +1997,8 @@ hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, h
 {
     // 8 row groups x rows_step blocks; each block takes every rows_step-th row
     // of its group (fewer, fuller blocks: one release + one ticket each)
-    launch_timed(k_icp_iteration, dim3(8 * a.rows_step), dim3(kIcpThreads), s, t0, t1, a, st);
+    launch_timed(k_icp_iteration, dim3((a.group_only >= 0 ? 1 : 8) * a.rows_step), dim3(kIcpThreads), s, t0, t1, a,
+                 st);
     return hipGetLastError();
 }
 hipError_t launch_icp_finish(DevState *st, int frame, float *lib, int lib_max, hipStream_t s, hipEvent_t t1)
@@ -1815,13 +2062,22 @@ hipError_t launch_batch_integrate(const BatchTick &t, int storage, int fast, int
 
 hipError_t launch_batch_bricks(const BatchTick &t, int total_coarse, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_dilate, dim3((total_coarse + 255) / 256), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(k_batch_dilate, dim3((total_coarse + 3) / 4), dim3(256), 0, s, t);
     return hipGetLastError();
 }
 
-hipError_t launch_batch_raycast(const BatchTick &t, int storage, int per, hipStream_t s)
+hipError_t launch_batch_raycast(const BatchTick &t, int storage, int lanes, int per, hipStream_t s)
 {
     const dim3 g(t.n * per), b(256);
+    if (lanes == 4) {
+        switch (storage) {
+        case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_batch_raycast_coop<StoreRG16I>, g, b, 0, s, t, per); break;
+        case KF_STORAGE_F32: hipLaunchKernelGGL(k_batch_raycast_coop<StoreF32>, g, b, 0, s, t, per); break;
+        case KF_STORAGE_F16: hipLaunchKernelGGL(k_batch_raycast_coop<StoreF16>, g, b, 0, s, t, per); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (storage) {
     case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_batch_raycast<StoreRG16I>, g, b, 0, s, t, per); break;
     case KF_STORAGE_F32: hipLaunchKernelGGL(k_batch_raycast<StoreF32>, g, b, 0, s, t, per); break;
@@ -1865,7 +2121,7 @@ hipError_t launch_brick_rebuild(const BrickArgs &a, int storage, hipStream_t s)
 hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s)
 {
     const int nc = a.NC[0] * a.NC[1] * a.NC[2];
-    hipLaunchKernelGGL(k_brick_dilate, dim3((nc + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_brick_dilate, dim3((nc + 3) / 4), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1940,6 +2196,15 @@ hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st,
                           hipEvent_t t1)
 {
     const dim3 g(a.blocks), b(256);
+    if (a.lanes == 4) {
+        switch (storage) {
+        case KF_STORAGE_RG16I: launch_timed(k_raycast_coop<StoreRG16I>, g, b, s, t0, t1, a, st); break;
+        case KF_STORAGE_F32: launch_timed(k_raycast_coop<StoreF32>, g, b, s, t0, t1, a, st); break;
+        case KF_STORAGE_F16: launch_timed(k_raycast_coop<StoreF16>, g, b, s, t0, t1, a, st); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (storage) {
     case KF_STORAGE_RG16I: launch_timed(k_raycast<StoreRG16I>, g, b, s, t0, t1, a, st); break;
     case KF_STORAGE_F32: launch_timed(k_raycast<StoreF32>, g, b, s, t0, t1, a, st); break;

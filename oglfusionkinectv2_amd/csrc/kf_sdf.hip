@@ -134,54 +134,55 @@ __device__ __forceinline__ int sdf_slot(const SdfArgs &a, int idx)
 
 // Eigen::LDLT<MatrixXd>(A).solve(b) (gFusion.cpp:918): symmetric pivoting on
 // the largest remaining |diagonal|, D pseudo-inverse with threshold DBL_MIN,
-// inner products summed in index order.
-__device__ void sdf_ldlt_solve(double (&m)[6][6], const double (&b)[6], double (&x)[6])
+// inner products summed in index order.  One thread; m (36, row-major), y, t
+// (6 each) and tr (6) are LDS scratch, indexed at run time (pivots) without
+// spilling to private memory.  On return y holds the solution.
+__device__ void sdf_ldlt_solve(double *m, double *y, double *t, int *tr, const double (&b)[6])
 {
-    int tr[6];
+#define M_(r, c) m[(r) * 6 + (c)]
     for (int k = 0; k < 6; ++k) {
         int big = k;
-        double bv = fabs(m[k][k]);
+        double bv = fabs(M_(k, k));
         for (int i = k + 1; i < 6; ++i)
-            if (fabs(m[i][i]) > bv) {
-                bv = fabs(m[i][i]);
+            if (fabs(M_(i, i)) > bv) {
+                bv = fabs(M_(i, i));
                 big = i;
             }
         tr[k] = big;
         if (big != k) {
             for (int j = 0; j < k; ++j) {
-                const double t = m[k][j];
-                m[k][j] = m[big][j];
-                m[big][j] = t;
+                const double u = M_(k, j);
+                M_(k, j) = M_(big, j);
+                M_(big, j) = u;
             }
             for (int i = big + 1; i < 6; ++i) {
-                const double t = m[i][k];
-                m[i][k] = m[i][big];
-                m[i][big] = t;
+                const double u = M_(i, k);
+                M_(i, k) = M_(i, big);
+                M_(i, big) = u;
             }
             {
-                const double t = m[k][k];
-                m[k][k] = m[big][big];
-                m[big][big] = t;
+                const double u = M_(k, k);
+                M_(k, k) = M_(big, big);
+                M_(big, big) = u;
             }
             for (int i = k + 1; i < big; ++i) {
-                const double t = m[i][k];
-                m[i][k] = m[big][i];
-                m[big][i] = t;
+                const double u = M_(i, k);
+                M_(i, k) = M_(big, i);
+                M_(big, i) = u;
             }
         }
         if (k > 0) {
-            double temp[6];
-            for (int j = 0; j < k; ++j) temp[j] = m[j][j] * m[k][j];
-            double s = 0.0;
-            for (int j = 0; j < k; ++j) s += m[k][j] * temp[j];
-            m[k][k] -= s;
+            for (int j = 0; j < k; ++j) t[j] = M_(j, j) * M_(k, j);
+            double sum = 0.0;
+            for (int j = 0; j < k; ++j) sum += M_(k, j) * t[j];
+            M_(k, k) -= sum;
             for (int i = k + 1; i < 6; ++i) {
-                double t = 0.0;
-                for (int j = 0; j < k; ++j) t += m[i][j] * temp[j];
-                m[i][k] -= t;
+                double u = 0.0;
+                for (int j = 0; j < k; ++j) u += M_(i, j) * t[j];
+                M_(i, k) -= u;
             }
         }
-        const double akk = m[k][k];
+        const double akk = M_(k, k);
         const bool valid = fabs(akk) > 0.0;
         if (k == 0 && !valid) {
             // the whole diagonal is zero (or NaN): Eigen stops with the
@@ -190,32 +191,31 @@ __device__ void sdf_ldlt_solve(double (&m)[6][6], const double (&b)[6], double (
             break;
         }
         if (valid)
-            for (int i = k + 1; i < 6; ++i) m[i][k] /= akk;
+            for (int i = k + 1; i < 6; ++i) M_(i, k) /= akk;
     }
-    double y[6];
     for (int i = 0; i < 6; ++i) y[i] = b[i];
     for (int k = 0; k < 6; ++k) {
-        const double t = y[k];
+        const double u = y[k];
         y[k] = y[tr[k]];
-        y[tr[k]] = t;
+        y[tr[k]] = u;
     }
     for (int i = 0; i < 6; ++i) {
-        double s = y[i];
-        for (int j = 0; j < i; ++j) s -= m[i][j] * y[j];
-        y[i] = s;
+        double sum = y[i];
+        for (int j = 0; j < i; ++j) sum -= M_(i, j) * y[j];
+        y[i] = sum;
     }
-    for (int i = 0; i < 6; ++i) y[i] = (fabs(m[i][i]) > DBL_MIN) ? y[i] / m[i][i] : 0.0;
+    for (int i = 0; i < 6; ++i) y[i] = (fabs(M_(i, i)) > DBL_MIN) ? y[i] / M_(i, i) : 0.0;
     for (int i = 5; i >= 0; --i) {
-        double s = y[i];
-        for (int j = i + 1; j < 6; ++j) s -= m[j][i] * y[j];
-        y[i] = s;
+        double sum = y[i];
+        for (int j = i + 1; j < 6; ++j) sum -= M_(j, i) * y[j];
+        y[i] = sum;
     }
     for (int k = 5; k >= 0; --k) {
-        const double t = y[k];
+        const double u = y[k];
         y[k] = y[tr[k]];
-        y[tr[k]] = t;
+        y[tr[k]] = u;
     }
-    for (int i = 0; i < 6; ++i) x[i] = y[i];
+#undef M_
 }
 
 // Twist(xi).exp() (gFusion.cpp:810-833): closed-form SE(3) exponential of the
@@ -292,6 +292,8 @@ __global__ __launch_bounds__(kSdfThreads) void k_sdf_track(SdfArgs a, DevState *
     __shared__ double gsum[8][kSdfTerms];
     __shared__ float T[16], prev[16];
     __shared__ double result[6], result_prev[6];
+    __shared__ double solve_m[36], solve_y[6], solve_t[6];
+    __shared__ int solve_tr[6];
     __shared__ int converged, sh_abort;
 
     const int blk = blockIdx.x, nblk = gridDim.x;
@@ -455,28 +457,30 @@ __global__ __launch_bounds__(kSdfThreads) void k_sdf_track(SdfArgs a, DevState *
                         count = (double)((h > 7) ? (h - 7 + 7) / 8 : 0) * (double)w;
                     }
                     const float ae = sqrtf((float)v[0] / (float)count);
-                    double m[6][6], bb[6], dx[6];
+                    double bb[6];
                     {
                         int q = 7;
+#pragma unroll
                         for (int p = 0; p < 6; ++p)
+#pragma unroll
                             for (int s2 = p; s2 < 6; ++s2) {
-                                m[p][s2] = m[s2][p] = (double)(float)v[q];
+                                solve_m[p * 6 + s2] = solve_m[s2 * 6 + p] = (double)(float)v[q];
                                 ++q;
                             }
                     }
-                    double mx = m[0][0];
-                    for (int r = 0; r < 6; ++r)
-                        for (int c = 0; c < 6; ++c)
-                            if (m[r][c] > mx) mx = m[r][c];
+                    double mx = solve_m[0];
+                    for (int k = 1; k < 36; ++k)
+                        if (solve_m[k] > mx) mx = solve_m[k];
                     const double scaling = 1.0 / mx;
-                    for (int r = 0; r < 6; ++r)
-                        for (int c = 0; c < 6; ++c) m[r][c] *= scaling;
+                    for (int k = 0; k < 36; ++k) solve_m[k] *= scaling;
+#pragma unroll
                     for (int k = 0; k < 6; ++k) bb[k] = (double)(float)v[1 + k] * scaling;
-                    for (int k = 0; k < 6; ++k) m[k][k] += (double)it * 0.01;
-                    sdf_ldlt_solve(m, bb, dx);
+                    for (int k = 0; k < 6; ++k) solve_m[k * 6 + k] += (double)it * 0.01;
+                    sdf_ldlt_solve(solve_m, solve_y, solve_t, solve_tr, bb);
                     double cn = 0.0, r[6];
+#pragma unroll
                     for (int k = 0; k < 6; ++k) {
-                        r[k] = result[k] - dx[k];
+                        r[k] = result[k] - solve_y[k];
                         const double d = r[k] - result_prev[k];
                         cn += d * d;
                     }

@@ -40,6 +40,8 @@ struct kf_session {
     int rebuild_period = 16;
     int xcd_group = 4;
     int icp_rows_per_block = 2;
+    int ray_lanes = 1;       // raycast lanes per ray (1 or 4)
+    int icp_all_groups = 0;  // ICP sums every row group, not only the one the combine rule reads
     // frame graph: ICP + gate + integrate + raycast, captured once
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -132,6 +132,26 @@ def test_track_pixels_and_partials(o32, seq20):
         assert abs(g.alignmentEnergy() - ae) <= 1e-4 * max(ae, 1e-6)
 
 
+def test_icp_group_7_only_equals_all_groups(seq20):
+    """Under the last-row rule only row group 7 reaches the solve: summing that
+    group alone (the default) gives the same bits as summing all eight
+    (KF_OPT_ICP_ALL_GROUPS) -- pose, AE, partial row 7, TSDF and maps over a
+    frame sequence."""
+    print("case: icp group 7 only vs all groups", flush=True)
+    depth, gt = seq20
+    out = []
+    for allg in (0, 1):
+        g, _ = make(fast=1)
+        g.setOption(K.OPT_ICP_ALL_GROUPS, allg)
+        g.Reset(gt[0])
+        for f in range(10):
+            g.frame(depth[f].astype(np.float32))
+        out.append((g.getPose(), g.readPartials()[7], g.alignmentEnergy(), g.getVolume()[0], g.getVerts()))
+        g.close()
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("rule", [0, 1])
 def test_track_full_matches_oracle(o32, seq20, rule):
     print("case: Track() rule %d" % rule, flush=True)
@@ -367,6 +387,29 @@ def test_raycast_skip_is_exact(o32, seq20, storage):
     np.testing.assert_array_equal(g.getVerts(), full_v)
     np.testing.assert_array_equal(g.getNorms(), full_n)
     assert (full_v[..., 3] > 0).mean() > 0.5
+
+
+@pytest.mark.parametrize("storage", ["rg16i", "f32", "f16"])
+def test_raycast_four_lanes_equal_one_lane(o32, seq20, storage):
+    """Marching a ray with four cooperating lanes (16 fine-step samples per
+    round) must give the bits of the one-lane march, with and without
+    empty-space skipping."""
+    print("case: raycast 4 lanes vs 1 lane %s" % storage, flush=True)
+    s = _oracle_state(o32, seq20, storage)
+    g, cfg = make(storage=storage)
+    g.Reset(np.eye(4, dtype=np.float32))
+    g.setVolume(*oracle_volume_planes(s.volume(), storage))
+    g.setPose(s.pose())
+    for skip in (0, 1):
+        g.setOption(K.OPT_SKIP_EMPTY, skip)
+        out = []
+        for lanes in (1, 4):
+            g.setOption(K.OPT_RAYCAST_LANES, lanes)
+            g.raycast()
+            out.append((g.getVerts(), g.getNorms()))
+        np.testing.assert_array_equal(out[0][0], out[1][0])
+        np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert (out[0][0][..., 3] > 0).mean() > 0.5
 
 
 def test_brick_flags_cover_every_negative_voxel(seq20):
