@@ -88,8 +88,8 @@ def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, count_every=25):
     integrate and raycast launches carry start / stop events in their own
     dispatch packets, so each figure is that kernel's duration; ICP is its
     first iteration's start to the finish kernel's end).  Every `count_every`
-    frames the integrate traffic of that frame is counted (a separate pass
-    after the frame)."""
+    frames the integrate and raycast traffic of that frame is counted
+    (separate passes after the frame)."""
     import ctypes as C
     from oglfusionkinectv2_amd import _lib as K
     s = g.session
@@ -101,15 +101,16 @@ def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, count_every=25):
         K.check(L.kf_frame_u16_dev(s, C.c_void_p(base + i * fb)))
     torch.cuda.synchronize()
     acc = {n: [] for n in ("pyramid", "icp", "integrate", "raycast")}
-    counts = []
+    counts, ray_counts = [], []
     for i in range(steps):
         K.check(L.kf_frame_u16_dev(s, C.c_void_p(base + (warmup + i) * fb)))
         for n, v in g.stageTimes().items():
             acc[n].append(v)
         if i % count_every == count_every - 1:
             counts.append(g.countIntegrateTraffic())
+            ray_counts.append(g.countRaycastTraffic())
     g.setOption(K.OPT_STAGE_TIMING, 0)
-    return {n: float(np.mean(v)) for n, v in acc.items()}, counts
+    return {n: float(np.mean(v)) for n, v in acc.items()}, counts, ray_counts
 
 
 def row_timings(torch, L, g, frames_dev, nframes, reps=3):
@@ -146,6 +147,35 @@ def row_timings(torch, L, g, frames_dev, nframes, reps=3):
     out["mesh_triangles"] = int(ntri)
     out["intensity_projection_ms"], _ = best(lambda: K.check(L.kf_intensity_projection(s)))
     return out
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "frame_loop_512_f32_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of each kernel at 512^3 FP32 from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines `traffic_bytes_per_launch
+    <kernel> <bytes>`, already scaled by the copy calibration)."""
+    out = {}
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 3 and p[0] == "traffic_bytes_per_launch":
+                    out[p[1]] = int(p[2])
+    except OSError:
+        pass
+    return out
+
+
+def roofline(kernel, nbytes, ms, traffic):
+    """HBM roofline of one kernel: algorithmic bytes per launch over its mean
+    launch duration (HIP events in the dispatch packets on the session stream)."""
+    ach = nbytes / (ms * 1e-3) / 1e9
+    key = kernel.replace(" ", "")
+    return {"kernel": kernel, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": traffic.get(key), "algorithmic_bytes": int(nbytes),
+            "avg_ms": ms, "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if key in traffic else None}
 
 
 def cpu_baseline(depth, poses, volume, frames, warm=2):
@@ -221,7 +251,7 @@ def main():
     st = g.status()
     est = g.getPose()
     ate_gt = float(np.linalg.norm(est[:3, 3] - poses[nframes - 1][:3, 3]))
-    kern, counts = run_kernels(torch, L, g, frames_dev, poses, args.warmup, args.steps)
+    kern, counts, ray_counts = run_kernels(torch, L, g, frames_dev, poses, args.warmup, args.steps)
     # integrate traffic under the reference semantics (sampled frames): each
     # read voxel moves value + weight in and out, each written-only voxel out;
     # plus the depth image in metres
@@ -229,6 +259,11 @@ def main():
     vr = float(np.mean([c[0] for c in counts])) if counts else 0.0
     vw = float(np.mean([c[1] for c in counts])) if counts else 0.0
     int_bytes = (vr + vw) * bv + 424 * 512 * 4
+    # raycast: distinct voxel values (4 B) and occupancy-flag bytes read, plus
+    # the vertex and normal maps written (2 x float4 per pixel)
+    rv = float(np.mean([c[0] for c in ray_counts])) if ray_counts else 0.0
+    rf = float(np.mean([c[1] for c in ray_counts])) if ray_counts else 0.0
+    ray_bytes = rv * 4 + rf + 424 * 512 * 32
     dom = max(kern, key=kern.get)
     extras = row_timings(torch, L, g, frames_dev, args.warmup + args.steps)
     g.close()
@@ -239,7 +274,7 @@ def main():
         g16 = make_session(args.volume, "f16", stream.cuda_stream, args.fast_math)
         e16, lat16 = run_frames(torch, L, g16, frames_dev, poses, args.warmup, args.steps, stream)
         fps16 = args.steps / e16
-        kern16, _ = run_kernels(torch, L, g16, frames_dev, poses, args.warmup, args.steps)
+        kern16, _, _ = run_kernels(torch, L, g16, frames_dev, poses, args.warmup, args.steps)
         g16.close()
 
     # --- multi-session -----------------------------------------------------
@@ -255,18 +290,12 @@ def main():
                "sample": "frames 2-%d of the same %d^3 FP32 sequence (frames 0-1 untimed), C oracle, "
                          "%d OpenMP threads, %.1f s" % (args.cpu_frames + 1, args.volume, threads, cdt)}
 
-    int_ms = kern["integrate"]
-    roof = {
-        "kernel": "k_integrate<StoreF32>",
-        "bound": "hbm",
-        "achieved": int_bytes / (int_ms * 1e-3) / 1e9,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": int_bytes / (int_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "traffic": None,
-        "algorithmic_bytes": int(int_bytes),
-        "avg_ms": int_ms,
+    traffic = measured_traffic() if args.volume == 512 else {}
+    rooflines = {
+        "raycast": roofline("k_raycast<StoreF32>", ray_bytes, kern["raycast"], traffic),
+        "integrate": roofline("k_integrate_fast<StoreF32, true>", int_bytes, kern["integrate"], traffic),
     }
+    roof = rooflines.get(dom) or rooflines["raycast"]
     pf = np.array(lat)
     out = {
         "metric": "frames/s per GPU at 512^3 TSDF (FP32 TSDF+FP32 weight, single session)",
@@ -294,6 +323,7 @@ def main():
         "fast_math": args.fast_math,
         "dominant_kernel": dom,
         "roofline": roof,
+        "roofline_by_kernel": rooflines,
         "cpu_baseline": cpu,
         "beside_the_loop": extras,
         "multi_session": multi,

@@ -231,6 +231,18 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         4 (the four evaluate 16 consecutive fine-step
  *                         samples per round).  Same results either way. */
 #define KF_OPT_RAYCAST_LANES 11
+/*   KF_OPT_ICP_PERSISTENT 1: Track() runs as one persistent launch (every
+ *                         level and iteration, grid barriers between them);
+ *                         0: one launch per iteration.  Same results.
+ *                         Default 1. */
+#define KF_OPT_ICP_PERSISTENT 12
+/*   KF_OPT_ICP_SOLO_ROWS  at a level whose row groups have at most this many
+ *                         rows one block takes all rows of a group; when
+ *                         only one group is summed (last-row rule) the
+ *                         persistent launch runs that level in that block
+ *                         alone, with no grid barrier.  0 = off (default:
+ *                         one 256-thread block is slower than a barrier). */
+#define KF_OPT_ICP_SOLO_ROWS 13
 /*   KF_OPT_TRACK_MODE     tracking of the frame loop (kinect.h:143-144):
  *                         KF_TRACK_POINT_TO_PLANE (Track, default),
  *                         KF_TRACK_POINT_TO_VOLUME (TrackSDF) or both (Track
@@ -264,6 +276,17 @@ int kf_count_integrate_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *v
  * as empty space, empty cells entered, rays that hit, the largest number of
  * samples one ray evaluated, and the largest evaluated + cells of one ray. */
 int kf_raycast_stats(kf_session *s, uint64_t stats[6]);
+
+/* The same with per-ray figures: counts_host[y * width + x] = samples the ray
+ * evaluated (low 16 bits) | empty cells it entered (high 16 bits), saturated
+ * at 65535. */
+int kf_raycast_ray_counts(kf_session *s, uint32_t *counts_host);
+
+/* Algorithmic traffic of raycast() for the current volume and pose: the
+ * number of distinct voxels whose value the march and the normals read, and
+ * of distinct occupancy-flag bytes it read (no timing; the maps written are
+ * identical to raycast()'s). */
+int kf_count_raycast_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *flag_bytes_read);
 
 /* ---- input adapters --------------------------------------------------------
  * camPams of kinect.h (the depth camera's intrinsics and distortion). */

@@ -45,6 +45,8 @@ OPT_TRACK_MODE = 8
 OPT_RECOVER = 9
 OPT_ICP_ALL_GROUPS = 10
 OPT_RAYCAST_LANES = 11
+OPT_ICP_PERSISTENT = 12
+OPT_ICP_SOLO_ROWS = 13
 
 TRACK_POINT_TO_PLANE = 1
 TRACK_POINT_TO_VOLUME = 2
@@ -132,6 +134,8 @@ def lib():
         "kf_rebuild_bricks": [sess],
         "kf_read_stage_times": [sess, C.POINTER(C.c_float)],
         "kf_raycast_stats": [sess, C.POINTER(C.c_uint64)],
+        "kf_raycast_ray_counts": [sess, C.POINTER(C.c_uint32)],
+        "kf_count_raycast_traffic": [sess, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
         "kf_track_sdf": [sess, C.POINTER(C.c_int)],
         "kf_recover_pose": [sess, C.POINTER(C.c_int)],
         "kf_read_pose_library": [sess, f32p, i32, C.POINTER(C.c_int)],

@@ -197,7 +197,8 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated)
     return a;
 }
 
-kf::RaycastArgs raycast_args(kf_session *s)
+// lanes: rays per lane group (1 or 4); -1 takes the session's option
+kf::RaycastArgs raycast_args(kf_session *s, int lanes = -1)
 {
     const kf_config &c = s->cfg;
     kf::RaycastArgs a{};
@@ -216,7 +217,7 @@ kf::RaycastArgs raycast_args(kf_session *s)
     a.tsdf = s->tsdf;
     a.w = c.width;
     a.h = c.height;
-    a.lanes = s->ray_lanes;
+    a.lanes = lanes < 0 ? s->ray_lanes : lanes;
     const int tile = a.lanes == 4 ? 8 : 16;  // 256 threads: 256 or 64 rays
     a.tiles_x = (c.width + tile - 1) / tile;
     a.tiles_y = (c.height + tile - 1) / tile;
@@ -306,6 +307,9 @@ kf::IcpArgs icp_args(kf_session *s, int level, int first, int save_pose)
     // getReduction's last-row rule reads only row group 7 (gFusion.cpp:1143-1149):
     // the other groups' sums are not needed unless they are to be inspected
     a.group_only = (s->cfg.reduce_rule == KF_RULE_LAST_ROW && !s->icp_all_groups && !s->debug) ? 7 : -1;
+    // small row groups: one block takes all rows of a group (the same
+    // partial rows whether one group or all eight are summed)
+    if (rows_per_group <= s->icp_solo_rows) a.rows_step = 1;
     return a;
 }
 
@@ -331,6 +335,26 @@ int run_icp(kf_session *s, int level, int first, int save_pose, hipEvent_t t0 = 
 // The last launch also applies the accept / reject (and the frame gate).
 int run_track(kf_session *s, int frame)
 {
+    if (s->icp_persistent && !s->debug) {
+        // one launch for every level and iteration when its grid fits the chip
+        kf::IcpTrackArgs a{};
+        int nblk = 0, total = 0;
+        for (int l = 0; l < 3; ++l) {
+            a.lv[l] = icp_args(s, l, 0, 0);
+            a.iters[l] = s->cfg.iterations[l];
+            total += a.iters[l];
+            if (a.iters[l] > 0) nblk = std::max(nblk, kf::icp_blocks(a.lv[l]));
+        }
+        if (total > 0 && nblk <= s->cus) {
+            a.frame = frame;
+            a.partials[0] = s->block_partials;
+            a.partials[1] = s->block_partials + (size_t)(s->cfg.height + 8) * 32;
+            a.spin_limit = 1u << 22;
+            hipError_t e = kf::launch_icp_track(a, s->st, nblk, s->stream, tev(s, 2), tev(s, 3));
+            if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("icp launch: ") + hipGetErrorString(e));
+            return KF_OK;
+        }
+    }
     int last_level = -1;  // levels run 2 -> 0: the last one with iterations
     for (int level = 0; level <= 2 && last_level < 0; ++level)
         if (s->cfg.iterations[level] > 0) last_level = level;
@@ -574,7 +598,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
               alloc((void **)&s->stage_f32, p0 * 4) && alloc((void **)&s->refV, p0 * 16) &&
               alloc((void **)&s->refN, p0 * 16) && alloc(&s->tsdf, vb) && alloc(&s->weight, vb) &&
               alloc((void **)&s->st, sizeof(kf::DevState)) &&
-              alloc((void **)&s->block_partials, (size_t)(cfg->height + 8) * 32 * 4) &&
+              alloc((void **)&s->block_partials, 2 * (size_t)(cfg->height + 8) * 32 * 4) &&
               alloc((void **)&s->counts, 8 * sizeof(unsigned long long)) &&
               alloc((void **)&s->library, (size_t)kf::kPoseLibraryMax * 64) &&
               alloc((void **)&s->sdf_partials, 2 * 256 * 32 * 4) &&
@@ -612,6 +636,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || cus < 8)
             return cleanup(KF_ERR_HIP, "cannot query the compute unit count");
         s->sdf_blocks = (cus < 256 ? cus : 256) & ~7;
+        s->cus = cus;
     }
     const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     if (!z || kf::launch_state_set(s->st, I, kf::kStateResetPose, s->stream) != hipSuccess ||
@@ -880,6 +905,11 @@ int kf_set_option(kf_session *s, int option, int value)
         s->track_mode = value;
         break;
     case KF_OPT_RECOVER: s->recover = value ? 1 : 0; break;
+    case KF_OPT_ICP_PERSISTENT: s->icp_persistent = value ? 1 : 0; break;
+    case KF_OPT_ICP_SOLO_ROWS:
+        if (value < 0) return fail(KF_ERR_INVALID, "solo rows must be >= 0");
+        s->icp_solo_rows = value;
+        break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     drop_graph(s);  // captured launches carry the old arguments
@@ -901,6 +931,8 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_ICP_ALL_GROUPS: *value = s->icp_all_groups; break;
     case KF_OPT_RAYCAST_LANES: *value = s->ray_lanes; break;
     case KF_OPT_RECOVER: *value = s->recover; break;
+    case KF_OPT_ICP_PERSISTENT: *value = s->icp_persistent; break;
+    case KF_OPT_ICP_SOLO_ROWS: *value = s->icp_solo_rows; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     return KF_OK;
@@ -987,6 +1019,7 @@ int kf_read_status(kf_session *s, int *tracked, int *iterations_run, int *counte
     kf::DevState h;
     KF_HIP(hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
     KF_HIP(hipStreamSynchronize(s->stream));
+    if (h.icp_error) return fail(KF_ERR_STATE, "a Track() grid barrier timed out (cleared by kf_reset)");
     if (tracked) *tracked = h.tracked;
     if (iterations_run) *iterations_run = h.iters_run;
     if (counter) *counter = h.counter;
@@ -1050,6 +1083,65 @@ int kf_raycast_stats(kf_session *s, uint64_t stats[6])
     KF_HIP(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, s->stream));
     KF_HIP(hipStreamSynchronize(s->stream));
     for (int k = 0; k < 6; ++k) stats[k] = h[k];
+    return KF_OK;
+}
+
+int kf_raycast_ray_counts(kf_session *s, uint32_t *counts_host)
+{
+    if (!s || !counts_host) return fail(KF_ERR_INVALID, "NULL argument");
+    const size_t np = (size_t)s->cfg.width * s->cfg.height;
+    uint32_t *d = nullptr;
+    KF_HIP(hipMalloc(&d, np * sizeof(uint32_t)));
+    kf::RaycastArgs a = raycast_args(s, 1);
+    a.stats = s->counts;
+    a.ray_counts = d;
+    hipError_t e = hipMemsetAsync(s->counts, 0, 6 * sizeof(unsigned long long), s->stream);
+    if (e == hipSuccess) e = kf::launch_raycast(a, s->cfg.storage, s->st, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts_host, d, np * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    const hipError_t ef = hipFree(d);
+    KF_HIP(e);
+    KF_HIP(ef);
+    return KF_OK;
+}
+
+int kf_count_raycast_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *flag_bytes_read)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    const int *N = s->cfg.volume_size;
+    const size_t nvox = (size_t)N[0] * N[1] * N[2];
+    const size_t nflag = (size_t)s->NB[0] * s->NB[1] * s->NB[2] + (size_t)s->NC[0] * s->NC[1] * s->NC[2];
+    const size_t words = (nvox + nflag + 31) / 32, whole = nvox / 32;
+    uint32_t *bits = nullptr;
+    KF_HIP(hipMalloc(&bits, words * sizeof(uint32_t)));
+    kf::RaycastArgs a = raycast_args(s, 1);
+    a.touch = bits;
+    unsigned long long h = 0;
+    std::vector<uint32_t> tail(words - whole);
+    // the words holding only voxel bits are counted on the device, the rest
+    // (the last partial voxel word and the flag bits) on the host
+    hipError_t e = hipMemsetAsync(bits, 0, words * sizeof(uint32_t), s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->counts, 0, sizeof(unsigned long long), s->stream);
+    if (e == hipSuccess) e = kf::launch_raycast_touch(a, s->cfg.storage, s->st, whole, s->counts, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, s->counts, sizeof(h), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(tail.data(), bits + whole, tail.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    const hipError_t ef = hipFree(bits);
+    KF_HIP(e);
+    KF_HIP(ef);
+    uint64_t vox = h, flags = 0;
+    for (size_t k = 0; k < tail.size(); ++k)
+        for (int b = 0; b < 32; ++b)
+            if ((tail[k] >> b) & 1u) {
+                if ((whole + k) * 32 + (size_t)b < nvox)
+                    ++vox;
+                else
+                    ++flags;
+            }
+    if (voxels_read) *voxels_read = vox;
+    if (flag_bytes_read) *flag_bytes_read = flags;
     return KF_OK;
 }
 
@@ -1126,7 +1218,7 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
             si->fast_math != s0->fast_math || si->icp_rows_per_block != s0->icp_rows_per_block ||
             si->xcd_group != s0->xcd_group || si->device != s0->device ||
             si->icp_all_groups != s0->icp_all_groups || si->cfg.reduce_rule != s0->cfg.reduce_rule ||
-            si->ray_lanes != s0->ray_lanes ||
+            si->ray_lanes != s0->ray_lanes || si->icp_solo_rows != s0->icp_solo_rows ||
             std::memcmp(si->cfg.iterations, s0->cfg.iterations, sizeof(s0->cfg.iterations)) != 0)
             return fail(KF_ERR_INVALID, "batched sessions must share depth size, ICP schedule, storage and options");
         if (si->track_mode != KF_TRACK_POINT_TO_PLANE)

@@ -45,7 +45,63 @@ struct DevState {
     unsigned int sdf_exit;    // its exit ticket (the last block clears the barrier)
     int sdf_abort;            // set by a block whose barrier wait timed out
     double sdf_log[64][10];   // per iteration: level, iteration, AE, |step|, result[6]
+    int icp_error;            // a grid barrier of the persistent Track() kernel timed out
+    unsigned int icp_bar;     // its grid barrier arrivals
+    unsigned int icp_exit;    // its exit ticket
+    int icp_abort;            // set by a block whose barrier wait timed out
 };
+
+// Grid barrier of a persistent launch whose blocks are all resident at once:
+// arrivals are counted in *bar (this barrier is complete at `target`), the
+// wait is bounded by `spin_limit` polls, after which the launch's abort flag
+// is raised; a raised flag releases every waiting block.  Returns false when
+// the launch is being aborted.  All threads of the block call it.
+__device__ __forceinline__ bool grid_barrier(unsigned int *bar, int *abort_flag, unsigned int target,
+                                             unsigned int spin_limit, int *sh_abort)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();  // release the block's stores
+        atomicAdd(bar, 1u);
+        unsigned int n = 0;
+        int ab = 0;
+        for (;;) {
+            if (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ab = 1;
+                break;
+            }
+            if (++n > spin_limit) {
+                __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ab = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        *sh_abort = ab;
+    }
+    __syncthreads();
+    __threadfence();  // acquire the other blocks' stores
+    return *sh_abort == 0;
+}
+
+// Exit ticket of such a launch: the last block out clears the barrier state
+// for the next launch.  All threads of the block call it.
+__device__ __forceinline__ void grid_exit(unsigned int *bar, unsigned int *exit_count, int *abort_flag,
+                                          unsigned int nblk)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned int t = atomicAdd(exit_count, 1u);
+        if (t == nblk - 1u) {
+            *bar = 0u;
+            *abort_flag = 0;
+            __threadfence();
+            *exit_count = 0u;
+        }
+    }
+}
 
 struct Mat16 {
     float m[16];

@@ -42,6 +42,16 @@ struct IcpArgs {
     float *dbg_error, *dbg_J;
 };
 
+// Track() in one persistent launch (k_icp_track): every level's arguments,
+// its iteration count, and two partial-row buffers used alternately.
+struct IcpTrackArgs {
+    IcpArgs lv[3];
+    int iters[3];
+    int frame;                  // apply the frame gate at the end
+    float *partials[2];
+    unsigned spin_limit;        // grid-barrier wait bound (polls)
+};
+
 struct IntegrateArgs {
     int N[3];
     float dim[3];
@@ -84,6 +94,8 @@ struct RaycastArgs {
     int off32;                  // voxel count < 2^32 (32-bit offsets)
     int xcd_group;              // tiles per XCD cluster (block -> tile mapping)
     int lanes;                  // lanes per ray: 1 (16x16-pixel tiles) or 4 (8x8-pixel tiles)
+    uint32_t *ray_counts;       // optional (with stats): per pixel, evaluated samples | skip events << 16
+    uint32_t *touch;            // k_raycast_touch: bitmap of voxels, then flag bytes, read
 };
 
 // Point-to-volume tracking (persistent k_sdf_track, one launch per TrackSDF /
@@ -153,6 +165,12 @@ hipError_t launch_pyramid_f32(const PyramidArgs &a, const float *in, hipStream_t
                               hipEvent_t t1 = nullptr);
 hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, hipEvent_t t0 = nullptr,
                                 hipEvent_t t1 = nullptr);
+// blocks of a k_icp_iteration launch for these arguments
+__host__ __device__ inline int icp_blocks(const IcpArgs &a) { return (a.group_only >= 0 ? 1 : 8) * a.rows_step; }
+// one launch of nblk blocks (nblk >= icp_blocks of every level with
+// iterations, all resident at once)
+hipError_t launch_icp_track(const IcpTrackArgs &a, DevState *st, int nblk, hipStream_t s, hipEvent_t t0,
+                            hipEvent_t t1);
 hipError_t launch_icp_finish(DevState *st, int frame, float *lib, int lib_max, hipStream_t s,
                              hipEvent_t t1 = nullptr);
 // state flags of launch_state_set (kStateSetPose is implied by a non-NULL pose)
@@ -166,6 +184,9 @@ hipError_t launch_count_integrate(const IntegrateArgs &a, const DevState *st, un
                                   hipStream_t s);
 hipError_t launch_intensity_projection(const RaycastArgs &a, int storage, const DevState *st, float4 *out,
                                        hipStream_t s);
+// k_raycast_touch then the population count of its `words`-word bitmap into *count
+hipError_t launch_raycast_touch(const RaycastArgs &a, int storage, const DevState *st, size_t words,
+                                unsigned long long *count, hipStream_t s);
 hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s,
                           hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 

@@ -214,6 +214,72 @@ __device__ __forceinline__ bool ldlt6(const double (&A)[36], const double (&b)[6
     return true;
 }
 
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// ldlt6 spread over a wavefront: lane i < 6 holds row i of A (Arow) and b[i]
+// (bi); every lane returns the same x.  Each element is formed by the same
+// operations in the same order as in ldlt6 (so the result has the same bits),
+// but the rows of a column are divided in parallel: six dependent divisions
+// instead of twenty-one.  All 64 lanes call it.
+__device__ __forceinline__ bool ldlt6_wave(const double (&Arow)[6], double bi, double (&x)[6])
+{
+    const int lane = threadIdx.x & 63;
+    double diag[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) diag[i] = readlane_d(Arow[i], i);
+    double dmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) dmax = fmax(dmax, fabs(diag[i]));
+    if (!(dmax > 0.0)) return false;
+    const double tol = dmax * 1e-12;
+    bool ok = true;
+    double L[6], D[6];  // L: this lane's row of L (entries below the diagonal)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) L[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double Lj[6];  // row j of L, k < j
+#pragma unroll
+        for (int k = 0; k < j; ++k) Lj[k] = readlane_d(L[k], j);
+        double d = diag[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) d -= Lj[k] * Lj[k] * D[k];
+        ok = ok && (d > tol);
+        D[j] = d;
+        if (lane > j && lane < 6) {
+            double sacc = Arow[j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) sacc -= L[k] * Lj[k] * D[k];
+            L[j] = sacc / d;
+        }
+    }
+    if (!ok) return false;
+    double y[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double sacc = bi;
+#pragma unroll
+        for (int k = 0; k < i; ++k) sacc -= L[k] * y[k];
+        y[i] = readlane_d(sacc, i);
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) y[i] /= D[i];
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+        double sacc = y[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; ++k) sacc -= readlane_d(L[i], k) * x[k];
+        x[i] = sacc;
+    }
+    return true;
+}
+
 // Minimum-norm least-squares solve of the symmetric 6x6 system (the contract
 // of Eigen::JacobiSVD::solve at gFusion.cpp:696-697): LDL^T in FP64 for the
 // positive-definite case, cyclic Jacobi eigen-decomposition otherwise.
@@ -577,12 +643,12 @@ __device__ __forceinline__ void icp_expose(const IcpArgs &a, double (*gsum)[kTer
     out[threadIdx.x] = val;
 }
 
-// Combine rule, AE (gFusion.cpp:1143-1167) and the 6x6 solve, on one lane with
-// everything in registers (the grid is smaller than the chip, so the register
-// budget costs no occupancy).  Returns |x|.
+// Combine rule, AE (gFusion.cpp:1143-1167) and the 6x6 solve on a whole
+// wavefront (every lane returns the same x and AE).  Returns |x|.
 __device__ __forceinline__ double icp_solve(const IcpArgs &a, double (*gsum)[kTerms], double *solve_ws,
                                             double (&x)[6], float &ae)
 {
+    const int lane = threadIdx.x & 63;
     double v[kTerms], count = 0.0;
 #pragma unroll
     for (int k = 0; k < kTerms; ++k) v[k] = 0.0;
@@ -599,28 +665,36 @@ __device__ __forceinline__ double icp_solve(const IcpArgs &a, double (*gsum)[kTe
         count = (double)((a.h > 7) ? (a.h - 7 + 7) / 8 : 0) * (double)a.w;
     }
     ae = sqrtf((float)v[0] / (float)count);  // gFusion.cpp:1167
-    double b[6], C[36];
+    // lane i < 6: row i of C and b[i]
+    double Arow[6] = {0, 0, 0, 0, 0, 0}, bi = 0.0;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) b[k] = v[1 + k];
+    for (int k = 0; k < 6; ++k)
+        if (lane == k) bi = v[1 + k];
     {
         int q = 7;
 #pragma unroll
         for (int p = 0; p < 6; ++p)
 #pragma unroll
             for (int s2 = p; s2 < 6; ++s2) {
-                C[p * 6 + s2] = v[q];
-                C[s2 * 6 + p] = v[q];
+                if (lane == p) Arow[s2] = v[q];
+                if (lane == s2) Arow[p] = v[q];
                 ++q;
             }
     }
-    if (!ldlt6(C, b, x)) {
-        // singular / indefinite: minimum-norm solve in LDS
+    if (!ldlt6_wave(Arow, bi, x)) {
+        // singular / indefinite: minimum-norm solve on lane 0 in LDS
         double *Cl = solve_ws + kSolveWs, *bl = Cl + 36, *xl = bl + 6;
-        for (int k = 0; k < 36; ++k) Cl[k] = C[k];
-        for (int k = 0; k < 6; ++k) bl[k] = b[k];
-        solve6(Cl, bl, xl, solve_ws);
+        if (lane < 6) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) x[k] = xl[k];
+            for (int k = 0; k < 6; ++k) Cl[lane * 6 + k] = Arow[k];
+            bl[lane] = bi;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (lane == 0) solve6(Cl, bl, xl, solve_ws);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) x[k] = readlane_d(lane == 0 ? xl[k] : 0.0, 0);
     }
     double nrm = 0.0;
 #pragma unroll
@@ -731,11 +805,10 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
     // lane 0 solves; lanes 0-2 then evaluate the three angles' cos / sin in
     // parallel (the same double routines, so the same bits)
     const int lane = threadIdx.x;
-    double x[6] = {0, 0, 0, 0, 0, 0}, nrm = 0.0;
+    double x[6] = {0, 0, 0, 0, 0, 0};
     float ae = 0.0f;
-    if (lane == 0) nrm = icp_solve(a, gsum, solve_ws, x, ae);
-    const double a3 = __shfl(x[3], 0, 64), a4 = __shfl(x[4], 0, 64), a5 = __shfl(x[5], 0, 64);
-    const double my_ang = lane == 1 ? a4 : lane == 2 ? a5 : a3;
+    const double nrm = icp_solve(a, gsum, solve_ws, x, ae);
+    const double my_ang = lane == 1 ? x[4] : lane == 2 ? x[5] : x[3];
     const double c1 = cos(my_ang), s1 = sin(my_ang);
     const double cs[3] = {c1, __shfl(c1, 1, 64), __shfl(c1, 2, 64)};
     const double sn[3] = {s1, __shfl(s1, 1, 64), __shfl(s1, 2, 64)};
@@ -771,6 +844,125 @@ __global__ void k_icp_finish(DevState *st, int frame, float *lib, int lib_max)
     icp_finish_wave(st, frame, lib, lib_max);
 }
 
+// Track() (gFusion.cpp:648-740) in one launch: every level (coarse to fine)
+// and iteration with a grid barrier between the partial sums of an iteration
+// and their use.  A block computes the partial row k_icp_iteration's block of
+// the same index would (blocks beyond the level's count only take part in the
+// barriers); after the barrier every block forms the same FP64 group sums
+// from the same rows in the same order, solves the same system and updates
+// the same pose, so the blocks agree on the pose and on the early exit
+// (gFusion.cpp:707-708) without a second barrier, and an exited level costs
+// nothing.  Partial rows alternate between two buffers: a block can be at
+// most one iteration ahead of the slowest reader.  Results are those of the
+// k_icp_iteration sequence bit for bit.
+__global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevState *__restrict__ st)
+{
+    __shared__ float wsum[kIcpThreads / 64][kTerms];
+    __shared__ double gsum[8][kTerms];
+    __shared__ double wgsum[kIcpThreads / 64][8][kTerms];
+    __shared__ double solve_ws[kSolveWs + 36 + 6 + 6];
+    __shared__ float Ts[16];
+    __shared__ int sh_abort, sh_done;
+    __shared__ __attribute__((aligned(16))) float solo_rows[8 * kPartStride];
+
+    const int blk = blockIdx.x, nblk = gridDim.x, tid = threadIdx.x, lane = tid & 63;
+    if (tid < 16) Ts[tid] = st->pose[tid];
+    if (tid == 0) {
+        sh_abort = 0;
+        sh_done = 0;
+    }
+    __syncthreads();
+    if (blk == 0 && tid < 16) st->old_pose[tid] = Ts[tid];  // oldPose (gFusion.cpp:653)
+
+    unsigned phase = 0;
+    int iters_run = 0;
+    bool aborted = false, solo_before = false;
+    for (int level = 2; level >= 0 && !aborted; --level) {
+        const IcpArgs &L = a.lv[level];
+        const int nb = icp_blocks(L);
+        if (a.iters[level] <= 0) continue;
+        // a level served by one block is run by block 0 alone, with no grid
+        // barrier (its partial row stays in LDS); the other blocks go on to
+        // wait for its pose at the next level that needs them
+        const bool solo = nb == 1;
+        if (!solo && solo_before) {
+            if (blk == 0 && tid < 16) st->pose[tid] = Ts[tid];
+            if (!grid_barrier(&st->icp_bar, &st->icp_abort, (phase + 1u) * (unsigned)nblk, a.spin_limit,
+                              &sh_abort)) {
+                aborted = true;
+                break;
+            }
+            ++phase;
+            if (tid < 16) Ts[tid] = st->pose[tid];
+            __syncthreads();
+        }
+        solo_before = solo;
+        if (solo && blk != 0) continue;
+        for (int it = 0; it < a.iters[level]; ++it) {
+            float T[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) T[i] = Ts[i];
+            float *part = solo ? solo_rows : a.partials[phase & 1u];
+            if (blk < nb) {
+                const int g = L.group_only >= 0 ? L.group_only : (blk & 7);
+                const int jb = L.group_only >= 0 ? blk : (blk >> 3);
+                icp_block_partials(L, T, g, jb, part + (8 * jb + g) * kPartStride, wsum);
+            }
+            if (!solo) {
+                if (!grid_barrier(&st->icp_bar, &st->icp_abort, (phase + 1u) * (unsigned)nblk, a.spin_limit,
+                                  &sh_abort)) {
+                    aborted = true;
+                    break;
+                }
+                ++phase;
+            }
+            icp_group_sums(part, 8 * L.rows_step, L.group_only, wgsum, gsum);
+            if (blk == 0) icp_expose(L, gsum, st->partials);
+            if (tid < 64) {
+                double x[6] = {0, 0, 0, 0, 0, 0};
+                float ae = 0.0f;
+                const double nrm = icp_solve(L, gsum, solve_ws, x, ae);
+                const double my_ang = lane == 1 ? x[4] : lane == 2 ? x[5] : x[3];
+                const double c1 = cos(my_ang), s1 = sin(my_ang);
+                const double cs[3] = {c1, __shfl(c1, 1, 64), __shfl(c1, 2, 64)};
+                const double sn[3] = {s1, __shfl(s1, 1, 64), __shfl(s1, 2, 64)};
+                if (lane == 0) {
+                    float P[16];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) P[i] = Ts[i];
+                    pose_update_cs(x, cs, sn, P);
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) Ts[i] = P[i];
+                    sh_done = (nrm < 1e-5 && nrm != 0.0) ? 1 : 0;
+                    if (blk == 0) {
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) st->x[k] = x[k];
+                        st->ae = ae;
+                    }
+                }
+            }
+            __syncthreads();
+            ++iters_run;
+            if (sh_done) break;  // uniform over the grid
+        }
+    }
+    if (blk == 0) {
+        if (!aborted && tid < 16) st->pose[tid] = Ts[tid];
+        if (tid == 0) {
+            st->iters_run = iters_run;
+            st->level_done = sh_done;
+            if (aborted) {
+                // the pose is left as it was; nothing is integrated
+                st->icp_error = 1;
+                st->ae = 0.0f;
+            }
+        }
+        __syncthreads();
+        if (tid < 64) icp_finish_wave(st, a.frame, a.lv[0].library, a.lv[0].lib_max);
+    }
+    grid_exit(&st->icp_bar, &st->icp_exit, &st->icp_abort, (unsigned)nblk);
+}
+
 // flags: kStateSetPose, kStateArmWindow (the 4-frame integrate window of
 // kinect.cpp:146, 677-682), kStateResetPose (resetPose, gFusion.cpp:275-286:
 // m_pose_eig = identity rotation with the pose's translation, m_cumTwist = 0,
@@ -784,6 +976,7 @@ __global__ void k_state_set(DevState *st, Mat16 pose, int flags)
         st->counter = 0;
         st->reset = 1;
         st->integrate_now = 0;
+        st->icp_error = 0;
     }
     if (flags & kStateResetPose) {
         for (int i = 0; i < 16; ++i) st->pose_eig[i] = (i % 5 == 0) ? 1.0f : 0.0f;
@@ -1293,17 +1486,27 @@ __global__ __launch_bounds__(256) void k_brick_dilate(BrickArgs a)
 // ---------------------------------------------------------------------------
 // Raycast
 // ---------------------------------------------------------------------------
-template <class S>
+template <class S, bool TOUCH = false>
 struct VolView {
     const typename S::V *__restrict__ v;
     int N0, N1, N2;
     size_t plane;
     uint32_t plane32;
-    int off32;  // every voxel index fits in 32 bits
+    int off32;        // every voxel index fits in 32 bits
+    uint32_t *touch;  // TOUCH: one bit per voxel, set for every voxel read
+    __device__ __forceinline__ void mark(size_t i) const
+    {
+        if (TOUCH) atomicOr(&touch[i >> 5], 1u << (i & 31));
+    }
+    __device__ __forceinline__ float ld(size_t i) const
+    {
+        mark(i);
+        return S::raw(v[i]);
+    }
     __device__ __forceinline__ float raw(int x, int y, int z) const
     {
         if ((unsigned)x >= (unsigned)N0 || (unsigned)y >= (unsigned)N1 || (unsigned)z >= (unsigned)N2) return 0.0f;
-        return S::raw(v[(size_t)z * plane + (size_t)y * N0 + x]);
+        return ld((size_t)z * plane + (size_t)y * N0 + x);
     }
 };
 
@@ -1346,8 +1549,8 @@ __device__ __forceinline__ void sample_setup(const RaycastArgs &a, float p0, flo
     }
 }
 
-template <class S>
-__device__ __forceinline__ float sample_eval(const VolView<S> &V, const Sample &sm)
+template <class S, bool TOUCH>
+__device__ __forceinline__ float sample_eval(const VolView<S, TOUCH> &V, const Sample &sm)
 {
     const int *lo = sm.lo, *hi = sm.hi;
     const float *f = sm.f;
@@ -1361,14 +1564,14 @@ __device__ __forceinline__ float sample_eval(const VolView<S> &V, const Sample &
         const uint32_t dx = (uint32_t)(hi[0] - lo[0]);
         const uint32_t dy = (uint32_t)(hi[1] - lo[1]) * (uint32_t)V.N0;
         const uint32_t dz = (uint32_t)(hi[2] - lo[2]) * V.plane32;
-        c000 = S::raw(V.v[b]);
-        c100 = S::raw(V.v[b + dx]);
-        c010 = S::raw(V.v[b + dy]);
-        c110 = S::raw(V.v[b + dy + dx]);
-        c001 = S::raw(V.v[b + dz]);
-        c101 = S::raw(V.v[b + dz + dx]);
-        c011 = S::raw(V.v[b + dz + dy]);
-        c111 = S::raw(V.v[b + dz + dy + dx]);
+        c000 = V.ld(b);
+        c100 = V.ld(b + dx);
+        c010 = V.ld(b + dy);
+        c110 = V.ld(b + dy + dx);
+        c001 = V.ld(b + dz);
+        c101 = V.ld(b + dz + dx);
+        c011 = V.ld(b + dz + dy);
+        c111 = V.ld(b + dz + dy + dx);
     } else {  // corners may be out of bounds: they read 0 (image load semantics)
         c000 = V.raw(lo[0], lo[1], lo[2]);
         c100 = V.raw(hi[0], lo[1], lo[2]);
@@ -1386,8 +1589,8 @@ __device__ __forceinline__ float sample_eval(const VolView<S> &V, const Sample &
     return S::kScaled ? r * 0.00003051944088f : r;
 }
 
-template <class S>
-__device__ __forceinline__ float interp_vol(const VolView<S> &V, const RaycastArgs &a, float p0, float p1, float p2)
+template <class S, bool TOUCH>
+__device__ __forceinline__ float interp_vol(const VolView<S, TOUCH> &V, const RaycastArgs &a, float p0, float p1, float p2)
 {
     Sample sm;
     sample_setup(a, p0, p1, p2, sm);
@@ -1395,8 +1598,8 @@ __device__ __forceinline__ float interp_vol(const VolView<S> &V, const RaycastAr
 }
 
 // getGradient: raycast.cs:229-276
-template <class S>
-__device__ void gradient(const VolView<S> &V, const RaycastArgs &a, const float hit[3], float g[3])
+template <class S, bool TOUCH>
+__device__ void gradient(const VolView<S, TOUCH> &V, const RaycastArgs &a, const float hit[3], float g[3])
 {
     int ll[3], lu[3], ul[3], uu[3];
     float f[3];
@@ -1457,7 +1660,7 @@ constexpr int kBatch = KF_RAYCAST_BATCH;  // fine-step samples evaluated per mar
 // One thread per pixel; 16x16 pixel tiles per block.  Block ids are remapped
 // so that the blocks an XCD receives (id mod 8) cover one contiguous band of
 // tiles, keeping the voxels neighbouring rays share in that XCD's L2.
-template <class S>
+template <class S, bool TOUCH = false>
 __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevState *__restrict__ st, int blk)
 {
     // block b runs on XCD b % 8; XCD x receives the clusters of xcd_group
@@ -1479,9 +1682,17 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
         inverse_camera_matrix(a.cam, iK);
         mul4(P, iK, view);  // m_pose * invK (gFusion.cpp:1298)
     }
-    VolView<S> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
-                 (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32};
+    VolView<S, TOUCH> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
+                 (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32, a.touch};
 
+    // TOUCH: the flag bytes read are marked after the voxel bits (dilated
+    // brick flags, then coarse cells)
+    const size_t nvox = (size_t)a.N[0] * a.N[1] * a.N[2], nbt = (size_t)a.NB[0] * a.NB[1] * a.NB[2];
+    auto flag_mark = [&](int bi, int ci) {
+        V.mark(nvox + (size_t)bi);
+        V.mark(nvox + nbt + (size_t)ci);
+    };
+    (void)flag_mark;
     const float o[3] = {view[12], view[13], view[14]};
     float d[3];
     rotate3(view, (float)x, (float)y, 1.0f, d);
@@ -1537,8 +1748,11 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
                         const int b0 = sm.lo[0] >> 3, b1 = sm.lo[1] >> 3, b2 = sm.lo[2] >> 3;
                         cf = ff = 1;  // outside the brick grid: never skipped
                         if (b0 < a.NB[0] && b1 < a.NB[1] && b2 < a.NB[2]) {
-                            cf = a.cflags[((b2 >> 2) * a.NC[1] + (b1 >> 2)) * a.NC[0] + (b0 >> 2)];
-                            ff = a.dflags[(b2 * a.NB[1] + b1) * a.NB[0] + b0];
+                            const int ci = ((b2 >> 2) * a.NC[1] + (b1 >> 2)) * a.NC[0] + (b0 >> 2);
+                            const int bi = (b2 * a.NB[1] + b1) * a.NB[0] + b0;
+                            cf = a.cflags[ci];
+                            ff = a.dflags[bi];
+                            flag_mark(bi, ci);
                         }
                     }
                     const int shift = cf == 0 ? 5 : (ff == 0 ? 3 : 0);
@@ -1599,8 +1813,11 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
                     sample_setup(a, o[0] + d[0] * t_next, o[1] + d[1] * t_next, o[2] + d[2] * t_next, sn);
                     const int b0 = sn.lo[0] >> 3, b1 = sn.lo[1] >> 3, b2 = sn.lo[2] >> 3;
                     if (sn.valid && b0 < a.NB[0] && b1 < a.NB[1] && b2 < a.NB[2]) {
-                        cf_n = a.cflags[((b2 >> 2) * a.NC[1] + (b1 >> 2)) * a.NC[0] + (b0 >> 2)];
-                        ff_n = a.dflags[(b2 * a.NB[1] + b1) * a.NB[0] + b0];
+                        const int ci = ((b2 >> 2) * a.NC[1] + (b1 >> 2)) * a.NC[0] + (b0 >> 2);
+                        const int bi = (b2 * a.NB[1] + b1) * a.NB[0] + b0;
+                        cf_n = a.cflags[ci];
+                        ff_n = a.dflags[bi];
+                        flag_mark(bi, ci);
                         have_flags = true;
                     }
                 }
@@ -1661,6 +1878,7 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
         atomicAdd(&a.stats[3], (unsigned long long)(hit && th > 0.0f));
         atomicMax(&a.stats[4], (unsigned long long)n_eval);
         atomicMax(&a.stats[5], (unsigned long long)(n_eval + n_brick));
+        if (a.ray_counts) a.ray_counts[i] = min(n_eval, 0xffffu) | (min(n_brick, 0xffffu) << 16);
     }
 }
 
@@ -1881,6 +2099,26 @@ __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *
     raycast_body<S>(a, st, blockIdx.x);
 }
 
+// k_raycast with every voxel and flag byte read marked in a.touch (counts the
+// distinct bytes the march reads; results identical to k_raycast)
+template <class S>
+__global__ __launch_bounds__(256) void k_raycast_touch(RaycastArgs a, const DevState *__restrict__ st)
+{
+    raycast_body<S, true>(a, st, blockIdx.x);
+}
+
+// population count of n words, added to *out
+__global__ __launch_bounds__(256) void k_popcount(const uint32_t *__restrict__ w, size_t n,
+                                                  unsigned long long *__restrict__ out)
+{
+    unsigned c = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        c += __popc(w[i]);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)c);
+}
+
 template <class S>
 __global__ __launch_bounds__(256) void k_raycast_coop(RaycastArgs a, const DevState *__restrict__ st)
 {
@@ -1997,10 +2235,18 @@ hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, h
 {
     // 8 row groups x rows_step blocks; each block takes every rows_step-th row
     // of its group (fewer, fuller blocks: one release + one ticket each)
-    launch_timed(k_icp_iteration, dim3((a.group_only >= 0 ? 1 : 8) * a.rows_step), dim3(kIcpThreads), s, t0, t1, a,
-                 st);
+    launch_timed(k_icp_iteration, dim3(icp_blocks(a)), dim3(kIcpThreads), s, t0, t1, a, st);
     return hipGetLastError();
 }
+hipError_t launch_icp_track(const IcpTrackArgs &a, DevState *st, int nblk, hipStream_t s, hipEvent_t t0,
+                            hipEvent_t t1)
+{
+    for (int l = 0; l < 3; ++l)
+        if (a.iters[l] > 0 && icp_blocks(a.lv[l]) > nblk) return hipErrorInvalidValue;
+    launch_timed(k_icp_track, dim3(nblk), dim3(kIcpThreads), s, t0, t1, a, st);
+    return hipGetLastError();
+}
+
 hipError_t launch_icp_finish(DevState *st, int frame, float *lib, int lib_max, hipStream_t s, hipEvent_t t1)
 {
     launch_timed(k_icp_finish, dim3(1), dim3(64), s, nullptr, t1, st, frame, lib, lib_max);
@@ -2189,6 +2435,20 @@ hipError_t launch_intensity_projection(const RaycastArgs &a, int storage, const 
     case KF_STORAGE_F16: hipLaunchKernelGGL(k_intensity_projection<StoreF16>, g, b, 0, s, a, st, out); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_raycast_touch(const RaycastArgs &a, int storage, const DevState *st, size_t words,
+                                unsigned long long *count, hipStream_t s)
+{
+    const dim3 g(a.blocks), b(256);
+    switch (storage) {
+    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_raycast_touch<StoreRG16I>, g, b, 0, s, a, st); break;
+    case KF_STORAGE_F32: hipLaunchKernelGGL(k_raycast_touch<StoreF32>, g, b, 0, s, a, st); break;
+    case KF_STORAGE_F16: hipLaunchKernelGGL(k_raycast_touch<StoreF16>, g, b, 0, s, a, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_popcount, dim3(1024), dim3(256), 0, s, a.touch, words, count);
     return hipGetLastError();
 }
 

@@ -42,6 +42,9 @@ struct kf_session {
     int icp_rows_per_block = 2;
     int ray_lanes = 1;       // raycast lanes per ray (1 or 4)
     int icp_all_groups = 0;  // ICP sums every row group, not only the one the combine rule reads
+    int icp_persistent = 1;  // Track() as one persistent launch
+    int icp_solo_rows = 0;   // levels with at most this many rows per row group: one block per group
+    int cus = 0;             // compute units of the device
     // frame graph: ICP + gate + integrate + raycast, captured once
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

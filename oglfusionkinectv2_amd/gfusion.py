@@ -391,6 +391,22 @@ class GFusion:
         K.check(K.lib().kf_raycast_stats(self.session, st))
         return dict(evaluated=st[0], skipped=st[1], bricks=st[2], hits=st[3], max_eval=st[4], max_iter=st[5])
 
+    def raycastRayCounts(self):
+        """Per-pixel (evaluated samples, empty cells entered) of one more
+        raycast(), as two (h, w) uint16 arrays."""
+        w, h = self._size
+        out = np.zeros(w * h, np.uint32)
+        K.check(K.lib().kf_raycast_ray_counts(self.session, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        out = out.reshape(h, w)
+        return (out & 0xFFFF).astype(np.uint16), (out >> 16).astype(np.uint16)
+
+    def countRaycastTraffic(self):
+        """(distinct voxels read, distinct occupancy-flag bytes read) by
+        raycast() for the current volume and pose."""
+        v, f = C.c_uint64(), C.c_uint64()
+        K.check(K.lib().kf_count_raycast_traffic(self.session, C.byref(v), C.byref(f)))
+        return v.value, f.value
+
     def stageTimes(self):
         """Last pyramid / ICP / integrate / raycast durations in ms (needs
         setOption(OPT_STAGE_TIMING, 1))."""

@@ -132,16 +132,18 @@ def test_track_pixels_and_partials(o32, seq20):
         assert abs(g.alignmentEnergy() - ae) <= 1e-4 * max(ae, 1e-6)
 
 
-def test_icp_group_7_only_equals_all_groups(seq20):
+@pytest.mark.parametrize("solo", [0, 16])
+def test_icp_group_7_only_equals_all_groups(seq20, solo):
     """Under the last-row rule only row group 7 reaches the solve: summing that
     group alone (the default) gives the same bits as summing all eight
     (KF_OPT_ICP_ALL_GROUPS) -- pose, AE, partial row 7, TSDF and maps over a
     frame sequence."""
-    print("case: icp group 7 only vs all groups", flush=True)
+    print("case: icp group 7 only vs all groups, solo %d" % solo, flush=True)
     depth, gt = seq20
     out = []
     for allg in (0, 1):
         g, _ = make(fast=1)
+        g.setOption(K.OPT_ICP_SOLO_ROWS, solo)
         g.setOption(K.OPT_ICP_ALL_GROUPS, allg)
         g.Reset(gt[0])
         for f in range(10):
@@ -150,6 +152,44 @@ def test_icp_group_7_only_equals_all_groups(seq20):
         g.close()
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("rule,iterations,solo", [(0, (4, 5, 10), 0), (1, (4, 5, 10), 0), (0, (3, 0, 6), 0),
+                                                  (0, (4, 5, 10), 16), (0, (4, 5, 10), 32), (1, (4, 5, 10), 16)])
+def test_icp_persistent_equals_per_iteration_launches(seq20, rule, iterations, solo):
+    """Track() as one persistent launch (the default) gives the bits of one
+    launch per iteration: pose, AE, exposed partials, iterations run, tracked
+    flag, pose library, and what integrate and raycast then make of them,
+    frame by frame; also for a schedule with an empty level and with levels
+    run by one block alone (KF_OPT_ICP_SOLO_ROWS)."""
+    print("case: icp persistent vs per-iteration rule %d iterations %s solo %d" % (rule, iterations, solo),
+          flush=True)
+    depth, gt = seq20
+    sess = []
+    for persistent in (1, 0):
+        g, _ = make(fast=1, rule=rule, iterations=iterations)
+        g.setOption(K.OPT_ICP_SOLO_ROWS, solo)
+        g.setOption(K.OPT_ICP_PERSISTENT, persistent)
+        assert g.getOption(K.OPT_ICP_PERSISTENT) == persistent
+        g.Reset(gt[0])
+        sess.append(g)
+    for f in range(10):
+        res = []
+        for g in sess:
+            g.frame(depth[f].astype(np.float32))
+            res.append((g.getPose(), g.readPartials(), np.float32(g.alignmentEnergy()), tuple(g.status().values()),
+                        g.getPoseLibrary()))
+        for a, b in zip(res[0], res[1]):
+            np.testing.assert_array_equal(a, b)
+    # Track() outside the frame loop
+    for g in sess:
+        g.depthToVertex(depth[12].astype(np.float32))
+        g.Track()
+    np.testing.assert_array_equal(sess[0].getPose(), sess[1].getPose())
+    np.testing.assert_array_equal(sess[0].getVolume()[0], sess[1].getVolume()[0])
+    np.testing.assert_array_equal(sess[0].getVerts(), sess[1].getVerts())
+    for g in sess:
+        g.close()
 
 
 @pytest.mark.parametrize("rule", [0, 1])
@@ -410,6 +450,45 @@ def test_raycast_four_lanes_equal_one_lane(o32, seq20, storage):
         np.testing.assert_array_equal(out[0][0], out[1][0])
         np.testing.assert_array_equal(out[0][1], out[1][1])
     assert (out[0][0][..., 3] > 0).mean() > 0.5
+
+
+def test_raycast_traffic_and_ray_counts():
+    """The counting variants of raycast (distinct voxels / flag bytes read,
+    per-ray sample counts) leave the maps as raycast() makes them, agree with
+    the aggregate statistics, and obey what skipping implies: the full march
+    reads no flags and a superset of the voxels the skipping march reads; every
+    hit's lower trilinear corner is among the voxels read."""
+    print("case: raycast traffic count f32 128^3", flush=True)
+    N = 128
+    depth, gt = synth.sequence(0, 6)
+    g, cfg = make(volume=N, storage="f32")
+    g.Reset(gt[0])
+    for f in range(6):
+        g.frame(depth[f].astype(np.float32))
+    res = {}
+    for skip in (0, 1):
+        g.setOption(K.OPT_SKIP_EMPTY, skip)
+        g.raycast()
+        v0, n0 = g.getVerts(), g.getNorms()
+        vox, fl = g.countRaycastTraffic()
+        np.testing.assert_array_equal(g.getVerts(), v0)
+        np.testing.assert_array_equal(g.getNorms(), n0)
+        ev, cells = g.raycastRayCounts()
+        st = g.raycastStats()
+        assert st["max_eval"] < 65535
+        assert int(ev.astype(np.int64).sum()) == st["evaluated"]
+        assert int(ev.max()) == st["max_eval"]
+        assert int(cells.astype(np.int64).sum()) == st["bricks"]
+        res[skip] = (vox, fl, v0)
+    (vox0, fl0, v0), (vox1, fl1, _) = res[0], res[1]
+    assert fl0 == 0 and fl1 > 0
+    assert 0 < vox1 <= vox0 <= N ** 3
+    hit = v0[..., 3] > 0
+    assert hit.mean() > 0.5
+    s = v0[hit][:, :3] * np.float32(N / 2.0) - np.float32(0.5)
+    corners = np.clip(np.floor(s).astype(np.int64), 0, N - 1)
+    distinct = np.unique(corners[:, 0] + N * (corners[:, 1] + N * corners[:, 2]))
+    assert vox1 >= len(distinct)
 
 
 def test_brick_flags_cover_every_negative_voxel(seq20):
