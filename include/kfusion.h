@@ -212,7 +212,7 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         XCD b % 8 and receives clusters of this many
  *                         consecutive 16x16 tiles.  Default 4. */
 #define KF_OPT_RAYCAST_XCD_GROUP 5
-/*   KF_OPT_ICP_ROWS_PER_BLOCK rows of one row group per ICP block.  Default 2. */
+/*   KF_OPT_ICP_ROWS_PER_BLOCK rows of one row group per ICP block.  Default 1. */
 #define KF_OPT_ICP_ROWS_PER_BLOCK 6
 /*   KF_OPT_STAGE_TIMING   1: frames run without the graph and the pyramid,
  *                         ICP (first iteration .. finish), integrate and
@@ -243,6 +243,12 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         alone, with no grid barrier.  0 = off (default:
  *                         one 256-thread block is slower than a barrier). */
 #define KF_OPT_ICP_SOLO_ROWS 13
+/*   KF_OPT_RAYCAST_SCHEDULE  0: one block per 16x16 tile; k > 0: k blocks
+ *                         per compute unit take 64-ray packets from a queue,
+ *                         heaviest first by their last durations (one-lane
+ *                         raycast only).  Same results.  Default 0 (the
+ *                         queue loses L2 locality and is slower here). */
+#define KF_OPT_RAYCAST_SCHEDULE 14
 /*   KF_OPT_TRACK_MODE     tracking of the frame loop (kinect.h:143-144):
  *                         KF_TRACK_POINT_TO_PLANE (Track, default),
  *                         KF_TRACK_POINT_TO_VOLUME (TrackSDF) or both (Track

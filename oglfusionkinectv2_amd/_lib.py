@@ -47,6 +47,7 @@ OPT_ICP_ALL_GROUPS = 10
 OPT_RAYCAST_LANES = 11
 OPT_ICP_PERSISTENT = 12
 OPT_ICP_SOLO_ROWS = 13
+OPT_RAYCAST_SCHEDULE = 14
 
 TRACK_POINT_TO_PLANE = 1
 TRACK_POINT_TO_VOLUME = 2

@@ -85,6 +85,41 @@ __device__ __forceinline__ bool grid_barrier(unsigned int *bar, int *abort_flag,
     return *sh_abort == 0;
 }
 
+// The same barrier for launches whose blocks exchange data only through
+// device-coherent accesses (agent-scope atomic stores and loads, which go
+// past the per-XCD L2s): each wavefront waits until its own stores are
+// complete, then the block arrives.  No L2 write-back or invalidate, so data
+// that the blocks only read (written by earlier launches) stays cached
+// across barriers.  Ordinary stores made before the barrier are not
+// published by it.
+__device__ __forceinline__ bool grid_barrier_coherent(unsigned int *bar, int *abort_flag, unsigned int target,
+                                                      unsigned int spin_limit, int *sh_abort)
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned int n = 0;
+        int ab = 0;
+        for (;;) {
+            if (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ab = 1;
+                break;
+            }
+            if (++n > spin_limit) {
+                __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ab = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        *sh_abort = ab;
+    }
+    __syncthreads();
+    return *sh_abort == 0;
+}
+
 // Exit ticket of such a launch: the last block out clears the barrier state
 // for the next launch.  All threads of the block call it.
 __device__ __forceinline__ void grid_exit(unsigned int *bar, unsigned int *exit_count, int *abort_flag,

@@ -98,6 +98,16 @@ struct RaycastArgs {
     uint32_t *touch;            // k_raycast_touch: bitmap of voxels, then flag bytes, read
 };
 
+// Dynamic raycast schedule (k_raycast_dyn): a queue of 64-ray packets.
+struct RayQueue {
+    unsigned *next, *exit;   // queue head and exit ticket (zero between launches)
+    const unsigned *order;   // packet order of this launch (a permutation of 0..npk-1)
+    unsigned *order_out;     // the same array, written by k_ray_order before the launch
+    unsigned *cost;          // per packet: its last march duration (10 ns ticks)
+    int npk;                 // packets: 4 per 16x16 tile
+    int nwaves;              // wavefronts of the launch
+};
+
 // Point-to-volume tracking (persistent k_sdf_track, one launch per TrackSDF /
 // recoverPose).
 constexpr int kSdfTrack = 0;          // TrackSDF() from m_pose_eig
@@ -187,6 +197,9 @@ hipError_t launch_intensity_projection(const RaycastArgs &a, int storage, const 
 // k_raycast_touch then the population count of its `words`-word bitmap into *count
 hipError_t launch_raycast_touch(const RaycastArgs &a, int storage, const DevState *st, size_t words,
                                 unsigned long long *count, hipStream_t s);
+// k_ray_order, then k_raycast_dyn over nblk blocks of four wavefronts
+hipError_t launch_raycast_dyn(const RaycastArgs &a, int storage, const DevState *st, const RayQueue &q, int nblk,
+                              hipStream_t s, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s,
                           hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 

@@ -579,7 +579,10 @@ __device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float
     __syncthreads();
     if (threadIdx.x < kTerms) {
         const int k = threadIdx.x;
-        dst[k] = (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]);
+        // device-coherent store: other blocks read the row after a grid
+        // barrier that does not write back or invalidate the L2
+        __hip_atomic_store(&dst[k], (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
 }
@@ -600,17 +603,14 @@ __device__ __forceinline__ void icp_group_sums(const float *partials, int nrows,
         for (int k = 0; k < kTerms; ++k) ps[k] = 0.0;
         const bool mine = group_only < 0 || (threadIdx.x & 7) == group_only;
         for (int pb = threadIdx.x; mine && pb < nrows; pb += kIcpThreads) {
-            const float4 *rp = reinterpret_cast<const float4 *>(partials + pb * kPartStride);
-            float4 q[kTerms / 4];
+            // device-coherent loads (the rows other blocks stored the same way)
+            const float *rp = partials + pb * kPartStride;
+            float q[kTerms];
 #pragma unroll
-            for (int m = 0; m < kTerms / 4; ++m) q[m] = rp[m];
+            for (int k = 0; k < kTerms; ++k)
+                q[k] = __hip_atomic_load(rp + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-            for (int m = 0; m < kTerms / 4; ++m) {
-                ps[4 * m + 0] += (double)q[m].x;
-                ps[4 * m + 1] += (double)q[m].y;
-                ps[4 * m + 2] += (double)q[m].z;
-                ps[4 * m + 3] += (double)q[m].w;
-            }
+            for (int k = 0; k < kTerms; ++k) ps[k] += (double)q[k];
         }
 #pragma unroll
         for (int k = 0; k < kTerms; ++k) {
@@ -886,14 +886,15 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
         // wait for its pose at the next level that needs them
         const bool solo = nb == 1;
         if (!solo && solo_before) {
-            if (blk == 0 && tid < 16) st->pose[tid] = Ts[tid];
-            if (!grid_barrier(&st->icp_bar, &st->icp_abort, (phase + 1u) * (unsigned)nblk, a.spin_limit,
-                              &sh_abort)) {
+            if (blk == 0 && tid < 16)
+                __hip_atomic_store(&st->pose[tid], Ts[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!grid_barrier_coherent(&st->icp_bar, &st->icp_abort, (phase + 1u) * (unsigned)nblk, a.spin_limit,
+                                       &sh_abort)) {
                 aborted = true;
                 break;
             }
             ++phase;
-            if (tid < 16) Ts[tid] = st->pose[tid];
+            if (tid < 16) Ts[tid] = __hip_atomic_load(&st->pose[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __syncthreads();
         }
         solo_before = solo;
@@ -909,8 +910,8 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
                 icp_block_partials(L, T, g, jb, part + (8 * jb + g) * kPartStride, wsum);
             }
             if (!solo) {
-                if (!grid_barrier(&st->icp_bar, &st->icp_abort, (phase + 1u) * (unsigned)nblk, a.spin_limit,
-                                  &sh_abort)) {
+                if (!grid_barrier_coherent(&st->icp_bar, &st->icp_abort, (phase + 1u) * (unsigned)nblk,
+                                           a.spin_limit, &sh_abort)) {
                     aborted = true;
                     break;
                 }
@@ -1549,6 +1550,18 @@ __device__ __forceinline__ void sample_setup(const RaycastArgs &a, float p0, flo
     }
 }
 
+// interpVol's trilinear blend (raycast.cs:69-82), in its expression order
+template <class S>
+__device__ __forceinline__ float trilerp(float c000, float c100, float c010, float c110, float c001, float c101,
+                                         float c011, float c111, const float *f)
+{
+    const float r = ((c000 * (1 - f[0]) + c100 * f[0]) * (1 - f[1]) + (c010 * (1 - f[0]) + c110 * f[0]) * f[1]) *
+                        (1 - f[2]) +
+                    ((c001 * (1 - f[0]) + c101 * f[0]) * (1 - f[1]) + (c011 * (1 - f[0]) + c111 * f[0]) * f[1]) *
+                        f[2];
+    return S::kScaled ? r * 0.00003051944088f : r;
+}
+
 template <class S, bool TOUCH>
 __device__ __forceinline__ float sample_eval(const VolView<S, TOUCH> &V, const Sample &sm)
 {
@@ -1582,12 +1595,40 @@ __device__ __forceinline__ float sample_eval(const VolView<S, TOUCH> &V, const S
         c011 = V.raw(lo[0], hi[1], hi[2]);
         c111 = V.raw(hi[0], hi[1], hi[2]);
     }
-    const float r = ((c000 * (1 - f[0]) + c100 * f[0]) * (1 - f[1]) + (c010 * (1 - f[0]) + c110 * f[0]) * f[1]) *
-                        (1 - f[2]) +
-                    ((c001 * (1 - f[0]) + c101 * f[0]) * (1 - f[1]) + (c011 * (1 - f[0]) + c111 * f[0]) * f[1]) *
-                        f[2];
-    return S::kScaled ? r * 0.00003051944088f : r;
+    return trilerp<S>(c000, c100, c010, c110, c001, c101, c011, c111, f);
 }
+
+// Fine-step sample whose trilinear corners are b and b + 1 on every axis with
+// no clamping (0 <= b_k <= N_k - 2): the index setup of sample_setup without
+// the clamps.  Its eight corners are four pairs of x-neighbours, read as pairs
+// from four row pointers; the values and the blend are those of
+// sample_setup + sample_eval.
+struct FastSample {
+    uint32_t base;  // voxel index of corner (b0, b1, b2)
+    int b[3];       // floor(s), INT_MIN when not a finite in-range value
+    float f[3];
+    bool inside;    // 0 <= b_k <= N_k - 2 on every axis
+};
+
+__device__ __forceinline__ void sample_fast(const RaycastArgs &a, float p0, float p1, float p2, uint32_t plane32,
+                                            FastSample &q)
+{
+    const float p[3] = {p0, p1, p2};
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float sk = a.dim_pow2 ? ((p[k] * (float)a.N[k]) * a.inv_dim[k]) - 0.5f
+                                    : (p[k] * (float)a.N[k] / a.dim[k]) - 0.5f;
+        const float fl = floorf(sk);
+        const int b = floor_index(fl);
+        q.b[k] = b;
+        q.f[k] = sk - fl;
+        in = in && ((unsigned)b < (unsigned)(a.N[k] - 1));
+    }
+    q.inside = in;
+    q.base = ((uint32_t)q.b[2] * plane32 + (uint32_t)q.b[1] * (uint32_t)a.N[0]) + (uint32_t)q.b[0];
+}
+
 
 template <class S, bool TOUCH>
 __device__ __forceinline__ float interp_vol(const VolView<S, TOUCH> &V, const RaycastArgs &a, float p0, float p1, float p2)
@@ -1657,31 +1698,20 @@ constexpr int kMaxMarchSteps = 1 << 20;
 #endif
 constexpr int kBatch = KF_RAYCAST_BATCH;  // fine-step samples evaluated per march iteration
 
-// One thread per pixel; 16x16 pixel tiles per block.  Block ids are remapped
-// so that the blocks an XCD receives (id mod 8) cover one contiguous band of
-// tiles, keeping the voxels neighbouring rays share in that XCD's L2.
-template <class S, bool TOUCH = false>
-__device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevState *__restrict__ st, int blk)
+// view = m_pose * invK (gFusion.cpp:1298)
+__device__ __forceinline__ void raycast_view(const RaycastArgs &a, const DevState *__restrict__ st, float view[16])
 {
-    // block b runs on XCD b % 8; XCD x receives the clusters of xcd_group
-    // consecutive tiles x, x + 8, x + 16, ... (1 = fully interleaved)
-    const int nb = a.tiles_x * a.tiles_y;
-    const int G = a.xcd_group;
-    const int j = blk / 8;
-    const int tile = ((blk % 8) + 8 * (j / G)) * G + (j % G);
-    if (tile >= nb) return;
-    const int x = (tile % a.tiles_x) * 16 + (threadIdx.x & 15);
-    const int y = (tile / a.tiles_x) * 16 + (threadIdx.x >> 4);
-    if (x >= a.w || y >= a.h) return;
-
-    float view[16];
-    {
-        float iK[16], P[16];
+    float iK[16], P[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
-        inverse_camera_matrix(a.cam, iK);
-        mul4(P, iK, view);  // m_pose * invK (gFusion.cpp:1298)
-    }
+    for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
+    inverse_camera_matrix(a.cam, iK);
+    mul4(P, iK, view);
+}
+
+// The ray of pixel (x, y): march, hit refinement, normal, map writes.
+template <class S, bool TOUCH>
+__device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float (&view)[16], int x, int y)
+{
     VolView<S, TOUCH> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
                  (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32, a.touch};
 
@@ -1797,22 +1827,69 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
                 // them, then consume the batch in march order.
                 float tj[kBatch], fv[kBatch];
                 tj[0] = t;
-                fv[0] = sample_eval(V, sm);
 #pragma unroll
-                for (int j = 1; j < kBatch; ++j) {
-                    tj[j] = tj[j - 1] + stepsize;
-                    Sample sj;
-                    sample_setup(a, o[0] + d[0] * tj[j], o[1] + d[1] * tj[j], o[2] + d[2] * tj[j], sj);
-                    fv[j] = sample_eval(V, sj);
+                for (int j = 1; j < kBatch; ++j) tj[j] = tj[j - 1] + stepsize;
+                // interior samples: corners b and b + 1 on every axis
+                FastSample fq[kBatch];
+                bool all_in = !TOUCH && a.off32;
+                if (all_in) {
+                    fq[0].inside = sm.valid;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        fq[0].b[k] = sm.lo[k];
+                        fq[0].f[k] = sm.f[k];
+                        fq[0].inside = fq[0].inside && sm.lo[k] >= 0 && sm.hi[k] == sm.lo[k] + 1;
+                    }
+                    fq[0].base = ((uint32_t)sm.lo[2] * V.plane32 + (uint32_t)sm.lo[1] * (uint32_t)a.N[0]) +
+                                 (uint32_t)sm.lo[0];
+#pragma unroll
+                    for (int j = 1; j < kBatch; ++j)
+                        sample_fast(a, o[0] + d[0] * tj[j], o[1] + d[1] * tj[j], o[2] + d[2] * tj[j], V.plane32,
+                                    fq[j]);
+#pragma unroll
+                    for (int j = 0; j < kBatch; ++j) all_in = all_in && fq[j].inside;
+                }
+                if (__all(all_in)) {
+                    // every active lane: all loads first, then the blends
+                    float c[kBatch][8];
+#pragma unroll
+                    for (int j = 0; j < kBatch; ++j) {
+                        const typename S::V *r0 = V.v + fq[j].base;
+                        const typename S::V *r1 = r0 + V.N0;
+                        const typename S::V *r2 = r0 + V.plane32;
+                        const typename S::V *r3 = r2 + V.N0;
+                        c[j][0] = S::raw(r0[0]);
+                        c[j][1] = S::raw(r0[1]);
+                        c[j][2] = S::raw(r1[0]);
+                        c[j][3] = S::raw(r1[1]);
+                        c[j][4] = S::raw(r2[0]);
+                        c[j][5] = S::raw(r2[1]);
+                        c[j][6] = S::raw(r3[0]);
+                        c[j][7] = S::raw(r3[1]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < kBatch; ++j)
+                        fv[j] = trilerp<S>(c[j][0], c[j][1], c[j][2], c[j][3], c[j][4], c[j][5], c[j][6], c[j][7],
+                                           fq[j].f);
+                } else {
+                    fv[0] = sample_eval(V, sm);
+#pragma unroll
+                    for (int j = 1; j < kBatch; ++j) {
+                        Sample sj;
+                        sample_setup(a, o[0] + d[0] * tj[j], o[1] + d[1] * tj[j], o[2] + d[2] * tj[j], sj);
+                        fv[j] = sample_eval(V, sj);
+                    }
                 }
                 const float t_next = tj[kBatch - 1] + stepsize;
                 if (a.skip && t_next < tfar) {
                     // the next sample lies in the box: its lower corner is a
-                    // voxel of the volume (checked all the same)
-                    Sample sn;
-                    sample_setup(a, o[0] + d[0] * t_next, o[1] + d[1] * t_next, o[2] + d[2] * t_next, sn);
-                    const int b0 = sn.lo[0] >> 3, b1 = sn.lo[1] >> 3, b2 = sn.lo[2] >> 3;
-                    if (sn.valid && b0 < a.NB[0] && b1 < a.NB[1] && b2 < a.NB[2]) {
+                    // voxel of the volume (checked all the same); the corner
+                    // index is floor(s) clamped at 0, as in sample_setup
+                    FastSample fn;
+                    sample_fast(a, o[0] + d[0] * t_next, o[1] + d[1] * t_next, o[2] + d[2] * t_next, V.plane32, fn);
+                    const bool valid = fn.b[0] != INT_MIN && fn.b[1] != INT_MIN && fn.b[2] != INT_MIN;
+                    const int b0 = max(fn.b[0], 0) >> 3, b1 = max(fn.b[1], 0) >> 3, b2 = max(fn.b[2], 0) >> 3;
+                    if (valid && b0 < a.NB[0] && b1 < a.NB[1] && b2 < a.NB[2]) {
                         const int ci = ((b2 >> 2) * a.NC[1] + (b1 >> 2)) * a.NC[0] + (b0 >> 2);
                         const int bi = (b2 * a.NB[1] + b1) * a.NB[0] + b0;
                         cf_n = a.cflags[ci];
@@ -2093,10 +2170,114 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
     }
 }
 
+// One thread per pixel; 16x16 pixel tiles per block.  Block ids are remapped
+// so that the blocks an XCD receives (id mod 8) cover one contiguous band of
+// tiles, keeping the voxels neighbouring rays share in that XCD's L2.
+template <class S, bool TOUCH = false>
+__device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevState *__restrict__ st, int blk)
+{
+    // block b runs on XCD b % 8; XCD x receives the clusters of xcd_group
+    // consecutive tiles x, x + 8, x + 16, ... (1 = fully interleaved)
+    const int nb = a.tiles_x * a.tiles_y;
+    const int G = a.xcd_group;
+    const int j = blk / 8;
+    const int tile = ((blk % 8) + 8 * (j / G)) * G + (j % G);
+    if (tile >= nb) return;
+    const int x = (tile % a.tiles_x) * 16 + (threadIdx.x & 15);
+    const int y = (tile / a.tiles_x) * 16 + (threadIdx.x >> 4);
+    if (x >= a.w || y >= a.h) return;
+    float view[16];
+    raycast_view(a, st, view);
+    raycast_pixel<S, TOUCH>(a, view, x, y);
+}
+
 template <class S>
 __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *__restrict__ st)
 {
     raycast_body<S>(a, st, blockIdx.x);
+}
+
+// The same rays with dynamic scheduling: the grid is a fixed number of
+// wavefronts (a few per SIMD), each of which takes 64-ray packets (16 x 4
+// pixels: the rays one wavefront of k_raycast marches) from a queue until it
+// is empty.  The queue hands packets out in the order q.order, which the
+// previous launch's per-packet durations sort heaviest first
+// (k_ray_order), so long packets start early and short ones fill the gaps;
+// a wavefront no longer waits on the slowest rays of a fixed set of tiles.
+// Every ray is computed exactly as in k_raycast.  The last wavefront out
+// rewinds the queue for the next launch.
+template <class S>
+__global__ __launch_bounds__(256) void k_raycast_dyn(RaycastArgs a, const DevState *__restrict__ st, RayQueue q)
+{
+    float view[16];
+    raycast_view(a, st, view);
+    const int lane = threadIdx.x & 63;
+    const unsigned npk = (unsigned)q.npk;
+    // Every lane takes part in the fetch (lane 0 adds one, the others zero),
+    // so no lane has left the loop's control flow when the ticket is read
+    // back: a fetch under `if (lane == 0)` lets the compiler run the other
+    // lanes round a loop of their own that re-reads a stale ticket.  Lane 0
+    // is the first active lane (all are active), so readfirstlane returns its
+    // ticket; the packet index and its queue entry live in scalar registers
+    // and the loop's exit and the entry check are uniform branches.
+    for (;;) {
+        const unsigned v = atomicAdd(q.next, lane == 0 ? 1u : 0u);
+        const unsigned p = __builtin_amdgcn_readfirstlane(v);
+        if (p >= npk) break;
+        const unsigned pk = __builtin_amdgcn_readfirstlane(q.order[p]);
+        if (pk < npk) {  // a permutation entry (anything else: nothing to do)
+            const unsigned long long c0 = wall_clock64();
+            const int tile = (int)(pk >> 2);
+            const int x = (tile % a.tiles_x) * 16 + (lane & 15);
+            const int y = (tile / a.tiles_x) * 16 + (int)(pk & 3u) * 4 + (lane >> 4);
+            if (x < a.w && y < a.h) raycast_pixel<S, false>(a, view, x, y);
+            if (lane == 0) {
+                const unsigned long long dt = wall_clock64() - c0;
+                q.cost[pk] = dt > 0xffffffffull ? 0xffffffffu : (unsigned)dt;
+            }
+        }
+    }
+    if (lane == 0) {
+        const unsigned t = atomicAdd(q.exit, 1u);
+        if (t == (unsigned)q.nwaves - 1u) {
+            *q.next = 0u;
+            *q.exit = 0u;
+        }
+    }
+}
+
+// Heaviest-first order of the raycast packets from their last durations: a
+// counting sort over 128 log-spaced buckets (four per octave), one block.
+// Any permutation is a valid order; this one only shortens the tail.
+constexpr int kOrderBuckets = 128;
+__device__ __forceinline__ int cost_bucket(unsigned c)
+{
+    if (c < 4u) return (int)c;
+    const int e = 31 - __clz(c);  // >= 2
+    const int b = (e << 2) | (int)((c >> (e - 2)) & 3u);
+    return b < kOrderBuckets ? b : kOrderBuckets - 1;
+}
+
+__global__ __launch_bounds__(1024) void k_ray_order(RayQueue q)
+{
+    __shared__ unsigned hist[kOrderBuckets], base[kOrderBuckets];
+    const int tid = threadIdx.x;
+    if (tid < kOrderBuckets) hist[tid] = 0u;
+    __syncthreads();
+    for (int i = tid; i < q.npk; i += 1024) atomicAdd(&hist[cost_bucket(q.cost[i])], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        unsigned acc = 0;
+        for (int b = kOrderBuckets - 1; b >= 0; --b) {
+            base[b] = acc;
+            acc += hist[b];
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < q.npk; i += 1024) {
+        const unsigned pos = atomicAdd(&base[cost_bucket(q.cost[i])], 1u);
+        q.order_out[pos] = (unsigned)i;
+    }
 }
 
 // k_raycast with every voxel and flag byte read marked in a.touch (counts the
@@ -2449,6 +2630,22 @@ hipError_t launch_raycast_touch(const RaycastArgs &a, int storage, const DevStat
     default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k_popcount, dim3(1024), dim3(256), 0, s, a.touch, words, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_raycast_dyn(const RaycastArgs &a, int storage, const DevState *st, const RayQueue &q, int nblk,
+                              hipStream_t s, hipEvent_t t0, hipEvent_t t1)
+{
+    if (a.lanes != 1 || q.nwaves != 4 * nblk || q.npk != 4 * a.tiles_x * a.tiles_y) return hipErrorInvalidValue;
+    // the stage timing spans both launches
+    launch_timed(k_ray_order, dim3(1), dim3(1024), s, t0, nullptr, q);
+    const dim3 g(nblk), b(256);
+    switch (storage) {
+    case KF_STORAGE_RG16I: launch_timed(k_raycast_dyn<StoreRG16I>, g, b, s, nullptr, t1, a, st, q); break;
+    case KF_STORAGE_F32: launch_timed(k_raycast_dyn<StoreF32>, g, b, s, nullptr, t1, a, st, q); break;
+    case KF_STORAGE_F16: launch_timed(k_raycast_dyn<StoreF16>, g, b, s, nullptr, t1, a, st, q); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

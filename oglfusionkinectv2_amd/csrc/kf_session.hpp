@@ -39,10 +39,15 @@ struct kf_session {
     int use_graph = 1;
     int rebuild_period = 16;
     int xcd_group = 4;
-    int icp_rows_per_block = 2;
+    int icp_rows_per_block = 1;
     int ray_lanes = 1;       // raycast lanes per ray (1 or 4)
     int icp_all_groups = 0;  // ICP sums every row group, not only the one the combine rule reads
     int icp_persistent = 1;  // Track() as one persistent launch
+    int ray_sched = 0;       // raycast: 0 = one block per tile, k > 0 = queue of packets, k blocks per CU
+    unsigned *ray_q = nullptr;      // queue head, exit ticket
+    unsigned *ray_order = nullptr;  // packet order
+    unsigned *ray_cost = nullptr;   // per-packet durations of the last dynamic raycast
+    int ray_npk = 0;                // packets (64 rays each)
     int icp_solo_rows = 0;   // levels with at most this many rows per row group: one block per group
     int cus = 0;             // compute units of the device
     // frame graph: ICP + gate + integrate + raycast, captured once
