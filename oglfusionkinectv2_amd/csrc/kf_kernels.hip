@@ -1638,55 +1638,110 @@ __device__ __forceinline__ float interp_vol(const VolView<S, TOUCH> &V, const Ra
     return sample_eval(V, sm);
 }
 
-// getGradient: raycast.cs:229-276
+// getGradient: raycast.cs:229-276.  Stencil indices per axis are named
+// kLL = b - 1, kLU = b (the trilinear "lo"), kUL = b + 1 ("hi"), kUU = b + 2,
+// each clamped to the volume; VR(X, Y, Z) reads the voxel at those indices.
+enum { kLL = 0, kLU = 1, kUL = 2, kUU = 3 };
+#define KF_GRADIENT(VR, fx, fy, fz, g)                                                                              \
+    g[0] = (((VR(kUL, kLU, kLU) - VR(kLL, kLU, kLU)) * (1 - fx) + (VR(kUU, kLU, kLU) - VR(kLU, kLU, kLU)) * fx) *    \
+                (1 - fy) +                                                                                          \
+            ((VR(kUL, kUL, kLU) - VR(kLL, kUL, kLU)) * (1 - fx) + (VR(kUU, kUL, kLU) - VR(kLU, kUL, kLU)) * fx) *    \
+                fy) *                                                                                               \
+               (1 - fz) +                                                                                           \
+           (((VR(kUL, kLU, kUL) - VR(kLL, kLU, kUL)) * (1 - fx) + (VR(kUU, kLU, kUL) - VR(kLU, kLU, kUL)) * fx) *    \
+                (1 - fy) +                                                                                          \
+            ((VR(kUL, kUL, kUL) - VR(kLL, kUL, kUL)) * (1 - fx) + (VR(kUU, kUL, kUL) - VR(kLU, kUL, kUL)) * fx) *    \
+                fy) *                                                                                               \
+               fz;                                                                                                  \
+    g[1] = (((VR(kLU, kUL, kLU) - VR(kLU, kLL, kLU)) * (1 - fx) + (VR(kUL, kUL, kLU) - VR(kUL, kLL, kLU)) * fx) *    \
+                (1 - fy) +                                                                                          \
+            ((VR(kLU, kUU, kLU) - VR(kLU, kLU, kLU)) * (1 - fx) + (VR(kUL, kUU, kLU) - VR(kUL, kLU, kLU)) * fx) *    \
+                fy) *                                                                                               \
+               (1 - fz) +                                                                                           \
+           (((VR(kLU, kUL, kUL) - VR(kLU, kLL, kUL)) * (1 - fx) + (VR(kUL, kUL, kUL) - VR(kUL, kLL, kUL)) * fx) *    \
+                (1 - fy) +                                                                                          \
+            ((VR(kLU, kUU, kUL) - VR(kLU, kLU, kUL)) * (1 - fx) + (VR(kUL, kUU, kUL) - VR(kUL, kLU, kUL)) * fx) *    \
+                fy) *                                                                                               \
+               fz;                                                                                                  \
+    g[2] = (((VR(kLU, kLU, kUL) - VR(kLU, kLU, kLL)) * (1 - fx) + (VR(kUL, kLU, kUL) - VR(kUL, kLU, kLL)) * fx) *    \
+                (1 - fy) +                                                                                          \
+            ((VR(kLU, kUL, kUL) - VR(kLU, kUL, kLL)) * (1 - fx) + (VR(kUL, kUL, kUL) - VR(kUL, kUL, kLL)) * fx) *    \
+                fy) *                                                                                               \
+               (1 - fz) +                                                                                           \
+           (((VR(kLU, kLU, kUU) - VR(kLU, kLU, kLU)) * (1 - fx) + (VR(kUL, kLU, kUU) - VR(kUL, kLU, kLU)) * fx) *    \
+                (1 - fy) +                                                                                          \
+            ((VR(kLU, kUL, kUU) - VR(kLU, kUL, kLU)) * (1 - fx) + (VR(kUL, kUL, kUU) - VR(kUL, kUL, kLU)) * fx) *    \
+                fy) *                                                                                               \
+               fz;
+
 template <class S, bool TOUCH>
 __device__ void gradient(const VolView<S, TOUCH> &V, const RaycastArgs &a, const float hit[3], float g[3])
 {
-    int ll[3], lu[3], ul[3], uu[3];
+    int ix[3][4];
+    int bb[3];
     float f[3];
+    bool interior = !TOUCH && a.off32;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float s = (hit[k] * (float)a.N[k] / a.dim[k]) - 0.5f;
         const float fl = floorf(s);
         const int b = floor_index(fl);
         f[k] = s - fl;
+        bb[k] = b;
         if (b == INT_MIN) {
-            ll[k] = lu[k] = ul[k] = uu[k] = b;
+            ix[k][kLL] = ix[k][kLU] = ix[k][kUL] = ix[k][kUU] = b;
         } else {
-            ll[k] = max(b - 1, 0);
-            lu[k] = max(b, 0);
-            ul[k] = min(b + 1, a.N[k] - 1);
-            uu[k] = min(b + 2, a.N[k] - 1);
+            ix[k][kLL] = max(b - 1, 0);
+            ix[k][kLU] = max(b, 0);
+            ix[k][kUL] = min(b + 1, a.N[k] - 1);
+            ix[k][kUU] = min(b + 2, a.N[k] - 1);
         }
+        // every stencil index in the volume without clamping: 1 <= b <= N - 3
+        interior = interior && ((unsigned)b - 1u < (unsigned)max(a.N[k] - 3, 0));
     }
-    const int *lo = lu, *hi = ul;
     const float fx = f[0], fy = f[1], fz = f[2];
-#define VR(x_, y_, z_) V.raw(x_, y_, z_)
-    g[0] = (((VR(ul[0], lo[1], lo[2]) - VR(ll[0], lo[1], lo[2])) * (1 - fx) +
-             (VR(uu[0], lo[1], lo[2]) - VR(lu[0], lo[1], lo[2])) * fx) * (1 - fy) +
-            ((VR(ul[0], hi[1], lo[2]) - VR(ll[0], hi[1], lo[2])) * (1 - fx) +
-             (VR(uu[0], hi[1], lo[2]) - VR(lu[0], hi[1], lo[2])) * fx) * fy) * (1 - fz) +
-           (((VR(ul[0], lo[1], hi[2]) - VR(ll[0], lo[1], hi[2])) * (1 - fx) +
-             (VR(uu[0], lo[1], hi[2]) - VR(lu[0], lo[1], hi[2])) * fx) * (1 - fy) +
-            ((VR(ul[0], hi[1], hi[2]) - VR(ll[0], hi[1], hi[2])) * (1 - fx) +
-             (VR(uu[0], hi[1], hi[2]) - VR(lu[0], hi[1], hi[2])) * fx) * fy) * fz;
-    g[1] = (((VR(lo[0], ul[1], lo[2]) - VR(lo[0], ll[1], lo[2])) * (1 - fx) +
-             (VR(hi[0], ul[1], lo[2]) - VR(hi[0], ll[1], lo[2])) * fx) * (1 - fy) +
-            ((VR(lo[0], uu[1], lo[2]) - VR(lo[0], lu[1], lo[2])) * (1 - fx) +
-             (VR(hi[0], uu[1], lo[2]) - VR(hi[0], lu[1], lo[2])) * fx) * fy) * (1 - fz) +
-           (((VR(lo[0], ul[1], hi[2]) - VR(lo[0], ll[1], hi[2])) * (1 - fx) +
-             (VR(hi[0], ul[1], hi[2]) - VR(hi[0], ll[1], hi[2])) * fx) * (1 - fy) +
-            ((VR(lo[0], uu[1], hi[2]) - VR(lo[0], lu[1], hi[2])) * (1 - fx) +
-             (VR(hi[0], uu[1], hi[2]) - VR(hi[0], lu[1], hi[2])) * fx) * fy) * fz;
-    g[2] = (((VR(lo[0], lo[1], ul[2]) - VR(lo[0], lo[1], ll[2])) * (1 - fx) +
-             (VR(hi[0], lo[1], ul[2]) - VR(hi[0], lo[1], ll[2])) * fx) * (1 - fy) +
-            ((VR(lo[0], hi[1], ul[2]) - VR(lo[0], hi[1], ll[2])) * (1 - fx) +
-             (VR(hi[0], hi[1], ul[2]) - VR(hi[0], hi[1], ll[2])) * fx) * fy) * (1 - fz) +
-           (((VR(lo[0], lo[1], uu[2]) - VR(lo[0], lo[1], lu[2])) * (1 - fx) +
-             (VR(hi[0], lo[1], uu[2]) - VR(hi[0], lo[1], lu[2])) * fx) * (1 - fy) +
-            ((VR(lo[0], hi[1], uu[2]) - VR(lo[0], hi[1], lu[2])) * (1 - fx) +
-             (VR(hi[0], hi[1], uu[2]) - VR(hi[0], hi[1], lu[2])) * fx) * fy) * fz;
-#undef VR
+    if (interior) {
+        // the 32 voxels of the stencil from twelve row reads: x = b - 1 .. b + 2
+        // at y, z in {b, b + 1}; x in {b, b + 1} at y in {b - 1, b + 2} and
+        // at z in {b - 1, b + 2}
+        const typename S::V *c =
+            V.v + (((uint32_t)bb[2] * V.plane32 + (uint32_t)bb[1] * (uint32_t)V.N0) + (uint32_t)bb[0]);
+        const ptrdiff_t py = (ptrdiff_t)V.N0, pz = (ptrdiff_t)V.plane32;
+        float A[2][2][4], B[2][2][2], C[2][2][2];
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int z = 0; z < 2; ++z) {
+                const typename S::V *r = c + z * pz + y * py - 1;
+#pragma unroll
+                for (int x = 0; x < 4; ++x) A[y][z][x] = S::raw(r[x]);
+            }
+#pragma unroll
+        for (int z = 0; z < 2; ++z)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const typename S::V *r = c + z * pz + (y ? 2 : -1) * py;
+                B[z][y][0] = S::raw(r[0]);
+                B[z][y][1] = S::raw(r[1]);
+            }
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int z = 0; z < 2; ++z) {
+                const typename S::V *r = c + (z ? 2 : -1) * pz + y * py;
+                C[y][z][0] = S::raw(r[0]);
+                C[y][z][1] = S::raw(r[1]);
+            }
+        auto VF = [&](int X, int Y, int Z) -> float {
+            if ((Y == kLU || Y == kUL) && (Z == kLU || Z == kUL)) return A[Y - kLU][Z - kLU][X];
+            if (Y == kLL || Y == kUU) return B[Z - kLU][Y == kLL ? 0 : 1][X - kLU];
+            return C[Y - kLU][Z == kLL ? 0 : 1][X - kLU];
+        };
+        KF_GRADIENT(VF, fx, fy, fz, g)
+    } else {
+        auto VR = [&](int X, int Y, int Z) -> float { return V.raw(ix[0][X], ix[1][Y], ix[2][Z]); };
+        KF_GRADIENT(VR, fx, fy, fz, g)
+    }
     const float gs = S::kScaled ? (0.5f * 0.00003051944088f) : 0.5f;
 #pragma unroll
     for (int k = 0; k < 3; ++k) g[k] = (g[k] * (a.dim[k] / (float)a.N[k])) * gs;
