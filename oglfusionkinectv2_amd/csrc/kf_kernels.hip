@@ -996,7 +996,11 @@ __global__ void k_state_set(DevState *st, Mat16 pose, int flags)
 // ---------------------------------------------------------------------------
 // Integrate
 // ---------------------------------------------------------------------------
-constexpr int kIntU = 8;  // z voxels per pipelined chunk (= brick depth)
+constexpr int kIntU = 8;      // z voxels per pipelined chunk (= brick depth)
+#ifndef KF_INT_GROUP
+#define KF_INT_GROUP 1
+#endif
+constexpr int kIntGroup = KF_INT_GROUP;  // chunks whose skip tests are issued together
 
 struct ColumnSetup {
     float pos[3], cx[3], dl[3], cd[3];
@@ -1227,29 +1231,32 @@ __device__ __forceinline__ bool chunk_unreachable(const IntegrateArgs &a, const 
                                                   float p2, float fx, float fy, float cxh, float cyh, float mu_lo,
                                                   float mu_hi)
 {
+    // straight-line: the four tile loads are always issued (at indices
+    // clamped into the tile grid), so the tests of several chunks can be in
+    // flight together; the result is that of the case analysis above
     const float q0 = fmaf(7.0f, c.dl[0], p0), q1 = fmaf(7.0f, c.dl[1], p1), q2 = fmaf(7.0f, c.dl[2], p2);
     const float zlo = fminf(p2, q2), zhi = fmaxf(p2, q2);
-    if (!(zlo > 0.001f)) return false;
+    const bool okz = zlo > 0.001f;
     const float rp = __builtin_amdgcn_rcpf(p2), rq = __builtin_amdgcn_rcpf(q2);
     const float xa = fmaf(fx, p0 * rp, cxh), xb = fmaf(fx, q0 * rq, cxh);
     const float ya = fmaf(fy, p1 * rp, cyh), yb = fmaf(fy, q1 * rq, cyh);
     const float W = (float)a.dw, H = (float)a.dh;
     const float x0 = fminf(xa, xb) - 1.0f, x1 = fmaxf(xa, xb) + 1.0f;
     const float y0 = fminf(ya, yb) - 1.0f, y1 = fmaxf(ya, yb) + 1.0f;
-    if (!(x0 == x0 && x1 == x1 && y0 == y0 && y1 == y1)) return false;
-    if (x1 < 0.0f || y1 < 0.0f || x0 >= W || y0 >= H) return true;  // whole box off the image
-    const int tx0 = (int)fmaxf(x0, 0.0f) >> 3, tx1 = (int)fminf(x1, W - 1.0f) >> 3;
-    const int ty0 = (int)fmaxf(y0, 0.0f) >> 3, ty1 = (int)fminf(y1, H - 1.0f) >> 3;
-    if (tx1 - tx0 > 1 || ty1 - ty0 > 1) return false;
-    float dmin = INFINITY, dmax = -INFINITY;
-    for (int ty = ty0; ty <= ty1; ++ty)
-        for (int tx = tx0; tx <= tx1; ++tx) {
-            const float2 t = a.dtiles[ty * a.tiles_x + tx];
-            dmin = fminf(dmin, t.x);
-            dmax = fmaxf(dmax, t.y);
-        }
+    const bool fin = x0 == x0 && x1 == x1 && y0 == y0 && y1 == y1;
+    const bool off = x1 < 0.0f || y1 < 0.0f || x0 >= W || y0 >= H;  // whole box off the image
+    const float cx0 = fin ? fminf(fmaxf(x0, 0.0f), W - 1.0f) : 0.0f, cx1 = fin ? fminf(fmaxf(x1, 0.0f), W - 1.0f) : 0.0f;
+    const float cy0 = fin ? fminf(fmaxf(y0, 0.0f), H - 1.0f) : 0.0f, cy1 = fin ? fminf(fmaxf(y1, 0.0f), H - 1.0f) : 0.0f;
+    const int tx0 = (int)cx0 >> 3, tx1 = (int)cx1 >> 3, ty0 = (int)cy0 >> 3, ty1 = (int)cy1 >> 3;
+    const bool small = tx1 - tx0 <= 1 && ty1 - ty0 <= 1;
+    const int txb = tx1 > tx0 ? tx0 + 1 : tx0, tyb = ty1 > ty0 ? ty0 + 1 : ty0;
+    const float2 t00 = a.dtiles[ty0 * a.tiles_x + tx0], t01 = a.dtiles[ty0 * a.tiles_x + txb];
+    const float2 t10 = a.dtiles[tyb * a.tiles_x + tx0], t11 = a.dtiles[tyb * a.tiles_x + txb];
+    const float dmin = fminf(fminf(t00.x, t01.x), fminf(t10.x, t11.x));
+    const float dmax = fmaxf(fmaxf(t00.y, t01.y), fmaxf(t10.y, t11.y));
     const float m = 1e-4f;
-    return !(zhi + m > dmin - mu_lo && zlo - m < dmax + mu_hi);
+    const bool reach = zhi + m > dmin - mu_lo && zlo - m < dmax + mu_hi;
+    return okz && fin && (off || (small && !reach));
 }
 
 // Reduced-arithmetic integrate (KF_OPT_FAST_MATH).  Same update rule as
@@ -1288,15 +1295,30 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
     const size_t brow = (size_t)(y >> 3) * a.NB[0] + (x >> 3);
     const size_t bplane = (size_t)a.NB[0] * a.NB[1];
 
-    for (int zb = zs; zb < ze; zb += kIntU) {
+    for (int zg = zs; zg < ze; zg += kIntGroup * kIntU) {
+        // the skip tests of the group's chunks first (their loads together):
+        // a brick with a clear zero flag holds only (0, 0), so a chunk in one
+        // that no depth can reach keeps (0, 0) and needs no work
+        unsigned need = 0u, zfs = 0u;
+#pragma unroll
+        for (int j = 0; j < kIntGroup; ++j) {
+            const int zb = zg + j * kIntU;
+            const int za = zb < ze ? zb : zg;  // an address inside the column either way
+            const bool zf = a.zflags[(size_t)(za >> 3) * bplane + brow] != 0;
+            const bool unr = chunk_unreachable(a, c, fmaf((float)za, c.dl[0], c.pos[0]),
+                                               fmaf((float)za, c.dl[1], c.pos[1]), fmaf((float)za, c.dl[2], c.pos[2]),
+                                               fx, fy, cxh, cyh, mu_lo, mu_hi);
+            if (zb < ze && !(unr && !zf)) need |= 1u << j;
+            if (zf) zfs |= 1u << j;
+        }
+      for (int j = 0; j < kIntGroup; ++j) {
+        if (!((need >> j) & 1u)) continue;
+        const int zb = zg + j * kIntU;
         float p0 = fmaf((float)zb, c.dl[0], c.pos[0]);
         float p1 = fmaf((float)zb, c.dl[1], c.pos[1]);
         float p2 = fmaf((float)zb, c.dl[2], c.pos[2]);
-        // a brick with a clear zero flag holds only (0, 0): its voxels need not
-        // be read, and writing (0, 0) to them changes nothing
         const size_t bidx = (size_t)(zb >> 3) * bplane + brow;
-        const bool zf = a.zflags[bidx] != 0;
-        if (chunk_unreachable(a, c, p0, p1, p2, fx, fy, cxh, cyh, mu_lo, mu_hi) && !zf) continue;
+        const bool zf = (zfs >> j) & 1u;
         int pix[kIntU];
         float pz[kIntU], fac[kIntU];
 #pragma unroll
@@ -1366,6 +1388,7 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
         }
         if (neg) a.bflags[bidx] = 1;
         if (nz && !zf) a.zflags[bidx] = 1;
+      }
     }
 }
 
