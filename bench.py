@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--volume", type=int, default=512)
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-frames", type=int, default=6)
+    ap.add_argument("--cpu-frames", type=int, default=60)
     ap.add_argument("--fast-math", type=int, default=1)
     ap.add_argument("--no-serve", action="store_true", help="skip the batched and served legs")
     ap.add_argument("--batch-ticks", type=int, default=100)
