@@ -249,6 +249,13 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         raycast only).  Same results.  Default 0 (the
  *                         queue loses L2 locality and is slower here). */
 #define KF_OPT_RAYCAST_SCHEDULE 14
+/*   KF_OPT_RAYCAST_BUDGET  a ray still marching after this many march-loop
+ *                         iterations is saved and finished by a second
+ *                         launch, with other long rays (one-lane,
+ *                         one-block-per-tile raycast).  0 = never.  Same
+ *                         results.  Default 0 (the second launch's long
+ *                         dependent chains cost more than the tail saved). */
+#define KF_OPT_RAYCAST_BUDGET 15
 /*   KF_OPT_TRACK_MODE     tracking of the frame loop (kinect.h:143-144):
  *                         KF_TRACK_POINT_TO_PLANE (Track, default),
  *                         KF_TRACK_POINT_TO_VOLUME (TrackSDF) or both (Track

@@ -48,6 +48,7 @@ OPT_RAYCAST_LANES = 11
 OPT_ICP_PERSISTENT = 12
 OPT_ICP_SOLO_ROWS = 13
 OPT_RAYCAST_SCHEDULE = 14
+OPT_RAYCAST_BUDGET = 15
 
 TRACK_POINT_TO_PLANE = 1
 TRACK_POINT_TO_VOLUME = 2

@@ -1393,7 +1393,11 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
 }
 
 template <class S, bool OFF32>
-__global__ __launch_bounds__(256) void k_integrate_fast(IntegrateArgs a, const DevState *__restrict__ st)
+#ifndef KF_INT_WAVES_PER_SIMD
+#define KF_INT_WAVES_PER_SIMD 6
+#endif
+__global__ __launch_bounds__(256, KF_INT_WAVES_PER_SIMD) void k_integrate_fast(IntegrateArgs a,
+                                                                              const DevState *__restrict__ st)
 {
     integrate_fast_body<S, OFF32>(a, st, blockIdx.x, blockIdx.y, blockIdx.z);
 }
@@ -1788,7 +1792,8 @@ __device__ __forceinline__ void raycast_view(const RaycastArgs &a, const DevStat
 
 // The ray of pixel (x, y): march, hit refinement, normal, map writes.
 template <class S, bool TOUCH>
-__device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float (&view)[16], int x, int y)
+__device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float (&view)[16], int x, int y,
+                                              const RayState *resume = nullptr)
 {
     VolView<S, TOUCH> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
                  (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32, a.touch};
@@ -1822,8 +1827,9 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
     float hp[3], th = 0.0f;
     unsigned n_eval = 0, n_skip = 0, n_brick = 0;  // statistics mode only
     if (tnear < tfar) {
-        float t = tnear, stepsize = a.large_step;
-        float f_t = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
+        // a ray resumed from a deferred march state continues at its loop top
+        float t = resume ? resume->t : tnear, stepsize = resume ? resume->step : a.large_step;
+        float f_t = resume ? resume->f_t : interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
         float f_tt = 0.0f;
         if (f_t >= 0.0f) {
             // Empty-space skipping.  A sample whose lower trilinear corner lies
@@ -1837,15 +1843,30 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
             // 1 / ds: the exit bound below is a skip limit with a 0.01-voxel
             // margin, far wider than the ulp a reciprocal adds
             const float rds[3] = {1.0f / ds[0], 1.0f / ds[1], 1.0f / ds[2]};
-            float t_prev = t;
-            bool have_prev = true;
-            n_eval++;
-            int guard = 0;
+            float t_prev = resume ? resume->t_prev : t;
+            bool have_prev = resume ? (resume->guard_prev < 0) : true;
+            if (!resume) n_eval++;
+            int guard = resume ? (resume->guard_prev & 0x7fffffff) : 0;
+            int iters = 0;
             // flags of the 32^3 cell and 8^3 brick of the next sample, loaded
             // together with the previous fine-step batch (have_flags)
             uint8_t cf_n = 1, ff_n = 1;
             bool have_flags = false;
             for (; t < tfar && guard < kMaxMarchSteps; ++guard) {
+                if (a.budget > 0 && !resume && ++iters > a.budget) {
+                    // a long ray: its state goes to the deferred list and a
+                    // second launch finishes it (k_raycast_resume)
+                    const unsigned k = atomicAdd(a.defer_n, 1u);
+                    RayState r;
+                    r.pixel = y * a.w + x;
+                    r.t = t;
+                    r.t_prev = t_prev;
+                    r.f_t = f_t;
+                    r.step = stepsize;
+                    r.guard_prev = guard | (have_prev ? (int)0x80000000u : 0);
+                    a.defer[k] = r;
+                    return;
+                }
                 const bool can_skip = a.skip && stepsize == a.step;
                 Sample sm;
                 sample_setup(a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t, sm);
@@ -2273,6 +2294,32 @@ template <class S>
 __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *__restrict__ st)
 {
     raycast_body<S>(a, st, blockIdx.x);
+}
+
+// The rays k_raycast deferred (more march-loop iterations than a.budget):
+// each lane resumes one from its saved loop-top state and finishes it exactly
+// as k_raycast would have.  Long rays then share wavefronts with other long
+// rays instead of holding up a wavefront of short ones.  The last wavefront
+// out empties the list for the next launch.
+template <class S>
+__global__ __launch_bounds__(256) void k_raycast_resume(RaycastArgs a, const DevState *__restrict__ st)
+{
+    float view[16];
+    raycast_view(a, st, view);
+    const unsigned n = *a.defer_n;  // every append of the k_raycast launch before this one
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const RayState r = a.defer[i];
+        if ((unsigned)r.pixel < (unsigned)(a.w * a.h))
+            raycast_pixel<S, false>(a, view, r.pixel % a.w, r.pixel / a.w, &r);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const unsigned t = atomicAdd(a.defer_n + 1, 1u);
+        if (t == gridDim.x * (blockDim.x / 64u) - 1u) {
+            a.defer_n[0] = 0u;
+            a.defer_n[1] = 0u;
+        }
+    }
 }
 
 // The same rays with dynamic scheduling: the grid is a fixed number of
@@ -2708,6 +2755,20 @@ hipError_t launch_raycast_touch(const RaycastArgs &a, int storage, const DevStat
     default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k_popcount, dim3(1024), dim3(256), 0, s, a.touch, words, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_raycast_resume(const RaycastArgs &a, int storage, const DevState *st, int nblk, hipStream_t s,
+                                 hipEvent_t t1)
+{
+    if (a.budget <= 0 || !a.defer || !a.defer_n || nblk < 1) return hipErrorInvalidValue;
+    const dim3 g(nblk), b(256);
+    switch (storage) {
+    case KF_STORAGE_RG16I: launch_timed(k_raycast_resume<StoreRG16I>, g, b, s, nullptr, t1, a, st); break;
+    case KF_STORAGE_F32: launch_timed(k_raycast_resume<StoreF32>, g, b, s, nullptr, t1, a, st); break;
+    case KF_STORAGE_F16: launch_timed(k_raycast_resume<StoreF16>, g, b, s, nullptr, t1, a, st); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 
 #include "../../include/kfusion.h"
 #include "kf_device.hpp"
+#include "kf_internal.hpp"
 
 struct kf_session {
     kf_config cfg;
@@ -48,6 +49,9 @@ struct kf_session {
     unsigned *ray_order = nullptr;  // packet order
     unsigned *ray_cost = nullptr;   // per-packet durations of the last dynamic raycast
     int ray_npk = 0;                // packets (64 rays each)
+    int ray_budget = 0;             // raycast: march-loop iterations before a ray is deferred (0 = never)
+    kf::RayState *ray_defer = nullptr;  // deferred rays, one slot per pixel
+    unsigned *ray_defer_n = nullptr;    // their count, exit ticket
     int icp_solo_rows = 0;   // levels with at most this many rows per row group: one block per group
     int cus = 0;             // compute units of the device
     // frame graph: ICP + gate + integrate + raycast, captured once
