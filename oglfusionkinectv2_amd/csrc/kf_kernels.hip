@@ -493,10 +493,45 @@ __device__ __forceinline__ void icp_view(const IcpArgs &a, const float T[16], fl
     mul4(K, ip, view);
 }
 
+// pixels of this thread in a block's rows (one chunk of kIcpPx at most when
+// icp_one_chunk holds)
+__device__ __forceinline__ int icp_block_pixels(const IcpArgs &a, int g, int jb)
+{
+    const int y0 = g + 8 * jb, ystep = 8 * a.rows_step;
+    const int per_row = (a.w + kIcpThreads - 1) / kIcpThreads;
+    const int nrows = y0 < a.h ? (a.h - 1 - y0) / ystep + 1 : 0;
+    return nrows * per_row;
+}
+
+// the map texels of the thread's first kIcpPx pixels (normal (2, 0, 0, 0) and
+// vertex 0 beyond its pixels): what icp_block_partials loads first
+__device__ __forceinline__ void icp_load_pixels(const IcpArgs &a, int g, int jb, float4 (&nv)[kIcpPx],
+                                                float4 (&vv)[kIcpPx])
+{
+    const int y0 = g + 8 * jb, ystep = 8 * a.rows_step;
+    const int per_row = (a.w + kIcpThreads - 1) / kIcpThreads;
+    const int npx = icp_block_pixels(a, g, jb);
+#pragma unroll
+    for (int p = 0; p < kIcpPx; ++p) {
+        const int r = p / per_row, m = p - r * per_row;
+        const int x = threadIdx.x + kIcpThreads * m, y = y0 + ystep * r;
+        nv[p] = make_float4(2.0f, 0.0f, 0.0f, 0.0f);
+        vv[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (p < npx && x < a.w) {
+            nv[p] = a.norm[y * a.w + x];
+            vv[p] = a.vert[y * a.w + x];
+        }
+    }
+}
+
 // this block's 28 sums at pose T written to dst[0..27] (threads < 28); ends
-// with the block synchronised.  g: row group, jb: block within the group
+// with the block synchronised.  g: row group, jb: block within the group.
+// CACHED: the block has at most kIcpPx pixels per thread and cn / cv hold
+// their map texels (icp_load_pixels), kept over the iterations of a level.
+template <bool CACHED = false>
 __device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float T[16], int g, int jb, float *dst,
-                                                   float (*wsum)[kTerms])
+                                                   float (*wsum)[kTerms], const float4 *cn = nullptr,
+                                                   const float4 *cv = nullptr)
 {
     float view[16];
     icp_view(a, T, view);
@@ -526,7 +561,10 @@ __device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float
         for (int p = 0; p < kIcpPx; ++p) {
             nv[p] = make_float4(2.0f, 0.0f, 0.0f, 0.0f);
             vv[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (idx[p] >= 0) {
+            if (CACHED) {
+                nv[p] = cn[p];
+                vv[p] = cv[p];
+            } else if (idx[p] >= 0) {
                 nv[p] = a.norm[idx[p]];
                 vv[p] = a.vert[idx[p]];
             }
@@ -587,46 +625,49 @@ __device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float
     __syncthreads();
 }
 
-// FP64 sums per row group of the partial rows 0..nrows-1 into gsum (row r
-// holds a block of row group r % 8; with group_only >= 0 only that group's
-// rows were written and the other groups sum to zero).  Thread t sums partial
-// rows t, t + 256, ..., lanes of the same group are combined with xor
-// shuffles (8, 16, 32), then the four waves through LDS.  Fixed order.  Ends
-// with the block synchronised.
+// FP64 sums per row group of the partial rows 0..nrows-1 into gsum (row
+// 8 jb + g holds block jb of row group g; with group_only >= 0 only that
+// group's rows were written and the other groups sum to zero).  For each
+// group, thread (j, k) = (tid / 32, tid % 32) sums term k of the rows
+// jb = j, j + 8, j + 16, ... in order (coalesced device-coherent loads, all
+// issued before the adds), the two halves of a wavefront are added with one
+// xor-32 shuffle and the four waves through LDS: a fixed order, the same in
+// every block and for a group whether or not the other groups are summed.
+// Ends with the block synchronised.
 __device__ __forceinline__ void icp_group_sums(const float *partials, int nrows, int group_only,
                                                double (*wgsum)[8][kTerms], double (*gsum)[kTerms])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    {
-        double ps[kTerms];
-#pragma unroll
-        for (int k = 0; k < kTerms; ++k) ps[k] = 0.0;
-        const bool mine = group_only < 0 || (threadIdx.x & 7) == group_only;
-        for (int pb = threadIdx.x; mine && pb < nrows; pb += kIcpThreads) {
-            // device-coherent loads (the rows other blocks stored the same way)
-            const float *rp = partials + pb * kPartStride;
-            float q[kTerms];
-#pragma unroll
-            for (int k = 0; k < kTerms; ++k)
-                q[k] = __hip_atomic_load(rp + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-            for (int k = 0; k < kTerms; ++k) ps[k] += (double)q[k];
+    const int j = threadIdx.x >> 5, k = threadIdx.x & 31;
+    const int per_group = nrows >> 3;
+    for (int g = 0; g < 8; ++g) {
+        if (group_only >= 0 && g != group_only) {  // uniform
+            if (lane < kTerms) wgsum[wave][g][lane] = 0.0;
+            continue;
         }
+        double ps = 0.0;
+        if (k < kTerms) {
+            for (int jb0 = j; jb0 < per_group; jb0 += 64) {
+                float q[8];
 #pragma unroll
-        for (int k = 0; k < kTerms; ++k) {
-            ps[k] += __shfl_xor(ps[k], 8, 64);
-            ps[k] += __shfl_xor(ps[k], 16, 64);
-            ps[k] += __shfl_xor(ps[k], 32, 64);
-        }
-        if (lane < 8) {
+                for (int u = 0; u < 8; ++u) {
+                    const int jb = jb0 + 8 * u;
+                    q[u] = jb < per_group ? __hip_atomic_load(partials + (8 * jb + g) * kPartStride + k,
+                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                          : 0.0f;
+                }
 #pragma unroll
-            for (int k = 0; k < kTerms; ++k) wgsum[wave][lane][k] = ps[k];
+                for (int u = 0; u < 8; ++u)
+                    if (jb0 + 8 * u < per_group) ps += (double)q[u];
+            }
         }
+        ps += __shfl_xor(ps, 32, 64);
+        if (lane < kTerms) wgsum[wave][g][lane] = ps;
     }
     __syncthreads();
     if (threadIdx.x < 8 * kTerms) {
-        const int g = threadIdx.x / kTerms, k = threadIdx.x % kTerms;
-        gsum[g][k] = (wgsum[0][g][k] + wgsum[1][g][k]) + (wgsum[2][g][k] + wgsum[3][g][k]);
+        const int g = threadIdx.x / kTerms, t = threadIdx.x % kTerms;
+        gsum[g][t] = (wgsum[0][g][t] + wgsum[1][g][t]) + (wgsum[2][g][t] + wgsum[3][g][t]);
     }
     __syncthreads();
 }
@@ -899,16 +940,22 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
         }
         solo_before = solo;
         if (solo && blk != 0) continue;
+        const int g = L.group_only >= 0 ? L.group_only : (blk & 7);
+        const int jb = L.group_only >= 0 ? blk : (blk >> 3);
+        // a block with one chunk of pixels per thread keeps their vertex and
+        // normal in registers for the level (they do not depend on the pose)
+        float4 cn[kIcpPx], cv[kIcpPx];
+        const bool cached = blk < nb && icp_block_pixels(L, g, jb) <= kIcpPx;  // uniform
+        if (cached) icp_load_pixels(L, g, jb, cn, cv);
         for (int it = 0; it < a.iters[level]; ++it) {
             float T[16];
 #pragma unroll
             for (int i = 0; i < 16; ++i) T[i] = Ts[i];
             float *part = solo ? solo_rows : a.partials[phase & 1u];
-            if (blk < nb) {
-                const int g = L.group_only >= 0 ? L.group_only : (blk & 7);
-                const int jb = L.group_only >= 0 ? blk : (blk >> 3);
+            if (cached)
+                icp_block_partials<true>(L, T, g, jb, part + (8 * jb + g) * kPartStride, wsum, cn, cv);
+            else if (blk < nb)
                 icp_block_partials(L, T, g, jb, part + (8 * jb + g) * kPartStride, wsum);
-            }
             if (!solo) {
                 if (!grid_barrier_coherent(&st->icp_bar, &st->icp_abort, (phase + 1u) * (unsigned)nblk,
                                            a.spin_limit, &sh_abort)) {
