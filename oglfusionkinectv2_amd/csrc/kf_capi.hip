@@ -99,7 +99,7 @@ size_t session_bytes(const kf_config *c)
     b += 2 * 16 * (p0 + p1 + p2);                  // vertex + normal pyramids
     b += 2 * 16 * p0;                              // reference maps
     b += vox * 2 * voxel_plane_bytes(c->storage);  // TSDF value + weight planes
-    b += sizeof(kf::DevState) + 2 * (size_t)(c->height + 8) * 32 * 4 + 64;  // state, ICP partial rows
+    b += sizeof(kf::DevState) + 2 * (size_t)(c->height + 8) * 32 * (4 + 8) + 64;  // state, ICP partial rows (plain, tagged)
     b += 8 + 2 * 4 * (size_t)4 * ((c->width + 15) / 16) * ((c->height + 15) / 16);  // raycast queue
     b += 8 + p0 * sizeof(kf::RayState);                                               // deferred rays
     b += (size_t)((c->width + 7) / 8) * ((c->height + 7) / 8) * 8;  // depth tiles
@@ -130,7 +130,7 @@ void free_all(kf_session *s)
         f(s->vert[l]);
         f(s->norm[l]);
     }
-    f(s->refV); f(s->refN); f(s->tsdf); f(s->weight); f(s->st); f(s->block_partials); f(s->counts);
+    f(s->refV); f(s->refN); f(s->tsdf); f(s->weight); f(s->st); f(s->block_partials); f(s->icp_rows); f(s->counts);
     f(s->dbg_result); f(s->dbg_error); f(s->dbg_J);
     f(s->library); f(s->sdf_partials); f(s->sdf_slots); f(s->ray_q); f(s->ray_order); f(s->ray_cost);
     f(s->ray_defer); f(s->ray_defer_n);
@@ -350,8 +350,8 @@ int run_track(kf_session *s, int frame)
         }
         if (total > 0 && nblk <= s->cus) {
             a.frame = frame;
-            a.partials[0] = s->block_partials;
-            a.partials[1] = s->block_partials + (size_t)(s->cfg.height + 8) * 32;
+            a.rows[0] = s->icp_rows;
+            a.rows[1] = s->icp_rows + (size_t)(s->cfg.height + 8) * 32;
             a.spin_limit = 1u << 22;
             hipError_t e = kf::launch_icp_track(a, s->st, nblk, s->stream, tev(s, 2), tev(s, 3));
             if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("icp launch: ") + hipGetErrorString(e));
@@ -624,6 +624,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
               alloc((void **)&s->refN, p0 * 16) && alloc(&s->tsdf, vb) && alloc(&s->weight, vb) &&
               alloc((void **)&s->st, sizeof(kf::DevState)) &&
               alloc((void **)&s->block_partials, 2 * (size_t)(cfg->height + 8) * 32 * 4) &&
+              alloc((void **)&s->icp_rows, 2 * (size_t)(cfg->height + 8) * 32 * 8) &&
               alloc((void **)&s->counts, 8 * sizeof(unsigned long long)) &&
               alloc((void **)&s->library, (size_t)kf::kPoseLibraryMax * 64) &&
               alloc((void **)&s->sdf_partials, 2 * 256 * 32 * 4) &&
@@ -653,6 +654,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
                             s->stream) == hipSuccess &&
              hipMemsetAsync(s->cflags, 0, s->ncoarse, s->stream) == hipSuccess &&
              hipMemsetAsync(s->st, 0, sizeof(kf::DevState), s->stream) == hipSuccess &&
+             hipMemsetAsync(s->icp_rows, 0, 2 * (size_t)(cfg->height + 8) * 32 * 8, s->stream) == hipSuccess &&
              hipMemsetAsync(s->library, 0, (size_t)kf::kPoseLibraryMax * 64, s->stream) == hipSuccess &&
              hipMemsetAsync(s->ray_q, 0, 2 * sizeof(unsigned), s->stream) == hipSuccess &&
              hipMemsetAsync(s->ray_order, 0, (size_t)s->ray_npk * sizeof(unsigned), s->stream) == hipSuccess &&
@@ -956,6 +958,13 @@ int kf_set_option(kf_session *s, int option, int value)
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     drop_graph(s);  // captured launches carry the old arguments
+    if (option == KF_OPT_ICP_ROWS_PER_BLOCK || option == KF_OPT_ICP_ALL_GROUPS || option == KF_OPT_ICP_SOLO_ROWS) {
+        // the persistent Track() may now read partial rows that no launch of
+        // the old layout wrote: clear their tags (a row is valid only when it
+        // carries the current launch's tag, and a row left from an old layout
+        // could carry a wrapped-around one)
+        KF_HIP(hipMemsetAsync(s->icp_rows, 0, 2 * (size_t)(s->cfg.height + 8) * 32 * 8, s->stream));
+    }
     return KF_OK;
 }
 

@@ -49,6 +49,7 @@ struct DevState {
     unsigned int icp_bar;     // its grid barrier arrivals
     unsigned int icp_exit;    // its exit ticket
     int icp_abort;            // set by a block whose barrier wait timed out
+    unsigned int icp_epoch;   // launches of that kernel so far (tags its partial rows)
 };
 
 // Grid barrier of a persistent launch whose blocks are all resident at once:
@@ -123,7 +124,7 @@ __device__ __forceinline__ bool grid_barrier_coherent(unsigned int *bar, int *ab
 // Exit ticket of such a launch: the last block out clears the barrier state
 // for the next launch.  All threads of the block call it.
 __device__ __forceinline__ void grid_exit(unsigned int *bar, unsigned int *exit_count, int *abort_flag,
-                                          unsigned int nblk)
+                                          unsigned int nblk, unsigned int *epoch = nullptr)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -132,6 +133,7 @@ __device__ __forceinline__ void grid_exit(unsigned int *bar, unsigned int *exit_
         if (t == nblk - 1u) {
             *bar = 0u;
             *abort_flag = 0;
+            if (epoch) *epoch += 1u;
             __threadfence();
             *exit_count = 0u;
         }

@@ -48,8 +48,8 @@ struct IcpTrackArgs {
     IcpArgs lv[3];
     int iters[3];
     int frame;                  // apply the frame gate at the end
-    float *partials[2];
-    unsigned spin_limit;        // grid-barrier wait bound (polls)
+    unsigned long long *rows[2];  // partial rows, alternately: per term (tag << 32) | float bits
+    unsigned spin_limit;        // grid-barrier / row wait bound (polls)
 };
 
 struct IntegrateArgs {

@@ -528,10 +528,13 @@ __device__ __forceinline__ void icp_load_pixels(const IcpArgs &a, int g, int jb,
 // with the block synchronised.  g: row group, jb: block within the group.
 // CACHED: the block has at most kIcpPx pixels per thread and cn / cv hold
 // their map texels (icp_load_pixels), kept over the iterations of a level.
+// dst64: the row is written as tagged words, (tag << 32) | float bits, for
+// readers that poll it (icp_group_sums_tagged); dst is then unused.
 template <bool CACHED = false>
 __device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float T[16], int g, int jb, float *dst,
                                                    float (*wsum)[kTerms], const float4 *cn = nullptr,
-                                                   const float4 *cv = nullptr)
+                                                   const float4 *cv = nullptr, unsigned long long *dst64 = nullptr,
+                                                   unsigned tag = 0u)
 {
     float view[16];
     icp_view(a, T, view);
@@ -617,10 +620,14 @@ __device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float
     __syncthreads();
     if (threadIdx.x < kTerms) {
         const int k = threadIdx.x;
+        const float v = (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]);
         // device-coherent store: other blocks read the row after a grid
-        // barrier that does not write back or invalidate the L2
-        __hip_atomic_store(&dst[k], (wsum[0][k] + wsum[1][k]) + (wsum[2][k] + wsum[3][k]), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        // barrier that does not write back or invalidate the L2, or poll it
+        if (dst64)
+            __hip_atomic_store(&dst64[k], ((unsigned long long)tag << 32) | __float_as_uint(v), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        else
+            __hip_atomic_store(&dst[k], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
 }
@@ -670,6 +677,79 @@ __device__ __forceinline__ void icp_group_sums(const float *partials, int nrows,
         gsum[g][t] = (wgsum[0][g][t] + wgsum[1][g][t]) + (wgsum[2][g][t] + wgsum[3][g][t]);
     }
     __syncthreads();
+}
+
+// icp_group_sums over rows of tagged words that other blocks of the same
+// launch are writing: a term is taken once its word carries `tag` (polled
+// device-coherent loads; no barrier, and no ordering between words is
+// needed since each word holds its own tag).  Same order as icp_group_sums,
+// so the same bits.  Returns false when a wait exceeded spin_limit polls
+// (the launch's abort flag is then raised) or the flag was raised by another
+// block.  Ends with the block synchronised.
+__device__ __forceinline__ bool icp_group_sums_tagged(unsigned long long *rows, int nrows, int group_only,
+                                                      unsigned tag, unsigned spin_limit, int *abort_flag,
+                                                      double (*wgsum)[8][kTerms], double (*gsum)[kTerms])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = threadIdx.x >> 5, k = threadIdx.x & 31;
+    const int per_group = nrows >> 3;
+    bool ab = false;
+    for (int g = 0; g < 8; ++g) {
+        if (group_only >= 0 && g != group_only) {  // uniform
+            if (lane < kTerms) wgsum[wave][g][lane] = 0.0;
+            continue;
+        }
+        double ps = 0.0;
+        if (k < kTerms) {
+            for (int jb0 = j; jb0 < per_group; jb0 += 64) {
+                unsigned long long q[8];
+                unsigned pending = 0u;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int jb = jb0 + 8 * u;
+                    q[u] = 0ull;
+                    if (jb < per_group) {
+                        q[u] = __hip_atomic_load(rows + (8 * jb + g) * kPartStride + k, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+                        if ((unsigned)(q[u] >> 32) != tag) pending |= 1u << u;
+                    }
+                }
+                unsigned n = 0;
+                while (pending != 0u && !ab) {
+                    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        ab = true;
+                        break;
+                    }
+                    if (++n > spin_limit) {
+                        __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ab = true;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (pending & (1u << u)) {
+                            q[u] = __hip_atomic_load(rows + (8 * (jb0 + 8 * u) + g) * kPartStride + k,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if ((unsigned)(q[u] >> 32) == tag) pending &= ~(1u << u);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (jb0 + 8 * u < per_group) ps += (double)__uint_as_float((unsigned)q[u]);
+            }
+        }
+        ps += __shfl_xor(ps, 32, 64);
+        if (lane < kTerms) wgsum[wave][g][lane] = ps;
+    }
+    const int any_ab = __syncthreads_or(ab ? 1 : 0);
+    if (threadIdx.x < 8 * kTerms) {
+        const int g = threadIdx.x / kTerms, t = threadIdx.x % kTerms;
+        gsum[g][t] = (wgsum[0][g][t] + wgsum[1][g][t]) + (wgsum[2][g][t] + wgsum[3][g][t]);
+    }
+    __syncthreads();
+    return any_ab == 0;
 }
 
 // exposed 8x32 partials in the reduce.cs layout (one slot per thread);
@@ -886,16 +966,16 @@ __global__ void k_icp_finish(DevState *st, int frame, float *lib, int lib_max)
 }
 
 // Track() (gFusion.cpp:648-740) in one launch: every level (coarse to fine)
-// and iteration with a grid barrier between the partial sums of an iteration
-// and their use.  A block computes the partial row k_icp_iteration's block of
-// the same index would (blocks beyond the level's count only take part in the
-// barriers); after the barrier every block forms the same FP64 group sums
-// from the same rows in the same order, solves the same system and updates
-// the same pose, so the blocks agree on the pose and on the early exit
-// (gFusion.cpp:707-708) without a second barrier, and an exited level costs
-// nothing.  Partial rows alternate between two buffers: a block can be at
-// most one iteration ahead of the slowest reader.  Results are those of the
-// k_icp_iteration sequence bit for bit.
+// and iteration.  A block computes the partial row k_icp_iteration's block of
+// the same index would (blocks beyond the level's count only read) and
+// writes it as tagged words; every block then polls all rows until each word
+// carries the iteration's tag (this exchange is the iteration's only
+// synchronisation), forms the same FP64 group sums from the same rows in the
+// same order, solves the same system and updates the same pose, so the
+// blocks agree on the pose and on the early exit (gFusion.cpp:707-708), and
+// an exited level costs nothing.  Partial rows alternate between two
+// buffers: a block can be at most one iteration ahead of the slowest reader.
+// Results are those of the k_icp_iteration sequence bit for bit.
 __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevState *__restrict__ st)
 {
     __shared__ float wsum[kIcpThreads / 64][kTerms];
@@ -915,7 +995,11 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
     __syncthreads();
     if (blk == 0 && tid < 16) st->old_pose[tid] = Ts[tid];  // oldPose (gFusion.cpp:653)
 
-    unsigned phase = 0;
+    // partial rows of exchange n carry the tag (launch << 8) + n + 1 (at
+    // most 3 x 64 exchanges per launch), so a row is current exactly when its
+    // words carry this exchange's tag
+    const unsigned epoch = __hip_atomic_load(&st->icp_epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned phase = 0, xchg = 0;  // grid barriers and row exchanges so far
     int iters_run = 0;
     bool aborted = false, solo_before = false;
     for (int level = 2; level >= 0 && !aborted; --level) {
@@ -951,20 +1035,28 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
             float T[16];
 #pragma unroll
             for (int i = 0; i < 16; ++i) T[i] = Ts[i];
-            float *part = solo ? solo_rows : a.partials[phase & 1u];
-            if (cached)
-                icp_block_partials<true>(L, T, g, jb, part + (8 * jb + g) * kPartStride, wsum, cn, cv);
-            else if (blk < nb)
-                icp_block_partials(L, T, g, jb, part + (8 * jb + g) * kPartStride, wsum);
-            if (!solo) {
-                if (!grid_barrier_coherent(&st->icp_bar, &st->icp_abort, (phase + 1u) * (unsigned)nblk,
-                                           a.spin_limit, &sh_abort)) {
+            if (solo) {
+                // block 0 alone: the row stays in LDS
+                icp_block_partials(L, T, g, jb, solo_rows + (8 * jb + g) * kPartStride, wsum);
+                icp_group_sums(solo_rows, 8 * L.rows_step, L.group_only, wgsum, gsum);
+            } else {
+                // rows alternate between two buffers: a block writes exchange
+                // n + 2 only after reading every row of n + 1, which every
+                // block wrote after reading all of n
+                unsigned long long *rows = a.rows[xchg & 1u];
+                const unsigned tag = (epoch << 8) + xchg + 1u;
+                unsigned long long *mine = rows + (8 * jb + g) * kPartStride;
+                if (cached)
+                    icp_block_partials<true>(L, T, g, jb, nullptr, wsum, cn, cv, mine, tag);
+                else if (blk < nb)
+                    icp_block_partials(L, T, g, jb, nullptr, wsum, nullptr, nullptr, mine, tag);
+                if (!icp_group_sums_tagged(rows, 8 * L.rows_step, L.group_only, tag, a.spin_limit, &st->icp_abort,
+                                           wgsum, gsum)) {
                     aborted = true;
                     break;
                 }
-                ++phase;
+                ++xchg;
             }
-            icp_group_sums(part, 8 * L.rows_step, L.group_only, wgsum, gsum);
             if (blk == 0) icp_expose(L, gsum, st->partials);
             if (tid < 64) {
                 double x[6] = {0, 0, 0, 0, 0, 0};
@@ -1008,7 +1100,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
         __syncthreads();
         if (tid < 64) icp_finish_wave(st, a.frame, a.lv[0].library, a.lv[0].lib_max);
     }
-    grid_exit(&st->icp_bar, &st->icp_exit, &st->icp_abort, (unsigned)nblk);
+    grid_exit(&st->icp_bar, &st->icp_exit, &st->icp_abort, (unsigned)nblk, &st->icp_epoch);
 }
 
 // flags: kStateSetPose, kStateArmWindow (the 4-frame integrate window of

@@ -25,6 +25,7 @@ struct kf_session {
     void *tsdf = nullptr, *weight = nullptr;
     kf::DevState *st = nullptr;
     float *block_partials = nullptr;
+    unsigned long long *icp_rows = nullptr;  // tagged partial rows of the persistent Track(): 2 x (height + 8) x 32
     unsigned long long *counts = nullptr;
     uint8_t *bflags = nullptr, *dflags = nullptr, *cflags = nullptr, *zflags = nullptr;
     float2 *dtiles = nullptr;  // 8x8-pixel (min, max) of the level-0 depth in metres
