@@ -264,7 +264,8 @@ __device__ __forceinline__ float dpp_move(float x)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xF, false));
 }
 
-__device__ __forceinline__ float wave_sum(float v)
+// the wavefront's sum of v, complete in lane 63
+__device__ __forceinline__ float wave_sum_lane63(float v)
 {
     v += dpp_move<0xB1, 0xF>(v);  // quad_perm(1,0,3,2)
     v += dpp_move<0x4E, 0xF>(v);  // quad_perm(2,3,0,1)
@@ -272,7 +273,27 @@ __device__ __forceinline__ float wave_sum(float v)
     v += dpp_move<0x128, 0xF>(v); // row_ror:8
     v += dpp_move<0x142, 0xA>(v); // row_bcast:15 into rows 1 and 3
     v += dpp_move<0x143, 0xC>(v); // row_bcast:31 into rows 2 and 3
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wave_sum_lane63(v)), 63));
+}
+
+// out[k] = wave_sum(v[k]) for every k, written by lane 63: the same
+// additions, with no branch between the terms so that their independent
+// DPP chains interleave
+template <int N>
+__device__ __forceinline__ void wave_sums_store(const float (&v)[N], float *out)
+{
+    float s[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) s[k] = wave_sum_lane63(v[k]);
+    if ((threadIdx.x & 63) == 63) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) out[k] = s[k];
+    }
 }
 
 }  // namespace kf

@@ -611,12 +611,7 @@ __device__ __forceinline__ void icp_block_partials(const IcpArgs &a, const float
         }
     }
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < kTerms; ++k) {
-        const float s = wave_sum(acc[k]);
-        if (lane == 0) wsum[wave][k] = s;
-    }
+    wave_sums_store(acc, wsum[threadIdx.x >> 6]);
     __syncthreads();
     if (threadIdx.x < kTerms) {
         const int k = threadIdx.x;

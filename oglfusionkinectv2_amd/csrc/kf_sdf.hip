@@ -383,11 +383,7 @@ __global__ __launch_bounds__(kSdfThreads) void k_sdf_track(SdfArgs a, DevState *
                             for (int s2 = p; s2 < 6; ++s2) acc[q++] += hh * rec[2 + p] * rec[2 + s2];
                     }
                 }
-#pragma unroll
-                for (int k = 0; k < kSdfTerms; ++k) {
-                    const float s = wave_sum(acc[k]);
-                    if (lane == 0) wsum[wave][k] = s;
-                }
+                wave_sums_store(acc, wsum[wave]);
                 __syncthreads();
                 float *part = a.partials + (size_t)(phase & 1u) * nblk * 32;
                 if (tid < kSdfTerms)
