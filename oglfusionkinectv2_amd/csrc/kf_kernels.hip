@@ -1686,19 +1686,20 @@ struct Sample {
     bool valid;
 };
 
+// p * N / dim - 0.5 on axis k.  When every dim is a power of two the
+// division is an exact scaling, done as a multiply by 1/dim (the same result
+// bit for bit).
+__device__ __forceinline__ float voxel_coord(const RaycastArgs &a, float p, int k)
+{
+    if (a.dim_pow2) return ((p * (float)a.N[k]) * a.inv_dim[k]) - 0.5f;
+    return (p * (float)a.N[k] / a.dim[k]) - 0.5f;
+}
+
 __device__ __forceinline__ void sample_setup(const RaycastArgs &a, float p0, float p1, float p2, Sample &sm)
 {
-    // p * N / dim; when every dim is a power of two the division is an exact
-    // scaling, done as a multiply by 1/dim (same result bit for bit)
-    if (a.dim_pow2) {
-        sm.s[0] = ((p0 * (float)a.N[0]) * a.inv_dim[0]) - 0.5f;
-        sm.s[1] = ((p1 * (float)a.N[1]) * a.inv_dim[1]) - 0.5f;
-        sm.s[2] = ((p2 * (float)a.N[2]) * a.inv_dim[2]) - 0.5f;
-    } else {
-        sm.s[0] = (p0 * (float)a.N[0] / a.dim[0]) - 0.5f;
-        sm.s[1] = (p1 * (float)a.N[1] / a.dim[1]) - 0.5f;
-        sm.s[2] = (p2 * (float)a.N[2] / a.dim[2]) - 0.5f;
-    }
+    sm.s[0] = voxel_coord(a, p0, 0);
+    sm.s[1] = voxel_coord(a, p1, 1);
+    sm.s[2] = voxel_coord(a, p2, 2);
     sm.valid = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -1734,10 +1735,12 @@ __device__ __forceinline__ float sample_eval(const VolView<S, TOUCH> &V, const S
                         (unsigned)lo[2] < (unsigned)V.N2 && (unsigned)hi[2] < (unsigned)V.N2;
     if (V.off32 && inside) {
         // all 8 corners in the volume: 32-bit element offsets from one base
-        const uint32_t b = ((uint32_t)lo[2] * V.plane32 + (uint32_t)lo[1] * (uint32_t)V.N0) + (uint32_t)lo[0];
+        // 24-bit products: lo, hi < N_k <= 2048 and plane32 = N0 N1 < 2^24
+        const uint32_t b = (__umul24((uint32_t)lo[2], V.plane32) + __umul24((uint32_t)lo[1], (uint32_t)V.N0)) +
+                           (uint32_t)lo[0];
         const uint32_t dx = (uint32_t)(hi[0] - lo[0]);
-        const uint32_t dy = (uint32_t)(hi[1] - lo[1]) * (uint32_t)V.N0;
-        const uint32_t dz = (uint32_t)(hi[2] - lo[2]) * V.plane32;
+        const uint32_t dy = __umul24((uint32_t)(hi[1] - lo[1]), (uint32_t)V.N0);
+        const uint32_t dz = __umul24((uint32_t)(hi[2] - lo[2]), V.plane32);
         c000 = V.ld(b);
         c100 = V.ld(b + dx);
         c010 = V.ld(b + dy);
@@ -1776,18 +1779,21 @@ __device__ __forceinline__ void sample_fast(const RaycastArgs &a, float p0, floa
 {
     const float p[3] = {p0, p1, p2};
     bool in = true;
+    uint32_t bi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float sk = a.dim_pow2 ? ((p[k] * (float)a.N[k]) * a.inv_dim[k]) - 0.5f
-                                    : (p[k] * (float)a.N[k] / a.dim[k]) - 0.5f;
+        const float sk = voxel_coord(a, p[k], k);
         const float fl = floorf(sk);
-        const int b = floor_index(fl);
-        q.b[k] = b;
+        q.b[k] = floor_index(fl);
         q.f[k] = sk - fl;
-        in = in && ((unsigned)b < (unsigned)(a.N[k] - 1));
+        // 0 <= b <= N - 2, tested on the integral float (false for NaN)
+        in = in && fl >= 0.0f && fl <= (float)(a.N[k] - 2);
+        bi[k] = (uint32_t)__float2int_rz(fl);  // = b when inside
     }
     q.inside = in;
-    q.base = ((uint32_t)q.b[2] * plane32 + (uint32_t)q.b[1] * (uint32_t)a.N[0]) + (uint32_t)q.b[0];
+    // 24-bit products: the base is used only when the sample is inside, where
+    // b_k < N_k <= 2048 and plane32 = N0 N1 < 2^24
+    q.base = (__umul24(bi[2], plane32) + __umul24(bi[1], (uint32_t)a.N[0])) + bi[0];
 }
 
 
@@ -2131,28 +2137,44 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
                         have_flags = true;
                     }
                 }
-                bool stop = false;
+                // consumed in march order: m is the first sample that ends
+                // the march (t past tfar, or a negative value), kBatch if
+                // none; the samples before it are taken (f >= 0 and < 0.8
+                // keep the fine step), the last of them becomes f_t / t_prev
+                int m = kBatch;
 #pragma unroll
-                for (int j = 0; j < kBatch; ++j) {
-                    if (!stop) {
-                        if (!(tj[j] < tfar)) {  // the march ends before this sample
-                            t = tj[j];
-                            stop = true;
-                        } else {
-                            f_tt = fv[j];
-                            n_eval++;
-                            if (f_tt < 0.0f) {
-                                t = tj[j];
-                                stop = true;
-                            } else {
-                                f_t = f_tt;  // f_tt < 0.8 keeps stepsize = step
-                                have_prev = true;
-                                t_prev = tj[j];
-                            }
+                for (int j = kBatch - 1; j >= 0; --j)
+                    if (!(tj[j] < tfar) || fv[j] < 0.0f) m = j;
+                if (m > 0) {
+                    float tp = tj[0], fp = fv[0];
+#pragma unroll
+                    for (int j = 1; j < kBatch; ++j)
+                        if (m > j) {
+                            tp = tj[j];
+                            fp = fv[j];
                         }
-                    }
+                    t_prev = tp;
+                    f_t = fp;
+                    have_prev = true;
                 }
-                if (stop) break;
+                if (m < kBatch) {
+                    float tm = tj[0], fm = fv[0];
+#pragma unroll
+                    for (int j = 1; j < kBatch; ++j)
+                        if (m == j) {
+                            tm = tj[j];
+                            fm = fv[j];
+                        }
+                    t = tm;
+                    if (tm < tfar) {  // a negative sample: the hit; else the march ended before it
+                        f_tt = fm;
+                        n_eval += m + 1;
+                    } else {
+                        n_eval += m;
+                    }
+                    break;
+                }
+                n_eval += kBatch;
                 t = t_next;
                 guard += kBatch - 1;
             }
