@@ -1131,10 +1131,6 @@ __global__ void k_state_set(DevState *st, Mat16 pose, int flags)
 // Integrate
 // ---------------------------------------------------------------------------
 constexpr int kIntU = 8;      // z voxels per pipelined chunk (= brick depth)
-#ifndef KF_INT_GROUP
-#define KF_INT_GROUP 1
-#endif
-constexpr int kIntGroup = KF_INT_GROUP;  // chunks whose skip tests are issued together
 
 struct ColumnSetup {
     float pos[3], cx[3], dl[3], cd[3];
@@ -1402,15 +1398,20 @@ __device__ __forceinline__ bool chunk_unreachable(const IntegrateArgs &a, const 
 // from its first chunk as in the reference.  Results agree with
 // the correctly rounded evaluation to a few ulps; voxels on a pixel edge or a
 // branch threshold can take the other side (counted as flips by the tests).
+#ifndef KF_INT_ALIGN
+#define KF_INT_ALIGN 1
+#endif
 template <class S, bool OFF32>
 __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, const DevState *__restrict__ st, int bx,
                                                     int by, int bz)
 {
     using Idx = typename std::conditional<OFF32, uint32_t, size_t>::type;
     if (a.gated && !st->integrate_now) return;
-    const int x = bx * 64 + threadIdx.x;
-    const int y = by * 4 + threadIdx.y;
-    if (x >= a.N[0] || y >= a.N[1]) return;
+    // columns beyond the volume take part with an empty range (the whole
+    // wavefront takes part in the range reduction below)
+    const bool col_in = bx * 64 + (int)threadIdx.x < a.N[0] && by * 4 + (int)threadIdx.y < a.N[1];
+    const int x = col_in ? bx * 64 + threadIdx.x : 0;
+    const int y = col_in ? by * 4 + threadIdx.y : 0;
 
     ColumnSetup c;
     column_setup(a, st->pose, x, y, c);
@@ -1420,6 +1421,23 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
     const int z1 = min(z0 + a.zlen, a.N[2]);
     int zs, ze;
     column_frustum(c, fx, fy, cxh, cyh, dmax_x, dmax_y, z0, z1, zs, ze);
+    if (!col_in) zs = ze = z0;
+#if KF_INT_ALIGN
+    // every lane walks the same chunks, from the wavefront's first chunk to
+    // its last (zs is on the segment's chunk grid for every lane), so the
+    // lanes of an iteration work on one z: coalesced voxel and flag accesses
+    // and neighbouring columns' similar skip decisions side by side; a lane
+    // works only inside its own [zs, ze)
+    int wzs = zs < ze ? zs : INT_MAX, wze = zs < ze ? ze : INT_MIN;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        wzs = min(wzs, __shfl_xor(wzs, o, 64));
+        wze = max(wze, __shfl_xor(wze, o, 64));
+    }
+    if (wzs >= wze) return;  // the whole wavefront
+#else
+    const int wzs = zs, wze = ze;
+#endif
     const float mu_lo = 0.2f * a.mu * 1.01f, mu_hi = a.mu * 1.01f;
     const float inv_mu = 1.0f / a.mu, neg_mu = -a.mu;
     typename S::V *__restrict__ tv = (typename S::V *)a.tsdf;
@@ -1429,25 +1447,30 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
     const size_t brow = (size_t)(y >> 3) * a.NB[0] + (x >> 3);
     const size_t bplane = (size_t)a.NB[0] * a.NB[1];
 
-    for (int zg = zs; zg < ze; zg += kIntGroup * kIntU) {
-        // the skip tests of the group's chunks first (their loads together):
-        // a brick with a clear zero flag holds only (0, 0), so a chunk in one
-        // that no depth can reach keeps (0, 0) and needs no work
-        unsigned need = 0u, zfs = 0u;
-#pragma unroll
-        for (int j = 0; j < kIntGroup; ++j) {
-            const int zb = zg + j * kIntU;
-            const int za = zb < ze ? zb : zg;  // an address inside the column either way
-            const bool zf = a.zflags[(size_t)(za >> 3) * bplane + brow] != 0;
-            const bool unr = chunk_unreachable(a, c, fmaf((float)za, c.dl[0], c.pos[0]),
-                                               fmaf((float)za, c.dl[1], c.pos[1]), fmaf((float)za, c.dl[2], c.pos[2]),
+    // windows of up to 32 chunks: first the skip tests of every chunk of the
+    // window, one z at a time for the whole wavefront (so neighbouring
+    // columns' tile and flag loads coincide) -- a brick with a clear zero
+    // flag holds only (0, 0), so a chunk in one that no depth can reach keeps
+    // (0, 0) and needs no work; then each lane's needed chunks in z order, the
+    // k-th of every lane in the k-th pass, so a pass has as many lanes at
+    // work as have a k-th chunk (a surface crosses neighbouring columns at
+    // different chunks)
+    for (int wz = wzs; wz < wze; wz += 32 * kIntU) {
+        const int wend = min(wze, wz + 32 * kIntU);
+        uint32_t need = 0u, zfs = 0u;
+        for (int i = 0, zb = wz; zb < wend; ++i, zb += kIntU) {
+            const bool mine = zb >= zs && zb < ze;
+            const bool zf = a.zflags[(size_t)(zb >> 3) * bplane + brow] != 0;
+            const bool unr = chunk_unreachable(a, c, fmaf((float)zb, c.dl[0], c.pos[0]),
+                                               fmaf((float)zb, c.dl[1], c.pos[1]), fmaf((float)zb, c.dl[2], c.pos[2]),
                                                fx, fy, cxh, cyh, mu_lo, mu_hi);
-            if (zb < ze && !(unr && !zf)) need |= 1u << j;
-            if (zf) zfs |= 1u << j;
+            if (mine && !(unr && !zf)) need |= 1u << i;
+            if (zf) zfs |= 1u << i;
         }
-      for (int j = 0; j < kIntGroup; ++j) {
-        if (!((need >> j) & 1u)) continue;
-        const int zb = zg + j * kIntU;
+      while (need != 0u) {
+        const int j = __builtin_ctz(need);
+        need &= need - 1u;
+        const int zb = wz + j * kIntU;
         float p0 = fmaf((float)zb, c.dl[0], c.pos[0]);
         float p1 = fmaf((float)zb, c.dl[1], c.pos[1]);
         float p2 = fmaf((float)zb, c.dl[2], c.pos[2]);
