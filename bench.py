@@ -89,7 +89,8 @@ def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, count_every=25):
     dispatch packets, so each figure is that kernel's duration; ICP is its
     first iteration's start to the finish kernel's end).  Every `count_every`
     frames the integrate and raycast traffic of that frame is counted
-    (separate passes after the frame)."""
+    (separate passes after the frame): integrate under the reference semantics
+    and as the default kernel moves it, the raycast's distinct bytes."""
     import ctypes as C
     from oglfusionkinectv2_amd import _lib as K
     s = g.session
@@ -107,7 +108,7 @@ def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, count_every=25):
         for n, v in g.stageTimes().items():
             acc[n].append(v)
         if i % count_every == count_every - 1:
-            counts.append(g.countIntegrateTraffic())
+            counts.append(g.countIntegrateTraffic() + g.countIntegrateFastTraffic())
             ray_counts.append(g.countRaycastTraffic())
     g.setOption(K.OPT_STAGE_TIMING, 0)
     return {n: float(np.mean(v)) for n, v in acc.items()}, counts, ray_counts
@@ -252,13 +253,18 @@ def main():
     est = g.getPose()
     ate_gt = float(np.linalg.norm(est[:3, 3] - poses[nframes - 1][:3, 3]))
     kern, counts, ray_counts = run_kernels(torch, L, g, frames_dev, poses, args.warmup, args.steps)
-    # integrate traffic under the reference semantics (sampled frames): each
+    # integrate traffic (sampled frames): under the reference semantics each
     # read voxel moves value + weight in and out, each written-only voxel out;
-    # plus the depth image in metres
+    # the default kernel skips chunks that stay (0, 0) and stores of (0, 0)
+    # over (0, 0), so its algorithmic bytes are the voxels it reads plus the
+    # voxels it writes (value + weight each); both plus the depth image
     bv = 8
     vr = float(np.mean([c[0] for c in counts])) if counts else 0.0
     vw = float(np.mean([c[1] for c in counts])) if counts else 0.0
-    int_bytes = (vr + vw) * bv + 424 * 512 * 4
+    fr = float(np.mean([c[2] for c in counts])) if counts else 0.0
+    fw = float(np.mean([c[3] for c in counts])) if counts else 0.0
+    int_ref_bytes = (vr + vw) * bv + 424 * 512 * 4
+    int_bytes = (fr + fw) * bv + 424 * 512 * 4
     # raycast: distinct voxel values (4 B) and occupancy-flag bytes read, plus
     # the vertex and normal maps written (2 x float4 per pixel)
     rv = float(np.mean([c[0] for c in ray_counts])) if ray_counts else 0.0
@@ -295,6 +301,8 @@ def main():
         "raycast": roofline("k_raycast<StoreF32>", ray_bytes, kern["raycast"], traffic),
         "integrate": roofline("k_integrate_fast<StoreF32, true>", int_bytes, kern["integrate"], traffic),
     }
+    rooflines["integrate"]["reference_semantics_bytes"] = int(int_ref_bytes)
+    rooflines["integrate"]["reference_semantics_GBps"] = int_ref_bytes / (kern["integrate"] * 1e-3) / 1e9
     roof = rooflines.get(dom) or rooflines["raycast"]
     pf = np.array(lat)
     out = {

@@ -285,6 +285,12 @@ int kf_get_stream(kf_session *s, void **stream);
  * voxels read and voxels written under the reference semantics (no timing). */
 int kf_count_integrate_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *voxels_written);
 
+/* The same for the default (KF_OPT_FAST_MATH) integrate as built: voxels it
+ * reads and writes for the current depth frame and pose (it skips chunks that
+ * stay (0, 0) and stores of (0, 0) over (0, 0)); a walk with no stores.  No
+ * counterpart in the reference (a measurement aid). */
+int kf_count_integrate_fast_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *voxels_written);
+
 /* Runs raycast() once more with counters: samples evaluated, samples skipped
  * as empty space, empty cells entered, rays that hit, the largest number of
  * samples one ray evaluated, and the largest evaluated + cells of one ray. */

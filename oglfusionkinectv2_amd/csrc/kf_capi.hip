@@ -1213,6 +1213,20 @@ int kf_count_integrate_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *v
     return KF_OK;
 }
 
+int kf_count_integrate_fast_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *voxels_written)
+{
+    if (!s) return fail(KF_ERR_INVALID, "session is NULL");
+    KF_HIP(hipMemsetAsync(s->counts, 0, 2 * sizeof(unsigned long long), s->stream));
+    kf::IntegrateArgs a = integrate_args(s, 0);
+    KF_HIP(kf::launch_count_integrate_fast(a, s->cfg.storage, s->st, s->counts, s->stream));
+    unsigned long long h[2];
+    KF_HIP(hipMemcpyAsync(h, s->counts, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    if (voxels_read) *voxels_read = h[0];
+    if (voxels_written) *voxels_written = h[1];
+    return KF_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Multi-session batches
 // ---------------------------------------------------------------------------

@@ -201,6 +201,9 @@ hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const
                             hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 hipError_t launch_brick_rebuild(const BrickArgs &a, int storage, hipStream_t s);
 hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s);
+// the voxels k_integrate_fast reads and writes (counts[0], counts[1]), no stores
+hipError_t launch_count_integrate_fast(const IntegrateArgs &a, int storage, const DevState *st,
+                                       unsigned long long *counts, hipStream_t s);
 hipError_t launch_count_integrate(const IntegrateArgs &a, const DevState *st, unsigned long long *counts,
                                   hipStream_t s);
 hipError_t launch_intensity_projection(const RaycastArgs &a, int storage, const DevState *st, float4 *out,

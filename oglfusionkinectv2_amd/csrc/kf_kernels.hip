@@ -1401,9 +1401,12 @@ __device__ __forceinline__ bool chunk_unreachable(const IntegrateArgs &a, const 
 #ifndef KF_INT_ALIGN
 #define KF_INT_ALIGN 1
 #endif
-template <class S, bool OFF32>
+// COUNT: the same walk, loads and decisions with no store and no flag
+// update; counts[0] += voxels read, counts[1] += voxels written (the bytes
+// this kernel moves for the volume are 2 x sizeof(V) per voxel of each).
+template <class S, bool OFF32, bool COUNT = false>
 __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, const DevState *__restrict__ st, int bx,
-                                                    int by, int bz)
+                                                    int by, int bz, unsigned long long *counts = nullptr)
 {
     using Idx = typename std::conditional<OFF32, uint32_t, size_t>::type;
     if (a.gated && !st->integrate_now) return;
@@ -1446,6 +1449,7 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
     const Idx col = (Idx)y * (Idx)a.N[0] + (Idx)x;
     const size_t brow = (size_t)(y >> 3) * a.NB[0] + (x >> 3);
     const size_t bplane = (size_t)a.NB[0] * a.NB[1];
+    unsigned long long n_read = 0, n_written = 0;  // COUNT only
 
     // windows of up to 32 chunks: first the skip tests of every chunk of the
     // window, one z at a time for the whole wavefront (so neighbouring
@@ -1518,6 +1522,7 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
                 const Idx vi = col + (Idx)(zb + k) * plane;
                 ot[k] = S::bits(tv[vi]);
                 ow[k] = S::bits(wv[vi]);
+                if (COUNT) ++n_read;
             }
         }
         bool neg = false, nz = false;
@@ -1537,15 +1542,25 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
             const typename S::V nt = S::from_value(t), nw = S::from_weight(w);
             const bool zero = S::zero_bits(nt) && S::zero_bits(nw);
             if (zf || !zero) {
-                tv[vi] = nt;
-                wv[vi] = nw;
+                if (COUNT) {
+                    ++n_written;
+                } else {
+                    tv[vi] = nt;
+                    wv[vi] = nw;
+                }
             }
             neg = neg || S::interesting(nt);
             nz = nz || !zero;
         }
-        if (neg) a.bflags[bidx] = 1;
-        if (nz && !zf) a.zflags[bidx] = 1;
+        if (!COUNT) {
+            if (neg) a.bflags[bidx] = 1;
+            if (nz && !zf) a.zflags[bidx] = 1;
+        }
       }
+    }
+    if (COUNT) {
+        atomicAdd(&counts[0], n_read);
+        atomicAdd(&counts[1], n_written);
     }
 }
 
@@ -1557,6 +1572,14 @@ __global__ __launch_bounds__(256, KF_INT_WAVES_PER_SIMD) void k_integrate_fast(I
                                                                               const DevState *__restrict__ st)
 {
     integrate_fast_body<S, OFF32>(a, st, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <class S, bool OFF32>
+__global__ __launch_bounds__(256, KF_INT_WAVES_PER_SIMD) void k_count_integrate_fast(IntegrateArgs a,
+                                                                                    const DevState *__restrict__ st,
+                                                                                    unsigned long long *counts)
+{
+    integrate_fast_body<S, OFF32, true>(a, st, blockIdx.x, blockIdx.y, blockIdx.z, counts);
 }
 
 // Same walk as k_integrate without memory traffic on the volume: counts the
@@ -2853,6 +2876,24 @@ hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s)
 {
     const int nc = a.NC[0] * a.NC[1] * a.NC[2];
     hipLaunchKernelGGL(k_brick_dilate, dim3((nc + 3) / 4), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_integrate_fast(const IntegrateArgs &a, int storage, const DevState *st,
+                                       unsigned long long *counts, hipStream_t s)
+{
+    const dim3 g = integrate_grid(a), b(64, 4);
+    const bool off32 = (size_t)a.N[0] * a.N[1] * a.N[2] < (size_t(1) << 32);
+#define KF_INT(ST)                                                                               \
+    if (off32) hipLaunchKernelGGL((k_count_integrate_fast<ST, true>), g, b, 0, s, a, st, counts); \
+    else hipLaunchKernelGGL((k_count_integrate_fast<ST, false>), g, b, 0, s, a, st, counts);
+    switch (storage) {
+    case KF_STORAGE_RG16I: KF_INT(StoreRG16I); break;
+    case KF_STORAGE_F32: KF_INT(StoreF32); break;
+    case KF_STORAGE_F16: KF_INT(StoreF16); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef KF_INT
     return hipGetLastError();
 }
 

@@ -377,6 +377,13 @@ class GFusion:
         K.check(K.lib().kf_count_integrate_traffic(self.session, C.byref(r), C.byref(w)))
         return r.value, w.value
 
+    def countIntegrateFastTraffic(self):
+        """(voxels read, voxels written) by the default integrate for the
+        current depth frame and pose."""
+        r, w = C.c_uint64(), C.c_uint64()
+        K.check(K.lib().kf_count_integrate_fast_traffic(self.session, C.byref(r), C.byref(w)))
+        return r.value, w.value
+
     def setOption(self, option, value):
         """kf_set_option: K.OPT_FAST_MATH, OPT_SKIP_EMPTY, OPT_GRAPH, OPT_BRICK_REBUILD_PERIOD."""
         K.check(K.lib().kf_set_option(self.session, int(option), int(value)))

@@ -491,6 +491,34 @@ def test_raycast_traffic_and_ray_counts():
     assert vox1 >= len(distinct)
 
 
+def test_integrate_fast_traffic_counts():
+    """The counting walk of the default integrate stores nothing; the voxels
+    it counts as read / written are within the reference semantics' (the same
+    depth and pose), and every voxel that a further integrate changes is among
+    the ones it counts as written."""
+    print("case: integrate fast traffic count f32 128^3", flush=True)
+    N = 128
+    depth, gt = synth.sequence(0, 6)
+    g, cfg = make(volume=N, storage="f32")
+    g.Reset(gt[0])
+    for f in range(6):
+        g.frame(depth[f].astype(np.float32))
+    v0, w0 = g.getVolume()
+    fr, fw = g.countIntegrateFastTraffic()
+    v1, w1 = g.getVolume()
+    np.testing.assert_array_equal(v0, v1)
+    np.testing.assert_array_equal(w0, w1)
+    vr, vw = g.countIntegrateTraffic()
+    print("fast read %d written %d; reference read %d written %d" % (fr, fw, vr, vw))
+    assert 0 < fr <= fw
+    # fast-math threshold flips aside, a subset of the reference's voxels
+    assert fr <= vr * 1.001 + 16 and fw <= vw * 1.001 + 16
+    g.integrate()  # the same depth and pose once more
+    v2, w2 = g.getVolume()
+    changed = int(((v2 != v0) | (w2 != w0)).sum())
+    assert 0 < changed <= fw
+
+
 def test_brick_flags_cover_every_negative_voxel(seq20):
     """Set-only flags maintained by integrate are a superset of the exact ones."""
     print("case: brick flags superset", flush=True)
