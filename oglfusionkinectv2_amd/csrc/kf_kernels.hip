@@ -1398,9 +1398,6 @@ __device__ __forceinline__ bool chunk_unreachable(const IntegrateArgs &a, const 
 // from its first chunk as in the reference.  Results agree with
 // the correctly rounded evaluation to a few ulps; voxels on a pixel edge or a
 // branch threshold can take the other side (counted as flips by the tests).
-#ifndef KF_INT_ALIGN
-#define KF_INT_ALIGN 1
-#endif
 // COUNT: the same walk, loads and decisions with no store and no flag
 // update; counts[0] += voxels read, counts[1] += voxels written (the bytes
 // this kernel moves for the volume are 2 x sizeof(V) per voxel of each).
@@ -1425,7 +1422,6 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
     int zs, ze;
     column_frustum(c, fx, fy, cxh, cyh, dmax_x, dmax_y, z0, z1, zs, ze);
     if (!col_in) zs = ze = z0;
-#if KF_INT_ALIGN
     // every lane walks the same chunks, from the wavefront's first chunk to
     // its last (zs is on the segment's chunk grid for every lane), so the
     // lanes of an iteration work on one z: coalesced voxel and flag accesses
@@ -1438,9 +1434,6 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
         wze = max(wze, __shfl_xor(wze, o, 64));
     }
     if (wzs >= wze) return;  // the whole wavefront
-#else
-    const int wzs = zs, wze = ze;
-#endif
     const float mu_lo = 0.2f * a.mu * 1.01f, mu_hi = a.mu * 1.01f;
     const float inv_mu = 1.0f / a.mu, neg_mu = -a.mu;
     typename S::V *__restrict__ tv = (typename S::V *)a.tsdf;
