@@ -185,6 +185,11 @@ int kf_write_tsdf(kf_session *s, const void *value, const void *weight);
  * frame counter and reset window flag. */
 int kf_read_reduction(kf_session *s, float partials[256], float *alignment_energy);
 int kf_read_status(kf_session *s, int *tracked, int *iterations_run, int *counter, int *reset);
+
+/* The 6-vector of the last ICP solve (gFusion.cpp:696-697: translation then
+ * the three angles of the pose update).  No counterpart in the reference (a
+ * test aid: the pose update can be checked against it exactly). */
+int kf_read_icp_solution(kf_session *s, double x[6]);
 /* Per-pixel output of the last ICP iteration at `level`
  * ({int result; float error; float J[6]} of track.cs:18-27, as planes).
  * Only recorded when enabled with kf_set_debug(s, 1). */

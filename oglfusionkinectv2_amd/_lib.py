@@ -132,6 +132,7 @@ def lib():
         "kf_get_stream": [sess, C.POINTER(C.c_void_p)],
         "kf_count_integrate_traffic": [sess, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
         "kf_count_integrate_fast_traffic": [sess, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+        "kf_read_icp_solution": [sess, C.c_void_p],
         "kf_set_option": [sess, i32, i32],
         "kf_get_option": [sess, i32, C.POINTER(C.c_int)],
         "kf_rebuild_bricks": [sess],

@@ -1067,6 +1067,14 @@ int kf_read_reduction(kf_session *s, float partials[256], float *ae)
     return KF_OK;
 }
 
+int kf_read_icp_solution(kf_session *s, double x[6])
+{
+    if (!s || !x) return fail(KF_ERR_INVALID, "NULL argument");
+    KF_HIP(hipMemcpyAsync(x, s->st->x, 6 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
+}
+
 int kf_read_status(kf_session *s, int *tracked, int *iterations_run, int *counter, int *reset)
 {
     if (!s) return fail(KF_ERR_INVALID, "session is NULL");

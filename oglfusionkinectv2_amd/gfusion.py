@@ -377,6 +377,12 @@ class GFusion:
         K.check(K.lib().kf_count_integrate_traffic(self.session, C.byref(r), C.byref(w)))
         return r.value, w.value
 
+    def readIcpSolution(self):
+        """The 6-vector of the last ICP solve (translation, then angles)."""
+        x = np.zeros(6, dtype=np.float64)
+        K.check(K.lib().kf_read_icp_solution(self.session, x.ctypes.data))
+        return x
+
     def countIntegrateFastTraffic(self):
         """(voxels read, voxels written) by the default integrate for the
         current depth frame and pose."""

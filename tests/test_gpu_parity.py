@@ -129,6 +129,10 @@ def test_track_pixels_and_partials(o32, seq20):
         got = g.getPose()
         assert np.abs(got[:3, 3] - want[:3, 3]).max() < 1e-4
         assert np.abs(got[:3, :3] - want[:3, :3]).max() < 1e-4
+        # the update from the GPU's own solution: the oracle's expressions
+        # (double eulerAngleXYZ, float product), up to an ulp of libm
+        xg = g.readIcpSolution()
+        np.testing.assert_allclose(got, o32.pose_update(xg, pose), rtol=0, atol=4e-7)
         assert abs(g.alignmentEnergy() - ae) <= 1e-4 * max(ae, 1e-6)
 
 
