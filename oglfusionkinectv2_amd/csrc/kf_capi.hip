@@ -182,11 +182,11 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated)
     a.dh = c.height;
     a.tsdf = s->tsdf;
     a.weight = s->weight;
-    // enough z segments that the launch holds ~1M column segments (16 waves
-    // per SIMD to hide the dependent depth -> voxel load latency)
+    // enough z segments that the launch holds ~2M column segments (64-voxel
+    // segments at 512^3: measured 106 us against 115 for 128 and 124 for 32)
     const long cols = (long)c.volume_size[0] * c.volume_size[1];
     int segs = (int)std::min<long>(std::max(1, c.volume_size[2] / 16),
-                                   std::max<long>(1, (1048576 + cols - 1) / cols));
+                                   std::max<long>(1, (2097152 + cols - 1) / cols));
     int zlen = (c.volume_size[2] + segs - 1) / segs;
     zlen = (zlen + 7) / 8 * 8;
     a.zlen = std::max(zlen, 8);
