@@ -166,7 +166,9 @@ kf::PyramidArgs pyramid_args(kf_session *s)
     return a;
 }
 
-kf::IntegrateArgs integrate_args(kf_session *s, int gated)
+// target_segments: column segments per launch for this session alone; a
+// batch, whose launch holds every member's, asks for fewer per session
+kf::IntegrateArgs integrate_args(kf_session *s, int gated, long target_segments = 2097152)
 {
     const kf_config &c = s->cfg;
     kf::IntegrateArgs a{};
@@ -186,7 +188,7 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated)
     // segments at 512^3: measured 106 us against 115 for 128 and 124 for 32)
     const long cols = (long)c.volume_size[0] * c.volume_size[1];
     int segs = (int)std::min<long>(std::max(1, c.volume_size[2] / 16),
-                                   std::max<long>(1, (2097152 + cols - 1) / cols));
+                                   std::max<long>(1, (target_segments + cols - 1) / cols));
     int zlen = (c.volume_size[2] + segs - 1) / segs;
     zlen = (zlen + 7) / 8 * 8;
     a.zlen = std::max(zlen, 8);
@@ -1321,7 +1323,7 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
         kf::BatchSession &bs = h[i];
         bs.pyr = pyramid_args(si);
         for (int l = 0; l < 3; ++l) bs.icp[l] = icp_args(si, l, 0, 0);
-        bs.integ = integrate_args(si, 1);
+        bs.integ = integrate_args(si, 1, 1048576);
         bs.ray = raycast_args(si);
         bs.brick = brick_args(si);
         bs.st = si->st;
