@@ -992,7 +992,9 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
 
     // partial rows of exchange n carry the tag (launch << 8) + n + 1 (at
     // most 3 x 64 exchanges per launch), so a row is current exactly when its
-    // words carry this exchange's tag
+    // words carry this exchange's tag.  The tag repeats after 2^24 launches;
+    // every row a launch reads is rewritten by every launch with the same row
+    // layout, and a change of layout (kf_set_option) clears the rows.
     const unsigned epoch = __hip_atomic_load(&st->icp_epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned phase = 0, xchg = 0;  // grid barriers and row exchanges so far
     int iters_run = 0;
