@@ -992,9 +992,10 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
 
     // partial rows of exchange n carry the tag (launch << 8) + n + 1 (at
     // most 3 x 64 exchanges per launch), so a row is current exactly when its
-    // words carry this exchange's tag.  The tag repeats after 2^24 launches;
-    // every row a launch reads is rewritten by every launch with the same row
-    // layout, and a change of layout (kf_set_option) clears the rows.
+    // words carry this exchange's tag.  Tags repeat after 2^24 launches: a
+    // stale row passes for a current one only if it was written exactly
+    // 2^24 k launches earlier at the same exchange and never since (a change
+    // of the row layout, kf_set_option, clears the rows).
     const unsigned epoch = __hip_atomic_load(&st->icp_epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned phase = 0, xchg = 0;  // grid barriers and row exchanges so far
     int iters_run = 0;
