@@ -9,11 +9,14 @@ are resident in HBM before the timed region.
 
 Also reported: the same loop with the FP16 TSDF (4 B/voxel), per-frame
 p50/p99 latency, the dominant kernel's roofline figures from HIP events
-carried by its dispatch packets on the session stream, the CPU
-oracle timed on a bounded sample of the same workload, and the multi-session
-FP16 legs (saturated 16 x 512^3 batch, the uneven 16-session mix, 16 x 512^3
-served at 30 Hz, Poisson arrivals under a 200 GB budget) under
-"multi_session".
+carried by its dispatch packets on the session stream, the tracked fraction
+and translation error against the ground-truth trajectory, the frame rate
+with the per-frame host upload (kf_frame_f32), the CPU oracle timed on a
+bounded sample of the same workload, and the multi-session legs (saturated
+16 x 512^3 batch in FP16 and FP32, the uneven 16-session mix, 16 and 64 x
+512^3 served at 30 Hz, Poisson arrivals under a 200 GB budget) under
+"multi_session".  Every leg after the headline records an "error" field
+instead of stopping the run, so the JSON line is always printed.
 """
 import argparse
 import json
@@ -29,7 +32,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def parse():
+def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
@@ -38,14 +41,17 @@ def parse():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=60)
+    ap.add_argument("--cpu-warm", type=int, default=2, help="untimed CPU oracle frames after its reset")
     ap.add_argument("--fast-math", type=int, default=1)
     ap.add_argument("--no-serve", action="store_true", help="skip the batched and served legs")
     ap.add_argument("--batch-ticks", type=int, default=100)
     ap.add_argument("--serve-seconds", type=float, default=8.0,
-                    help="length of the fixed 16 x 512^3 at 30 Hz served run")
+                    help="length of each fixed-membership served run (16 and 64 x 512^3 at 30 Hz)")
     ap.add_argument("--poisson-seconds", type=float, default=40.0,
                     help="how much of the Poisson arrival process (64 sessions at 1/s) to serve")
-    return ap.parse_args()
+    ap.add_argument("--plan", action="store_true",
+                    help="print the frame plan for these arguments and exit (no GPU)")
+    return ap.parse_args(argv)
 
 
 def make_session(volume, storage, stream, fast_math):
@@ -83,14 +89,18 @@ def run_frames(torch, L, g, frames_dev, poses, warmup, steps, stream):
     return elapsed, lat
 
 
-def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, count_every=25):
+def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, count_every=25, count=True):
     """Per-kernel pass: the same frames with stage timing on (the pyramid,
     integrate and raycast launches carry start / stop events in their own
     dispatch packets, so each figure is that kernel's duration; ICP is its
-    first iteration's start to the finish kernel's end).  Every `count_every`
-    frames the integrate and raycast traffic of that frame is counted
-    (separate passes after the frame): integrate under the reference semantics
-    and as the default kernel moves it, the raycast's distinct bytes."""
+    first iteration's start to the finish kernel's end).  After the frames of
+    count_schedule() the integrate and raycast traffic of that frame is
+    counted (separate passes after the frame): integrate under the reference
+    semantics and as the default kernel moves it, the raycast's distinct
+    bytes.  Every frame's tracking decision and pose are read back, giving
+    the tracked fraction and the translation error against the ground-truth
+    trajectory (the frames are the headline pass's, from the same reset, so
+    the poses are the same)."""
     import ctypes as C
     from oglfusionkinectv2_amd import _lib as K
     s = g.session
@@ -98,20 +108,42 @@ def run_kernels(torch, L, g, frames_dev, poses, warmup, steps, count_every=25):
     base = frames_dev.data_ptr()
     g.setOption(K.OPT_STAGE_TIMING, 1)
     g.Reset(poses[0])
+    tracked, err = [], []
+
+    def record(f):
+        tracked.append(g.status()["tracked"])
+        err.append(float(np.linalg.norm(g.getPose()[:3, 3] - poses[f][:3, 3])))
+
     for i in range(warmup):
         K.check(L.kf_frame_u16_dev(s, C.c_void_p(base + i * fb)))
+        record(i)
     torch.cuda.synchronize()
     acc = {n: [] for n in ("pyramid", "icp", "integrate", "raycast")}
     counts, ray_counts = [], []
+    sched = set(count_schedule(steps, count_every)) if count else set()
     for i in range(steps):
         K.check(L.kf_frame_u16_dev(s, C.c_void_p(base + (warmup + i) * fb)))
         for n, v in g.stageTimes().items():
             acc[n].append(v)
-        if i % count_every == count_every - 1:
+        record(warmup + i)
+        if i in sched:
             counts.append(g.countIntegrateTraffic() + g.countIntegrateFastTraffic())
             ray_counts.append(g.countRaycastTraffic())
     g.setOption(K.OPT_STAGE_TIMING, 0)
-    return {n: float(np.mean(v)) for n, v in acc.items()}, counts, ray_counts
+    e = np.array(err)
+    t = np.array(tracked)
+    timed = slice(warmup, warmup + steps)
+    accuracy = {
+        "frames": len(e),
+        "tracked_fraction_timed": float(t[timed].mean()) if steps else None,
+        "tracked_fraction_all": float(t.mean()),
+        "translation_error_vs_gt_m": {"timed_mean": float(e[timed].mean()) if steps else None,
+                                      "timed_max": float(e[timed].max()) if steps else None,
+                                      "ate_rmse_all": float(np.sqrt(np.mean(e ** 2))),
+                                      "final": float(e[-1])},
+        "first_frame_over_1cm": int(np.argmax(e > 0.01)) if (e > 0.01).any() else None,
+    }
+    return {n: float(np.mean(v)) for n, v in acc.items()}, counts, ray_counts, accuracy
 
 
 def row_timings(torch, L, g, frames_dev, nframes, reps=3):
@@ -181,8 +213,10 @@ def roofline(kernel, nbytes, ms, traffic):
 
 def cpu_baseline(depth, poses, volume, frames, warm=2):
     """The FP32 C oracle on the same workload: `warm` untimed frames after
-    reset, then `frames` timed frames."""
+    reset, then `frames` timed frames (depth must hold warm + frames)."""
     import oracle
+    if len(depth) < warm + frames:
+        raise ValueError("cpu_baseline needs %d frames, got %d" % (warm + frames, len(depth)))
     o = oracle.Oracle("f32")
     vox = np.float32(2.0) / np.float32(volume)
     mu = float(vox * np.float32(3.0))
@@ -199,31 +233,93 @@ def cpu_baseline(depth, poses, volume, frames, warm=2):
 
 
 def serving_legs(torch, args, fps32):
-    """Multi-session legs, all FP16 TSDF:
+    """Multi-session legs (FP16 TSDF unless named _f32):
     - saturated: 16 x 512^3 sessions with a frame due every tick (offline
-      throughput of the batched path), compared with the FP32 single session;
+      throughput of the batched path), FP16 and FP32, compared with the FP32
+      single session;
     - batched: the uneven mix 4 x 128^3, 6 x 256^3, 4 x 384^3, 2 x 512^3, each
       session its own seeded scene, every session due every tick;
-    - served: 16 x 512^3 sessions at 30 Hz, first frames spread over one
-      period, real-time request loop with continuous batching;
+    - served: 16 and 64 x 512^3 sessions at 30 Hz (480 and 1920 frames/s
+      offered), first frames spread over one period, real-time request loop
+      with continuous batching;
     - poisson: sessions arriving at 1/s (64 in all), sizes {256..768}^3, 15 or
       30 Hz, 200-900 frames each, 200 GB HBM budget; the first
-      `poisson_seconds` of the process are served."""
+      `poisson_seconds` of the process are served.
+    Each leg that fails records its error and the others still run."""
     from oglfusionkinectv2_amd import serve
     out = {}
-    sat = serve.saturate(torch, [512] * 16, args.batch_ticks, sequences=16,
-                         pool_frames=args.batch_ticks + 10)
-    sat["vs_fp32_single_session"] = sat["frames_per_s"] / fps32
-    out["saturated_16x512_f16"] = sat
+
+    def sat(volumes, storage):
+        r = serve.saturate(torch, volumes, args.batch_ticks, sequences=len(volumes), storage=storage,
+                           pool_frames=args.batch_ticks + 10)
+        r["vs_fp32_single_session"] = r["frames_per_s"] / fps32
+        return r
+
+    run_leg(out, "saturated_16x512_f16", lambda: sat([512] * 16, "f16"))
+    run_leg(out, "saturated_16x512_f32", lambda: sat([512] * 16, "f32"))
     mix = [128] * 4 + [256] * 6 + [384] * 4 + [512] * 2
-    out["batched_mix_f16"] = serve.saturate(torch, mix, args.batch_ticks, sequences=len(mix),
-                                            pool_frames=args.batch_ticks + 10)
-    out["served_16x512_30hz_f16"] = serve.serve(torch, serve.fixed_sessions(16, 512, 30.0, args.serve_seconds),
-                                                args.serve_seconds, sequences=16)
+    run_leg(out, "batched_mix_f16", lambda: sat(mix, "f16"))
+    run_leg(out, "served_16x512_30hz_f16",
+            lambda: serve.serve(torch, serve.fixed_sessions(16, 512, 30.0, args.serve_seconds),
+                                args.serve_seconds, sequences=16))
+    run_leg(out, "served_64x512_30hz_f16",
+            lambda: serve.serve(torch, serve.fixed_sessions(64, 512, 30.0, args.serve_seconds),
+                                args.serve_seconds, sequences=16))
     if args.poisson_seconds > 0:
-        out["served_poisson_f16"] = serve.serve(torch, serve.poisson_sessions(), args.poisson_seconds,
-                                                sequences=16)
+        run_leg(out, "served_poisson_f16",
+                lambda: serve.serve(torch, serve.poisson_sessions(), args.poisson_seconds, sequences=16))
     return out
+
+
+def frame_plan(warmup, steps, cpu_frames, cpu_warm=2, with_cpu=True):
+    """Frames of the seeded sequence each leg reads: the GPU legs run frames
+    0 .. warmup + steps - 1; the CPU oracle runs `cpu_warm` untimed frames
+    after its reset, then `cpu_frames` timed ones.  One sequence long enough
+    for both is generated."""
+    n_gpu = warmup + steps
+    n_cpu = (cpu_warm + cpu_frames) if with_cpu and cpu_frames > 0 else 0
+    return {"gpu_frames": n_gpu, "cpu_frames": n_cpu, "generate": max(n_gpu, n_cpu)}
+
+
+def count_schedule(steps, every=25):
+    """Frames (0-based within the timed steps) after which the integrate and
+    raycast traffic is counted: every `every`-th frame, and at least the last
+    one, so a short run still has a sample."""
+    idx = [i for i in range(steps) if i % every == every - 1]
+    if not idx and steps > 0:
+        idx = [steps - 1]
+    return idx
+
+
+def run_leg(out, name, fn):
+    """Runs one optional leg; a failure is recorded as {"error": ...} and the
+    benchmark carries on, so the JSON line is always printed."""
+    t0 = time.perf_counter()
+    try:
+        out[name] = fn()
+    except Exception as e:  # noqa: BLE001 -- reported in the JSON line
+        import traceback
+        traceback.print_exc(file=sys.stderr)
+        out[name] = {"error": "%s: %s" % (type(e).__name__, e)}
+    print("bench: %s done in %.1f s" % (name, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    return out[name]
+
+
+def run_with_upload(torch, L, g, depth_f32, poses, warmup, steps):
+    """The boundary's host-buffer path (kf_frame_f32, what gFusion::
+    depthToVertex(float*) takes): each frame's float mm image is copied from
+    pageable host memory before the pyramid.  Returns frames/s."""
+    from oglfusionkinectv2_amd import _lib as K
+    s = g.session
+    g.Reset(poses[0])
+    for i in range(warmup):
+        K.check(L.kf_frame_f32(s, depth_f32[i].ctypes.data))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        K.check(L.kf_frame_f32(s, depth_f32[warmup + i].ctypes.data))
+    torch.cuda.synchronize()
+    return steps / (time.perf_counter() - t0)
 
 
 def main():
@@ -232,16 +328,20 @@ def main():
         print("bench.py: this benchmark runs on one GPU; --gpus %d is not supported (sessions are "
               "independent and shard across GPUs with no communication)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    plan = frame_plan(args.warmup, args.steps, args.cpu_frames, args.cpu_warm, not args.no_cpu)
+    if args.plan:
+        print(json.dumps({"plan": plan, "count_frames": count_schedule(args.steps)}))
+        return
     import torch
     from oglfusionkinectv2_amd import _lib as K
     from oglfusionkinectv2_amd import synth
 
     L = K.lib()
-    nframes = args.warmup + args.steps
+    nframes = plan["gpu_frames"]
     t_gen = time.perf_counter()
-    depth, poses = synth.sequence(0, nframes)
+    depth, poses = synth.sequence(0, plan["generate"])
     t_gen = time.perf_counter() - t_gen
-    frames_dev = torch.from_numpy(depth.view(np.int16)).to("cuda")
+    frames_dev = torch.from_numpy(depth[:nframes].view(np.int16)).to("cuda")
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
 
@@ -249,62 +349,63 @@ def main():
     g = make_session(args.volume, "f32", stream.cuda_stream, args.fast_math)
     elapsed, lat = run_frames(torch, L, g, frames_dev, poses, args.warmup, args.steps, stream)
     fps = args.steps / elapsed
-    st = g.status()
-    est = g.getPose()
-    ate_gt = float(np.linalg.norm(est[:3, 3] - poses[nframes - 1][:3, 3]))
-    kern, counts, ray_counts = run_kernels(torch, L, g, frames_dev, poses, args.warmup, args.steps)
-    # integrate traffic (sampled frames): under the reference semantics each
-    # read voxel moves value + weight in and out, each written-only voxel out;
-    # the default kernel skips chunks that stay (0, 0) and stores of (0, 0)
-    # over (0, 0), so its algorithmic bytes are the voxels it reads plus the
-    # voxels it writes (value + weight each); both plus the depth image
-    bv = 8
-    vr = float(np.mean([c[0] for c in counts])) if counts else 0.0
-    vw = float(np.mean([c[1] for c in counts])) if counts else 0.0
-    fr = float(np.mean([c[2] for c in counts])) if counts else 0.0
-    fw = float(np.mean([c[3] for c in counts])) if counts else 0.0
-    int_ref_bytes = (vr + vw) * bv + 424 * 512 * 4
-    int_bytes = (fr + fw) * bv + 424 * 512 * 4
-    # raycast: distinct voxel values (4 B) and occupancy-flag bytes read, plus
-    # the vertex and normal maps written (2 x float4 per pixel)
-    rv = float(np.mean([c[0] for c in ray_counts])) if ray_counts else 0.0
-    rf = float(np.mean([c[1] for c in ray_counts])) if ray_counts else 0.0
-    ray_bytes = rv * 4 + rf + 424 * 512 * 32
-    dom = max(kern, key=kern.get)
-    extras = row_timings(torch, L, g, frames_dev, args.warmup + args.steps)
+    print("bench: headline %.1f frames/s" % fps, file=sys.stderr, flush=True)
+    legs = {}
+    kr = run_leg(legs, "kernels", lambda: run_kernels(torch, L, g, frames_dev, poses, args.warmup, args.steps))
+    kern, counts, ray_counts, acc = (kr if isinstance(kr, tuple) else ({}, [], [], {}))
+    run_leg(legs, "beside_the_loop", lambda: row_timings(torch, L, g, frames_dev, nframes))
     g.close()
+    depth_f32 = [depth[i].astype(np.float32) for i in range(nframes)]
+    run_leg(legs, "with_upload", lambda: with_upload_leg(torch, L, args, stream, depth_f32, poses))
+    del depth_f32
 
     # --- FP16 --------------------------------------------------------------
-    fps16, kern16, lat16 = None, {}, None
+    fp16 = {}
     if not args.no_fp16:
-        g16 = make_session(args.volume, "f16", stream.cuda_stream, args.fast_math)
-        e16, lat16 = run_frames(torch, L, g16, frames_dev, poses, args.warmup, args.steps, stream)
-        fps16 = args.steps / e16
-        kern16, _, _ = run_kernels(torch, L, g16, frames_dev, poses, args.warmup, args.steps)
-        g16.close()
+        fp16 = run_leg(legs, "fp16", lambda: fp16_leg(torch, L, args, stream, frames_dev, poses))
+
+    # --- CPU baseline (before the long serving legs) -------------------------
+    cpu = None
+    if not args.no_cpu:
+        cpu = run_leg(legs, "cpu_baseline", lambda: cpu_leg(depth, poses, args))
 
     # --- multi-session -----------------------------------------------------
     multi = None
     if not args.no_serve:
         multi = serving_legs(torch, args, fps)
 
-    # --- CPU baseline --------------------------------------------------------
-    cpu = None
-    if not args.no_cpu:
-        cfps, threads, cdt = cpu_baseline(depth, poses, args.volume, args.cpu_frames)
-        cpu = {"value": cfps, "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": "frames 2-%d of the same %d^3 FP32 sequence (frames 0-1 untimed), C oracle, "
-                         "%d OpenMP threads, %.1f s" % (args.cpu_frames + 1, args.volume, threads, cdt)}
-
     traffic = measured_traffic() if args.volume == 512 else {}
-    rooflines = {
-        "raycast": roofline("k_raycast<StoreF32>", ray_bytes, kern["raycast"], traffic),
-        "integrate": roofline("k_integrate_fast<StoreF32, true>", int_bytes, kern["integrate"], traffic),
-    }
-    rooflines["integrate"]["reference_semantics_bytes"] = int(int_ref_bytes)
-    rooflines["integrate"]["reference_semantics_GBps"] = int_ref_bytes / (kern["integrate"] * 1e-3) / 1e9
-    roof = rooflines.get(dom) or rooflines["raycast"]
+    rooflines, roof, dom = {}, None, None
+    if kern:
+        # integrate traffic (sampled frames): under the reference semantics each
+        # read voxel moves value + weight in and out, each written-only voxel
+        # out; the default kernel skips chunks that stay (0, 0) and stores of
+        # (0, 0) over (0, 0), so its algorithmic bytes are the voxels it reads
+        # plus the voxels it writes (value + weight each); both plus the depth
+        bv = 8
+        vr = float(np.mean([c[0] for c in counts])) if counts else 0.0
+        vw = float(np.mean([c[1] for c in counts])) if counts else 0.0
+        fr = float(np.mean([c[2] for c in counts])) if counts else 0.0
+        fw = float(np.mean([c[3] for c in counts])) if counts else 0.0
+        int_ref_bytes = (vr + vw) * bv + 424 * 512 * 4
+        int_bytes = (fr + fw) * bv + 424 * 512 * 4
+        # raycast: distinct voxel values (4 B) and occupancy-flag bytes read,
+        # plus the vertex and normal maps written (2 x float4 per pixel)
+        rv = float(np.mean([c[0] for c in ray_counts])) if ray_counts else 0.0
+        rf = float(np.mean([c[1] for c in ray_counts])) if ray_counts else 0.0
+        ray_bytes = rv * 4 + rf + 424 * 512 * 32
+        dom = max(kern, key=kern.get)
+        rooflines = {
+            "raycast": roofline("k_raycast<StoreF32>", ray_bytes, kern["raycast"], traffic),
+            "integrate": roofline("k_integrate_fast<StoreF32, true>", int_bytes, kern["integrate"], traffic),
+        }
+        rooflines["integrate"]["reference_semantics_bytes"] = int(int_ref_bytes)
+        rooflines["integrate"]["reference_semantics_GBps"] = int_ref_bytes / (kern["integrate"] * 1e-3) / 1e9
+        rooflines["raycast"]["samples"] = len(ray_counts)
+        rooflines["integrate"]["samples"] = len(counts)
+        roof = rooflines.get(dom) or rooflines["raycast"]
     pf = np.array(lat)
+    fps16 = fp16.get("frames_per_s") if isinstance(fp16, dict) else None
     out = {
         "metric": "frames/s per GPU at 512^3 TSDF (FP32 TSDF+FP32 weight, single session)",
         "value": fps,
@@ -325,21 +426,48 @@ def main():
         "voxel_updates_per_s": fps * args.volume ** 3,
         "latency_ms": {"p50": float(np.percentile(pf, 50)), "p99": float(np.percentile(pf, 99))},
         "kernel_ms": kern,
-        "kernel_ms_fp16": kern16,
-        "latency_ms_fp16": ({"p50": float(np.percentile(lat16, 50)), "p99": float(np.percentile(lat16, 99))}
-                            if lat16 else None),
         "fast_math": args.fast_math,
         "dominant_kernel": dom,
         "roofline": roof,
         "roofline_by_kernel": rooflines,
         "cpu_baseline": cpu,
-        "beside_the_loop": extras,
+        "with_upload_frames_per_s": legs.get("with_upload"),
+        "fp16": fp16,
+        "accuracy": acc,
+        "beside_the_loop": legs.get("beside_the_loop"),
         "multi_session": multi,
-        "tracked_last": st["tracked"],
-        "final_translation_error_vs_gt_m": ate_gt,
+        "errors": {k: v["error"] for k, v in legs.items() if isinstance(v, dict) and "error" in v} or None,
         "generator_s": t_gen,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+
+
+def with_upload_leg(torch, L, args, stream, depth_f32, poses):
+    g = make_session(args.volume, "f32", stream.cuda_stream, args.fast_math)
+    try:
+        return run_with_upload(torch, L, g, depth_f32, poses, args.warmup, args.steps)
+    finally:
+        g.close()
+
+
+def fp16_leg(torch, L, args, stream, frames_dev, poses):
+    g = make_session(args.volume, "f16", stream.cuda_stream, args.fast_math)
+    try:
+        e16, lat16 = run_frames(torch, L, g, frames_dev, poses, args.warmup, args.steps, stream)
+        kern16, _, _, acc16 = run_kernels(torch, L, g, frames_dev, poses, args.warmup, args.steps, count=False)
+    finally:
+        g.close()
+    return {"frames_per_s": args.steps / e16, "kernel_ms": kern16,
+            "latency_ms": {"p50": float(np.percentile(lat16, 50)), "p99": float(np.percentile(lat16, 99))},
+            "accuracy": acc16}
+
+
+def cpu_leg(depth, poses, args):
+    cfps, threads, cdt = cpu_baseline(depth, poses, args.volume, args.cpu_frames, args.cpu_warm)
+    return {"value": cfps, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": "frames %d-%d of the same %d^3 FP32 sequence (frames 0-%d untimed after the reset), "
+                      "C oracle, %d OpenMP threads, %.1f s" % (args.cpu_warm, args.cpu_warm + args.cpu_frames - 1,
+                                                                args.volume, args.cpu_warm - 1, threads, cdt)}
 
 
 if __name__ == "__main__":

@@ -235,6 +235,19 @@ def test_track_rejects_empty_and_far_frames(o32, seq20):
     g.depthToVertex(np.zeros((424, 512), np.uint16))
     assert g.Track() is False
     np.testing.assert_array_equal(g.getPose(), gt[0])
+    # a model built from three frames, then a far frame: a flat wall at 4 m,
+    # every pixel beyond dist_threshold of the model (result -4) or off it,
+    # so the error sum is 0, AE = 0 and Track() rejects (gFusion.cpp:715-723)
+    g.Reset(gt[0])
+    for f in range(3):
+        g.frame(depth[f].astype(np.float32))
+    g.synchronize()
+    before = g.getPose()
+    g.depthToVertex(np.full((424, 512), 4000, np.uint16))
+    assert g.Track() is False
+    np.testing.assert_array_equal(g.getPose(), before)
+    assert g.status()["tracked"] == 0
+    assert g.alignmentEnergy() == 0.0
 
 
 # ---------------------------------------------------------------------------
