@@ -261,6 +261,18 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         results.  Default 0 (the second launch's long
  *                         dependent chains cost more than the tail saved). */
 #define KF_OPT_RAYCAST_BUDGET 15
+/*   KF_OPT_ICP_REFERENCE_ORDER  1: every ICP iteration records track.cs's
+ *                         per-pixel output and forms reduce.cs's sums in
+ *                         reduce.cs's own order (8 groups x 112 threads,
+ *                         serial FP32 per thread, serial 112-row add,
+ *                         reduce.cs:27-127), combines them in FP32
+ *                         (gFusion.cpp:1137-1167) and solves by Jacobi
+ *                         alone: the partials, AE and pose of the CPU
+ *                         restatement bit for bit.  Two launches per
+ *                         iteration, single session only (a parity mode).
+ *                         0 (default): the fused kernels (same products,
+ *                         wave-tree and FP64 group sums). */
+#define KF_OPT_ICP_REFERENCE_ORDER 16
 /*   KF_OPT_TRACK_MODE     tracking of the frame loop (kinect.h:143-144):
  *                         KF_TRACK_POINT_TO_PLANE (Track, default),
  *                         KF_TRACK_POINT_TO_VOLUME (TrackSDF) or both (Track

@@ -49,6 +49,7 @@ OPT_ICP_PERSISTENT = 12
 OPT_ICP_SOLO_ROWS = 13
 OPT_RAYCAST_SCHEDULE = 14
 OPT_RAYCAST_BUDGET = 15
+OPT_ICP_REFERENCE_ORDER = 16
 
 TRACK_POINT_TO_PLANE = 1
 TRACK_POINT_TO_VOLUME = 2

@@ -324,13 +324,14 @@ int run_icp(kf_session *s, int level, int first, int save_pose, hipEvent_t t0 = 
     kf::IcpArgs a = icp_args(s, level, first, save_pose);
     a.finish = finish;
     a.frame = frame;
-    if (s->debug) {
+    if (s->debug || s->icp_ref_order) {
         a.dbg_result = s->dbg_result;
         a.dbg_error = s->dbg_error;
         a.dbg_J = s->dbg_J;
         s->dbg_level = level;
     }
-    hipError_t e = kf::launch_icp_iteration(a, s->st, s->stream, t0, t1);
+    hipError_t e = s->icp_ref_order ? kf::launch_icp_ref_iteration(a, s->st, s->stream, t0, t1)
+                                    : kf::launch_icp_iteration(a, s->st, s->stream, t0, t1);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("icp launch: ") + hipGetErrorString(e));
     return KF_OK;
 }
@@ -340,7 +341,7 @@ int run_icp(kf_session *s, int level, int first, int save_pose, hipEvent_t t0 = 
 // The last launch also applies the accept / reject (and the frame gate).
 int run_track(kf_session *s, int frame)
 {
-    if (s->icp_persistent && !s->debug) {
+    if (s->icp_persistent && !s->debug && !s->icp_ref_order) {
         // one launch for every level and iteration when its grid fits the chip
         kf::IcpTrackArgs a{};
         int nblk = 0, total = 0;
@@ -513,6 +514,21 @@ int run_raycast(kf_session *s)
         e = kf::launch_raycast(a, s->cfg.storage, s->st, s->stream, tev(s, 6), tev(s, 7));
     }
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("raycast launch: ") + hipGetErrorString(e));
+    return KF_OK;
+}
+
+// per-pixel track output of one ICP iteration ({result, error, J[6]} of
+// track.cs:18-27 as planes), for kf_set_debug and the reference-order Track()
+int alloc_track_records(kf_session *s)
+{
+    if (s->dbg_result) return KF_OK;
+    const size_t p0 = (size_t)s->cfg.width * s->cfg.height;
+    if (hipMalloc((void **)&s->dbg_result, p0 * 4) != hipSuccess ||
+        hipMalloc((void **)&s->dbg_error, p0 * 4) != hipSuccess || hipMalloc((void **)&s->dbg_J, p0 * 24) != hipSuccess)
+        return fail(KF_ERR_OOM, "per-pixel track records");
+    KF_HIP(hipMemsetAsync(s->dbg_result, 0, p0 * 4, s->stream));
+    KF_HIP(hipMemsetAsync(s->dbg_error, 0, p0 * 4, s->stream));
+    KF_HIP(hipMemsetAsync(s->dbg_J, 0, p0 * 24, s->stream));
     return KF_OK;
 }
 
@@ -957,6 +973,13 @@ int kf_set_option(kf_session *s, int option, int value)
         if (value < 0) return fail(KF_ERR_INVALID, "solo rows must be >= 0");
         s->icp_solo_rows = value;
         break;
+    case KF_OPT_ICP_REFERENCE_ORDER:
+        if (value) {
+            const int rc = alloc_track_records(s);
+            if (rc) return rc;
+        }
+        s->icp_ref_order = value ? 1 : 0;
+        break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     drop_graph(s);  // captured launches carry the old arguments
@@ -987,6 +1010,7 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_RECOVER: *value = s->recover; break;
     case KF_OPT_ICP_PERSISTENT: *value = s->icp_persistent; break;
     case KF_OPT_ICP_SOLO_ROWS: *value = s->icp_solo_rows; break;
+    case KF_OPT_ICP_REFERENCE_ORDER: *value = s->icp_ref_order; break;
     case KF_OPT_RAYCAST_SCHEDULE: *value = s->ray_sched; break;
     case KF_OPT_RAYCAST_BUDGET: *value = s->ray_budget; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
@@ -1094,15 +1118,9 @@ int kf_read_status(kf_session *s, int *tracked, int *iterations_run, int *counte
 int kf_set_debug(kf_session *s, int record)
 {
     if (!s) return fail(KF_ERR_INVALID, "session is NULL");
-    if (record && !s->dbg_result) {
-        const size_t p0 = (size_t)s->cfg.width * s->cfg.height;
-        if (hipMalloc((void **)&s->dbg_result, p0 * 4) != hipSuccess ||
-            hipMalloc((void **)&s->dbg_error, p0 * 4) != hipSuccess ||
-            hipMalloc((void **)&s->dbg_J, p0 * 24) != hipSuccess)
-            return fail(KF_ERR_OOM, "debug buffers");
-        KF_HIP(hipMemsetAsync(s->dbg_result, 0, p0 * 4, s->stream));
-        KF_HIP(hipMemsetAsync(s->dbg_error, 0, p0 * 4, s->stream));
-        KF_HIP(hipMemsetAsync(s->dbg_J, 0, p0 * 24, s->stream));
+    if (record) {
+        const int rc = alloc_track_records(s);
+        if (rc) return rc;
     }
     s->debug = record ? 1 : 0;
     drop_graph(s);
@@ -1301,6 +1319,8 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
             return fail(KF_ERR_INVALID, "batched sessions must share depth size, ICP schedule, storage and options");
         if (si->track_mode != KF_TRACK_POINT_TO_PLANE)
             return fail(KF_ERR_INVALID, "batched sessions track point-to-plane only (KF_OPT_TRACK_MODE = 1)");
+        if (si->icp_ref_order)
+            return fail(KF_ERR_INVALID, "reference-order ICP sums (KF_OPT_ICP_REFERENCE_ORDER) are single-session only");
         for (int j = 0; j < i; ++j)
             if (sessions[j] == si) return fail(KF_ERR_INVALID, "a session appears twice in the batch");
     }

@@ -190,6 +190,10 @@ hipError_t launch_icp_iteration(const IcpArgs &a, DevState *st, hipStream_t s, h
 __host__ __device__ inline int icp_blocks(const IcpArgs &a) { return (a.group_only >= 0 ? 1 : 8) * a.rows_step; }
 // one launch of nblk blocks (nblk >= icp_blocks of every level with
 // iterations, all resident at once)
+// one ICP iteration with the reference's summation order: per-pixel records
+// into a.dbg_* (which must be set), then reduce.cs's sums, combine, solve
+hipError_t launch_icp_ref_iteration(const IcpArgs &a, DevState *st, hipStream_t s, hipEvent_t t0 = nullptr,
+                                    hipEvent_t t1 = nullptr);
 hipError_t launch_icp_track(const IcpTrackArgs &a, DevState *st, int nblk, hipStream_t s, hipEvent_t t0,
                             hipEvent_t t1);
 hipError_t launch_icp_finish(DevState *st, int frame, float *lib, int lib_max, hipStream_t s,

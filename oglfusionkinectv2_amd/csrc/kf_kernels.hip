@@ -280,56 +280,17 @@ __device__ __forceinline__ bool ldlt6_wave(const double (&Arow)[6], double bi, d
     return true;
 }
 
-// Minimum-norm least-squares solve of the symmetric 6x6 system (the contract
-// of Eigen::JacobiSVD::solve at gFusion.cpp:696-697): LDL^T in FP64 for the
-// positive-definite case, cyclic Jacobi eigen-decomposition otherwise.
-// Runs on one thread; `ws` (>= 120 doubles) is LDS scratch so that the solve
+// LDS scratch of the one-thread 6x6 solves below (doubles), so that a solve
 // does not inflate the register budget of the kernel that calls it.
 constexpr int kSolveWs = 120;
-__device__ void solve6(const double *Cin, const double *b, double *x, double *ws)
+// Minimum-norm solve of the symmetric system held in ws[0..35] (A, already
+// symmetrised) by cyclic Jacobi eigen-decomposition, singular values below
+// 6*eps*sigma_max treated as zero (Eigen's JacobiSVD threshold); V in
+// ws[72..107].  One thread.
+__device__ void jacobi6(const double *b, double *x, double *ws)
 {
     double(*A)[6] = (double(*)[6])ws;
-    double(*L)[6] = (double(*)[6])(ws + 36);
     double(*V)[6] = (double(*)[6])(ws + 72);
-    double *Dg = ws + 108, *y = ws + 114;
-    double dmax = 0.0;
-    for (int i = 0; i < 6; ++i) {
-        for (int j = 0; j < 6; ++j) A[i][j] = 0.5 * (Cin[i * 6 + j] + Cin[j * 6 + i]);
-        dmax = fmax(dmax, fabs(A[i][i]));
-    }
-    // LDL^T
-    bool spd = dmax > 0.0;
-    const double tol = dmax * 1e-12;
-    for (int j = 0; j < 6 && spd; ++j) {
-        double d = A[j][j];
-        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k] * Dg[k];
-        if (!(d > tol)) {
-            spd = false;
-            break;
-        }
-        Dg[j] = d;
-        L[j][j] = 1.0;
-        for (int i = j + 1; i < 6; ++i) {
-            double s = A[i][j];
-            for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k] * Dg[k];
-            L[i][j] = s / d;
-        }
-    }
-    if (spd) {
-        for (int i = 0; i < 6; ++i) {
-            double s = b[i];
-            for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
-            y[i] = s;
-        }
-        for (int i = 0; i < 6; ++i) y[i] /= Dg[i];
-        for (int i = 5; i >= 0; --i) {
-            double s = y[i];
-            for (int k = i + 1; k < 6; ++k) s -= L[k][i] * x[k];
-            x[i] = s;
-        }
-        return;
-    }
-    // Jacobi eigen-decomposition, pseudo-inverse with threshold 6*eps*sigma_max
     for (int i = 0; i < 6; ++i)
         for (int j = 0; j < 6; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
     for (int sweep = 0; sweep < 64; ++sweep) {
@@ -375,6 +336,66 @@ __device__ void solve6(const double *Cin, const double *b, double *x, double *ws
         proj /= lam;
         for (int k = 0; k < 6; ++k) x[k] += V[k][i] * proj;
     }
+}
+
+// The same contract solved by Jacobi alone (no LDL^T shortcut), for the
+// reference-order Track(): A = (C + C^T) / 2 then jacobi6, the operations of
+// the CPU restatement of JacobiSVD::solve in the same order.
+__device__ void solve6_jacobi(const double *Cin, const double *b, double *x, double *ws)
+{
+    double(*A)[6] = (double(*)[6])ws;
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) A[i][j] = 0.5 * (Cin[i * 6 + j] + Cin[j * 6 + i]);
+    jacobi6(b, x, ws);
+}
+
+// Minimum-norm least-squares solve of the symmetric 6x6 system (the contract
+// of Eigen::JacobiSVD::solve at gFusion.cpp:696-697): LDL^T in FP64 for the
+// positive-definite case, cyclic Jacobi eigen-decomposition otherwise.
+// Runs on one thread; `ws` holds kSolveWs doubles.
+__device__ void solve6(const double *Cin, const double *b, double *x, double *ws)
+{
+    double(*A)[6] = (double(*)[6])ws;
+    double(*L)[6] = (double(*)[6])(ws + 36);
+    double *Dg = ws + 108, *y = ws + 114;
+    double dmax = 0.0;
+    for (int i = 0; i < 6; ++i) {
+        for (int j = 0; j < 6; ++j) A[i][j] = 0.5 * (Cin[i * 6 + j] + Cin[j * 6 + i]);
+        dmax = fmax(dmax, fabs(A[i][i]));
+    }
+    // LDL^T
+    bool spd = dmax > 0.0;
+    const double tol = dmax * 1e-12;
+    for (int j = 0; j < 6 && spd; ++j) {
+        double d = A[j][j];
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k] * Dg[k];
+        if (!(d > tol)) {
+            spd = false;
+            break;
+        }
+        Dg[j] = d;
+        L[j][j] = 1.0;
+        for (int i = j + 1; i < 6; ++i) {
+            double s = A[i][j];
+            for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k] * Dg[k];
+            L[i][j] = s / d;
+        }
+    }
+    if (spd) {
+        for (int i = 0; i < 6; ++i) {
+            double s = b[i];
+            for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+            y[i] = s;
+        }
+        for (int i = 0; i < 6; ++i) y[i] /= Dg[i];
+        for (int i = 5; i >= 0; --i) {
+            double s = y[i];
+            for (int k = i + 1; k < 6; ++k) s -= L[k][i] * x[k];
+            x[i] = s;
+        }
+        return;
+    }
+    jacobi6(b, x, ws);
 }
 
 // glm::eulerAngleXYZ(x3,x4,x5) in double, cast to float, translation x0..2,
@@ -882,6 +903,40 @@ __device__ __forceinline__ void icp_finish_wave(DevState *st, int frame, float *
     }
 }
 
+// The end of one ICP iteration on the first wavefront (every lane holds the
+// same x, |x| and AE): pose update (lanes 0-2 evaluate the three angles' cos
+// / sin in parallel, the same double routines, so the same bits), the early-
+// exit flag of gFusion.cpp:707-708, and on the last launch of Track() the
+// accept / reject.
+__device__ __forceinline__ void icp_apply(const IcpArgs &a, DevState *__restrict__ st, const double (&x)[6],
+                                          double nrm, float ae)
+{
+    const int lane = threadIdx.x & 63;
+    const double my_ang = lane == 1 ? x[4] : lane == 2 ? x[5] : x[3];
+    const double c1 = cos(my_ang), s1 = sin(my_ang);
+    const double cs[3] = {c1, __shfl(c1, 1, 64), __shfl(c1, 2, 64)};
+    const double sn[3] = {s1, __shfl(s1, 1, 64), __shfl(s1, 2, 64)};
+    if (lane == 0) {
+        float P[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
+        if (a.save_pose) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) st->old_pose[i] = P[i];
+        }
+        pose_update_cs(x, cs, sn, P);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) st->pose[i] = P[i];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st->x[k] = x[k];
+        st->ae = ae;
+        st->level_done = (nrm < 1e-5 && nrm != 0.0) ? 1 : 0;
+        st->iters_run = (a.save_pose ? 0 : st->iters_run) + 1;
+        st->ticket = 0u;
+    }
+    if (a.finish) icp_finish_wave(st, a.frame, a.library, a.lib_max);
+}
+
 // One ICP iteration per launch (track.cs + reduce.cs + getReduction + solve +
 // pose update); the last block to finish (ticket) does the FP64 part.
 // blk / nblk: this block's index and the block count of the session's launch
@@ -918,35 +973,10 @@ __device__ __forceinline__ void icp_body(const IcpArgs &a, DevState *__restrict_
     icp_group_sums(a.block_partials, 8 * a.rows_step, a.group_only, wgsum, gsum);
     icp_expose(a, gsum, st->partials);
     if (threadIdx.x >= 64) return;
-    // lane 0 solves; lanes 0-2 then evaluate the three angles' cos / sin in
-    // parallel (the same double routines, so the same bits)
-    const int lane = threadIdx.x;
     double x[6] = {0, 0, 0, 0, 0, 0};
     float ae = 0.0f;
     const double nrm = icp_solve(a, gsum, solve_ws, x, ae);
-    const double my_ang = lane == 1 ? x[4] : lane == 2 ? x[5] : x[3];
-    const double c1 = cos(my_ang), s1 = sin(my_ang);
-    const double cs[3] = {c1, __shfl(c1, 1, 64), __shfl(c1, 2, 64)};
-    const double sn[3] = {s1, __shfl(s1, 1, 64), __shfl(s1, 2, 64)};
-    if (lane == 0) {
-        float P[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
-        if (a.save_pose) {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) st->old_pose[i] = P[i];
-        }
-        pose_update_cs(x, cs, sn, P);
-#pragma unroll
-        for (int i = 0; i < 16; ++i) st->pose[i] = P[i];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) st->x[k] = x[k];
-        st->ae = ae;
-        st->level_done = (nrm < 1e-5 && nrm != 0.0) ? 1 : 0;
-        st->iters_run = (a.save_pose ? 0 : st->iters_run) + 1;
-        st->ticket = 0u;
-    }
-    if (a.finish) icp_finish_wave(st, a.frame, a.library, a.lib_max);
+    icp_apply(a, st, x, nrm, ae);
 }
 
 __global__ __launch_bounds__(kIcpThreads) void k_icp_iteration(IcpArgs a, DevState *__restrict__ st)
@@ -958,6 +988,142 @@ __global__ void k_icp_finish(DevState *st, int frame, float *lib, int lib_max)
 {
     if (threadIdx.x >= 64 || blockIdx.x != 0) return;
     icp_finish_wave(st, frame, lib, lib_max);
+}
+
+// ---------------------------------------------------------------------------
+// Track() with the reference's summation order (KF_OPT_ICP_REFERENCE_ORDER).
+// Per iteration: k_icp_records writes track.cs's per-pixel output
+// ({result, error, J[6]}, track.cs:18-27) for every pixel of the level; then
+// k_icp_reduce_ref forms reduce.cs's sums in reduce.cs's own order -- block g
+// of 8 over rows y = g (mod 8), thread s of 112 over columns x = s (mod 112),
+// serial FP32 sums per thread in row-major order, the count term counting
+// every pixel, then threads 0..31 adding the 112 per-thread rows serially
+// (reduce.cs:27-127) -- and its last block applies getReduction's combine in
+// FP32 (gFusion.cpp:1137-1167) and the JacobiSVD-contract solve by Jacobi
+// alone.  These are the operations of the CPU restatement in the same order,
+// so partials, AE and pose come out as the same bits; the fused kernels above
+// sum the same products in another order (wave trees, FP64 group sums).
+constexpr int kRefThreads = 112;  // reduce.cs work-group size (gFusion.cpp:795)
+
+__global__ __launch_bounds__(256) void k_icp_records(IcpArgs a, const DevState *__restrict__ st)
+{
+    if (!a.first && st->level_done) return;  // early exit of this level (uniform)
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.w * a.h) return;
+    float T[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) T[k] = st->pose[k];
+    float view[16];
+    icp_view(a, T, view);
+    TrackPx t;
+    track_project(a, T, view, a.norm[i], a.vert[i], t);
+    float4 N4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), V4 = N4;
+    if (t.code == 0 && t.ri >= 0) {
+        N4 = a.refN[t.ri];
+        V4 = a.refV[t.ri];
+    }
+    float e = 0.0f, J[6] = {0, 0, 0, 0, 0, 0};
+    const int r = track_finish(a, T, t, N4, V4, e, J);
+    a.dbg_result[i] = r;
+    a.dbg_error[i] = e;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.dbg_J[6 * i + k] = J[k];
+}
+
+__global__ __launch_bounds__(128) void k_icp_reduce_ref(IcpArgs a, DevState *__restrict__ st)
+{
+    __shared__ float S[kRefThreads][32];
+    __shared__ int last;
+    __shared__ double solve_ws[kSolveWs + 36 + 6 + 6];
+    if (!a.first && st->level_done) {  // early exit of this level (uniform)
+        if (a.finish && blockIdx.x == 0 && threadIdx.x < 64) icp_finish_wave(st, a.frame, a.library, a.lib_max);
+        return;
+    }
+    const int g = blockIdx.x, sl = threadIdx.x;
+    if (sl < kRefThreads) {
+        float sums[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) sums[k] = 0.0f;
+        for (int y = g; y < a.h; y += 8)
+            for (int x = sl; x < a.w; x += kRefThreads) {
+                const int i = y * a.w + x;
+                const int r = a.dbg_result[i];
+                if (r < 1) {  // codes -1 .. -5: each counted once (reduce.cs:47-62)
+                    sums[28] += 1.0f;
+                    continue;
+                }
+                const float e = a.dbg_error[i];
+                float J[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) J[k] = a.dbg_J[6 * i + k];
+                sums[0] += e * e;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sums[k + 1] += e * J[k];
+                int q = 7;
+#pragma unroll
+                for (int p = 0; p < 6; ++p)
+#pragma unroll
+                    for (int s2 = p; s2 < 6; ++s2) sums[q++] += J[p] * J[s2];
+                sums[28] += 1.0f;
+            }
+#pragma unroll
+        for (int k = 0; k < 32; ++k) S[sl][k] = sums[k];
+    }
+    __syncthreads();
+    if (sl < 32) {
+        float acc = S[0][sl];
+        for (int k = 1; k < kRefThreads; ++k) acc += S[k][sl];
+        __hip_atomic_store(&st->partials[g * 32 + sl], acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned t = atomicAdd(&st->ticket, 1u);
+        last = (t == 7u);
+    }
+    __syncthreads();
+    if (!last || threadIdx.x >= 64) return;
+    __threadfence();
+    const int lane = threadIdx.x;
+    double x[6] = {0, 0, 0, 0, 0, 0};
+    float ae = 0.0f;
+    double nrm = 0.0;
+    if (lane == 0) {
+        float P[256];
+        for (int k = 0; k < 256; ++k)
+            P[k] = __hip_atomic_load(&st->partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // getReduction (gFusion.cpp:1137-1167): last-row rule keeps work group
+        // 7's row; the sum rule adds the rows in order, in FP32
+        float row[32];
+        for (int k = 0; k < 32; ++k) {
+            if (a.rule == 1) {
+                float acc = P[k];
+                for (int r = 1; r < 8; ++r) acc += P[r * 32 + k];
+                row[k] = acc;
+            } else {
+                row[k] = P[7 * 32 + k];
+            }
+        }
+        ae = sqrtf(row[0] / row[28]);
+        double *C = solve_ws + kSolveWs, *b = C + 36, *xs = b + 6;
+        for (int k = 0; k < 6; ++k) b[k] = (double)row[1 + k];
+        int q = 7;
+        for (int p = 0; p < 6; ++p)
+            for (int s2 = p; s2 < 6; ++s2) {
+                C[p * 6 + s2] = (double)row[q];
+                C[s2 * 6 + p] = (double)row[q];
+                ++q;
+            }
+        solve6_jacobi(C, b, xs, solve_ws);
+        for (int k = 0; k < 6; ++k) x[k] = xs[k];
+        for (int k = 0; k < 6; ++k) nrm += x[k] * x[k];
+        nrm = sqrt(nrm);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) x[k] = readlane_d(x[k], 0);
+    nrm = readlane_d(nrm, 0);
+    ae = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ae), 0));
+    icp_apply(a, st, x, nrm, ae);
 }
 
 // Track() (gFusion.cpp:648-740) in one launch: every level (coarse to fine)
@@ -2748,6 +2914,14 @@ hipError_t launch_icp_track(const IcpTrackArgs &a, DevState *st, int nblk, hipSt
     for (int l = 0; l < 3; ++l)
         if (a.iters[l] > 0 && icp_blocks(a.lv[l]) > nblk) return hipErrorInvalidValue;
     launch_timed(k_icp_track, dim3(nblk), dim3(kIcpThreads), s, t0, t1, a, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_icp_ref_iteration(const IcpArgs &a, DevState *st, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
+{
+    if (!a.dbg_result || !a.dbg_error || !a.dbg_J) return hipErrorInvalidValue;
+    launch_timed(k_icp_records, dim3((a.w * a.h + 255) / 256), dim3(256), s, t0, nullptr, a, (const DevState *)st);
+    launch_timed(k_icp_reduce_ref, dim3(8), dim3(128), s, nullptr, t1, a, st);
     return hipGetLastError();
 }
 

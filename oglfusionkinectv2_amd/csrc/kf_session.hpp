@@ -54,6 +54,7 @@ struct kf_session {
     kf::RayState *ray_defer = nullptr;  // deferred rays, one slot per pixel
     unsigned *ray_defer_n = nullptr;    // their count, exit ticket
     int icp_solo_rows = 0;   // levels with at most this many rows per row group: one block per group
+    int icp_ref_order = 0;   // Track() sums in reduce.cs's order (per-pixel records + serial FP32 sums)
     int cus = 0;             // compute units of the device
     // frame graph: ICP + gate + integrate + raycast, captured once
     hipGraph_t graph = nullptr;
