@@ -137,6 +137,11 @@ void free_all(kf_session *s)
     f(s->mc_tab); f(s->mc_count); f(s->mc_offset); f(s->mc_part); f(s->mc_total); f(s->mc_verts); f(s->mip);
     for (int i = 0; i < 8; ++i)
         if (s->tev[i]) (void)hipEventDestroy(s->tev[i]);
+    for (int k = 0; k < 2; ++k) {
+        if (s->h_stage_ev[k]) (void)hipEventSynchronize(s->h_stage_ev[k]);
+        if (s->h_stage[k]) (void)hipHostFree(s->h_stage[k]);
+        if (s->h_stage_ev[k]) (void)hipEventDestroy(s->h_stage_ev[k]);
+    }
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -556,6 +561,33 @@ int map_ptr(kf_session *s, int which, int level, void **ptr, size_t *bytes)
     return fail(KF_ERR_INVALID, "unknown map");
 }
 
+// Host frame -> device staging buffer through a pinned slot: the caller's
+// buffer is copied on the host (it may be reused as soon as the call
+// returns, as depthToVertex's float* is), then sent by DMA on the session
+// stream.  A slot is refilled only after its previous DMA completed.
+int upload_host(kf_session *s, const void *src, size_t bytes, void *dst)
+{
+    const size_t cap = (size_t)s->cfg.width * s->cfg.height * 4;
+    if (bytes > cap) return fail(KF_ERR_INVALID, "upload larger than a depth frame");
+    if (!s->h_stage[0]) {
+        for (int k = 0; k < 2; ++k) {
+            if (hipHostMalloc(&s->h_stage[k], cap, hipHostMallocDefault) != hipSuccess)
+                return fail(KF_ERR_OOM, "pinned staging buffer");
+            if (hipEventCreateWithFlags(&s->h_stage_ev[k], hipEventDisableTiming) != hipSuccess)
+                return fail(KF_ERR_HIP, "hipEventCreate failed");
+        }
+    }
+    const int k = s->h_stage_next;
+    if (s->h_stage_pending[k]) KF_HIP(hipEventSynchronize(s->h_stage_ev[k]));
+    s->h_stage_pending[k] = false;
+    std::memcpy(s->h_stage[k], src, bytes);
+    KF_HIP(hipMemcpyAsync(dst, s->h_stage[k], bytes, hipMemcpyHostToDevice, s->stream));
+    KF_HIP(hipEventRecord(s->h_stage_ev[k], s->stream));
+    s->h_stage_pending[k] = true;
+    s->h_stage_next = k ^ 1;
+    return KF_OK;
+}
+
 }  // namespace
 
 int kf_fail(int code, const char *msg) { return fail(code, msg); }
@@ -759,16 +791,16 @@ int kf_depth_to_vertex_f32(kf_session *s, const float *depth_mm)
 {
     if (!s || !depth_mm) return fail(KF_ERR_INVALID, "NULL argument");
     const size_t n = (size_t)s->cfg.width * s->cfg.height;
-    KF_HIP(hipMemcpyAsync(s->stage_f32, depth_mm, n * 4, hipMemcpyHostToDevice, s->stream));
-    return run_pyramid_from_stage(s, s->stage_f32, 0);
+    const int rc = upload_host(s, depth_mm, n * 4, s->stage_f32);
+    return rc ? rc : run_pyramid_from_stage(s, s->stage_f32, 0);
 }
 
 int kf_depth_to_vertex_u16(kf_session *s, const uint16_t *depth_mm)
 {
     if (!s || !depth_mm) return fail(KF_ERR_INVALID, "NULL argument");
     const size_t n = (size_t)s->cfg.width * s->cfg.height;
-    KF_HIP(hipMemcpyAsync(s->stage_u16, depth_mm, n * 2, hipMemcpyHostToDevice, s->stream));
-    return run_pyramid_from_stage(s, s->stage_u16, 1);
+    const int rc = upload_host(s, depth_mm, n * 2, s->stage_u16);
+    return rc ? rc : run_pyramid_from_stage(s, s->stage_u16, 1);
 }
 
 int kf_depth_to_vertex_u16_dev(kf_session *s, const uint16_t *depth_mm_dev)

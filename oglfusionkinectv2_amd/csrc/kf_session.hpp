@@ -19,6 +19,13 @@ struct kf_session {
     float *depth_m = nullptr;
     uint16_t *stage_u16 = nullptr;
     float *stage_f32 = nullptr;
+    // pinned host staging of the host-buffer uploads (two slots, allocated on
+    // first use): the caller's frame is copied into a slot and sent by DMA in
+    // stream order, so the call returns without waiting for the GPU
+    void *h_stage[2] = {nullptr, nullptr};
+    hipEvent_t h_stage_ev[2] = {nullptr, nullptr};
+    bool h_stage_pending[2] = {false, false};
+    int h_stage_next = 0;
     float4 *vert[3] = {nullptr, nullptr, nullptr};
     float4 *norm[3] = {nullptr, nullptr, nullptr};
     float4 *refV = nullptr, *refN = nullptr;

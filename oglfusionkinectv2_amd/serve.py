@@ -63,17 +63,25 @@ class Session:
 class FusionServer:
     """Admission, free list and continuous batching over a backend with
     bytes(volume), create(volume), destroy(h), start(h, seq), members(hs),
-    frame(ptrs, active), depth_ptr(seq, i) and mark().
+    frame(ptrs, active), depth_ptr(seq, i), mark() and done(mark).
 
     Admission is first-come first-served: a request waits while the live and
     free-listed sessions plus its own bytes would exceed the budget (free-list
-    entries of other sizes are released first) or the batch is full."""
+    entries of other sizes are released first) or the batch is full.
 
-    def __init__(self, backend, budget_bytes, sequences, on_retire=None):
+    At most `max_inflight` launches are queued on the GPU at once (0: no
+    limit): while that many are still running, frames that fall due wait on
+    the host and all ride the next launch together, so under load the batch
+    grows instead of the stream filling with one-frame launches."""
+
+    def __init__(self, backend, budget_bytes, sequences, on_retire=None, max_inflight=2):
         self.b = backend
         self.budget = budget_bytes
         self.sequences = sequences
         self.on_retire = on_retire
+        self.max_inflight = max_inflight
+        self.inflight = []  # marks of launches not yet seen complete, oldest first
+        self.blocked = 0  # passes that found every launch slot busy
         self.live = []
         self.free = {}  # volume -> [handle]
         self.waiting = []  # FIFO of (sid, volume, rate, frames, t_request)
@@ -146,9 +154,21 @@ class FusionServer:
     def next_due(self):
         return min((d for d in (s.next_due() for s in self.live) if d is not None), default=None)
 
+    def busy(self):
+        """True while max_inflight launches are still running."""
+        if not self.max_inflight:
+            return False
+        while self.inflight and self.b.done(self.inflight[0]):
+            self.inflight.pop(0)
+        return len(self.inflight) >= self.max_inflight
+
     def tick(self, now):
-        """One batched frame for every session with a frame due by `now`;
-        returns the number of frames launched."""
+        """One batched frame for every session with a frame due by `now`
+        (none while every launch slot is busy); returns the number of frames
+        launched."""
+        if self.busy():
+            self.blocked += 1
+            return 0
         active, ptrs, dues = [], [], []
         for s in self.live:
             d = s.next_due()
@@ -163,7 +183,10 @@ class FusionServer:
         if not dues:
             return 0
         self.b.frame(ptrs, active)
-        self.records.append((self.b.mark(), dues))
+        m = self.b.mark()
+        self.records.append((m, dues))
+        if self.max_inflight:
+            self.inflight.append(m)
         self._retire(now)
         return len(dues)
 
@@ -199,6 +222,8 @@ def drive(server, arrivals, seconds, clock, sleep):
         wait = min(cands) - clock()
         if wait > 0:
             sleep(wait)
+        elif server.busy():
+            sleep(1e-4)  # frames are due but every launch slot is busy: poll
     return launched
 
 
@@ -295,6 +320,9 @@ class GpuBackend:
         ev.record(self.stream)
         return ev
 
+    def done(self, ev):
+        return ev.query()
+
     def close(self):
         if self.batch is not None:
             self.batch.close()
@@ -312,12 +340,12 @@ def latency_stats(records, origin, warm_s=0.5):
 
 
 def serve(torch, arrivals, seconds, budget_bytes=200e9, storage="f16", sequences=8, pool_frames=64,
-          warm_s=0.5, options=None):
+          warm_s=0.5, options=None, max_inflight=2):
     """Real-time run of `arrivals` for at most `seconds`.  Returns a dict of
     frames/s, latency percentiles and admission counts."""
     pool = DepthPool(torch, sequences, pool_frames)
     be = GpuBackend(torch, pool, storage, options)
-    srv = FusionServer(be, budget_bytes, sequences)
+    srv = FusionServer(be, budget_bytes, sequences, max_inflight=max_inflight)
     for v in sorted({a[1] for a in arrivals}):  # warm the per-size code paths
         g = be.create(v)
         be.destroy(g)
@@ -357,6 +385,8 @@ def serve(torch, arrivals, seconds, budget_bytes=200e9, storage="f16", sequences
         "max_hbm_GB": srv.max_used / 1e9,
         "hbm_budget_GB": budget_bytes / 1e9,
         "storage": storage,
+        "max_inflight": max_inflight,
+        "passes_blocked": srv.blocked,
     }
 
 

@@ -1,4 +1,6 @@
-"""Precision of the GPU path against the FP64 oracle (test infrastructure).
+"""Precision of the GPU path against the FP64 oracle, or against the FP32
+oracle with the GPU's own storage type (the reference semantics; `against`),
+as test infrastructure.
 
 Two measurements on the same seeded sequence:
 
@@ -50,7 +52,7 @@ def _tsdf_err(v, w, V64):
     if d.size == 0:
         return dict(voxels=0)
     return dict(voxels=int(d.size), mean=float(d.mean()), p99=float(np.percentile(d, 99)),
-                max=float(d.max()), frac_gt_1e3=float((d > 1e-3).mean()),
+                max=float(d.max()), frac_gt_1e3=float((d > 1e-3).mean()), frac_exact=float((d == 0).mean()),
                 weight_differs=float(((w > 0) != (V64[..., 1] > 0)).mean()))
 
 
@@ -66,9 +68,14 @@ def _ray_err(R, R64, voxel):
                 hit_flips=float((a != b).mean()))
 
 
-def evaluate(volume=512, frames=20, storage="f16", pinned=(), session=0, with_f32_oracle=True):
+def evaluate(volume=512, frames=20, storage="f16", pinned=(), session=0, with_f32_oracle=True, against="f64",
+             free_running=True, options=None):
     """Runs the comparison; returns a JSON-able dict.  `pinned`: frame
-    indices at which the one-frame pinned comparison is made."""
+    indices at which the one-frame pinned comparison is made.  against:
+    "f64" (the FP64 oracle, FP64 TSDF) or "f32" (the FP32 oracle with the
+    GPU's own storage type, i.e. the reference semantics).  free_running:
+    also run a second GPU session on its own and compare it at every frame.
+    options: {kf option: value} set on the GPU sessions."""
     import torch  # noqa: F401  (initialises the device for the library)
 
     from oglfusionkinectv2_amd import GFusion, kinect_config, synth
@@ -82,12 +89,18 @@ def evaluate(volume=512, frames=20, storage="f16", pinned=(), session=0, with_f3
         g = GFusion()
         g.setConfig(kinect_config(volume, 2.0, storage))
         g.allocateBuffers()
+        for k, v in (options or {}).items():
+            g.setOption(k, v)
         g.Reset(_gt[0])
         return g
 
-    g, gp = gpu(), (gpu() if pinned else None)
-    o64 = oracle.Oracle("f64")
-    s64 = oracle.OracleSession(o64, volume, 2.0, mu, oracle.F64X2)
+    g, gp = (gpu() if free_running else None), (gpu() if pinned else None)
+    if against == "f64":
+        s64 = oracle.OracleSession(oracle.Oracle("f64"), volume, 2.0, mu, oracle.F64X2)
+    else:
+        mode = {"f16": oracle.F16X2, "f32": oracle.F32X2}[storage]
+        s64 = oracle.OracleSession(oracle.Oracle("f32"), volume, 2.0, mu, mode)
+        with_f32_oracle = False
     s64.reset(_gt[0])
     s32 = None
     if with_f32_oracle:
@@ -99,14 +112,18 @@ def evaluate(volume=512, frames=20, storage="f16", pinned=(), session=0, with_f3
         d = depth[f].astype(np.float32)
         if f in pinned:
             V0 = s64.volume()
-            gp.setVolume(V0[..., 0].astype(plane), V0[..., 1].astype(plane))
+            if against == "f64":
+                gp.setVolume(V0[..., 0].astype(plane), V0[..., 1].astype(plane))
+            else:  # the oracle's own planes in the GPU's storage type
+                gp.setVolume(V0[..., 0].view(plane), V0[..., 1].view(plane))
             del V0
             gp.setReferenceMaps(s64.map(3).astype(np.float32), s64.map(4).astype(np.float32))
             gp.setPose(s64.pose().astype(np.float32))
         s64.frame(d)
         Q = s64.pose()
-        g.frame(d)
-        ate.append(_pose_err(g.getPose(), Q))
+        if g is not None:
+            g.frame(d)
+            ate.append(_pose_err(g.getPose(), Q))
         if s32 is not None:
             s32.frame(d)
             ate32.append(_pose_err(s32.pose(), Q))
@@ -114,25 +131,36 @@ def evaluate(volume=512, frames=20, storage="f16", pinned=(), session=0, with_f3
             gp.frame(d)  # every frame, so its reset window and counters match the oracle's
         if f in pinned:
             dt, ang = _pose_err(gp.getPose(), Q)
-            V64 = s64.volume()
+            V64 = _as_values(s64.volume(), against, storage)
             v, w = gp.getVolume()
             pin.append(dict(frame=f, pose_mm=dt * 1e3, pose_mrad=ang * 1e3, tsdf=_tsdf_err(v, w, V64),
-                            raycast=_ray_err(gp.getVerts(), s64.map(3), vox)))
+                            raycast=_ray_err(gp.getVerts(), s64.map(3), vox),
+                            tracked=gp.status()["tracked"], oracle_tracked=s64.stats()["tracked"]))
             del V64
-    V64 = s64.volume()
-    v, w = g.getVolume()
-    out = dict(volume=volume, frames=frames, storage=storage, voxel_mm=vox * 1e3,
-               free_running=dict(ate_max_mm=max(a for a, _ in ate) * 1e3, ate_last_mm=ate[-1][0] * 1e3,
-                                 rot_max_mrad=max(r for _, r in ate) * 1e3,
-                                 tsdf=_tsdf_err(v, w, V64), raycast=_ray_err(g.getVerts(), s64.map(3), vox)),
-               pinned=pin)
+    out = dict(volume=volume, frames=frames, storage=storage, voxel_mm=vox * 1e3, against=against, pinned=pin)
+    if g is not None:
+        V64 = _as_values(s64.volume(), against, storage)
+        v, w = g.getVolume()
+        out["free_running"] = dict(ate_max_mm=max(a for a, _ in ate) * 1e3, ate_last_mm=ate[-1][0] * 1e3,
+                                   rot_max_mrad=max(r for _, r in ate) * 1e3,
+                                   tsdf=_tsdf_err(v, w, V64), raycast=_ray_err(g.getVerts(), s64.map(3), vox))
+        del V64
     if s32 is not None:
         out["f32_oracle_vs_f64_oracle"] = dict(ate_max_mm=max(a for a, _ in ate32) * 1e3,
                                                ate_last_mm=ate32[-1][0] * 1e3)
-    g.close()
+    if g is not None:
+        g.close()
     if gp is not None:
         gp.close()
     return out
+
+
+def _as_values(vol, against, storage):
+    """The oracle volume as float (value, weight): FP64 planes as they are,
+    the FP32 oracle's FP16 planes decoded."""
+    if against == "f32" and storage == "f16":
+        return vol.view(np.float16).astype(np.float32)
+    return vol
 
 
 def main():
@@ -141,9 +169,10 @@ def main():
     ap.add_argument("--frames", type=int, default=30)
     ap.add_argument("--storage", default="f16")
     ap.add_argument("--pinned", default="5,15,25")
+    ap.add_argument("--against", default="f64", choices=("f64", "f32"))
     args = ap.parse_args()
     pinned = tuple(int(x) for x in args.pinned.split(",") if x)
-    print(json.dumps(evaluate(args.volume, args.frames, args.storage, pinned)))
+    print(json.dumps(evaluate(args.volume, args.frames, args.storage, pinned, against=args.against)))
 
 
 if __name__ == "__main__":

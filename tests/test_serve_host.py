@@ -17,6 +17,9 @@ class FakeBackend:
         self.member_calls = []
         self.frames = []  # (members, active) per launch
         self.n = 0
+        self.gpu_s = 0.0  # simulated duration of one launch (0: complete at once)
+        self.clock = None
+        self.finish = []
 
     def bytes(self, volume):
         return self.bytes_of[volume]
@@ -46,9 +49,19 @@ class FakeBackend:
 
     def frame(self, ptrs, active):
         self.frames.append((list(self.member_calls[-1]), list(ptrs), list(active)))
+        if self.gpu_s:
+            start = max(self.clock(), self.finish[-1] if self.finish else 0.0)
+            self.finish.append(start + self.gpu_s)
 
     def mark(self):
         return len(self.frames)
+
+    def done(self, mark):
+        # with a simulated GPU (gpu_s > 0) launch `mark` finishes gpu_s after
+        # the later of its launch time and the previous launch's end
+        if not self.gpu_s:
+            return True
+        return self.clock() >= self.finish[mark - 1]
 
 
 class VirtualClock:
@@ -66,10 +79,12 @@ class VirtualClock:
         self.t = math.ceil(t / self.dt - 1e-9) * self.dt if self.dt else t
 
 
-def run_virtual(arrivals, budget, bytes_of, seconds=1e9, dt=0.0, on_retire=None, sequences=4):
+def run_virtual(arrivals, budget, bytes_of, seconds=1e9, dt=0.0, on_retire=None, sequences=4, gpu_s=0.0,
+                max_inflight=2):
     be = FakeBackend(bytes_of)
-    srv = serve.FusionServer(be, budget, sequences, on_retire=on_retire)
+    srv = serve.FusionServer(be, budget, sequences, on_retire=on_retire, max_inflight=max_inflight)
     clk = VirtualClock(dt)
+    be.gpu_s, be.clock = gpu_s, clk
     used_trace = []
     orig_tick = srv.tick
 
@@ -170,3 +185,33 @@ def test_poisson_generator_matches_config():
     assert {a[1] for a in arr} <= {256, 384, 512, 768}
     assert {a[2] for a in arr} <= {15.0, 30.0}
     assert all(200 <= a[3] <= 900 for a in arr)
+
+
+def test_busy_gpu_batches_the_frames_that_fall_due():
+    """Sixteen 30 Hz sessions with first frames spread over one period and a
+    GPU that needs 10 ms per launch: with one launch in flight at a time the
+    frames that fall due meanwhile ride the next launch together (several per
+    launch, every frame once and in order); with no cap every frame would be
+    its own launch queued behind the others."""
+    arr = [(k / (16 * 30.0), 64, 30.0, 30) for k in range(16)]
+    for cap in (1, 2):
+        srv, be, _ = run_virtual(arr, 10 ** 9, {64: 1}, gpu_s=0.010, max_inflight=cap)
+        sizes = [len(d) for _, d in srv.records]
+        assert sum(sizes) == 16 * 30
+        assert srv.blocked > 0
+        assert np.mean(sizes) >= 3.0, (cap, np.mean(sizes))
+        # a launch never holds more work than max_inflight launches behind it
+        # can make wait: no frame is launched more than (cap + 1) launches
+        # late, i.e. its launch ends within (cap + 1) * gpu_s + one period
+        for k, (m, dues) in enumerate(srv.records):
+            end = be.finish[m - 1]
+            for sid, d in dues:
+                assert end - d <= (cap + 1) * 0.010 + 1.0 / 30.0 + 1e-9
+        seen = {}
+        for _, dues in srv.records:
+            for sid, d in dues:
+                seen.setdefault(sid, []).append(d)
+        for sid in range(16):
+            assert np.allclose(np.diff(seen[sid]), 1.0 / 30.0)
+    srv, be, _ = run_virtual(arr, 10 ** 9, {64: 1}, gpu_s=0.010, max_inflight=0)
+    assert max(len(d) for _, d in srv.records) == 1
