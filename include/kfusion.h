@@ -356,6 +356,25 @@ int kf_load_calibration(const char *path, kf_cam_params *out);
 int kf_mesh_generate(kf_session *s, float iso, uint64_t *triangles);
 /* Copies min(triangles, max_triangles) triangles, 12 floats each. */
 int kf_mesh_read(kf_session *s, float *vertices, uint64_t max_triangles);
+/* No counterpart in the reference: kf_mesh_generate for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same triangles, in the same order
+ * and with the same bits, written to the caller's device buffer vertices_dev
+ * (room for max_triangles triangles of 3 float4 (x, y, z, 0) each, 16-byte
+ * aligned); the total triangle count goes to the caller's device word
+ * *triangles_dev (8-byte aligned) -- when it exceeds max_triangles only the
+ * first max_triangles triangles are written.  Allocates nothing and does not
+ * synchronise: it only queues kernels on the session stream (synchronise,
+ * e.g. kf_synchronize, before reading the results).  The per-brick counts and
+ * offsets live in a fixed device scratch array of about 16 MiB per GPU, set
+ * up by kf_session_create; it holds volumes up to 1024 voxels per side
+ * (2^21 Morton brick slots) and larger volumes are rejected with
+ * KF_ERR_INVALID.  Every call on a GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (other streams, other threads) race; calls queued on
+ * one stream are ordered.  The session's own mesh (kf_mesh_read /
+ * kf_mesh_export_*) is untouched. */
+int kf_mesh_generate_noalloc(kf_session *s, float iso, float *vertices_dev, uint64_t max_triangles,
+                             uint64_t *triangles_dev);
 /* exportMesh (mcubes.cpp:429-509): binary STL of the last kf_mesh_generate,
  * with the reference's layout (zero 80-byte header, a count field of four
  * times the triangle count, normals (1, 1, 1), zero attribute words). */

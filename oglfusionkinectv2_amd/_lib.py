@@ -149,6 +149,7 @@ def lib():
         "kf_mesh_generate": [sess, C.c_float, C.POINTER(C.c_uint64)],
         "kf_intensity_projection": [sess],
         "kf_mesh_read": [sess, f32p, C.c_uint64],
+        "kf_mesh_generate_noalloc": [sess, C.c_float, C.c_void_p, C.c_uint64, C.c_void_p],
         "kf_mesh_export_stl": [sess, C.c_char_p],
         "kf_mesh_export_ply": [sess, C.c_char_p],
         "kf_read_sdf_state": [sess, f32p, f32p, f32p, f32p, C.POINTER(C.c_int), C.POINTER(C.c_double)],

@@ -647,6 +647,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
         return fail(code, msg);
     };
     if (hipGetDevice(&s->device) != hipSuccess) return cleanup(KF_ERR_HIP, "no HIP device");
+    if (kf::mesh_noalloc_setup(s->device) != KF_OK) return cleanup(KF_ERR_HIP, kf_last_error());
     if (stream) {
         s->stream = (hipStream_t)stream;
     } else {

@@ -22,6 +22,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -391,6 +392,29 @@ int ensure_mesh_buffers(kf_session *s, unsigned long long slots)
     return KF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of kf_mesh_generate_noalloc: one file-scope device array per GPU
+// (each device the library runs on holds its own copy until the process
+// ends), laid out as the triangle table, the scan block totals, and the
+// per-brick triangle counts and their prefix sums for up to kScratchSlots
+// Morton brick slots (volumes up to 1024 voxels per side).  Host code never
+// uses the array's name as an address: set-up takes each device's address
+// with hipGetSymbolAddress and keeps it in g_scratch[device].
+// ---------------------------------------------------------------------------
+constexpr unsigned long long kScratchSlots = 1ull << 21;
+constexpr size_t kScratchTab = 0;                                  // 256 x 16 + 256 bytes
+constexpr size_t kScratchPart = 4608;                              // block totals, kScratchSlots / 1024 words
+constexpr size_t kScratchCount = kScratchPart + (kScratchSlots / kScanBlock) * 4;
+constexpr size_t kScratchOffset = kScratchCount + kScratchSlots * 4;
+constexpr size_t kScratchBytes = kScratchOffset + kScratchSlots * 4;
+static_assert(kScratchCount % 256 == 0 && kScratchOffset % 256 == 0, "scratch regions are 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kScratchMaxDevices] = {};
+
 template <class S>
 void launch_mc(const McArgs &a, bool emit, hipStream_t st)
 {
@@ -408,7 +432,47 @@ void launch_mc_storage(int storage, const McArgs &a, bool emit, hipStream_t st)
     }
 }
 
+// Morton brick slots of a volume's surface export (power-of-two cube of
+// 8^3 bricks covering the volume) and its level count
+unsigned long long mesh_slots(const int N[3], int &levels)
+{
+    int P = 8;
+    while (P < N[0] || P < N[1] || P < N[2]) P *= 2;
+    levels = 0;
+    while ((8 << levels) < P) ++levels;
+    return 1ull << (3 * levels);
+}
+
+const uint8_t *host_tables()
+{
+    static uint8_t tab[256 * 16 + 256];
+    static std::once_flag once;
+    std::call_once(once, [] { build_tables(reinterpret_cast<uint8_t(*)[16]>(tab), tab + 256 * 16); });
+    return tab;
+}
+
 }  // namespace
+
+// Once per device, with that device current: the address of the device's
+// scratch array and the triangle table copied into it.  Called by
+// kf_session_create, so kf_mesh_generate_noalloc itself neither allocates
+// nor synchronises.
+int mesh_noalloc_setup(int device)
+{
+    if (device < 0 || device >= kScratchMaxDevices) return kf_fail(KF_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return KF_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return kf_fail(KF_ERR_HIP, (std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)).c_str());
+    e = hipMemcpy((unsigned char *)p + kScratchTab, host_tables(), 256 * 16 + 256, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return kf_fail(KF_ERR_HIP, (std::string("triangle table copy: ") + hipGetErrorString(e)).c_str());
+    g_scratch[device] = (unsigned char *)p;
+    return KF_OK;
+}
+
 }  // namespace kf
 
 using kf::kScanBlock;
@@ -426,11 +490,8 @@ int kf_mesh_generate(kf_session *s, float iso, uint64_t *triangles)
 {
     if (!s) return kf_fail(KF_ERR_INVALID, "session is NULL");
     const int *N = s->cfg.volume_size;
-    int P = 8;
-    while (P < N[0] || P < N[1] || P < N[2]) P *= 2;
     int levels = 0;
-    while ((8 << levels) < P) ++levels;
-    const unsigned long long slots = 1ull << (3 * levels);
+    const unsigned long long slots = kf::mesh_slots(N, levels);
     const size_t nb = (size_t)((slots + kScanBlock - 1) / kScanBlock);
     if (nb > 256 * 16) return kf_fail(KF_ERR_INVALID, "volume too large for the surface export scan");
     int rc = kf::ensure_mesh_buffers(s, slots);
@@ -476,6 +537,60 @@ int kf_mesh_generate(kf_session *s, float iso, uint64_t *triangles)
     }
     s->mc_triangles = total;
     if (triangles) *triangles = total;
+    return KF_OK;
+}
+
+int kf_mesh_generate_noalloc(kf_session *s, float iso, float *vertices_dev, uint64_t max_triangles,
+                             uint64_t *triangles_dev)
+{
+    if (!s || !triangles_dev) return kf_fail(KF_ERR_INVALID, "NULL argument");
+    if (max_triangles && !vertices_dev) return kf_fail(KF_ERR_INVALID, "vertices_dev is NULL");
+    if ((uintptr_t)vertices_dev % 16 || (uintptr_t)triangles_dev % 8)
+        return kf_fail(KF_ERR_INVALID, "vertices_dev must be 16-byte and triangles_dev 8-byte aligned");
+    const int *N = s->cfg.volume_size;
+    int levels = 0;
+    const unsigned long long slots = kf::mesh_slots(N, levels);
+    if (slots > kf::kScratchSlots)
+        return kf_fail(KF_ERR_INVALID,
+                       ("kf_mesh_generate_noalloc: a " + std::to_string(N[0]) + "x" + std::to_string(N[1]) + "x" +
+                        std::to_string(N[2]) + " volume needs " + std::to_string(slots) +
+                        " brick slots; its scratch holds " + std::to_string(kf::kScratchSlots) +
+                        " (volumes up to 1024 voxels per side)")
+                           .c_str());
+    unsigned char *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(kf::g_scratch_mu);
+        if (s->device >= 0 && s->device < kf::kScratchMaxDevices) base = kf::g_scratch[s->device];
+    }
+    if (!base) return kf_fail(KF_ERR_STATE, "surface-export scratch not set up on this device");
+    uint32_t *count = reinterpret_cast<uint32_t *>(base + kf::kScratchCount);
+    uint32_t *offset = reinterpret_cast<uint32_t *>(base + kf::kScratchOffset);
+    uint32_t *part = reinterpret_cast<uint32_t *>(base + kf::kScratchPart);
+    const size_t nb = (size_t)((slots + kScanBlock - 1) / kScanBlock);
+    kf::McArgs a{};
+    a.tsdf = s->tsdf;
+    for (int k = 0; k < 3; ++k) {
+        a.N[k] = N[k];
+        a.NB[k] = s->NB[k];
+    }
+    a.iso = iso;
+    a.levels = levels;
+    a.dflags = (iso <= 0.0f) ? s->dflags : nullptr;
+    a.tri = base + kf::kScratchTab;
+    a.ntri = base + kf::kScratchTab + 256 * 16;
+    a.count = count;
+    a.offset = offset;
+    a.nslots = slots;
+    a.verts = reinterpret_cast<float4 *>(vertices_dev);
+    a.cap = max_triangles;
+    // every slot's count, offset and block total is written before it is read
+    kf::launch_mc_storage(s->cfg.storage, a, false, s->stream);
+    hipLaunchKernelGGL(kf::k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, s->stream, count, offset, part, slots);
+    hipLaunchKernelGGL(kf::k_scan_top, dim3(1), dim3(256), 0, s->stream, part, (int)nb,
+                       reinterpret_cast<unsigned long long *>(triangles_dev));
+    hipLaunchKernelGGL(kf::k_scan_add, dim3((unsigned)nb), dim3(256), 0, s->stream, offset, part, slots);
+    if (max_triangles) kf::launch_mc_storage(s->cfg.storage, a, true, s->stream);
+    MC_HIP(hipGetLastError());
     return KF_OK;
 }
 

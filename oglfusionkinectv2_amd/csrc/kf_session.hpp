@@ -95,3 +95,9 @@ struct kf_session {
 
 // sets the calling thread's error message (kf_last_error); returns code
 int kf_fail(int code, const char *msg);
+
+namespace kf {
+// once per device (kf_session_create): the device's surface-export scratch
+// address and its triangle table (kf_mcubes.hip)
+int mesh_noalloc_setup(int device);
+}  // namespace kf
