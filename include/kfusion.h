@@ -409,8 +409,25 @@ int kf_batch_destroy(kf_batch *b);
 int kf_batch_synchronize(kf_batch *b);
 int kf_batch_get_stream(kf_batch *b, void **stream);
 /* One frame for every session i with active[i] != 0 (active NULL: all), its
- * 512x424 uint16 mm depth at depth_mm_dev[i] (device memory). */
+ * 512x424 uint16 mm depth at depth_mm_dev[i] (device memory).  The tick's
+ * record (depth pointers, active mask) goes to the device through a ring of
+ * four pinned slots, so the call does not block unless four ticks are still
+ * queued. */
 int kf_batch_frame_u16_dev(kf_batch *b, const uint16_t *const *depth_mm_dev, const int *active);
+/* Batch options (no counterpart in the reference; results are the same bits
+ * for every setting):
+ *   KF_OPT_GRAPH          1: each tick replays a captured hipGraph of its
+ *                         record copy and launches (one per ring slot,
+ *                         captured again after a membership change); 0
+ *                         (default): direct launches, which the host queues
+ *                         ahead of the GPU anyway (measured as fast).
+ *   KF_OPT_ICP_PERSISTENT 1 (default): Track() of all members in one
+ *                         persistent launch, each session on its own group of
+ *                         blocks (at most one block per compute unit in all);
+ *                         0: one launch per ICP iteration.  Off when a member
+ *                         has per-pixel recording on. */
+int kf_batch_set_option(kf_batch *b, int option, int value);
+int kf_batch_get_option(kf_batch *b, int option, int *value);
 
 const char *kf_last_error(void);
 const char *kf_version(void);

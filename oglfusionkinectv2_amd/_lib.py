@@ -173,6 +173,8 @@ def lib():
         "kf_batch_synchronize": [vp],
         "kf_batch_get_stream": [vp, C.POINTER(C.c_void_p)],
         "kf_batch_frame_u16_dev": [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_int)],
+        "kf_batch_set_option": [vp, i32, i32],
+        "kf_batch_get_option": [vp, i32, C.POINTER(C.c_int)],
     }
     for name, args in batch.items():
         fn = getattr(L, name)

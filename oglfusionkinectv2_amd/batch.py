@@ -17,6 +17,15 @@ class SessionBatch:
         K.check(K.lib().kf_batch_create(arr, len(self._members), stream, C.byref(h)))
         self._b = h
 
+    def setOption(self, option, value):
+        """KF_OPT_GRAPH / KF_OPT_ICP_PERSISTENT (kf_batch_set_option)."""
+        K.check(K.lib().kf_batch_set_option(self._b, int(option), int(value)))
+
+    def getOption(self, option):
+        v = C.c_int()
+        K.check(K.lib().kf_batch_get_option(self._b, int(option), C.byref(v)))
+        return v.value
+
     @property
     def members(self):
         return list(self._members)

@@ -1315,12 +1315,50 @@ struct kf_batch {
     bool slot_pending[kSlots] = {false, false, false, false};
     int next_slot = 0;
     hipEvent_t join_ev = nullptr;  // orders the members' own streams before the batch stream
+    // Tick records (depth pointers, active mask): a ring of device slots, each
+    // filled by a stream-ordered copy from its own pinned host slot; a host
+    // slot is rewritten only after the tick that read it completed.
+    static constexpr int kTicks = 4;
+    kf::BatchTick *d_ticks = nullptr;
+    kf::BatchTick *h_ticks = nullptr;
+    hipEvent_t tick_ev[kTicks] = {nullptr, nullptr, nullptr, nullptr};
+    bool tick_pending[kTicks] = {false, false, false, false};
+    int next_tick = 0;
+    // Track() of all members as one persistent launch (icp_per blocks per
+    // session), or one launch per ICP iteration
+    int icp_persistent = 1, icp_per = 0;
+    // each ring slot's tick captured as a hipGraph (copy + launches), rebuilt
+    // after a membership change (version); off by default: the direct
+    // launches are queued ahead of the GPU already and measured as fast
+    int use_graph = 0;
+    bool graph_failed = false;
+    unsigned version = 0;
+    hipGraph_t graph[kTicks] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t gexec[kTicks] = {nullptr, nullptr, nullptr, nullptr};
+    unsigned graph_version[kTicks] = {0, 0, 0, 0};
+    int cus = 0;
+    int iters[3] = {0, 0, 0};
 };
 
 namespace {
 
+void drop_batch_graphs(kf_batch *b)
+{
+    for (int k = 0; k < kf_batch::kTicks; ++k) {
+        if (b->gexec[k]) (void)hipGraphExecDestroy(b->gexec[k]);
+        if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+        b->gexec[k] = nullptr;
+        b->graph[k] = nullptr;
+    }
+}
+
 void release_batch(kf_batch *b)
 {
+    drop_batch_graphs(b);
+    for (int k = 0; k < kf_batch::kTicks; ++k)
+        if (b->tick_ev[k]) (void)hipEventDestroy(b->tick_ev[k]);
+    if (b->d_ticks) (void)hipFree(b->d_ticks);
+    if (b->h_ticks) (void)hipHostFree(b->h_ticks);
     if (b->d_sess) (void)hipFree(b->d_sess);
     if (b->d_prefix) (void)hipFree(b->d_prefix);
     for (int k = 0; k < kf_batch::kSlots; ++k) {
@@ -1376,6 +1414,15 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
         kf::BatchSession &bs = h[i];
         bs.pyr = pyramid_args(si);
         for (int l = 0; l < 3; ++l) bs.icp[l] = icp_args(si, l, 0, 0);
+        bs.trk = kf::IcpTrackArgs{};
+        for (int l = 0; l < 3; ++l) {
+            bs.trk.lv[l] = bs.icp[l];
+            bs.trk.iters[l] = si->cfg.iterations[l];
+        }
+        bs.trk.frame = 1;
+        bs.trk.rows[0] = si->icp_rows;
+        bs.trk.rows[1] = si->icp_rows + (size_t)(si->cfg.height + 8) * 32;
+        bs.trk.spin_limit = 1u << 22;
         bs.integ = integrate_args(si, 1, 1048576);
         bs.ray = raycast_args(si);
         bs.brick = brick_args(si);
@@ -1403,6 +1450,85 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
     for (int l = 0; l < 3; ++l) b->nblk[l] = (h[0].icp[l].group_only >= 0 ? 1 : 8) * h[0].icp[l].rows_step;
     b->total_int = ip[n];
     b->total_coarse = cp[n];
+    for (int l = 0; l < 3; ++l) b->iters[l] = s0->cfg.iterations[l];
+    // persistent Track(): every member's blocks resident at once, at most one
+    // block per compute unit, no more blocks per session than the largest
+    // level has partial rows
+    int maxnb = 0;
+    bool pers = true;
+    for (int l = 0; l < 3; ++l)
+        if (h[0].icp[l].rows_step > 0 && b->iters[l] > 0) maxnb = std::max(maxnb, kf::icp_blocks(h[0].icp[l]));
+    for (int i = 0; i < n; ++i) pers = pers && sessions[i]->icp_persistent && !sessions[i]->debug;
+    b->icp_per = pers ? std::min(maxnb, b->cus / n) : 0;
+    ++b->version;  // captured ticks hold the old launch shapes
+    return KF_OK;
+}
+
+// The launches of one tick reading ring slot k: the record copy, then the
+// frame block of every member (kinect.cpp:654-695).
+int enqueue_tick(kf_batch *b, int k)
+{
+    const kf::BatchTick *tp = b->d_ticks + k;
+    kf::BatchTable tb{};
+    tb.sess = b->d_sess;
+    tb.int_prefix = b->d_prefix;
+    tb.coa_prefix = b->d_prefix + (kf::kMaxBatch + 1);
+    tb.n = b->n;
+    KF_HIP(hipMemcpyAsync(b->d_ticks + k, b->h_ticks + k, sizeof(kf::BatchTick), hipMemcpyHostToDevice, b->stream));
+    hipError_t e = kf::launch_batch_pyramid(tb, tp, b->pyr_blocks, b->stream);
+    if (b->icp_persistent && b->icp_per > 0) {
+        if (e == hipSuccess) e = kf::launch_batch_icp_track(tb, tp, b->icp_per, b->stream);
+    } else {
+        int last_level = -1;
+        for (int level = 0; level <= 2 && last_level < 0; ++level)
+            if (b->iters[level] > 0) last_level = level;
+        bool first_any = true;
+        for (int level = 2; level >= 0 && e == hipSuccess; --level)
+            for (int it = 0; it < b->iters[level] && e == hipSuccess; ++it) {
+                const int fin = level == last_level && it == b->iters[level] - 1;
+                e = kf::launch_batch_icp(tb, tp, level, it == 0, first_any ? 1 : 0, fin, b->nblk[level], b->stream);
+                first_any = false;
+            }
+    }
+    if (e == hipSuccess)
+        e = kf::launch_batch_integrate(tb, tp, b->storage, b->fast, b->off32, b->total_int, b->stream);
+    if (e == hipSuccess) e = kf::launch_batch_bricks(tb, tp, b->total_coarse, b->stream);
+    if (e == hipSuccess) e = kf::launch_batch_raycast(tb, tp, b->storage, b->ray_lanes, b->ray_blocks, b->stream);
+    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+    return KF_OK;
+}
+
+// Replays ring slot k's captured tick, capturing it first when the
+// membership changed since; falls back to direct launches when capture is
+// not possible on the batch stream.
+int run_tick(kf_batch *b, int k)
+{
+    if (!b->use_graph || b->graph_failed) return enqueue_tick(b, k);
+    if (!b->gexec[k] || b->graph_version[k] != b->version) {
+        if (b->gexec[k]) (void)hipGraphExecDestroy(b->gexec[k]);
+        if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+        b->gexec[k] = nullptr;
+        b->graph[k] = nullptr;
+        if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            b->graph_failed = true;
+            return enqueue_tick(b, k);
+        }
+        const int rc = enqueue_tick(b, k);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+        if (rc != KF_OK || e2 != hipSuccess || !g ||
+            hipGraphInstantiate(&b->gexec[k], g, nullptr, nullptr, 0) != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            b->gexec[k] = nullptr;
+            (void)hipGetLastError();
+            b->graph_failed = true;
+            return rc != KF_OK ? rc : enqueue_tick(b, k);
+        }
+        b->graph[k] = g;
+        b->graph_version[k] = b->version;
+    }
+    KF_HIP(hipGraphLaunch(b->gexec[k], b->stream));
     return KF_OK;
 }
 
@@ -1437,6 +1563,19 @@ int kf_batch_create(kf_session *const *sessions, int n, void *stream, kf_batch *
             return cleanup(fail(KF_ERR_OOM, "batch staging"));
     if (hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming) != hipSuccess)
         return cleanup(fail(KF_ERR_HIP, "hipEventCreate failed"));
+    if (hipMalloc((void **)&b->d_ticks, sizeof(kf::BatchTick) * kf_batch::kTicks) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_ticks, sizeof(kf::BatchTick) * kf_batch::kTicks, hipHostMallocDefault) !=
+            hipSuccess)
+        return cleanup(fail(KF_ERR_OOM, "batch tick records"));
+    for (int k = 0; k < kf_batch::kTicks; ++k)
+        if (hipEventCreateWithFlags(&b->tick_ev[k], hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(KF_ERR_HIP, "hipEventCreate failed"));
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return cleanup(fail(KF_ERR_HIP, "cannot query the compute unit count"));
+    }
     int rc = batch_fill(b, sessions, n);
     if (rc) return cleanup(rc);
     *out = b;
@@ -1471,14 +1610,40 @@ int kf_batch_get_stream(kf_batch *b, void **stream)
     return KF_OK;
 }
 
+int kf_batch_set_option(kf_batch *b, int option, int value)
+{
+    if (!b) return fail(KF_ERR_INVALID, "batch is NULL");
+    switch (option) {
+    case KF_OPT_GRAPH:
+        b->use_graph = value ? 1 : 0;
+        b->graph_failed = false;
+        break;
+    case KF_OPT_ICP_PERSISTENT: b->icp_persistent = value ? 1 : 0; break;
+    default: return fail(KF_ERR_INVALID, "unknown batch option");
+    }
+    drop_batch_graphs(b);
+    return KF_OK;
+}
+
+int kf_batch_get_option(kf_batch *b, int option, int *value)
+{
+    if (!b || !value) return fail(KF_ERR_INVALID, "NULL argument");
+    switch (option) {
+    case KF_OPT_GRAPH: *value = b->use_graph && !b->graph_failed; break;
+    case KF_OPT_ICP_PERSISTENT: *value = b->icp_persistent && b->icp_per > 0; break;
+    default: return fail(KF_ERR_INVALID, "unknown batch option");
+    }
+    return KF_OK;
+}
+
 int kf_batch_frame_u16_dev(kf_batch *b, const uint16_t *const *depth_mm_dev, const int *active)
 {
     if (!b || !depth_mm_dev) return fail(KF_ERR_INVALID, "NULL argument");
-    kf::BatchTick t{};
-    t.n = b->n;
-    t.sess = b->d_sess;
-    t.int_prefix = b->d_prefix;
-    t.coa_prefix = b->d_prefix + (kf::kMaxBatch + 1);
+    const int k = b->next_tick;
+    if (b->tick_pending[k]) KF_HIP(hipEventSynchronize(b->tick_ev[k]));
+    b->tick_pending[k] = false;
+    kf::BatchTick &t = b->h_ticks[k];
+    std::memset(&t, 0, sizeof(t));
     int nact = 0;
     for (int i = 0; i < b->n; ++i) {
         const bool on = active ? active[i] != 0 : true;
@@ -1489,22 +1654,12 @@ int kf_batch_frame_u16_dev(kf_batch *b, const uint16_t *const *depth_mm_dev, con
         ++nact;
     }
     if (!nact) return KF_OK;
-    hipError_t e = kf::launch_batch_pyramid(t, b->pyr_blocks, b->stream);
-    const kf_config &c = b->sessions[0]->cfg;
-    int last_level = -1;
-    for (int level = 0; level <= 2 && last_level < 0; ++level)
-        if (c.iterations[level] > 0) last_level = level;
-    bool first_any = true;
-    for (int level = 2; level >= 0 && e == hipSuccess; --level)
-        for (int it = 0; it < c.iterations[level] && e == hipSuccess; ++it) {
-            const int fin = level == last_level && it == c.iterations[level] - 1;
-            e = kf::launch_batch_icp(t, level, it == 0, first_any ? 1 : 0, fin, b->nblk[level], b->stream);
-            first_any = false;
-        }
-    if (e == hipSuccess) e = kf::launch_batch_integrate(t, b->storage, b->fast, b->off32, b->total_int, b->stream);
-    if (e == hipSuccess) e = kf::launch_batch_bricks(t, b->total_coarse, b->stream);
-    if (e == hipSuccess) e = kf::launch_batch_raycast(t, b->storage, b->ray_lanes, b->ray_blocks, b->stream);
-    if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+    int rc = run_tick(b, k);
+    if (rc) return rc;
+    KF_HIP(hipEventRecord(b->tick_ev[k], b->stream));
+    b->tick_pending[k] = true;
+    b->next_tick = (k + 1) % kf_batch::kTicks;
+    hipError_t e = hipSuccess;
     // per-session exact brick rebuild on the same schedule as kf_frame_*
     for (int i = 0; i < b->n; ++i) {
         if (!((t.active >> i) & 1ull)) continue;

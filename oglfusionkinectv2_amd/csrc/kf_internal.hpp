@@ -153,6 +153,7 @@ constexpr int kMaxBatch = 64;
 struct BatchSession {
     PyramidArgs pyr;
     IcpArgs icp[3];
+    IcpTrackArgs trk;  // the persistent Track() of this session (k_batch_icp_track)
     IntegrateArgs integ;
     RaycastArgs ray;
     BrickArgs brick;
@@ -160,23 +161,31 @@ struct BatchSession {
     int int_grid[3];  // integrate grid of this session
 };
 
-// per-launch arguments (by value): depth inputs and the active mask
+// per-tick record, read by the batch kernels from device memory: the depth
+// inputs and the active mask
 struct BatchTick {
     const uint16_t *depth[kMaxBatch];
     unsigned long long active;
-    int n;
-    const BatchSession *sess;                      // n entries
+};
+
+// the batch's member tables (kernel arguments; fixed between membership changes)
+struct BatchTable {
+    const BatchSession *sess;            // n entries
     const int *int_prefix, *coa_prefix;  // n + 1 entries each
+    int n;
 };
 
 void integrate_grid_dims(const IntegrateArgs &a, int g[3]);
-hipError_t launch_batch_pyramid(const BatchTick &t, int blocks_per_session, hipStream_t s);
-hipError_t launch_batch_icp(const BatchTick &t, int level, int first, int save_pose, int finish, int nblk,
-                            hipStream_t s);
-hipError_t launch_batch_integrate(const BatchTick &t, int storage, int fast, int off32, int total_blocks,
-                                  hipStream_t s);
-hipError_t launch_batch_bricks(const BatchTick &t, int total_coarse, hipStream_t s);
-hipError_t launch_batch_raycast(const BatchTick &t, int storage, int lanes, int blocks_per_session, hipStream_t s);
+// tb: the member tables; tp: the tick record in device memory
+hipError_t launch_batch_pyramid(const BatchTable &tb, const BatchTick *tp, int blocks_per_session, hipStream_t s);
+hipError_t launch_batch_icp(const BatchTable &tb, const BatchTick *tp, int level, int first, int save_pose,
+                            int finish, int nblk, hipStream_t s);
+hipError_t launch_batch_icp_track(const BatchTable &tb, const BatchTick *tp, int blocks_per_session, hipStream_t s);
+hipError_t launch_batch_integrate(const BatchTable &tb, const BatchTick *tp, int storage, int fast, int off32,
+                                  int total_blocks, hipStream_t s);
+hipError_t launch_batch_bricks(const BatchTable &tb, const BatchTick *tp, int total_coarse, hipStream_t s);
+hipError_t launch_batch_raycast(const BatchTable &tb, const BatchTick *tp, int storage, int lanes,
+                                int blocks_per_session, hipStream_t s);
 
 // Optional t0 / t1: events recorded by the kernel's own dispatch packet at its
 // start / end (exact kernel duration, no separate marker packets).

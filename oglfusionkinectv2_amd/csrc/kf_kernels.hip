@@ -1137,7 +1137,15 @@ __global__ __launch_bounds__(128) void k_icp_reduce_ref(IcpArgs a, DevState *__r
 // an exited level costs nothing.  Partial rows alternate between two
 // buffers: a block can be at most one iteration ahead of the slowest reader.
 // Results are those of the k_icp_iteration sequence bit for bit.
-__global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevState *__restrict__ st)
+// The launch body of Track() for one session whose grid is `nblk` blocks,
+// this one being `blk`: a level of nb blocks of the per-iteration launch
+// (virtual blocks) is covered by the session's blocks in turn, block blk
+// computing the partial rows of virtual blocks blk, blk + nblk, ...; the rows,
+// and so the sums, pose and decisions, are those of the per-iteration
+// launches whatever nblk is.  With nblk >= nb a block keeps its one virtual
+// block's texels in registers over the level.
+__device__ __forceinline__ void icp_track_body(const IcpTrackArgs &a, DevState *__restrict__ st, const int blk,
+                                               const int nblk)
 {
     __shared__ float wsum[kIcpThreads / 64][kTerms];
     __shared__ double gsum[8][kTerms];
@@ -1147,7 +1155,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
     __shared__ int sh_abort, sh_done;
     __shared__ __attribute__((aligned(16))) float solo_rows[8 * kPartStride];
 
-    const int blk = blockIdx.x, nblk = gridDim.x, tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
     if (tid < 16) Ts[tid] = st->pose[tid];
     if (tid == 0) {
         sh_abort = 0;
@@ -1167,7 +1175,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
     int iters_run = 0;
     bool aborted = false, solo_before = false;
     for (int level = 2; level >= 0 && !aborted; --level) {
-        const IcpArgs &L = a.lv[level];
+        const IcpArgs L = a.lv[level];
         const int nb = icp_blocks(L);
         if (a.iters[level] <= 0) continue;
         // a level served by one block is run by block 0 alone, with no grid
@@ -1193,7 +1201,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
         // a block with one chunk of pixels per thread keeps their vertex and
         // normal in registers for the level (they do not depend on the pose)
         float4 cn[kIcpPx], cv[kIcpPx];
-        const bool cached = blk < nb && icp_block_pixels(L, g, jb) <= kIcpPx;  // uniform
+        const bool cached = nblk >= nb && blk < nb && icp_block_pixels(L, g, jb) <= kIcpPx;  // uniform
         if (cached) icp_load_pixels(L, g, jb, cn, cv);
         for (int it = 0; it < a.iters[level]; ++it) {
             float T[16];
@@ -1210,10 +1218,16 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
                 unsigned long long *rows = a.rows[xchg & 1u];
                 const unsigned tag = (epoch << 8) + xchg + 1u;
                 unsigned long long *mine = rows + (8 * jb + g) * kPartStride;
-                if (cached)
+                if (cached) {
                     icp_block_partials<true>(L, T, g, jb, nullptr, wsum, cn, cv, mine, tag);
-                else if (blk < nb)
-                    icp_block_partials(L, T, g, jb, nullptr, wsum, nullptr, nullptr, mine, tag);
+                } else {
+                    for (int v = blk; v < nb; v += nblk) {
+                        const int gv = L.group_only >= 0 ? L.group_only : (v & 7);
+                        const int jv = L.group_only >= 0 ? v : (v >> 3);
+                        icp_block_partials(L, T, gv, jv, nullptr, wsum, nullptr, nullptr,
+                                           rows + (8 * jv + gv) * kPartStride, tag);
+                    }
+                }
                 if (!icp_group_sums_tagged(rows, 8 * L.rows_step, L.group_only, tag, a.spin_limit, &st->icp_abort,
                                            wgsum, gsum)) {
                     aborted = true;
@@ -1265,6 +1279,11 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevSt
         if (tid < 64) icp_finish_wave(st, a.frame, a.lv[0].library, a.lv[0].lib_max);
     }
     grid_exit(&st->icp_bar, &st->icp_exit, &st->icp_abort, (unsigned)nblk, &st->icp_epoch);
+}
+
+__global__ __launch_bounds__(kIcpThreads) void k_icp_track(IcpTrackArgs a, DevState *__restrict__ st)
+{
+    icp_track_body(a, st, blockIdx.x, gridDim.x);
 }
 
 // flags: kStateSetPose, kStateArmWindow (the 4-frame integrate window of
@@ -2801,7 +2820,10 @@ __global__ __launch_bounds__(256) void k_raycast_coop(RaycastArgs a, const DevSt
 // may differ (integrate / brick launches map blocks through prefix tables).
 // Blocks of sessions without a frame in this launch return at once.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool batch_active(const BatchTick &t, int i) { return (t.active >> i) & 1ull; }
+__device__ __forceinline__ bool batch_active(const BatchTick *tp, int i) { return (tp->active >> i) & 1ull; }
+// the tick record (depth pointers, active mask) is read from device memory, so
+// that a captured launch sequence can be replayed for every tick; the member
+// tables are kernel arguments
 
 // largest i with prefix[i] <= b
 __device__ __forceinline__ int batch_find(const int *prefix, int n, int b)
@@ -2815,68 +2837,80 @@ __device__ __forceinline__ int batch_find(const int *prefix, int n, int b)
     return lo;
 }
 
-__global__ __launch_bounds__(256) void k_batch_pyramid(BatchTick t, int per)
+__global__ __launch_bounds__(256) void k_batch_pyramid(BatchTable tb, const BatchTick *__restrict__ tp, int per)
 {
     const int i = blockIdx.x / per;
-    if (!batch_active(t, i)) return;
-    const PyramidArgs a = t.sess[i].pyr;
-    pyramid_body<uint16_t>(a, t.depth[i], blockIdx.x % per);
+    if (!batch_active(tp, i)) return;
+    const PyramidArgs a = tb.sess[i].pyr;
+    pyramid_body<uint16_t>(a, tp->depth[i], blockIdx.x % per);
 }
 
-__global__ __launch_bounds__(kIcpThreads) void k_batch_icp(BatchTick t, int level, int first, int save_pose,
-                                                           int finish, int nblk)
+__global__ __launch_bounds__(kIcpThreads) void k_batch_icp(BatchTable tb, const BatchTick *__restrict__ tp, int level,
+                                                           int first, int save_pose, int finish, int nblk)
 {
     const int i = blockIdx.x / nblk;
-    if (!batch_active(t, i)) return;
-    IcpArgs a = t.sess[i].icp[level];
+    if (!batch_active(tp, i)) return;
+    IcpArgs a = tb.sess[i].icp[level];
     a.first = first;
     a.save_pose = save_pose;
     a.finish = finish;
     a.frame = 1;
-    icp_body(a, t.sess[i].st, blockIdx.x % nblk, nblk);
+    icp_body(a, tb.sess[i].st, blockIdx.x % nblk, nblk);
+}
+
+// Track() of every active session in one persistent launch: session i owns
+// blocks [i * per, (i + 1) * per) and runs icp_track_body over them (its own
+// state, rows and tags), so its result is that of its launch alone.  The
+// whole grid fits the device at once (per * n <= compute units).
+__global__ __launch_bounds__(kIcpThreads) void k_batch_icp_track(BatchTable tb, const BatchTick *__restrict__ tp,
+                                                                 int per)
+{
+    const int i = blockIdx.x / per;
+    if (!batch_active(tp, i)) return;
+    icp_track_body(tb.sess[i].trk, tb.sess[i].st, blockIdx.x % per, per);
 }
 
 template <class S, bool FAST, bool OFF32>
-__global__ __launch_bounds__(256) void k_batch_integrate(BatchTick t)
+__global__ __launch_bounds__(256) void k_batch_integrate(BatchTable tb, const BatchTick *__restrict__ tp)
 {
-    const int i = batch_find(t.int_prefix, t.n, blockIdx.x);
-    if (!batch_active(t, i)) return;
-    const IntegrateArgs a = t.sess[i].integ;
-    const int gx = t.sess[i].int_grid[0], gy = t.sess[i].int_grid[1];
-    const int local = blockIdx.x - t.int_prefix[i];
+    const int i = batch_find(tb.int_prefix, tb.n, blockIdx.x);
+    if (!batch_active(tp, i)) return;
+    const IntegrateArgs a = tb.sess[i].integ;
+    const int gx = tb.sess[i].int_grid[0], gy = tb.sess[i].int_grid[1];
+    const int local = blockIdx.x - tb.int_prefix[i];
     const int bx = local % gx, by = (local / gx) % gy, bz = local / (gx * gy);
-    if (FAST) integrate_fast_body<S, OFF32>(a, t.sess[i].st, bx, by, bz);
-    else integrate_body<S, false>(a, t.sess[i].st, bx, by, bz);
+    if (FAST) integrate_fast_body<S, OFF32>(a, tb.sess[i].st, bx, by, bz);
+    else integrate_body<S, false>(a, tb.sess[i].st, bx, by, bz);
 }
 
-__global__ __launch_bounds__(256) void k_batch_dilate(BatchTick t)
+__global__ __launch_bounds__(256) void k_batch_dilate(BatchTable tb, const BatchTick *__restrict__ tp)
 {
     const int b = blockIdx.x * 4 + (threadIdx.x >> 6);  // one wavefront per coarse cell
-    if (b >= t.coa_prefix[t.n]) return;
-    const int i = batch_find(t.coa_prefix, t.n, b);
-    if (!batch_active(t, i)) return;
-    const BrickArgs a = t.sess[i].brick;
-    brick_dilate_body(a, b - t.coa_prefix[i]);
+    if (b >= tb.coa_prefix[tb.n]) return;
+    const int i = batch_find(tb.coa_prefix, tb.n, b);
+    if (!batch_active(tp, i)) return;
+    const BrickArgs a = tb.sess[i].brick;
+    brick_dilate_body(a, b - tb.coa_prefix[i]);
 }
 
 // blocks_per_session is a multiple of 8, so a block's XCD (global id mod 8)
 // equals its session-local id mod 8 and the per-session tile clustering holds
 template <class S>
-__global__ __launch_bounds__(256) void k_batch_raycast(BatchTick t, int per)
+__global__ __launch_bounds__(256) void k_batch_raycast(BatchTable tb, const BatchTick *__restrict__ tp, int per)
 {
     const int i = blockIdx.x / per;
-    if (!batch_active(t, i)) return;
-    const RaycastArgs a = t.sess[i].ray;
-    raycast_body<S>(a, t.sess[i].st, blockIdx.x % per);
+    if (!batch_active(tp, i)) return;
+    const RaycastArgs a = tb.sess[i].ray;
+    raycast_body<S>(a, tb.sess[i].st, blockIdx.x % per);
 }
 
 template <class S>
-__global__ __launch_bounds__(256) void k_batch_raycast_coop(BatchTick t, int per)
+__global__ __launch_bounds__(256) void k_batch_raycast_coop(BatchTable tb, const BatchTick *__restrict__ tp, int per)
 {
     const int i = blockIdx.x / per;
-    if (!batch_active(t, i)) return;
-    const RaycastArgs a = t.sess[i].ray;
-    raycast_coop_body<S>(a, t.sess[i].st, blockIdx.x % per);
+    if (!batch_active(tp, i)) return;
+    const RaycastArgs a = tb.sess[i].ray;
+    raycast_coop_body<S>(a, tb.sess[i].st, blockIdx.x % per);
 }
 
 // ---------------------------------------------------------------------------
@@ -2953,27 +2987,34 @@ void integrate_grid_dims(const IntegrateArgs &a, int g[3])
     g[2] = (int)d.z;
 }
 
-hipError_t launch_batch_pyramid(const BatchTick &t, int per, hipStream_t s)
+hipError_t launch_batch_pyramid(const BatchTable &tb, const BatchTick *tp, int per, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_pyramid, dim3(t.n * per), dim3(256), 0, s, t, per);
+    hipLaunchKernelGGL(k_batch_pyramid, dim3(tb.n * per), dim3(256), 0, s, tb, tp, per);
     return hipGetLastError();
 }
 
-hipError_t launch_batch_icp(const BatchTick &t, int level, int first, int save_pose, int finish, int nblk,
-                            hipStream_t s)
+hipError_t launch_batch_icp(const BatchTable &tb, const BatchTick *tp, int level, int first, int save_pose,
+                            int finish, int nblk, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_icp, dim3(t.n * nblk), dim3(kIcpThreads), 0, s, t, level, first, save_pose, finish,
-                       nblk);
+    hipLaunchKernelGGL(k_batch_icp, dim3(tb.n * nblk), dim3(kIcpThreads), 0, s, tb, tp, level, first, save_pose,
+                       finish, nblk);
     return hipGetLastError();
 }
 
-hipError_t launch_batch_integrate(const BatchTick &t, int storage, int fast, int off32, int total, hipStream_t s)
+hipError_t launch_batch_icp_track(const BatchTable &tb, const BatchTick *tp, int per, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_icp_track, dim3(tb.n * per), dim3(kIcpThreads), 0, s, tb, tp, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_integrate(const BatchTable &tb, const BatchTick *tp, int storage, int fast, int off32,
+                                  int total, hipStream_t s)
 {
     const dim3 g(total), b(64, 4);
-#define KF_BINT(ST)                                                                             \
-    if (fast && off32) hipLaunchKernelGGL((k_batch_integrate<ST, true, true>), g, b, 0, s, t);  \
-    else if (fast) hipLaunchKernelGGL((k_batch_integrate<ST, true, false>), g, b, 0, s, t);     \
-    else hipLaunchKernelGGL((k_batch_integrate<ST, false, false>), g, b, 0, s, t);
+#define KF_BINT(ST)                                                                                  \
+    if (fast && off32) hipLaunchKernelGGL((k_batch_integrate<ST, true, true>), g, b, 0, s, tb, tp);  \
+    else if (fast) hipLaunchKernelGGL((k_batch_integrate<ST, true, false>), g, b, 0, s, tb, tp);     \
+    else hipLaunchKernelGGL((k_batch_integrate<ST, false, false>), g, b, 0, s, tb, tp);
     switch (storage) {
     case KF_STORAGE_RG16I: KF_BINT(StoreRG16I); break;
     case KF_STORAGE_F32: KF_BINT(StoreF32); break;
@@ -2984,28 +3025,29 @@ hipError_t launch_batch_integrate(const BatchTick &t, int storage, int fast, int
     return hipGetLastError();
 }
 
-hipError_t launch_batch_bricks(const BatchTick &t, int total_coarse, hipStream_t s)
+hipError_t launch_batch_bricks(const BatchTable &tb, const BatchTick *tp, int total_coarse, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_dilate, dim3((total_coarse + 3) / 4), dim3(256), 0, s, t);
+    hipLaunchKernelGGL(k_batch_dilate, dim3((total_coarse + 3) / 4), dim3(256), 0, s, tb, tp);
     return hipGetLastError();
 }
 
-hipError_t launch_batch_raycast(const BatchTick &t, int storage, int lanes, int per, hipStream_t s)
+hipError_t launch_batch_raycast(const BatchTable &tb, const BatchTick *tp, int storage, int lanes, int per,
+                                hipStream_t s)
 {
-    const dim3 g(t.n * per), b(256);
+    const dim3 g(tb.n * per), b(256);
     if (lanes == 4) {
         switch (storage) {
-        case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_batch_raycast_coop<StoreRG16I>, g, b, 0, s, t, per); break;
-        case KF_STORAGE_F32: hipLaunchKernelGGL(k_batch_raycast_coop<StoreF32>, g, b, 0, s, t, per); break;
-        case KF_STORAGE_F16: hipLaunchKernelGGL(k_batch_raycast_coop<StoreF16>, g, b, 0, s, t, per); break;
+        case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_batch_raycast_coop<StoreRG16I>, g, b, 0, s, tb, tp, per); break;
+        case KF_STORAGE_F32: hipLaunchKernelGGL(k_batch_raycast_coop<StoreF32>, g, b, 0, s, tb, tp, per); break;
+        case KF_STORAGE_F16: hipLaunchKernelGGL(k_batch_raycast_coop<StoreF16>, g, b, 0, s, tb, tp, per); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
     }
     switch (storage) {
-    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_batch_raycast<StoreRG16I>, g, b, 0, s, t, per); break;
-    case KF_STORAGE_F32: hipLaunchKernelGGL(k_batch_raycast<StoreF32>, g, b, 0, s, t, per); break;
-    case KF_STORAGE_F16: hipLaunchKernelGGL(k_batch_raycast<StoreF16>, g, b, 0, s, t, per); break;
+    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_batch_raycast<StoreRG16I>, g, b, 0, s, tb, tp, per); break;
+    case KF_STORAGE_F32: hipLaunchKernelGGL(k_batch_raycast<StoreF32>, g, b, 0, s, tb, tp, per); break;
+    case KF_STORAGE_F16: hipLaunchKernelGGL(k_batch_raycast<StoreF16>, g, b, 0, s, tb, tp, per); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

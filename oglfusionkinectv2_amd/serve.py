@@ -269,9 +269,10 @@ class GpuBackend:
     """Sessions on their own streams (allocation and reset never wait for the
     batch), one kf_batch on the serving stream."""
 
-    def __init__(self, torch, pool, storage="f16", options=None):
+    def __init__(self, torch, pool, storage="f16", options=None, batch_options=None):
         self.torch, self.pool, self.storage = torch, pool, storage
         self.options = dict(options or {})
+        self.batch_options = dict(batch_options or {})
         self.stream = torch.cuda.Stream()
         self.batch = None
         self._bytes = {}
@@ -306,6 +307,8 @@ class GpuBackend:
         from .batch import SessionBatch
         if self.batch is None:
             self.batch = SessionBatch(gs, self.stream.cuda_stream)
+            for k, v in self.batch_options.items():
+                self.batch.setOption(k, v)
         else:
             self.batch.set_sessions(gs)
 
@@ -340,11 +343,11 @@ def latency_stats(records, origin, warm_s=0.5):
 
 
 def serve(torch, arrivals, seconds, budget_bytes=200e9, storage="f16", sequences=8, pool_frames=64,
-          warm_s=0.5, options=None, max_inflight=2):
+          warm_s=0.5, options=None, max_inflight=2, batch_options=None):
     """Real-time run of `arrivals` for at most `seconds`.  Returns a dict of
     frames/s, latency percentiles and admission counts."""
     pool = DepthPool(torch, sequences, pool_frames)
-    be = GpuBackend(torch, pool, storage, options)
+    be = GpuBackend(torch, pool, storage, options, batch_options)
     srv = FusionServer(be, budget_bytes, sequences, max_inflight=max_inflight)
     for v in sorted({a[1] for a in arrivals}):  # warm the per-size code paths
         g = be.create(v)
@@ -390,12 +393,13 @@ def serve(torch, arrivals, seconds, budget_bytes=200e9, storage="f16", sequences
     }
 
 
-def saturate(torch, volumes, ticks, warmup=10, storage="f16", sequences=8, pool_frames=64, options=None):
+def saturate(torch, volumes, ticks, warmup=10, storage="f16", sequences=8, pool_frames=64, options=None,
+             batch_options=None):
     """Every session has a frame due every tick (offline throughput): one
     batched frame per tick for `ticks` ticks.  Returns aggregate frames/s and
     voxel updates/s."""
     pool = DepthPool(torch, sequences, pool_frames)
-    be = GpuBackend(torch, pool, storage, options)
+    be = GpuBackend(torch, pool, storage, options, batch_options)
     hs = [be.create(v) for v in volumes]
     for i, g in enumerate(hs):
         be.start(g, i % sequences)

@@ -26,9 +26,14 @@ def _state(g):
     return g.getPose(), v.copy(), w.copy(), g.getVerts(), g.getNorms(), g.status()
 
 
-@pytest.mark.parametrize("storage", ["f16", "f32"])
-def test_batch_equals_alone(storage):
-    print("case: batch vs alone %s" % storage, flush=True)
+@pytest.mark.parametrize("storage,graph,persistent", [("f16", 1, 1), ("f32", 1, 1), ("f16", 0, 0), ("f16", 1, 0),
+                                                      ("f16", 0, 1)])
+def test_batch_equals_alone(storage, graph, persistent):
+    """Each session's bits after 8 frames in a batch (with ticks skipped)
+    equal its bits alone, whether the ticks replay captured graphs or launch
+    directly and whether Track() runs as one persistent launch for all
+    members or one launch per iteration."""
+    print("case: batch vs alone %s graph %d persistent %d" % (storage, graph, persistent), flush=True)
     vols = [64, 96, 128]
     F = 8
     seqs = [synth.sequence(i, F) for i in range(3)]
@@ -48,6 +53,8 @@ def test_batch_equals_alone(storage):
     for i, g in enumerate(gs):
         g.Reset(seqs[i][1][0])
     b = SessionBatch(gs)
+    b.setOption(K.OPT_GRAPH, graph)
+    b.setOption(K.OPT_ICP_PERSISTENT, persistent)
     nxt = [0, 0, 0]
     tick = 0
     while min(nxt) < F:
@@ -62,6 +69,8 @@ def test_batch_equals_alone(storage):
             nxt[i] += 1 if active[i] else 0
         tick += 1
     b.synchronize()
+    assert b.getOption(K.OPT_GRAPH) == graph
+    assert b.getOption(K.OPT_ICP_PERSISTENT) == persistent
     for i, g in enumerate(gs):
         got = _state(g)
         for a, c in zip(alone[i][:5], got[:5]):
