@@ -273,6 +273,17 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         0 (default): the fused kernels (same products,
  *                         wave-tree and FP64 group sums). */
 #define KF_OPT_ICP_REFERENCE_ORDER 16
+/*   KF_OPT_INTEGRATE_ZLEN z voxels one integrate thread walks (a multiple of
+ *                         8, up to 4096); 0 (default): enough segments for
+ *                         about 2M column segments per launch.  Same results.
+ *   KF_OPT_INTEGRATE_ROWS y rows of an integrate block of 64 x rows threads:
+ *                         1, 2 or 4 (default).  Same results.
+ *   Both, with KF_OPT_RAYCAST_XCD_GROUP, KF_OPT_RAYCAST_LANES and
+ *   KF_OPT_BRICK_REBUILD_PERIOD, are picked per volume size by the autotuner
+ *   (oglfusionkinectv2_amd/autotune.py), whose table kf_session_create
+ *   applies; kf_set_option overrides it. */
+#define KF_OPT_INTEGRATE_ZLEN 17
+#define KF_OPT_INTEGRATE_ROWS 18
 /*   KF_OPT_TRACK_MODE     tracking of the frame loop (kinect.h:143-144):
  *                         KF_TRACK_POINT_TO_PLANE (Track, default),
  *                         KF_TRACK_POINT_TO_VOLUME (TrackSDF) or both (Track

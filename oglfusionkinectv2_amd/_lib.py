@@ -50,6 +50,8 @@ OPT_ICP_SOLO_ROWS = 13
 OPT_RAYCAST_SCHEDULE = 14
 OPT_RAYCAST_BUDGET = 15
 OPT_ICP_REFERENCE_ORDER = 16
+OPT_INTEGRATE_ZLEN = 17
+OPT_INTEGRATE_ROWS = 18
 
 TRACK_POINT_TO_PLANE = 1
 TRACK_POINT_TO_VOLUME = 2

@@ -195,8 +195,10 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated, long target_segments 
     int segs = (int)std::min<long>(std::max(1, c.volume_size[2] / 16),
                                    std::max<long>(1, (target_segments + cols - 1) / cols));
     int zlen = (c.volume_size[2] + segs - 1) / segs;
+    if (s->int_zlen > 0) zlen = s->int_zlen;  // tuned voxels per thread
     zlen = (zlen + 7) / 8 * 8;
     a.zlen = std::max(zlen, 8);
+    a.rows = s->int_rows;
     a.gated = gated;
     a.bflags = s->bflags;
     a.zflags = s->zflags;
@@ -208,7 +210,7 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated, long target_segments 
 }
 
 // lanes: rays per lane group (1 or 4); -1 takes the session's option
-kf::RaycastArgs raycast_args(kf_session *s, int lanes = -1)
+kf::RaycastArgs raycast_args(kf_session *s, int lanes = -1, int xcd_group = -1)
 {
     const kf_config &c = s->cfg;
     kf::RaycastArgs a{};
@@ -231,7 +233,7 @@ kf::RaycastArgs raycast_args(kf_session *s, int lanes = -1)
     const int tile = a.lanes == 4 ? 8 : 16;  // 256 threads: 256 or 64 rays
     a.tiles_x = (c.width + tile - 1) / tile;
     a.tiles_y = (c.height + tile - 1) / tile;
-    a.xcd_group = std::max(1, s->xcd_group);
+    a.xcd_group = std::max(1, xcd_group < 0 ? s->xcd_group : xcd_group);
     a.blocks = (a.tiles_x * a.tiles_y + 8 * a.xcd_group - 1) / (8 * a.xcd_group) * (8 * a.xcd_group);
     a.off32 = s->voxels < (size_t(1) << 32) ? 1 : 0;
     a.refV = s->refV;
@@ -632,6 +634,36 @@ int kf_session_bytes(const kf_config *cfg, size_t *bytes)
     return KF_OK;
 }
 
+// Launch shapes picked by the autotuner (oglfusionkinectv2_amd/autotune.py)
+// per storage type and volume size; the session takes the row of its storage
+// with the nearest size (largest side), RG16I the FP16 rows (both 4 B per voxel).
+namespace {
+struct TunedRow {
+    int storage, volume, int_zlen, int_rows, xcd_group, ray_lanes, rebuild_period;
+};
+static const TunedRow kTuned[] = {
+#include "kf_tuned.inc"
+    {-1, 0, 0, 4, 4, 1, 16},  // end marker
+};
+
+void apply_tuned(kf_session *s)
+{
+    const int st = s->cfg.storage == KF_STORAGE_F32 ? KF_STORAGE_F32 : KF_STORAGE_F16;
+    const int n = std::max(std::max(s->cfg.volume_size[0], s->cfg.volume_size[1]), s->cfg.volume_size[2]);
+    const TunedRow *best = nullptr;
+    for (const TunedRow &r : kTuned) {
+        if (r.storage != st) continue;
+        if (!best || std::abs(r.volume - n) < std::abs(best->volume - n)) best = &r;
+    }
+    if (!best) return;
+    s->int_zlen = best->int_zlen;
+    s->int_rows = best->int_rows;
+    s->xcd_group = best->xcd_group;
+    s->ray_lanes = best->ray_lanes;
+    s->rebuild_period = best->rebuild_period;
+}
+}  // namespace
+
 int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
 {
     int rc = validate(cfg);
@@ -641,6 +673,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
     kf_session *s = new (std::nothrow) kf_session();
     if (!s) return fail(KF_ERR_OOM, "host allocation failed");
     s->cfg = *cfg;
+    apply_tuned(s);
     auto cleanup = [&](int code, const std::string &msg) {
         free_all(s);
         delete s;
@@ -1006,6 +1039,14 @@ int kf_set_option(kf_session *s, int option, int value)
         if (value < 0) return fail(KF_ERR_INVALID, "solo rows must be >= 0");
         s->icp_solo_rows = value;
         break;
+    case KF_OPT_INTEGRATE_ZLEN:
+        if (value < 0 || value > 4096 || value % 8) return fail(KF_ERR_INVALID, "integrate z length must be 0 or a multiple of 8 up to 4096");
+        s->int_zlen = value;
+        break;
+    case KF_OPT_INTEGRATE_ROWS:
+        if (value != 1 && value != 2 && value != 4) return fail(KF_ERR_INVALID, "integrate block rows must be 1, 2 or 4");
+        s->int_rows = value;
+        break;
     case KF_OPT_ICP_REFERENCE_ORDER:
         if (value) {
             const int rc = alloc_track_records(s);
@@ -1044,6 +1085,8 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_ICP_PERSISTENT: *value = s->icp_persistent; break;
     case KF_OPT_ICP_SOLO_ROWS: *value = s->icp_solo_rows; break;
     case KF_OPT_ICP_REFERENCE_ORDER: *value = s->icp_ref_order; break;
+    case KF_OPT_INTEGRATE_ZLEN: *value = s->int_zlen; break;
+    case KF_OPT_INTEGRATE_ROWS: *value = s->int_rows; break;
     case KF_OPT_RAYCAST_SCHEDULE: *value = s->ray_sched; break;
     case KF_OPT_RAYCAST_BUDGET: *value = s->ray_budget; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
@@ -1338,6 +1381,7 @@ struct kf_batch {
     unsigned graph_version[kTicks] = {0, 0, 0, 0};
     int cus = 0;
     int iters[3] = {0, 0, 0};
+    int int_rows = 4;
 };
 
 namespace {
@@ -1383,9 +1427,9 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
         const kf_session *si = sessions[i];
         if (si->cfg.width != s0->cfg.width || si->cfg.height != s0->cfg.height || si->cfg.storage != s0->cfg.storage ||
             si->fast_math != s0->fast_math || si->icp_rows_per_block != s0->icp_rows_per_block ||
-            si->xcd_group != s0->xcd_group || si->device != s0->device ||
+            si->device != s0->device ||
             si->icp_all_groups != s0->icp_all_groups || si->cfg.reduce_rule != s0->cfg.reduce_rule ||
-            si->ray_lanes != s0->ray_lanes || si->icp_solo_rows != s0->icp_solo_rows ||
+            si->icp_solo_rows != s0->icp_solo_rows ||
             std::memcmp(si->cfg.iterations, s0->cfg.iterations, sizeof(s0->cfg.iterations)) != 0)
             return fail(KF_ERR_INVALID, "batched sessions must share depth size, ICP schedule, storage and options");
         if (si->track_mode != KF_TRACK_POINT_TO_PLANE)
@@ -1423,8 +1467,11 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
         bs.trk.rows[0] = si->icp_rows;
         bs.trk.rows[1] = si->icp_rows + (size_t)(si->cfg.height + 8) * 32;
         bs.trk.spin_limit = 1u << 22;
+        // launch-wide shapes (integrate block rows, raycast lanes and XCD
+        // clusters) are the first member's; they do not change any result
         bs.integ = integrate_args(si, 1, 1048576);
-        bs.ray = raycast_args(si);
+        bs.integ.rows = s0->int_rows;
+        bs.ray = raycast_args(si, s0->ray_lanes, s0->xcd_group);
         bs.brick = brick_args(si);
         bs.st = si->st;
         kf::integrate_grid_dims(bs.integ, bs.int_grid);
@@ -1451,6 +1498,7 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
     b->total_int = ip[n];
     b->total_coarse = cp[n];
     for (int l = 0; l < 3; ++l) b->iters[l] = s0->cfg.iterations[l];
+    b->int_rows = s0->int_rows;
     // persistent Track(): every member's blocks resident at once, at most one
     // block per compute unit, no more blocks per session than the largest
     // level has partial rows
@@ -1491,7 +1539,7 @@ int enqueue_tick(kf_batch *b, int k)
             }
     }
     if (e == hipSuccess)
-        e = kf::launch_batch_integrate(tb, tp, b->storage, b->fast, b->off32, b->total_int, b->stream);
+        e = kf::launch_batch_integrate(tb, tp, b->storage, b->fast, b->off32, b->total_int, b->int_rows, b->stream);
     if (e == hipSuccess) e = kf::launch_batch_bricks(tb, tp, b->total_coarse, b->stream);
     if (e == hipSuccess) e = kf::launch_batch_raycast(tb, tp, b->storage, b->ray_lanes, b->ray_blocks, b->stream);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));

@@ -60,7 +60,8 @@ struct IntegrateArgs {
     const float *depth_m; // level-0 depth / 1000 (metres)
     int dw, dh;
     void *tsdf, *weight;  // planes
-    int zlen;             // z voxels per segment (blockIdx.z), multiple of 8
+    int zlen;             // z voxels per segment (blockIdx.z), multiple of 8: voxels per thread
+    int rows;             // y rows per block (blockDim.y: 1, 2 or 4), 64 x per row
     int gated;
     uint8_t *bflags;      // 8^3 brick occupancy flags (set-only)
     uint8_t *zflags;      // 8^3 bricks that may hold a voxel other than (0, 0) (set-only)
@@ -182,7 +183,7 @@ hipError_t launch_batch_icp(const BatchTable &tb, const BatchTick *tp, int level
                             int finish, int nblk, hipStream_t s);
 hipError_t launch_batch_icp_track(const BatchTable &tb, const BatchTick *tp, int blocks_per_session, hipStream_t s);
 hipError_t launch_batch_integrate(const BatchTable &tb, const BatchTick *tp, int storage, int fast, int off32,
-                                  int total_blocks, hipStream_t s);
+                                  int total_blocks, int rows, hipStream_t s);
 hipError_t launch_batch_bricks(const BatchTable &tb, const BatchTick *tp, int total_coarse, hipStream_t s);
 hipError_t launch_batch_raycast(const BatchTable &tb, const BatchTick *tp, int storage, int lanes,
                                 int blocks_per_session, hipStream_t s);

@@ -1382,7 +1382,7 @@ __device__ __forceinline__ void integrate_body(const IntegrateArgs &a, const Dev
 {
     if (a.gated && !st->integrate_now) return;
     const int x = bx * 64 + threadIdx.x;
-    const int y = by * 4 + threadIdx.y;
+    const int y = by * a.rows + threadIdx.y;
     if (x >= a.N[0] || y >= a.N[1]) return;
 
     ColumnSetup c;
@@ -1597,9 +1597,9 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
     if (a.gated && !st->integrate_now) return;
     // columns beyond the volume take part with an empty range (the whole
     // wavefront takes part in the range reduction below)
-    const bool col_in = bx * 64 + (int)threadIdx.x < a.N[0] && by * 4 + (int)threadIdx.y < a.N[1];
+    const bool col_in = bx * 64 + (int)threadIdx.x < a.N[0] && by * a.rows + (int)threadIdx.y < a.N[1];
     const int x = col_in ? bx * 64 + threadIdx.x : 0;
-    const int y = col_in ? by * 4 + threadIdx.y : 0;
+    const int y = col_in ? by * a.rows + threadIdx.y : 0;
 
     ColumnSetup c;
     column_setup(a, st->pose, x, y, c);
@@ -2976,7 +2976,7 @@ hipError_t launch_state_set(DevState *st, const float *pose, int flags, hipStrea
 
 static dim3 integrate_grid(const IntegrateArgs &a)
 {
-    return dim3((a.N[0] + 63) / 64, (a.N[1] + 3) / 4, (a.N[2] + a.zlen - 1) / a.zlen);
+    return dim3((a.N[0] + 63) / 64, (a.N[1] + a.rows - 1) / a.rows, (a.N[2] + a.zlen - 1) / a.zlen);
 }
 
 void integrate_grid_dims(const IntegrateArgs &a, int g[3])
@@ -3008,9 +3008,9 @@ hipError_t launch_batch_icp_track(const BatchTable &tb, const BatchTick *tp, int
 }
 
 hipError_t launch_batch_integrate(const BatchTable &tb, const BatchTick *tp, int storage, int fast, int off32,
-                                  int total, hipStream_t s)
+                                  int total, int rows, hipStream_t s)
 {
-    const dim3 g(total), b(64, 4);
+    const dim3 g(total), b(64, rows);
 #define KF_BINT(ST)                                                                                  \
     if (fast && off32) hipLaunchKernelGGL((k_batch_integrate<ST, true, true>), g, b, 0, s, tb, tp);  \
     else if (fast) hipLaunchKernelGGL((k_batch_integrate<ST, true, false>), g, b, 0, s, tb, tp);     \
@@ -3056,7 +3056,7 @@ hipError_t launch_batch_raycast(const BatchTable &tb, const BatchTick *tp, int s
 hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s,
                             hipEvent_t t0, hipEvent_t t1)
 {
-    const dim3 g = integrate_grid(a), b(64, 4);
+    const dim3 g = integrate_grid(a), b(64, a.rows);
     const bool off32 = (size_t)a.N[0] * a.N[1] * a.N[2] < (size_t(1) << 32);
 #define KF_INT(ST)                                                                     \
     if (fast && off32) launch_timed(k_integrate_fast<ST, true>, g, b, s, t0, t1, a, st);  \
@@ -3094,7 +3094,7 @@ hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s)
 hipError_t launch_count_integrate_fast(const IntegrateArgs &a, int storage, const DevState *st,
                                        unsigned long long *counts, hipStream_t s)
 {
-    const dim3 g = integrate_grid(a), b(64, 4);
+    const dim3 g = integrate_grid(a), b(64, a.rows);
     const bool off32 = (size_t)a.N[0] * a.N[1] * a.N[2] < (size_t(1) << 32);
 #define KF_INT(ST)                                                                               \
     if (off32) hipLaunchKernelGGL((k_count_integrate_fast<ST, true>), g, b, 0, s, a, st, counts); \

@@ -62,6 +62,8 @@ struct kf_session {
     unsigned *ray_defer_n = nullptr;    // their count, exit ticket
     int icp_solo_rows = 0;   // levels with at most this many rows per row group: one block per group
     int icp_ref_order = 0;   // Track() sums in reduce.cs's order (per-pixel records + serial FP32 sums)
+    int int_zlen = 0;        // integrate z voxels per thread (0: about 2M column segments per launch)
+    int int_rows = 4;        // integrate block rows (64 x int_rows threads)
     int cus = 0;             // compute units of the device
     // frame graph: ICP + gate + integrate + raycast, captured once
     hipGraph_t graph = nullptr;
