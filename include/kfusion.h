@@ -248,19 +248,6 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         alone, with no grid barrier.  0 = off (default:
  *                         one 256-thread block is slower than a barrier). */
 #define KF_OPT_ICP_SOLO_ROWS 13
-/*   KF_OPT_RAYCAST_SCHEDULE  0: one block per 16x16 tile; k > 0: k blocks
- *                         per compute unit take 64-ray packets from a queue,
- *                         heaviest first by their last durations (one-lane
- *                         raycast only).  Same results.  Default 0 (the
- *                         queue loses L2 locality and is slower here). */
-#define KF_OPT_RAYCAST_SCHEDULE 14
-/*   KF_OPT_RAYCAST_BUDGET  a ray still marching after this many march-loop
- *                         iterations is saved and finished by a second
- *                         launch, with other long rays (one-lane,
- *                         one-block-per-tile raycast).  0 = never.  Same
- *                         results.  Default 0 (the second launch's long
- *                         dependent chains cost more than the tail saved). */
-#define KF_OPT_RAYCAST_BUDGET 15
 /*   KF_OPT_ICP_REFERENCE_ORDER  1: every ICP iteration records track.cs's
  *                         per-pixel output and forms reduce.cs's sums in
  *                         reduce.cs's own order (8 groups x 112 threads,

@@ -47,8 +47,6 @@ OPT_ICP_ALL_GROUPS = 10
 OPT_RAYCAST_LANES = 11
 OPT_ICP_PERSISTENT = 12
 OPT_ICP_SOLO_ROWS = 13
-OPT_RAYCAST_SCHEDULE = 14
-OPT_RAYCAST_BUDGET = 15
 OPT_ICP_REFERENCE_ORDER = 16
 OPT_INTEGRATE_ZLEN = 17
 OPT_INTEGRATE_ROWS = 18

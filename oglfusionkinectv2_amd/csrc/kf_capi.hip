@@ -100,8 +100,6 @@ size_t session_bytes(const kf_config *c)
     b += 2 * 16 * p0;                              // reference maps
     b += vox * 2 * voxel_plane_bytes(c->storage);  // TSDF value + weight planes
     b += sizeof(kf::DevState) + 2 * (size_t)(c->height + 8) * 32 * (4 + 8) + 64;  // state, ICP partial rows (plain, tagged)
-    b += 8 + 2 * 4 * (size_t)4 * ((c->width + 15) / 16) * ((c->height + 15) / 16);  // raycast queue
-    b += 8 + p0 * sizeof(kf::RayState);                                               // deferred rays
     b += (size_t)((c->width + 7) / 8) * ((c->height + 7) / 8) * 8;  // depth tiles
     b += (size_t)kf::kPoseLibraryMax * 64 + 2 * 256 * 32 * 4;       // pose library, TrackSDF partials
     b += (size_t)(c->height + (c->height >> 1) + (c->height >> 2)) * 32;  // TrackSDF record slots
@@ -132,8 +130,7 @@ void free_all(kf_session *s)
     }
     f(s->refV); f(s->refN); f(s->tsdf); f(s->weight); f(s->st); f(s->block_partials); f(s->icp_rows); f(s->counts);
     f(s->dbg_result); f(s->dbg_error); f(s->dbg_J);
-    f(s->library); f(s->sdf_partials); f(s->sdf_slots); f(s->ray_q); f(s->ray_order); f(s->ray_cost);
-    f(s->ray_defer); f(s->ray_defer_n);
+    f(s->library); f(s->sdf_partials); f(s->sdf_slots);
     f(s->mc_tab); f(s->mc_count); f(s->mc_offset); f(s->mc_part); f(s->mc_total); f(s->mc_verts); f(s->mip);
     for (int i = 0; i < 8; ++i)
         if (s->tev[i]) (void)hipEventDestroy(s->tev[i]);
@@ -497,29 +494,7 @@ int run_frame_body(kf_session *s)
 int run_raycast(kf_session *s)
 {
     kf::RaycastArgs a = raycast_args(s);
-    hipError_t e;
-    if (s->ray_sched > 0 && a.lanes == 1) {
-        kf::RayQueue q{};
-        q.next = s->ray_q;
-        q.exit = s->ray_q + 1;
-        q.order = s->ray_order;
-        q.order_out = s->ray_order;
-        q.cost = s->ray_cost;
-        q.npk = s->ray_npk;
-        const int nblk = std::max(1, s->cus * s->ray_sched);
-        q.nwaves = 4 * nblk;
-        e = kf::launch_raycast_dyn(a, s->cfg.storage, s->st, q, nblk, s->stream, tev(s, 6), tev(s, 7));
-    } else if (s->ray_budget > 0 && a.lanes == 1) {
-        // long rays are deferred to a second launch
-        a.budget = s->ray_budget;
-        a.defer = s->ray_defer;
-        a.defer_n = s->ray_defer_n;
-        e = kf::launch_raycast(a, s->cfg.storage, s->st, s->stream, tev(s, 6), nullptr);
-        if (e == hipSuccess)
-            e = kf::launch_raycast_resume(a, s->cfg.storage, s->st, std::max(1, s->cus), s->stream, tev(s, 7));
-    } else {
-        e = kf::launch_raycast(a, s->cfg.storage, s->st, s->stream, tev(s, 6), tev(s, 7));
-    }
+    const hipError_t e = kf::launch_raycast(a, s->cfg.storage, s->st, s->stream, tev(s, 6), tev(s, 7));
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("raycast launch: ") + hipGetErrorString(e));
     return KF_OK;
 }
@@ -713,12 +688,6 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
               alloc((void **)&s->library, (size_t)kf::kPoseLibraryMax * 64) &&
               alloc((void **)&s->sdf_partials, 2 * 256 * 32 * 4) &&
               alloc((void **)&s->sdf_slots, (size_t)(cfg->height + (cfg->height >> 1) + (cfg->height >> 2)) * 32);
-    s->ray_npk = 4 * ((cfg->width + 15) / 16) * ((cfg->height + 15) / 16);
-    ok = ok && alloc((void **)&s->ray_q, 2 * sizeof(unsigned)) &&
-         alloc((void **)&s->ray_order, (size_t)s->ray_npk * sizeof(unsigned)) &&
-         alloc((void **)&s->ray_cost, (size_t)s->ray_npk * sizeof(unsigned)) &&
-         alloc((void **)&s->ray_defer, (size_t)cfg->width * cfg->height * sizeof(kf::RayState)) &&
-         alloc((void **)&s->ray_defer_n, 2 * sizeof(unsigned));
     for (int l = 0; l < 3 && ok; ++l) {
         const size_t n = (size_t)(cfg->width >> l) * (cfg->height >> l);
         ok = alloc((void **)&s->vert[l], n * 16) && alloc((void **)&s->norm[l], n * 16);
@@ -740,10 +709,6 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
              hipMemsetAsync(s->st, 0, sizeof(kf::DevState), s->stream) == hipSuccess &&
              hipMemsetAsync(s->icp_rows, 0, 2 * (size_t)(cfg->height + 8) * 32 * 8, s->stream) == hipSuccess &&
              hipMemsetAsync(s->library, 0, (size_t)kf::kPoseLibraryMax * 64, s->stream) == hipSuccess &&
-             hipMemsetAsync(s->ray_q, 0, 2 * sizeof(unsigned), s->stream) == hipSuccess &&
-             hipMemsetAsync(s->ray_order, 0, (size_t)s->ray_npk * sizeof(unsigned), s->stream) == hipSuccess &&
-             hipMemsetAsync(s->ray_cost, 0, (size_t)s->ray_npk * sizeof(unsigned), s->stream) == hipSuccess &&
-             hipMemsetAsync(s->ray_defer_n, 0, 2 * sizeof(unsigned), s->stream) == hipSuccess &&
              hipMemsetAsync(s->sdf_slots, 0, (size_t)(cfg->height + (cfg->height >> 1) + (cfg->height >> 2)) * 32,
                             s->stream) == hipSuccess;
     for (int l = 0; l < 3 && z; ++l) {
@@ -1027,14 +992,6 @@ int kf_set_option(kf_session *s, int option, int value)
         break;
     case KF_OPT_RECOVER: s->recover = value ? 1 : 0; break;
     case KF_OPT_ICP_PERSISTENT: s->icp_persistent = value ? 1 : 0; break;
-    case KF_OPT_RAYCAST_SCHEDULE:
-        if (value < 0 || value > 16) return fail(KF_ERR_INVALID, "raycast schedule must be in [0, 16]");
-        s->ray_sched = value;
-        break;
-    case KF_OPT_RAYCAST_BUDGET:
-        if (value < 0) return fail(KF_ERR_INVALID, "raycast budget must be >= 0");
-        s->ray_budget = value;
-        break;
     case KF_OPT_ICP_SOLO_ROWS:
         if (value < 0) return fail(KF_ERR_INVALID, "solo rows must be >= 0");
         s->icp_solo_rows = value;
@@ -1087,8 +1044,6 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_ICP_REFERENCE_ORDER: *value = s->icp_ref_order; break;
     case KF_OPT_INTEGRATE_ZLEN: *value = s->int_zlen; break;
     case KF_OPT_INTEGRATE_ROWS: *value = s->int_rows; break;
-    case KF_OPT_RAYCAST_SCHEDULE: *value = s->ray_sched; break;
-    case KF_OPT_RAYCAST_BUDGET: *value = s->ray_budget; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     return KF_OK;

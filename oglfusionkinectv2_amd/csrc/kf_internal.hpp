@@ -76,14 +76,6 @@ struct BrickArgs {
     uint8_t *flags, *dflags, *cflags, *zflags;
 };
 
-// March state of a deferred ray at the top of its march loop (24 B used)
-struct alignas(16) RayState {
-    int pixel;       // y * width + x
-    float t, t_prev, f_t, step;
-    int guard_prev;  // step counter | 0x80000000 when f_t is the value at t_prev
-    int pad[2];
-};
-
 struct RaycastArgs {
     int N[3];
     float dim[3];
@@ -105,19 +97,6 @@ struct RaycastArgs {
     int lanes;                  // lanes per ray: 1 (16x16-pixel tiles) or 4 (8x8-pixel tiles)
     uint32_t *ray_counts;       // optional (with stats): per pixel, evaluated samples | skip events << 16
     uint32_t *touch;            // k_raycast_touch: bitmap of voxels, then flag bytes, read
-    int budget;                 // > 0: march-loop iterations after which a ray is deferred
-    RayState *defer;            // deferred rays (one entry per pixel at most)
-    unsigned *defer_n;          // their count; [1]: exit ticket of k_raycast_resume
-};
-
-// Dynamic raycast schedule (k_raycast_dyn): a queue of 64-ray packets.
-struct RayQueue {
-    unsigned *next, *exit;   // queue head and exit ticket (zero between launches)
-    const unsigned *order;   // packet order of this launch (a permutation of 0..npk-1)
-    unsigned *order_out;     // the same array, written by k_ray_order before the launch
-    unsigned *cost;          // per packet: its last march duration (10 ns ticks)
-    int npk;                 // packets: 4 per 16x16 tile
-    int nwaves;              // wavefronts of the launch
 };
 
 // Point-to-volume tracking (persistent k_sdf_track, one launch per TrackSDF /
@@ -225,12 +204,6 @@ hipError_t launch_intensity_projection(const RaycastArgs &a, int storage, const 
 // k_raycast_touch then the population count of its `words`-word bitmap into *count
 hipError_t launch_raycast_touch(const RaycastArgs &a, int storage, const DevState *st, size_t words,
                                 unsigned long long *count, hipStream_t s);
-// k_raycast_resume over nblk blocks: finishes the rays k_raycast deferred
-hipError_t launch_raycast_resume(const RaycastArgs &a, int storage, const DevState *st, int nblk, hipStream_t s,
-                                 hipEvent_t t1 = nullptr);
-// k_ray_order, then k_raycast_dyn over nblk blocks of four wavefronts
-hipError_t launch_raycast_dyn(const RaycastArgs &a, int storage, const DevState *st, const RayQueue &q, int nblk,
-                              hipStream_t s, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st, hipStream_t s,
                           hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 

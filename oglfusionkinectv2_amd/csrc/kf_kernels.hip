@@ -2159,8 +2159,7 @@ __device__ __forceinline__ void raycast_view(const RaycastArgs &a, const DevStat
 
 // The ray of pixel (x, y): march, hit refinement, normal, map writes.
 template <class S, bool TOUCH>
-__device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float (&view)[16], int x, int y,
-                                              const RayState *resume = nullptr)
+__device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float (&view)[16], int x, int y)
 {
     VolView<S, TOUCH> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
                  (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32, a.touch};
@@ -2194,9 +2193,8 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
     float hp[3], th = 0.0f;
     unsigned n_eval = 0, n_skip = 0, n_brick = 0;  // statistics mode only
     if (tnear < tfar) {
-        // a ray resumed from a deferred march state continues at its loop top
-        float t = resume ? resume->t : tnear, stepsize = resume ? resume->step : a.large_step;
-        float f_t = resume ? resume->f_t : interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
+        float t = tnear, stepsize = a.large_step;
+        float f_t = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
         float f_tt = 0.0f;
         if (f_t >= 0.0f) {
             // Empty-space skipping.  A sample whose lower trilinear corner lies
@@ -2210,30 +2208,15 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
             // 1 / ds: the exit bound below is a skip limit with a 0.01-voxel
             // margin, far wider than the ulp a reciprocal adds
             const float rds[3] = {1.0f / ds[0], 1.0f / ds[1], 1.0f / ds[2]};
-            float t_prev = resume ? resume->t_prev : t;
-            bool have_prev = resume ? (resume->guard_prev < 0) : true;
-            if (!resume) n_eval++;
-            int guard = resume ? (resume->guard_prev & 0x7fffffff) : 0;
-            int iters = 0;
+            float t_prev = t;
+            bool have_prev = true;
+            n_eval++;
+            int guard = 0;
             // flags of the 32^3 cell and 8^3 brick of the next sample, loaded
             // together with the previous fine-step batch (have_flags)
             uint8_t cf_n = 1, ff_n = 1;
             bool have_flags = false;
             for (; t < tfar && guard < kMaxMarchSteps; ++guard) {
-                if (a.budget > 0 && !resume && ++iters > a.budget) {
-                    // a long ray: its state goes to the deferred list and a
-                    // second launch finishes it (k_raycast_resume)
-                    const unsigned k = atomicAdd(a.defer_n, 1u);
-                    RayState r;
-                    r.pixel = y * a.w + x;
-                    r.t = t;
-                    r.t_prev = t_prev;
-                    r.f_t = f_t;
-                    r.step = stepsize;
-                    r.guard_prev = guard | (have_prev ? (int)0x80000000u : 0);
-                    a.defer[k] = r;
-                    return;
-                }
                 const bool can_skip = a.skip && stepsize == a.step;
                 Sample sm;
                 sample_setup(a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t, sm);
@@ -2679,115 +2662,6 @@ __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *
     raycast_body<S>(a, st, blockIdx.x);
 }
 
-// The rays k_raycast deferred (more march-loop iterations than a.budget):
-// each lane resumes one from its saved loop-top state and finishes it exactly
-// as k_raycast would have.  Long rays then share wavefronts with other long
-// rays instead of holding up a wavefront of short ones.  The last wavefront
-// out empties the list for the next launch.
-template <class S>
-__global__ __launch_bounds__(256) void k_raycast_resume(RaycastArgs a, const DevState *__restrict__ st)
-{
-    float view[16];
-    raycast_view(a, st, view);
-    const unsigned n = *a.defer_n;  // every append of the k_raycast launch before this one
-    const unsigned stride = gridDim.x * blockDim.x;
-    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const RayState r = a.defer[i];
-        if ((unsigned)r.pixel < (unsigned)(a.w * a.h))
-            raycast_pixel<S, false>(a, view, r.pixel % a.w, r.pixel / a.w, &r);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        const unsigned t = atomicAdd(a.defer_n + 1, 1u);
-        if (t == gridDim.x * (blockDim.x / 64u) - 1u) {
-            a.defer_n[0] = 0u;
-            a.defer_n[1] = 0u;
-        }
-    }
-}
-
-// The same rays with dynamic scheduling: the grid is a fixed number of
-// wavefronts (a few per SIMD), each of which takes 64-ray packets (16 x 4
-// pixels: the rays one wavefront of k_raycast marches) from a queue until it
-// is empty.  The queue hands packets out in the order q.order, which the
-// previous launch's per-packet durations sort heaviest first
-// (k_ray_order), so long packets start early and short ones fill the gaps;
-// a wavefront no longer waits on the slowest rays of a fixed set of tiles.
-// Every ray is computed exactly as in k_raycast.  The last wavefront out
-// rewinds the queue for the next launch.
-template <class S>
-__global__ __launch_bounds__(256) void k_raycast_dyn(RaycastArgs a, const DevState *__restrict__ st, RayQueue q)
-{
-    float view[16];
-    raycast_view(a, st, view);
-    const int lane = threadIdx.x & 63;
-    const unsigned npk = (unsigned)q.npk;
-    // Every lane takes part in the fetch (lane 0 adds one, the others zero),
-    // so no lane has left the loop's control flow when the ticket is read
-    // back: a fetch under `if (lane == 0)` lets the compiler run the other
-    // lanes round a loop of their own that re-reads a stale ticket.  Lane 0
-    // is the first active lane (all are active), so readfirstlane returns its
-    // ticket; the packet index and its queue entry live in scalar registers
-    // and the loop's exit and the entry check are uniform branches.
-    for (;;) {
-        const unsigned v = atomicAdd(q.next, lane == 0 ? 1u : 0u);
-        const unsigned p = __builtin_amdgcn_readfirstlane(v);
-        if (p >= npk) break;
-        const unsigned pk = __builtin_amdgcn_readfirstlane(q.order[p]);
-        if (pk < npk) {  // a permutation entry (anything else: nothing to do)
-            const unsigned long long c0 = wall_clock64();
-            const int tile = (int)(pk >> 2);
-            const int x = (tile % a.tiles_x) * 16 + (lane & 15);
-            const int y = (tile / a.tiles_x) * 16 + (int)(pk & 3u) * 4 + (lane >> 4);
-            if (x < a.w && y < a.h) raycast_pixel<S, false>(a, view, x, y);
-            if (lane == 0) {
-                const unsigned long long dt = wall_clock64() - c0;
-                q.cost[pk] = dt > 0xffffffffull ? 0xffffffffu : (unsigned)dt;
-            }
-        }
-    }
-    if (lane == 0) {
-        const unsigned t = atomicAdd(q.exit, 1u);
-        if (t == (unsigned)q.nwaves - 1u) {
-            *q.next = 0u;
-            *q.exit = 0u;
-        }
-    }
-}
-
-// Heaviest-first order of the raycast packets from their last durations: a
-// counting sort over 128 log-spaced buckets (four per octave), one block.
-// Any permutation is a valid order; this one only shortens the tail.
-constexpr int kOrderBuckets = 128;
-__device__ __forceinline__ int cost_bucket(unsigned c)
-{
-    if (c < 4u) return (int)c;
-    const int e = 31 - __clz(c);  // >= 2
-    const int b = (e << 2) | (int)((c >> (e - 2)) & 3u);
-    return b < kOrderBuckets ? b : kOrderBuckets - 1;
-}
-
-__global__ __launch_bounds__(1024) void k_ray_order(RayQueue q)
-{
-    __shared__ unsigned hist[kOrderBuckets], base[kOrderBuckets];
-    const int tid = threadIdx.x;
-    if (tid < kOrderBuckets) hist[tid] = 0u;
-    __syncthreads();
-    for (int i = tid; i < q.npk; i += 1024) atomicAdd(&hist[cost_bucket(q.cost[i])], 1u);
-    __syncthreads();
-    if (tid == 0) {
-        unsigned acc = 0;
-        for (int b = kOrderBuckets - 1; b >= 0; --b) {
-            base[b] = acc;
-            acc += hist[b];
-        }
-    }
-    __syncthreads();
-    for (int i = tid; i < q.npk; i += 1024) {
-        const unsigned pos = atomicAdd(&base[cost_bucket(q.cost[i])], 1u);
-        q.order_out[pos] = (unsigned)i;
-    }
-}
-
 // k_raycast with every voxel and flag byte read marked in a.touch (counts the
 // distinct bytes the march reads; results identical to k_raycast)
 template <class S>
@@ -3187,36 +3061,6 @@ hipError_t launch_raycast_touch(const RaycastArgs &a, int storage, const DevStat
     default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k_popcount, dim3(1024), dim3(256), 0, s, a.touch, words, count);
-    return hipGetLastError();
-}
-
-hipError_t launch_raycast_resume(const RaycastArgs &a, int storage, const DevState *st, int nblk, hipStream_t s,
-                                 hipEvent_t t1)
-{
-    if (a.budget <= 0 || !a.defer || !a.defer_n || nblk < 1) return hipErrorInvalidValue;
-    const dim3 g(nblk), b(256);
-    switch (storage) {
-    case KF_STORAGE_RG16I: launch_timed(k_raycast_resume<StoreRG16I>, g, b, s, nullptr, t1, a, st); break;
-    case KF_STORAGE_F32: launch_timed(k_raycast_resume<StoreF32>, g, b, s, nullptr, t1, a, st); break;
-    case KF_STORAGE_F16: launch_timed(k_raycast_resume<StoreF16>, g, b, s, nullptr, t1, a, st); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_raycast_dyn(const RaycastArgs &a, int storage, const DevState *st, const RayQueue &q, int nblk,
-                              hipStream_t s, hipEvent_t t0, hipEvent_t t1)
-{
-    if (a.lanes != 1 || q.nwaves != 4 * nblk || q.npk != 4 * a.tiles_x * a.tiles_y) return hipErrorInvalidValue;
-    // the stage timing spans both launches
-    launch_timed(k_ray_order, dim3(1), dim3(1024), s, t0, nullptr, q);
-    const dim3 g(nblk), b(256);
-    switch (storage) {
-    case KF_STORAGE_RG16I: launch_timed(k_raycast_dyn<StoreRG16I>, g, b, s, nullptr, t1, a, st, q); break;
-    case KF_STORAGE_F32: launch_timed(k_raycast_dyn<StoreF32>, g, b, s, nullptr, t1, a, st, q); break;
-    case KF_STORAGE_F16: launch_timed(k_raycast_dyn<StoreF16>, g, b, s, nullptr, t1, a, st, q); break;
-    default: return hipErrorInvalidValue;
-    }
     return hipGetLastError();
 }
 

@@ -52,14 +52,6 @@ struct kf_session {
     int ray_lanes = 1;       // raycast lanes per ray (1 or 4)
     int icp_all_groups = 0;  // ICP sums every row group, not only the one the combine rule reads
     int icp_persistent = 1;  // Track() as one persistent launch
-    int ray_sched = 0;       // raycast: 0 = one block per tile, k > 0 = queue of packets, k blocks per CU
-    unsigned *ray_q = nullptr;      // queue head, exit ticket
-    unsigned *ray_order = nullptr;  // packet order
-    unsigned *ray_cost = nullptr;   // per-packet durations of the last dynamic raycast
-    int ray_npk = 0;                // packets (64 rays each)
-    int ray_budget = 0;             // raycast: march-loop iterations before a ray is deferred (0 = never)
-    kf::RayState *ray_defer = nullptr;  // deferred rays, one slot per pixel
-    unsigned *ray_defer_n = nullptr;    // their count, exit ticket
     int icp_solo_rows = 0;   // levels with at most this many rows per row group: one block per group
     int icp_ref_order = 0;   // Track() sums in reduce.cs's order (per-pixel records + serial FP32 sums)
     int int_zlen = 0;        // integrate z voxels per thread (0: about 2M column segments per launch)

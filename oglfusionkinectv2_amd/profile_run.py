@@ -32,7 +32,6 @@ def main():
     ap.add_argument("--stats", type=int, default=0, help="print raycast / integrate statistics every N frames")
     ap.add_argument("--fast-math", type=int, default=1)
     ap.add_argument("--xcd-group", type=int, default=4)
-    ap.add_argument("--ray-budget", type=int, default=-1, help="KF_OPT_RAYCAST_BUDGET (-1: library default)")
     ap.add_argument("--time", type=int, default=0, help="time raycast per frame with events and print the mean")
     args = ap.parse_args()
 
@@ -48,8 +47,6 @@ def main():
     g.allocateBuffers()
     g.setOption(K.OPT_FAST_MATH, args.fast_math)
     g.setOption(K.OPT_RAYCAST_XCD_GROUP, args.xcd_group)
-    if args.ray_budget >= 0:
-        g.setOption(K.OPT_RAYCAST_BUDGET, args.ray_budget)
     g.Reset(poses[0])
     fb = depth[0].nbytes
     for i in range(args.frames):

@@ -624,75 +624,3 @@ def test_stage_timing_matches_graph_frames(seq20):
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     np.testing.assert_array_equal(runs[0][1][0].view(np.uint16), runs[1][1][0].view(np.uint16))
     np.testing.assert_array_equal(runs[0][2], runs[1][2])
-
-
-
-@pytest.mark.parametrize("storage", ["rg16i", "f32", "f16"])
-def test_raycast_dynamic_schedule_equals_static(seq20, storage):
-    """Raycast packets taken from a queue (heaviest first by the previous
-    launch's durations, KF_OPT_RAYCAST_SCHEDULE > 0) give the bits of one
-    block per tile: maps after every frame of a sequence, and the whole frame
-    loop (whose tracking reads the maps) in lockstep."""
-    print("case: raycast dynamic vs static schedule %s" % storage, flush=True)
-    depth, gt = seq20
-    sess = []
-    for sched in (0, 2, 1):
-        g, _ = make(storage=storage, fast=1)
-        g.setOption(K.OPT_RAYCAST_SCHEDULE, sched)
-        assert g.getOption(K.OPT_RAYCAST_SCHEDULE) == sched
-        g.Reset(gt[0])
-        sess.append(g)
-    for f in range(8):
-        out = []
-        for g in sess:
-            g.frame(depth[f].astype(np.float32))
-            out.append((g.getPose(), g.getVerts(), g.getNorms()))
-        for o in out[1:]:
-            for a, b in zip(out[0], o):
-                np.testing.assert_array_equal(a, b)
-    assert (out[0][1][..., 3] > 0).mean() > 0.5
-    # raycast() alone, twice (the second launch orders by the first's durations)
-    for g in sess:
-        g.raycast()
-        g.raycast()
-    for g in sess[1:]:
-        np.testing.assert_array_equal(g.getVerts(), sess[0].getVerts())
-        np.testing.assert_array_equal(g.getNorms(), sess[0].getNorms())
-    for g in sess:
-        g.close()
-
-
-@pytest.mark.parametrize("storage", ["rg16i", "f32", "f16"])
-def test_raycast_deferred_rays_equal_single_pass(seq20, storage):
-    """Rays still marching after KF_OPT_RAYCAST_BUDGET loop iterations are
-    saved and finished by a second launch: the maps equal those of the
-    single-pass march for every budget (0 = never defer, 1 = defer nearly
-    every ray, the default), with and without empty-space skipping, over a
-    frame sequence whose tracking reads them."""
-    print("case: raycast deferred rays %s" % storage, flush=True)
-    depth, gt = seq20
-    sess = []
-    for budget in (0, 1, 24):
-        g, _ = make(storage=storage, fast=1)
-        g.setOption(K.OPT_RAYCAST_BUDGET, budget)
-        assert g.getOption(K.OPT_RAYCAST_BUDGET) == budget
-        g.Reset(gt[0])
-        sess.append(g)
-    for f in range(6):
-        out = []
-        for g in sess:
-            g.frame(depth[f].astype(np.float32))
-            out.append((g.getPose(), g.getVerts(), g.getNorms()))
-        for o in out[1:]:
-            for a, b in zip(out[0], o):
-                np.testing.assert_array_equal(a, b)
-    assert (out[0][1][..., 3] > 0).mean() > 0.5
-    for skip in (0, 1):
-        for g in sess:
-            g.setOption(K.OPT_SKIP_EMPTY, skip)
-            g.raycast()
-        for g in sess[1:]:
-            np.testing.assert_array_equal(g.getVerts(), sess[0].getVerts())
-            np.testing.assert_array_equal(g.getNorms(), sess[0].getNorms())
-    for g in sess:
-        g.close()
