@@ -271,6 +271,13 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *   applies; kf_set_option overrides it. */
 #define KF_OPT_INTEGRATE_ZLEN 17
 #define KF_OPT_INTEGRATE_ROWS 18
+/*   KF_OPT_POSE_FP64      1: every ICP pose update is composed in FP64
+ *                         (delta in double times an FP64 copy of the pose,
+ *                         then rounded to the float pose the kernels use),
+ *                         the precision variant's SE(3) composition; the
+ *                         reference multiplies in float (gFusion.cpp:700-705).
+ *                         0 (default): float, as the reference. */
+#define KF_OPT_POSE_FP64 19
 /*   KF_OPT_TRACK_MODE     tracking of the frame loop (kinect.h:143-144):
  *                         KF_TRACK_POINT_TO_PLANE (Track, default),
  *                         KF_TRACK_POINT_TO_VOLUME (TrackSDF) or both (Track

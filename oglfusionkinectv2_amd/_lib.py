@@ -50,6 +50,7 @@ OPT_ICP_SOLO_ROWS = 13
 OPT_ICP_REFERENCE_ORDER = 16
 OPT_INTEGRATE_ZLEN = 17
 OPT_INTEGRATE_ROWS = 18
+OPT_POSE_FP64 = 19
 
 TRACK_POINT_TO_PLANE = 1
 TRACK_POINT_TO_VOLUME = 2

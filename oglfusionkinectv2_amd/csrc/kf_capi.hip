@@ -316,6 +316,7 @@ kf::IcpArgs icp_args(kf_session *s, int level, int first, int save_pose)
     // getReduction's last-row rule reads only row group 7 (gFusion.cpp:1143-1149):
     // the other groups' sums are not needed unless they are to be inspected
     a.group_only = (s->cfg.reduce_rule == KF_RULE_LAST_ROW && !s->icp_all_groups && !s->debug) ? 7 : -1;
+    a.pose_fp64 = s->pose_fp64;
     // small row groups: one block takes all rows of a group (the same
     // partial rows whether one group or all eight are summed)
     if (rows_per_group <= s->icp_solo_rows) a.rows_step = 1;
@@ -1004,6 +1005,7 @@ int kf_set_option(kf_session *s, int option, int value)
         if (value != 1 && value != 2 && value != 4) return fail(KF_ERR_INVALID, "integrate block rows must be 1, 2 or 4");
         s->int_rows = value;
         break;
+    case KF_OPT_POSE_FP64: s->pose_fp64 = value ? 1 : 0; break;
     case KF_OPT_ICP_REFERENCE_ORDER:
         if (value) {
             const int rc = alloc_track_records(s);
@@ -1044,6 +1046,7 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_ICP_REFERENCE_ORDER: *value = s->icp_ref_order; break;
     case KF_OPT_INTEGRATE_ZLEN: *value = s->int_zlen; break;
     case KF_OPT_INTEGRATE_ROWS: *value = s->int_rows; break;
+    case KF_OPT_POSE_FP64: *value = s->pose_fp64; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     return KF_OK;

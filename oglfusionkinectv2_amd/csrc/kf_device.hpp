@@ -50,6 +50,7 @@ struct DevState {
     unsigned int icp_exit;    // its exit ticket
     int icp_abort;            // set by a block whose barrier wait timed out
     unsigned int icp_epoch;   // launches of that kernel so far (tags its partial rows)
+    double pose_d[16];        // FP64 pose of KF_OPT_POSE_FP64 (valid while it casts to `pose`)
 };
 
 // Grid barrier of a persistent launch whose blocks are all resident at once:

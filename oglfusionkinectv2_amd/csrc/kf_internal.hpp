@@ -38,6 +38,7 @@ struct IcpArgs {
     int lib_max;
     int rows_step;              // blocks per row group
     int group_only;             // -1: all 8 row groups; g: only row group g (the one the combine rule reads)
+    int pose_fp64;              // compose the pose update in FP64 (KF_OPT_POSE_FP64)
     int32_t *dbg_result;        // optional per-pixel output
     float *dbg_error, *dbg_J;
 };

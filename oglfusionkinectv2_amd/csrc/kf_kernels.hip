@@ -424,6 +424,45 @@ __device__ void pose_update_cs(const double x[6], const double cs[3], const doub
 }
 
 
+// The same update composed in FP64 (the precision variant): delta in double
+// times the double pose pd, then the float pose is pd rounded.  pd shadows the
+// float pose: when the float pose no longer equals pd rounded (it was set by
+// setPose, a reset, a rejected frame or TrackSDF), pd starts again from it.
+__device__ void pose_update_cs_d(const double x[6], const double cs[3], const double sn[3], float pose[16],
+                                 double pd[16])
+{
+    bool same = true;
+    for (int i = 0; i < 16; ++i) same = same && ((float)pd[i] == pose[i]);
+    if (!same)
+        for (int i = 0; i < 16; ++i) pd[i] = (double)pose[i];
+    const double ca = cs[0], sa = sn[0];
+    const double cb = cs[1], sb = sn[1];
+    const double cc = cs[2], sc = sn[2];
+    const double Rm[3][3] = {
+        {cb * cc, -cb * sc, sb},
+        {sa * sb * cc + ca * sc, -sa * sb * sc + ca * cc, -sa * cb},
+        {-ca * sb * cc + sa * sc, ca * sb * sc + sa * cc, ca * cb},
+    };
+    double d[16];
+    for (int c = 0; c < 3; ++c) {
+        for (int r = 0; r < 3; ++r) d[c * 4 + r] = Rm[r][c];
+        d[c * 4 + 3] = 0.0;
+    }
+    d[12] = x[0];
+    d[13] = x[1];
+    d[14] = x[2];
+    d[15] = 1.0;
+    double t[16];
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i)
+            t[j * 4 + i] = ((d[i] * pd[j * 4 + 0] + d[4 + i] * pd[j * 4 + 1]) + d[8 + i] * pd[j * 4 + 2]) +
+                           d[12 + i] * pd[j * 4 + 3];
+    for (int i = 0; i < 16; ++i) {
+        pd[i] = t[i];
+        pose[i] = (float)t[i];
+    }
+}
+
 // track.cs:47-163 for one pixel, split in two so that a thread can issue the
 // loads of several pixels before it waits for any: track_project takes the
 // pixel's normal / vertex and finds the reference-map texel (code -1 / -2
@@ -924,7 +963,14 @@ __device__ __forceinline__ void icp_apply(const IcpArgs &a, DevState *__restrict
 #pragma unroll
             for (int i = 0; i < 16; ++i) st->old_pose[i] = P[i];
         }
-        pose_update_cs(x, cs, sn, P);
+        if (a.pose_fp64) {
+            double Pd[16];
+            for (int i = 0; i < 16; ++i) Pd[i] = st->pose_d[i];
+            pose_update_cs_d(x, cs, sn, P, Pd);
+            for (int i = 0; i < 16; ++i) st->pose_d[i] = Pd[i];
+        } else {
+            pose_update_cs(x, cs, sn, P);
+        }
 #pragma unroll
         for (int i = 0; i < 16; ++i) st->pose[i] = P[i];
 #pragma unroll
@@ -1152,11 +1198,15 @@ __device__ __forceinline__ void icp_track_body(const IcpTrackArgs &a, DevState *
     __shared__ double wgsum[kIcpThreads / 64][8][kTerms];
     __shared__ double solve_ws[kSolveWs + 36 + 6 + 6];
     __shared__ float Ts[16];
+    __shared__ double Tsd[16];  // FP64 pose (KF_OPT_POSE_FP64)
     __shared__ int sh_abort, sh_done;
     __shared__ __attribute__((aligned(16))) float solo_rows[8 * kPartStride];
 
     const int tid = threadIdx.x, lane = tid & 63;
-    if (tid < 16) Ts[tid] = st->pose[tid];
+    if (tid < 16) {
+        Ts[tid] = st->pose[tid];
+        Tsd[tid] = st->pose_d[tid];
+    }
     if (tid == 0) {
         sh_abort = 0;
         sh_done = 0;
@@ -1183,15 +1233,24 @@ __device__ __forceinline__ void icp_track_body(const IcpTrackArgs &a, DevState *
         // wait for its pose at the next level that needs them
         const bool solo = nb == 1;
         if (!solo && solo_before) {
-            if (blk == 0 && tid < 16)
+            if (blk == 0 && tid < 16) {
                 __hip_atomic_store(&st->pose[tid], Ts[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(reinterpret_cast<unsigned long long *>(st->pose_d) + tid,
+                                   (unsigned long long)__double_as_longlong(Tsd[tid]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
             if (!grid_barrier_coherent(&st->icp_bar, &st->icp_abort, (phase + 1u) * (unsigned)nblk, a.spin_limit,
                                        &sh_abort)) {
                 aborted = true;
                 break;
             }
             ++phase;
-            if (tid < 16) Ts[tid] = __hip_atomic_load(&st->pose[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tid < 16) {
+                Ts[tid] = __hip_atomic_load(&st->pose[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                Tsd[tid] = __longlong_as_double((long long)__hip_atomic_load(
+                    reinterpret_cast<unsigned long long *>(st->pose_d) + tid, __ATOMIC_RELAXED,
+                    __HIP_MEMORY_SCOPE_AGENT));
+            }
             __syncthreads();
         }
         solo_before = solo;
@@ -1248,7 +1307,14 @@ __device__ __forceinline__ void icp_track_body(const IcpTrackArgs &a, DevState *
                     float P[16];
 #pragma unroll
                     for (int i = 0; i < 16; ++i) P[i] = Ts[i];
-                    pose_update_cs(x, cs, sn, P);
+                    if (L.pose_fp64) {
+                        double Pd[16];
+                        for (int i = 0; i < 16; ++i) Pd[i] = Tsd[i];
+                        pose_update_cs_d(x, cs, sn, P, Pd);
+                        for (int i = 0; i < 16; ++i) Tsd[i] = Pd[i];
+                    } else {
+                        pose_update_cs(x, cs, sn, P);
+                    }
 #pragma unroll
                     for (int i = 0; i < 16; ++i) Ts[i] = P[i];
                     sh_done = (nrm < 1e-5 && nrm != 0.0) ? 1 : 0;
@@ -1265,7 +1331,10 @@ __device__ __forceinline__ void icp_track_body(const IcpTrackArgs &a, DevState *
         }
     }
     if (blk == 0) {
-        if (!aborted && tid < 16) st->pose[tid] = Ts[tid];
+        if (!aborted && tid < 16) {
+            st->pose[tid] = Ts[tid];
+            st->pose_d[tid] = Tsd[tid];
+        }
         if (tid == 0) {
             st->iters_run = iters_run;
             st->level_done = sh_done;

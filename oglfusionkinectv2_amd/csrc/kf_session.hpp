@@ -56,6 +56,7 @@ struct kf_session {
     int icp_ref_order = 0;   // Track() sums in reduce.cs's order (per-pixel records + serial FP32 sums)
     int int_zlen = 0;        // integrate z voxels per thread (0: about 2M column segments per launch)
     int int_rows = 4;        // integrate block rows (64 x int_rows threads)
+    int pose_fp64 = 0;       // ICP pose updates composed in FP64 (KF_OPT_POSE_FP64)
     int cus = 0;             // compute units of the device
     // frame graph: ICP + gate + integrate + raycast, captured once
     hipGraph_t graph = nullptr;

@@ -151,3 +151,47 @@ def test_reference_order_is_single_session_only():
     with pytest.raises(K.KfError):
         batch.SessionBatch([g])
     g.close()
+
+
+def test_pose_fp64_composes_in_double(o32, o64, seq20):
+    """KF_OPT_POSE_FP64: each ICP update is delta (double) times the FP64
+    pose, then rounded: after two iterations the float pose is the FP64
+    oracle's composition of the GPU's own two solutions (up to the last ulp of
+    libm's cos / sin), the FP64 pose carried between them."""
+    print("case: FP64 pose composition", flush=True)
+    depth, gt = seq20
+    refV, refN, pose = oracle_state(o32, seq20, 128, 3)
+    start = (pose @ PERTURB["offset"]).astype(np.float32)
+    g = GFusion()
+    g.setConfig(kinect_config(128, 2.0, "f32"))
+    g.allocateBuffers()
+    g.setOption(K.OPT_POSE_FP64, 1)
+    assert g.getOption(K.OPT_POSE_FP64) == 1
+    g.Reset(np.eye(4, dtype=np.float32))
+    g.setReferenceMaps(refV, refN)
+    g.setPose(start)
+    g.depthToVertex(depth[3].astype(np.float32))
+    pd = start.astype(np.float64)
+    for it in range(2):
+        g.track(0, first=(it == 0), save_pose=(it == 0))
+        x = g.readIcpSolution()
+        pd = o64.pose_update(x, pd)
+        got = g.getPose()
+        np.testing.assert_allclose(got, pd.astype(np.float32), rtol=0, atol=2.5e-7)
+    g.close()
+
+
+def test_pose_fp64_precision_variant_holds_tolerances():
+    """The precision variant (FP16 TSDF, FP64 pose composition) against the
+    FP64 oracle: one frame from pinned state within the per-frame
+    tolerances."""
+    import json
+
+    from precision import evaluate
+    print("case: 128^3 f16 + FP64 pose against the FP64 oracle", flush=True)
+    r = evaluate(volume=128, frames=9, storage="f16", pinned=(4, 8), against="f64",
+                 with_f32_oracle=False, options={K.OPT_POSE_FP64: 1})
+    print(json.dumps(r))
+    for p in r["pinned"]:
+        assert p["pose_mm"] <= 0.1 and p["pose_mrad"] <= 0.1, p
+        assert p["tsdf"]["frac_gt_1e3"] <= 1e-3, p
