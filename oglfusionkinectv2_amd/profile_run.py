@@ -1,8 +1,12 @@
 """Profiling driver: runs the frame loop on resident synthetic depth for a
 fixed number of frames (for rocprofv3 kernel traces and counters), followed by
 a plain device copy of known size that calibrates the HBM byte counters.
+With --sessions N > 1 the frames are ticks of a saturated batch of N sessions
+(kf_batch_frame_u16_dev, every session due every tick, session k on its own
+synthetic sequence).
 
     python -m oglfusionkinectv2_amd.profile_run --volume 512 --storage f32 --frames 40
+    python -m oglfusionkinectv2_amd.profile_run --volume 512 --storage f16 --frames 40 --sessions 16
 """
 import argparse
 import ctypes as C
@@ -31,7 +35,8 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--stats", type=int, default=0, help="print raycast / integrate statistics every N frames")
     ap.add_argument("--fast-math", type=int, default=1)
-    ap.add_argument("--xcd-group", type=int, default=4)
+    ap.add_argument("--xcd-group", type=int, default=-1, help="raycast XCD cluster (-1: the tuned default)")
+    ap.add_argument("--sessions", type=int, default=1, help="> 1: a saturated batch of this many sessions")
     ap.add_argument("--time", type=int, default=0, help="time raycast per frame with events and print the mean")
     args = ap.parse_args()
 
@@ -40,13 +45,18 @@ def main():
     from oglfusionkinectv2_amd import _lib as K
 
     L = K.lib()
+    if args.sessions > 1:
+        batch_run(torch, args)
+        copy_calibration(torch, args.copy_mib)
+        return
     depth, poses = synth.sequence(0, args.frames)
     dev = torch.from_numpy(depth.view(np.int16)).to("cuda")
     g = GFusion()
     g.setConfig(kinect_config(args.volume, 2.0, args.storage))
     g.allocateBuffers()
     g.setOption(K.OPT_FAST_MATH, args.fast_math)
-    g.setOption(K.OPT_RAYCAST_XCD_GROUP, args.xcd_group)
+    if args.xcd_group > 0:
+        g.setOption(K.OPT_RAYCAST_XCD_GROUP, args.xcd_group)
     g.Reset(poses[0])
     fb = depth[0].nbytes
     for i in range(args.frames):
@@ -75,7 +85,7 @@ def main():
             torch.cuda.synchronize()
             g.synchronize()
             print("raycast xcd_group %d: %.1f us" % (grp, e0.elapsed_time(e1) * 1000 / args.time), flush=True)
-        g.setOption(K.OPT_RAYCAST_XCD_GROUP, args.xcd_group)
+        g.setOption(K.OPT_RAYCAST_XCD_GROUP, args.xcd_group if args.xcd_group > 0 else 4)
         pose = g.getPose()
         for rpb in (1, 2, 4, 6, 8, 14, 27):
             g.setOption(K.OPT_ICP_ROWS_PER_BLOCK, rpb)
@@ -92,13 +102,43 @@ def main():
             torch.cuda.synchronize()
             print("icp rows_per_block %d: %.1f us per Track (%d iterations)"
                   % (rpb, e0.elapsed_time(e1) * 1000 / args.time, g.status()["iterations"]), flush=True)
-    n = args.copy_mib * 1024 * 1024 // 4
+    copy_calibration(torch, args.copy_mib)
+
+
+def copy_calibration(torch, mib):
+    """Five device copies of `mib` MiB: known bytes for the HBM counters."""
+    n = mib * 1024 * 1024 // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(5):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print("copy done: %d MiB x 5" % args.copy_mib, flush=True)
+    print("copy done: %d MiB x 5" % mib, flush=True)
+
+
+def batch_run(torch, args):
+    """`frames` ticks of a saturated batch of `sessions` sessions."""
+    from oglfusionkinectv2_amd import GFusion, kinect_config, synth
+    from oglfusionkinectv2_amd.batch import SessionBatch
+    fb = 512 * 424 * 2
+    devs, gs = [], []
+    for k in range(args.sessions):
+        depth, poses = synth.sequence(k, args.frames)
+        devs.append(torch.from_numpy(depth.view(np.int16)).to("cuda"))
+        g = GFusion()
+        g.setConfig(kinect_config(args.volume, 2.0, args.storage))
+        g.allocateBuffers()
+        g.Reset(poses[0])
+        gs.append(g)
+    b = SessionBatch(gs)
+    for t in range(args.frames):
+        b.frame([d.data_ptr() + t * fb for d in devs])
+    b.synchronize()
+    print("ticks done:", args.frames, "sessions", args.sessions, "tracked",
+          sum(g.status()["tracked"] for g in gs), flush=True)
+    b.close()
+    for g in gs:
+        g.close()
 
 
 if __name__ == "__main__":
