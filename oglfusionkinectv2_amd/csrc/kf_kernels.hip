@@ -2784,8 +2784,9 @@ __global__ __launch_bounds__(256) void k_batch_pyramid(BatchTable tb, const Batc
 {
     const int i = blockIdx.x / per;
     if (!batch_active(tp, i)) return;
-    const PyramidArgs a = tb.sess[i].pyr;
-    pyramid_body<uint16_t>(a, tp->depth[i], blockIdx.x % per);
+    // by reference: the body indexes the level arrays with a run-time level,
+    // which a private copy of the arguments would put in scratch memory
+    pyramid_body<uint16_t>(tb.sess[i].pyr, tp->depth[i], blockIdx.x % per);
 }
 
 __global__ __launch_bounds__(kIcpThreads) void k_batch_icp(BatchTable tb, const BatchTick *__restrict__ tp, int level,
