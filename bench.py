@@ -49,6 +49,7 @@ def parse(argv=None):
                     help="length of each fixed-membership served run (16 and 64 x 512^3 at 30 Hz)")
     ap.add_argument("--poisson-seconds", type=float, default=40.0,
                     help="how much of the Poisson arrival process (64 sessions at 1/s) to serve")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the 600-frame accuracy leg")
     ap.add_argument("--plan", action="store_true",
                     help="print the frame plan for these arguments and exit (no GPU)")
     return ap.parse_args(argv)
@@ -271,6 +272,59 @@ def serving_legs(torch, args, fps32):
     return out
 
 
+ORACLE_TRAJ = os.path.join(ROOT, "tests", "golden", "oracle_trajectories.npz")
+
+
+def free_run_poses(torch, L, volume, storage, frames_dev, poses, n, options=None):
+    """(n, 4, 4) poses and tracked flags of a free-running session over the
+    first n resident frames (read back after every frame)."""
+    import ctypes as C
+    from oglfusionkinectv2_amd import _lib as K
+    g = make_session(volume, storage, None, 1)
+    for k, v in (options or {}).items():
+        g.setOption(k, v)
+    g.Reset(poses[0])
+    fb = frames_dev[0].numel() * 2
+    out, tracked = np.zeros((n, 4, 4)), np.zeros(n, np.int8)
+    for f in range(n):
+        K.check(L.kf_frame_u16_dev(g.session, C.c_void_p(frames_dev.data_ptr() + f * fb)))
+        out[f] = g.getPose()
+        tracked[f] = g.status()["tracked"]
+    g.close()
+    return out, tracked
+
+
+def accuracy_leg(torch, L, volume):
+    """The 600-frame sequences of the 512^3 configurations, free-running on
+    the GPU, against the oracle trajectories (tests/golden/
+    oracle_trajectories.npz): ATE (RMS and largest translation difference)
+    against the FP64 oracle (the precision variant's reference) and the FP32
+    oracle (the reference semantics), and against ground truth."""
+    if volume != 512 or not os.path.exists(ORACLE_TRAJ):
+        return {"skipped": "needs 512^3 and %s" % os.path.relpath(ORACLE_TRAJ, ROOT)}
+    from oglfusionkinectv2_amd import _lib as K
+    from oglfusionkinectv2_amd import synth
+    ref = np.load(ORACLE_TRAJ)
+    n = len(ref["c3_512_f64_poses"])
+    depth, gt = synth.sequence(0, n)
+    dev = torch.from_numpy(depth.view(np.int16)).to("cuda")
+
+    def ate(P, Q):
+        d = np.linalg.norm(P[:, :3, 3] - Q[:, :3, 3], axis=1) * 1e3
+        return {"rms_mm": float(np.sqrt(np.mean(d ** 2))), "max_mm": float(d.max()),
+                "frame_20_mm": float(d[min(19, len(d) - 1)])}
+
+    out = {"frames": n, "oracle_f32_vs_f64": ate(ref["c2_512_f32_poses"], ref["c3_512_f64_poses"]),
+           "oracle_f64_vs_ground_truth": ate(ref["c3_512_f64_poses"], gt)}
+    legs = {"f16_fp64_pose": ("f16", {K.OPT_POSE_FP64: 1}), "f16": ("f16", {}), "f32": ("f32", {})}
+    for name, (st, opts) in legs.items():
+        P, t = free_run_poses(torch, L, volume, st, dev, gt, n, opts)
+        out[name] = {"vs_fp64_oracle": ate(P, ref["c3_512_f64_poses"]),
+                     "vs_fp32_oracle": ate(P, ref["c2_512_f32_poses"]),
+                     "vs_ground_truth": ate(P, gt), "tracked_fraction": float(t.mean())}
+    return out
+
+
 def frame_plan(warmup, steps, cpu_frames, cpu_warm=2, with_cpu=True):
     """Frames of the seeded sequence each leg reads: the GPU legs run frames
     0 .. warmup + steps - 1; the CPU oracle runs `cpu_warm` untimed frames
@@ -364,6 +418,10 @@ def main():
     if not args.no_fp16:
         fp16 = run_leg(legs, "fp16", lambda: fp16_leg(torch, L, args, stream, frames_dev, poses))
 
+    # --- accuracy of the 600-frame configurations against the oracle --------
+    if not args.no_accuracy:
+        run_leg(legs, "accuracy_600", lambda: accuracy_leg(torch, L, args.volume))
+
     # --- CPU baseline (before the long serving legs) -------------------------
     cpu = None
     if not args.no_cpu:
@@ -434,6 +492,7 @@ def main():
         "with_upload_frames_per_s": legs.get("with_upload"),
         "fp16": fp16,
         "accuracy": acc,
+        "accuracy_600_frames": legs.get("accuracy_600"),
         "beside_the_loop": legs.get("beside_the_loop"),
         "multi_session": multi,
         "errors": {k: v["error"] for k, v in legs.items() if isinstance(v, dict) and "error" in v} or None,
