@@ -17,9 +17,16 @@
  *
  * Parity pinning (tests/test_oracle_*.py): closed forms, special cases, brute
  * force on small inputs, numpy.linalg for the 6x6 solve, numpy.float16 for the
- * half conversion.  Functions only the reference driver could pin (the GL mip
- * filter's exact rounding, glm's float expression order) are "parity unpinned"
- * at the ulp level and pinned only to their mathematical contract.
+ * half conversion; and the reference's own compute shaders, run by Mesa's
+ * llvmpipe on identical inputs (tests/test_oracle_reference_shaders.py):
+ * depthToVert, reduce and first-frame integrate bit-exact, track's result
+ * codes and Jacobians bit-exact, vertToNorm / raycast within the ulps a GL
+ * compiler may spend on FMA contraction, the driver's mip filter the box mean
+ * up to its bilinear-weight rounding, and the frame loop within 1 mm over 8
+ * frames.  The host-side arithmetic of glm (eulerAngleXYZ, inverse) and Eigen
+ * (JacobiSVD, ldlt, the matrix exponential) cannot be built here and stays
+ * "parity unpinned" at the ulp level: it is pinned to its mathematical
+ * contract only.
  */
 #include <math.h>
 #include <stdint.h>
@@ -235,9 +242,10 @@ KFO_EXPORT void kfo_inverse4(const R *M, R *out)
 /* Pyramid: gFusion.cpp:496-547, depthToVert.cs:33-72, vertToNorm.cs:9-57     */
 /* ------------------------------------------------------------------------- */
 
-/* glGenerateMipmap of the R32F depth texture (gFusion.cpp:508): 2x2 box mean.
- * The exact rounding of the driver's filter is parity unpinned; the closed
- * form (mean of the 4 texels, zeros included) is pinned by tests. */
+/* glGenerateMipmap of the R32F depth texture (gFusion.cpp:508): 2x2 box mean,
+ * zeros included.  The filter is the driver's choice; llvmpipe's agrees with
+ * this mean to within 2.5e-4 of the block's range (its bilinear weights are
+ * rounded), which the tests pin. */
 KFO_EXPORT void kfo_mip_depth(const R *src, int w, int h, R *dst)
 {
     int w2 = w >> 1, h2 = h >> 1;
