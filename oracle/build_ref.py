@@ -1,7 +1,8 @@
 """Builds oracle/_ref/libkfref.so: the reference's compute shaders
 (shaders/depthToVert.cs, vertToNorm.cs, track.cs, reduce.cs, integrate.cs,
-raycast.cs of the reference tree) bound into ref_llvmpipe.c, which runs them
-on Mesa's llvmpipe (headless, host cores).
+raycast.cs, trackSDF.cs, reduceSDF.cs and mip.cs of the reference tree)
+bound into ref_llvmpipe.c, which runs them on Mesa's llvmpipe (headless,
+host cores).
 
 Test infrastructure only.  The shader files are read where they lie in the
 reference tree at build time; the generated C file lives in a temporary
@@ -20,7 +21,8 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT_DIR = os.path.join(HERE, "_ref")
 OUT = os.path.join(OUT_DIR, "libkfref.so")
-SHADERS = ("depthToVert", "vertToNorm", "track", "reduce", "integrate", "raycast")
+SHADERS = ("depthToVert", "vertToNorm", "track", "reduce", "integrate", "raycast", "trackSDF", "reduceSDF",
+           "mip")
 DRI_HEADER = "/usr/include/GL/internal/dri_interface.h"
 SWRAST = "/usr/lib/x86_64-linux-gnu/dri/swrast_dri.so"
 

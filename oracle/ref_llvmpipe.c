@@ -1,6 +1,7 @@
 /*
  * ref_llvmpipe.c -- the reference's own GLSL compute shaders (depthToVert,
- * vertToNorm, track, reduce, integrate, raycast) executed by Mesa's llvmpipe
+ * vertToNorm, track, reduce, integrate, raycast, trackSDF, reduceSDF, mip)
+ * executed by Mesa's llvmpipe
  * on the host cores, headless: the swrast DRI driver is loaded through its
  * loader interface (GL/internal/dri_interface.h) and a surfaceless OpenGL 4.3
  * core context is made current, so no display, EGL or OSMesa is needed.
@@ -10,16 +11,17 @@
  * reference tree; nothing in the product loads this library.  The host side
  * below restates what gFusion.cpp does around each dispatch (textures and
  * formats gFusion.cpp:200-305, depthToVertex :496-526, vertexToNormal
- * :528-547, track :742-777, reduce :779-809, integrate :1234-1287, raycast
- * :1289-1366): the same texture formats, image bindings, uniforms and work
+ * :528-547, track :742-777, reduce :779-809, trackSDF :1040-1098, reduceSDF
+ * :1100-1125, integrate :1234-1287, raycast :1289-1366, intensityProjection
+ * :1368-1409): the same texture formats, image bindings, uniforms and work
  * group counts.  Matrices are 16 floats, column-major, as glm passes them.
  *
- * One deliberate difference: gFusion.cpp:1254 and :1312 bind the 3D volume
- * with layered = GL_FALSE.  By the GL specification that binds layer 0 alone
- * as a 2D image, so an iimage3D access is a type mismatch (Mesa drops the
- * loads and stores); NVIDIA's driver binds the whole volume, which is what the
- * reference was written and run against.  Here the volume is bound layered,
- * the behaviour the reference relies on.
+ * One deliberate difference: gFusion.cpp:1056, :1254, :1312 and :1372 bind
+ * the 3D volume with layered = GL_FALSE.  By the GL specification that binds
+ * layer 0 alone as a 2D image, so an iimage3D access is a type mismatch
+ * (Mesa drops the loads and stores); NVIDIA's driver binds the whole volume,
+ * which is what the reference was written and run against.  Here the volume
+ * is bound layered, the behaviour the reference relies on.
  */
 #include <dlfcn.h>
 #include <stdint.h>
@@ -34,7 +36,7 @@
 
 /* shader sources, generated from the reference tree by build_ref.py */
 extern const char *kfr_src_depthToVert, *kfr_src_vertToNorm, *kfr_src_track, *kfr_src_reduce,
-    *kfr_src_integrate, *kfr_src_raycast;
+    *kfr_src_integrate, *kfr_src_raycast, *kfr_src_trackSDF, *kfr_src_reduceSDF, *kfr_src_mip;
 
 /* ---- GL entry points through the shared dispatch of libglapi ------------- */
 typedef void *(*getproc_t)(const char *);
@@ -99,6 +101,7 @@ GLFN(GLenum, glGetError, (void))
 #define K_RG_INTEGER 0x8228
 #define K_READ_WRITE 0x88BA
 #define K_WRITE_ONLY 0x88B9
+#define K_READ_ONLY 0x88B8
 #define K_ALL_BARRIERS 0xFFFFFFFFu
 
 static char g_err[2048];
@@ -131,7 +134,7 @@ static const __DRIextension *g_loader_exts[] = {&g_loader.base, NULL};
 static int g_ready = 0;
 static char g_info[512];
 
-enum { P_D2V, P_V2N, P_TRACK, P_REDUCE, P_INTEGRATE, P_RAYCAST, P_N };
+enum { P_D2V, P_V2N, P_TRACK, P_REDUCE, P_INTEGRATE, P_RAYCAST, P_TSDF, P_RSDF, P_MIP, P_N };
 static GLuint g_prog[P_N];
 
 static int compile(const char *src, GLuint *prog)
@@ -208,7 +211,8 @@ KFR_EXPORT int kfr_init(void)
     snprintf(g_info, sizeof(g_info), "%s | %s", (const char *)p_glGetString(GL_RENDERER),
              (const char *)p_glGetString(GL_VERSION));
     const char *srcs[P_N] = {kfr_src_depthToVert, kfr_src_vertToNorm, kfr_src_track,
-                             kfr_src_reduce,      kfr_src_integrate,  kfr_src_raycast};
+                             kfr_src_reduce,      kfr_src_integrate,  kfr_src_raycast,
+                             kfr_src_trackSDF,    kfr_src_reduceSDF,  kfr_src_mip};
     for (int i = 0; i < P_N; ++i)
         if (compile(srcs[i], &g_prog[i])) return -1;
     p_glPixelStorei(GL_PACK_ALIGNMENT, 1);
@@ -221,7 +225,8 @@ KFR_EXPORT const char *kfr_info(void) { return g_info; }
 
 /* ---- gFusion's objects (initTextures / initVolume / allocateBuffers) ------- */
 static int g_w = 0, g_h = 0, g_n[3] = {0, 0, 0};
-static GLuint t_depth, t_vert, t_norm, t_refV, t_refN, t_diff, t_trackim, t_vol, b_red, b_out;
+static GLuint t_depth, t_vert, t_norm, t_refV, t_refN, t_diff, t_trackim, t_test, t_mip, t_vol, b_red, b_out,
+    b_sdf, b_sdfout;
 
 static GLuint make_tex(GLenum target, int levels, int w, int h, int d, GLenum fmt)
 {
@@ -244,10 +249,10 @@ KFR_EXPORT int kfr_setup(int w, int h, int nx, int ny, int nz)
     if (!g_ready && kfr_init()) return -1;
     if (w == g_w && h == g_h && nx == g_n[0] && ny == g_n[1] && nz == g_n[2]) return 0;
     if (g_w) {
-        GLuint ts[8] = {t_depth, t_vert, t_norm, t_refV, t_refN, t_diff, t_trackim, t_vol};
-        p_glDeleteTextures(8, ts);
-        GLuint bs[2] = {b_red, b_out};
-        p_glDeleteBuffers(2, bs);
+        GLuint ts[10] = {t_depth, t_vert, t_norm, t_refV, t_refN, t_diff, t_trackim, t_test, t_mip, t_vol};
+        p_glDeleteTextures(10, ts);
+        GLuint bs[4] = {b_red, b_out, b_sdf, b_sdfout};
+        p_glDeleteBuffers(4, bs);
     }
     t_depth = make_tex(GL_TEXTURE_2D, 3, w, h, 1, K_R32F);
     t_vert = make_tex(GL_TEXTURE_2D, 3, w, h, 1, K_RGBA32F);
@@ -256,6 +261,8 @@ KFR_EXPORT int kfr_setup(int w, int h, int nx, int ny, int nz)
     t_refN = make_tex(GL_TEXTURE_2D, 1, w, h, 1, K_RGBA32F);
     t_diff = make_tex(GL_TEXTURE_2D, 1, w, h, 1, K_R32F);
     t_trackim = make_tex(GL_TEXTURE_2D, 1, w, h, 1, K_RGBA32F);
+    t_test = make_tex(GL_TEXTURE_2D, 1, w, h, 1, K_RGBA32F);
+    t_mip = make_tex(GL_TEXTURE_2D, 1, w, h, 1, K_RGBA32F);
     t_vol = make_tex(K_TEXTURE_3D, 1, nx, ny, nz, K_RG16I);
     /* the reduction buffer holds one 32-byte track record per level-0 pixel
        (allocateBuffers, gFusion.cpp:297-305); depthToVert's dead write of
@@ -266,8 +273,18 @@ KFR_EXPORT int kfr_setup(int w, int h, int nx, int ny, int nz)
     p_glGenBuffers(1, &b_out);
     p_glBindBuffer(K_SSBO, b_out);
     p_glBufferData(K_SSBO, 256 * 4, NULL, K_DYNAMIC_COPY);
+    /* point-to-volume records {h, D, J[6]} per level-0 pixel and their
+       8 x 32 partials (gFusion.cpp:345-353) */
+    p_glGenBuffers(1, &b_sdf);
+    p_glBindBuffer(K_SSBO, b_sdf);
+    p_glBufferData(K_SSBO, (long)w * h * 32, NULL, K_DYNAMIC_COPY);
+    p_glGenBuffers(1, &b_sdfout);
+    p_glBindBuffer(K_SSBO, b_sdfout);
+    p_glBufferData(K_SSBO, 256 * 4, NULL, K_DYNAMIC_COPY);
     p_glBindBufferBase(K_SSBO, 0, b_red);
     p_glBindBufferBase(K_SSBO, 1, b_out);
+    p_glBindBufferBase(K_SSBO, 12, b_sdf);
+    p_glBindBufferBase(K_SSBO, 13, b_sdfout);
     g_w = w;
     g_h = h;
     g_n[0] = nx;
@@ -491,6 +508,83 @@ KFR_EXPORT int kfr_raycast(const float view[16], float near_plane, float far_pla
     p_glDispatchCompute(divup(g_w, 32), divup(g_h, 32), 1);
     p_glMemoryBarrier(K_ALL_BARRIERS);
     return p_glGetError() == GL_NO_ERROR ? 0 : fail("%s", "GL error in raycast");
+}
+
+/* trackSDF(layer, camToWorld), gFusion.cpp:1040-1098: c = 0.02 * 0.1,
+   eps = 10e-9; records go to the SSBO at binding 12 */
+KFR_EXPORT int kfr_track_sdf(int layer, const float camToWorld[16], const float vol_dim[3])
+{
+    const float vsize[3] = {(float)g_n[0], (float)g_n[1], (float)g_n[2]};
+    const GLint size[2] = {g_w >> layer, g_h >> layer};
+    p_glUseProgram(g_prog[P_TSDF]);
+    p_glActiveTexture(K_TEXTURE0);
+    p_glBindTexture(K_TEXTURE_3D, t_vol);
+    p_glBindImageTexture(0, t_vol, 0, GL_TRUE, 0, K_READ_WRITE, K_RG16I);
+    p_glBindImageTexture(1, t_vert, layer, GL_FALSE, 0, K_READ_WRITE, K_RGBA32F);
+    p_glBindImageTexture(2, t_test, 0, GL_FALSE, 0, K_WRITE_ONLY, K_RGBA32F);
+    p_glUniformMatrix4fv(U(P_TSDF, "Ttrack"), 1, GL_FALSE, camToWorld);
+    p_glUniform2iv(U(P_TSDF, "imageSize"), 1, size);
+    p_glUniform1f(U(P_TSDF, "c"), 0.02f * 0.1f);
+    p_glUniform1f(U(P_TSDF, "eps"), 10e-9f);
+    p_glUniform3fv(U(P_TSDF, "volDim"), 1, vol_dim);
+    p_glUniform3fv(U(P_TSDF, "volSize"), 1, vsize);
+    p_glBindBufferBase(K_SSBO, 12, b_sdf);
+    p_glDispatchCompute(divup(g_w >> layer, 32), divup(g_h >> layer, 32), 1);
+    p_glMemoryBarrier(K_ALL_BARRIERS);
+    return p_glGetError() == GL_NO_ERROR ? 0 : fail("%s", "GL error in trackSDF");
+}
+
+/* The first n point-to-volume records, 8 floats each ({h, D, J[6]}). */
+KFR_EXPORT int kfr_read_sdf(int n, float *rec)
+{
+    p_glBindBuffer(K_SSBO, b_sdf);
+    p_glGetBufferSubData(K_SSBO, 0, (long)n * 32, rec);
+    return p_glGetError() == GL_NO_ERROR ? 0 : fail("%s", "GL error in read_sdf");
+}
+
+KFR_EXPORT int kfr_write_sdf(int n, const float *rec)
+{
+    p_glBindBuffer(K_SSBO, b_sdf);
+    p_glBufferSubData(K_SSBO, 0, (long)n * 32, rec);
+    return p_glGetError() == GL_NO_ERROR ? 0 : fail("%s", "GL error in write_sdf");
+}
+
+/* reduceSDF(layer), gFusion.cpp:1100-1125: 8 work groups of 112 */
+KFR_EXPORT int kfr_reduce_sdf(int layer, float out[256])
+{
+    const GLint size[2] = {g_w >> layer, g_h >> layer};
+    p_glUseProgram(g_prog[P_RSDF]);
+    p_glBindBufferBase(K_SSBO, 12, b_sdf);
+    p_glBindBufferBase(K_SSBO, 13, b_sdfout);
+    p_glUniform2iv(U(P_RSDF, "imageSize"), 1, size);
+    p_glDispatchCompute(8, 1, 1);
+    p_glMemoryBarrier(K_ALL_BARRIERS);
+    p_glBindBuffer(K_SSBO, b_sdfout);
+    p_glGetBufferSubData(K_SSBO, 0, 256 * 4, out);
+    return p_glGetError() == GL_NO_ERROR ? 0 : fail("%s", "GL error in reduceSDF");
+}
+
+/* intensityProjection(), gFusion.cpp:1368-1409 (mip.cs): view = m_pose * invK;
+   the RGBA32F result image is returned in out (h x w x 4) */
+KFR_EXPORT int kfr_intensity_projection(const float view[16], float near_plane, float far_plane, float step,
+                                        float large_step, const float vol_dim[3], float *out)
+{
+    const float vsize[3] = {(float)g_n[0], (float)g_n[1], (float)g_n[2]};
+    p_glUseProgram(g_prog[P_MIP]);
+    p_glBindImageTexture(0, t_vol, 0, GL_TRUE, 0, K_READ_ONLY, K_RG16I);
+    p_glBindImageTexture(1, t_mip, 0, GL_FALSE, 0, K_WRITE_ONLY, K_RGBA32F);
+    p_glUniformMatrix4fv(U(P_MIP, "view"), 1, GL_FALSE, view);
+    p_glUniform1f(U(P_MIP, "nearPlane"), near_plane);
+    p_glUniform1f(U(P_MIP, "farPlane"), far_plane);
+    p_glUniform1f(U(P_MIP, "step"), step);
+    p_glUniform1f(U(P_MIP, "largeStep"), large_step);
+    p_glUniform3fv(U(P_MIP, "volDim"), 1, vol_dim);
+    p_glUniform3fv(U(P_MIP, "volSize"), 1, vsize);
+    p_glDispatchCompute(divup(g_w, 32), divup(g_h, 32), 1);
+    p_glMemoryBarrier(K_ALL_BARRIERS);
+    p_glBindTexture(GL_TEXTURE_2D, t_mip);
+    p_glGetTexImage(GL_TEXTURE_2D, 0, GL_RGBA, GL_FLOAT, out);
+    return p_glGetError() == GL_NO_ERROR ? 0 : fail("%s", "GL error in intensityProjection");
 }
 
 KFR_EXPORT void kfr_finish(void) { p_glFinish(); }

@@ -127,6 +127,36 @@ class RefShaders:
         _check(self.L.kfr_reduce(layer, _p(out)))
         return out.reshape(8, 32)
 
+    # -- point-to-volume tracking (TrackSDF) ---------------------------------------
+    def track_sdf(self, layer, cam_to_world, dim):
+        """trackSDF.cs at `layer` (gFusion::trackSDF, gFusion.cpp:1040-1098)
+        with Ttrack = camToWorld; the records stay in the SSBO."""
+        _check(self.L.kfr_track_sdf(layer, _p(_cm(cam_to_world)), _p(_f(dim))))
+
+    def sdf_records(self):
+        """The (h*w, 8) record buffer {h, D, J[6]} of all level-0 pixels."""
+        out = np.zeros((self.h * self.w, 8), np.float32)
+        _check(self.L.kfr_read_sdf(self.h * self.w, _p(out)))
+        return out
+
+    def set_sdf_records(self, rec):
+        r = _f(rec).reshape(-1, 8)
+        _check(self.L.kfr_write_sdf(len(r), _p(r)))
+
+    def reduce_sdf(self, layer):
+        """reduceSDF.cs over level `layer`: the 8 x 32 partials."""
+        out = np.zeros(256, np.float32)
+        _check(self.L.kfr_reduce_sdf(layer, _p(out)))
+        return out.reshape(8, 32)
+
+    def intensity_projection(self, view, near, far, step, large_step, dim):
+        """mip.cs (gFusion::intensityProjection, gFusion.cpp:1368-1409): the
+        (h, w, 4) image; `view` = m_pose * invK."""
+        out = np.zeros((self.h, self.w, 4), np.float32)
+        _check(self.L.kfr_intensity_projection(_p(_cm(view)), C.c_float(near), C.c_float(far), C.c_float(step),
+                                               C.c_float(large_step), _p(_f(dim)), _p(out)))
+        return out
+
     # -- volume ------------------------------------------------------------------
     def set_volume(self, vol):
         v = np.ascontiguousarray(vol, np.int16)
@@ -158,7 +188,8 @@ class RefSession:
     matrices, getReduction, the 6x6 solve, the Euler pose update) around them:
     depthToVertex + vertexToNormal, Track() (gFusion.cpp:648-740), integrate
     while tracked or within the first frames after Reset (kinect.cpp:654-695),
-    raycast.  Point-to-plane tracking only (no TrackSDF / recoverPose).
+    raycast; TrackSDF (gFusion.cpp:835-1038), the pose library and
+    recoverPose (:1570-1653) as the failure path.
 
     `times` accumulates wall-clock seconds per phase (each phase ends with
     glFinish)."""
@@ -183,6 +214,12 @@ class RefSession:
         self.K = self.o.camera_matrix(self.cam)
         self.invK = self.o.inverse_camera_matrix(self.cam)
         self.pose_ = np.eye(4, dtype=np.float32)
+        self.pose_eig = np.eye(4, dtype=np.float32)
+        self.cum_twist = np.zeros(6)
+        self.library = []
+        self.track_mode, self.recover = 1, 1  # kinect.cpp's defaults
+        # glBufferData(..., NULL) (gFusion.cpp:349): contents taken as zero
+        self.g.set_sdf_records(np.zeros((width * height, 8), np.float32))
         self.counter, self.reset_flag = 0, 1
         self.ae, self.tracked, self.iters_run = 0.0, 0, 0
         self.times = dict.fromkeys(self.PHASES, 0.0)
@@ -196,6 +233,12 @@ class RefSession:
         resetVolume zeroes every texel."""
         self.g.set_volume(np.zeros((self.N[2], self.N[1], self.N[0], 2), np.int16))
         self.pose_ = np.asarray(pose, np.float32).copy()
+        # resetPose (gFusion.cpp:275-286): m_pose_eig = identity rotation with
+        # the pose's translation, m_cumTwist = 0, the library emptied
+        self.pose_eig = np.eye(4, dtype=np.float32)
+        self.pose_eig[:3, 3] = self.pose_[:3, 3]
+        self.cum_twist = np.zeros(6)
+        self.library = []
         self.raycast()
         self.counter, self.reset_flag = 0, 1
 
@@ -234,9 +277,68 @@ class RefSession:
             self.pose_, self.tracked = old, 0
         else:
             self.tracked = 1
+            self._update_pose_finder()
         self.ae = float(ae)
         self._t("track", t0)
         return self.tracked
+
+    def _update_pose_finder(self):
+        """updatePoseFinder (gFusion.cpp:1570-1597): keep the pose when its
+        translation is at least 0.2 m from every kept pose's."""
+        t = self.pose_[:3, 3]
+        for q in self.library:
+            d = (t - q[:3, 3]).astype(np.float32)
+            if np.sqrt(np.float32(np.dot(d, d))) < np.float32(0.2):
+                return
+        if len(self.library) < 256:
+            self.library.append(self.pose_.copy())
+
+    def track_sdf(self):
+        """TrackSDF() from m_pose_eig: per level 2 -> 0 and iteration,
+        camToWorld = exp(twist(result)) * m_pose_eig_prev, trackSDF +
+        reduceSDF + getSDFReduction, a scaled and damped LDLT step; a converged
+        iteration ends its level and marks the frame tracked."""
+        t0 = time.perf_counter()
+        old, prev = self.pose_.copy(), self.pose_eig.copy()
+        result, result_prev = np.zeros(6), np.zeros(6)
+        tracked = 0
+        for level in (2, 1, 0):
+            for it in range(self.iterations[level]):
+                cam_to_world = self.o.mul4(self.o.twist_exp(result), prev)
+                self.g.track_sdf(level, cam_to_world, self.dim)
+                partials = self.g.reduce_sdf(level)
+                b, Cm, ae = self.o.get_reduction(partials, self.rule)
+                cn, result, result_prev = self.o.sdf_step(b, Cm, it, result, result_prev)
+                if (float(ae) != 0.0 and float(ae) < 1e-3) or cn < 1e-3:
+                    tracked = 1
+                    break
+        if np.isnan(result.sum()):
+            result = np.zeros(6)
+        if tracked:
+            self.pose_eig = self.o.mul4(self.o.twist_exp(result), prev).astype(np.float32)
+            self.pose_ = self.pose_eig.copy()
+            self.pose_[3, :3] = 0
+            self.pose_[3, 3] = 1
+            self.cum_twist = self.cum_twist + result
+        else:
+            self.pose_eig, self.pose_ = prev, old
+        self.tracked = tracked
+        self._t("track", t0)
+        return tracked
+
+    def recover_pose(self):
+        """recoverPose() (gFusion.cpp:1616-1653): each kept pose in turn as
+        m_pose, then TrackSDF(); stop at the first success."""
+        for q in self.library:
+            self.pose_ = q.copy()
+            if self.track_sdf():
+                return 1
+        return 0
+
+    def set_mode(self, track_mode=1, recover=1):
+        """track_mode: 1 = Track, 2 = TrackSDF, 3 = both; recover: call
+        recoverPose() on a frame whose tracking failed."""
+        self.track_mode, self.recover = int(track_mode), int(recover)
 
     def integrate(self):
         t0 = time.perf_counter()
@@ -250,13 +352,20 @@ class RefSession:
         self._t("raycast", t0)
 
     def frame(self, depth_mm):
-        """The frame block of kinect.cpp:654-695 (integration on, no recovery)."""
+        """The frame block of kinect.cpp:654-695 with integration on."""
         self.depth(depth_mm)
-        tracked = self.track()
+        tracked = 0
+        if self.track_mode & 1:
+            tracked = self.track()
+        if self.track_mode & 2:
+            tracked = self.track_sdf()
+        self.tracked = tracked
         if tracked or self.reset_flag:
             self.integrate()
             if self.counter > 2:
                 self.reset_flag = 0
+        if not tracked and self.recover:
+            self.recover_pose()
         self.counter += 1
         self.raycast()
         return tracked

@@ -246,3 +246,102 @@ def test_frame_loop_follows_the_oracle(seq):
         assert np.abs(ps[:3, 3] - pr[:3, 3]).max() <= 1e-3, f
         assert np.abs(ps[:3, :3] - pr[:3, :3]).max() <= 1e-3, f
     assert np.linalg.norm(r.pose()[:3, 3] - gt[7][:3, 3]) < 0.02
+
+
+def test_track_sdf_records(o, fused, gl):
+    """trackSDF.cs from identical vertex maps, volume and camToWorld, over a
+    record buffer holding random values (pixels the shader leaves unwritten
+    keep them, on both sides): the Huber weight h and the distance D of every
+    record bit-exact; the Jacobian (central differences of interpolated TSDF
+    values, which amplify an ulp of contraction) within 2e-6 of its row's
+    largest entry (16 ulp of the element at worst, 7e-7 of the row here)."""
+    s, d, _ = fused
+    vol, pose = s.volume(), s.pose()
+    gl.set_volume(vol)
+    gl.depth_to_vertex(d, CAM)
+    rng = np.random.default_rng(3)
+    for level in range(3):
+        gl.set_map("vertex", s.map(1, level), level)
+    for level in range(3):
+        n = (512 >> level) * (424 >> level)
+        init = rng.standard_normal((512 * 424, 8)).astype(np.float32)
+        gl.set_sdf_records(init)
+        gl.track_sdf(level, pose, DIM)
+        got = gl.sdf_records()[:n]
+        want = o.track_sdf(vol, oracle.RG16I, DIM, s.map(1, level), pose, buf=init[:n])
+        assert np.array_equal(got[:, :2], want[:, :2]), level
+        assert (want[:, 0] != init[:n, 0]).sum() > 1000
+        row = np.maximum(np.abs(want[:, 2:]).max(-1, keepdims=True), 1)
+        assert np.all(np.abs(got[:, 2:] - want[:, 2:]) <= 2e-6 * row), level
+
+
+def test_reduce_sdf_is_bit_exact(o, fused, gl):
+    """reduceSDF.cs (D^2 summed twice, reduceSDF.cs:52/60) from identical
+    records: every partial bit-exact on all three levels."""
+    s, _, _ = fused
+    vol, pose = s.volume(), s.pose()
+    for level in range(3):
+        w, h = 512 >> level, 424 >> level
+        rec = o.track_sdf(vol, oracle.RG16I, DIM, s.map(1, level), pose)
+        gl.set_sdf_records(rec)
+        assert np.array_equal(gl.reduce_sdf(level), o.reduce_sdf(rec, w, h)), level
+
+
+def test_intensity_projection(o, fused, gl):
+    """mip.cs over the oracle's volume: within 1e-5 (sums along each ray)."""
+    s, _, mu = fused
+    vol, pose = s.volume(), s.pose()
+    step = float(np.float32(2.0) / np.float32(N))
+    large = float(np.float32(0.75) * np.float32(mu))
+    gl.set_volume(vol)
+    got = gl.intensity_projection(o.mul4(pose, o.inverse_camera_matrix(CAM)), 0.1, 3.0, step, large, DIM)
+    want = o.intensity_projection(vol, oracle.RG16I, DIM, pose, CAM, 0.1, 3.0, large)
+    assert (want != 0).any(-1).sum() > 100000
+    assert np.abs(got - want).max() <= 1e-5
+
+
+def test_frame_loop_track_sdf_follows_the_oracle(seq):
+    """TrackSDF as the tracker (track mode 2), 6 frames at 128^3: the same
+    decision every frame, poses within 1 mm, the cumulative twist within 1e-3."""
+    depth, gt = seq
+    mu = kinect_config(N, 2.0, "f32").mu
+    s = oracle.OracleSession(oracle.Oracle("f32"), N, 2.0, mu, oracle.RG16I)
+    r = R.RefSession(N, 2.0, mu)
+    s.set_mode(2, 1)
+    r.set_mode(2, 1)
+    s.reset(gt[0])
+    r.reset(gt[0])
+    for f in range(6):
+        d = depth[f].astype(np.float32)
+        assert s.frame(d) == r.frame(d), f
+        assert np.abs(s.pose() - r.pose()).max() <= 1e-3, f
+    assert np.abs(s.cum_twist() - r.cum_twist).max() <= 1e-3
+    assert np.abs(s.cum_twist()).max() > 1e-3
+
+
+def test_recover_pose_follows_the_oracle(seq):
+    """recoverPose: after 5 frames of Track (one pose kept in the library) the
+    pose is moved 12 cm off; Track fails on both sides, both recover through
+    the library with TrackSDF, to poses within 1 mm of each other."""
+    depth, gt = seq
+    mu = kinect_config(N, 2.0, "f32").mu
+    s = oracle.OracleSession(oracle.Oracle("f32"), N, 2.0, mu, oracle.RG16I)
+    r = R.RefSession(N, 2.0, mu)
+    s.reset(gt[0])
+    r.reset(gt[0])
+    for f in range(5):
+        d = depth[f].astype(np.float32)
+        s.frame(d)
+        r.frame(d)
+    assert len(s.library()) == len(r.library) >= 1
+    d = depth[5].astype(np.float32)
+    s.depth(d)
+    r.depth(d)
+    P = s.pose().copy()
+    P[:3, 3] += np.float32(0.12)
+    s.set_pose(P)
+    r.set_pose(P)
+    assert s.track() == r.track() == 0
+    assert s.recover_pose() == r.recover_pose() == 1
+    assert np.abs(s.pose() - r.pose()).max() <= 1e-3
+    assert np.linalg.norm(r.pose()[:3, 3] - gt[5][:3, 3]) < 0.08
