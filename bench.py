@@ -11,8 +11,9 @@ Also reported: the same loop with the FP16 TSDF (4 B/voxel), per-frame
 p50/p99 latency, the dominant kernel's roofline figures from HIP events
 carried by its dispatch packets on the session stream, the tracked fraction
 and translation error against the ground-truth trajectory, the frame rate
-with the per-frame host upload (kf_frame_f32), the CPU oracle timed on a
-bounded sample of the same workload, and the multi-session legs (saturated
+with the per-frame host upload (kf_frame_f32), the CPU baseline on bounded
+samples of the same workload (the reference's own compute shaders on Mesa
+llvmpipe, in a child process, and the C oracle), and the multi-session legs (saturated
 16 x 512^3 batch in FP16 and FP32, the uneven 16-session mix, 16 and 64 x
 512^3 served at 30 Hz, Poisson arrivals under a 200 GB budget) under
 "multi_session".  Every leg after the headline records an "error" field
@@ -21,6 +22,7 @@ instead of stopping the run, so the JSON line is always printed.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -42,6 +44,8 @@ def parse(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=60)
     ap.add_argument("--cpu-warm", type=int, default=2, help="untimed CPU oracle frames after its reset")
+    ap.add_argument("--ref-frames", type=int, default=20,
+                    help="timed frames of the reference's shaders on llvmpipe (0: the C oracle alone)")
     ap.add_argument("--fast-math", type=int, default=1)
     ap.add_argument("--no-serve", action="store_true", help="skip the batched and served legs")
     ap.add_argument("--batch-ticks", type=int, default=100)
@@ -521,12 +525,44 @@ def fp16_leg(torch, L, args, stream, frames_dev, poses):
             "accuracy": acc16}
 
 
+def reference_baseline(args, threads):
+    """The reference's own compute shaders on Mesa's llvmpipe
+    (oracle/ref_baseline.py), in a child process of its own so that Mesa's
+    LLVM never shares this process with the GPU runtime."""
+    cmd = [sys.executable, "-m", "oracle.ref_baseline", "--volume", str(args.volume), "--frames",
+           str(args.ref_frames), "--warm", str(args.cpu_warm), "--threads", str(threads)]
+    p = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        raise RuntimeError("reference baseline exited %d: %s" % (p.returncode, p.stderr[-600:]))
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
 def cpu_leg(depth, poses, args):
+    """cpu_baseline: the reference's shaders on llvmpipe (kind "reference"),
+    with the C oracle's time on the same cores beside it (kind "port"); the
+    port alone when the reference library is not built."""
     cfps, threads, cdt = cpu_baseline(depth, poses, args.volume, args.cpu_frames, args.cpu_warm)
-    return {"value": cfps, "unit": "frames/s", "cores": threads, "kind": "port",
+    port = {"value": cfps, "unit": "frames/s", "cores": threads, "kind": "port",
             "sample": "frames %d-%d of the same %d^3 FP32 sequence (frames 0-%d untimed after the reset), "
                       "C oracle, %d OpenMP threads, %.1f s" % (args.cpu_warm, args.cpu_warm + args.cpu_frames - 1,
                                                                 args.volume, args.cpu_warm - 1, threads, cdt)}
+    if args.ref_frames <= 0:
+        return port
+    try:
+        r = reference_baseline(args, threads)
+    except Exception as e:  # noqa: BLE001 - the port's figure still stands
+        port["reference_error"] = "%s: %s" % (type(e).__name__, e)
+        return port
+    print("bench: reference shaders on llvmpipe %.2f frames/s" % r["frames_per_s"], file=sys.stderr, flush=True)
+    return {"value": r["frames_per_s"], "unit": "frames/s", "cores": r["lp_num_threads"], "kind": "reference",
+            "sample": "frames %d-%d of the same synthetic sequence at %d^3 RG16I (the reference's volume format): "
+                      "the reference's compute shaders (depthToVert, vertToNorm, track, reduce, integrate, raycast) "
+                      "on Mesa llvmpipe with %d threads, its host loop (matrices, reduction combine, solve, pose "
+                      "update) by the C oracle; frames 0-%d untimed after the reset, %.1f s"
+                      % (args.cpu_warm, args.cpu_warm + args.ref_frames - 1, args.volume, r["lp_num_threads"],
+                         args.cpu_warm - 1, r["seconds"]),
+            "ms_per_frame_by_phase": r["ms_per_frame_by_phase"], "host_cores": r["host_cores"],
+            "renderer": r["renderer"], "tracked_frames": r["tracked"], "port": port}
 
 
 if __name__ == "__main__":
