@@ -25,8 +25,14 @@ LIB = os.path.join(HERE, "_ref", "libkfref.so")
 _lib = None
 
 
+def built():
+    """True when libkfref.so exists (nothing is loaded)."""
+    return os.path.exists(LIB)
+
+
 def available():
-    """True when libkfref.so is built and its context and shaders come up."""
+    """True when libkfref.so is built and its context and shaders come up
+    (loads Mesa into this process)."""
     try:
         lib()
         return True
