@@ -278,6 +278,12 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         reference multiplies in float (gFusion.cpp:700-705).
  *                         0 (default): float, as the reference. */
 #define KF_OPT_POSE_FP64 19
+/*   KF_OPT_INTEGRATE_COLUMNS  1 (default): the fast integrate's per-column
+ *                         set-up (inverse pose, the column's position at
+ *                         z = 0, its frustum range) runs once per frame in
+ *                         one thread per column, not in every z segment's
+ *                         thread; 0: per thread.  Same bits either way. */
+#define KF_OPT_INTEGRATE_COLUMNS 20
 /*   KF_OPT_TRACK_MODE     tracking of the frame loop (kinect.h:143-144):
  *                         KF_TRACK_POINT_TO_PLANE (Track, default),
  *                         KF_TRACK_POINT_TO_VOLUME (TrackSDF) or both (Track

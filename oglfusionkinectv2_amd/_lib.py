@@ -51,6 +51,7 @@ OPT_ICP_REFERENCE_ORDER = 16
 OPT_INTEGRATE_ZLEN = 17
 OPT_INTEGRATE_ROWS = 18
 OPT_POSE_FP64 = 19
+OPT_INTEGRATE_COLUMNS = 20
 
 TRACK_POINT_TO_PLANE = 1
 TRACK_POINT_TO_VOLUME = 2

@@ -106,6 +106,7 @@ size_t session_bytes(const kf_config *c)
     const size_t nb = (size_t)((c->volume_size[0] + 7) / 8) * ((c->volume_size[1] + 7) / 8) *
                       ((c->volume_size[2] + 7) / 8);
     b += 4 * nb;                                   // brick flags, dilated, zero, coarse (bound)
+    b += (size_t)c->volume_size[0] * c->volume_size[1] * 24 + 16;  // integrate's per-column set-up
     return b;
 }
 
@@ -123,7 +124,7 @@ void free_all(kf_session *s)
         if (p) (void)hipFree(p);
     };
     drop_graph(s);
-    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags); f(s->cflags); f(s->zflags); f(s->dtiles);
+    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags); f(s->cflags); f(s->zflags); f(s->dtiles); f(s->col_pos); f(s->col_range); f(s->col_dl);
     for (int l = 0; l < 3; ++l) {
         f(s->vert[l]);
         f(s->norm[l]);
@@ -430,6 +431,11 @@ int run_tracking(kf_session *s, int frame)
 int run_integrate(kf_session *s, int gated)
 {
     kf::IntegrateArgs a = integrate_args(s, gated);
+    if (s->fast_math && s->int_columns) {
+        a.col_pos = s->col_pos;
+        a.col_range = s->col_range;
+        a.col_dl = s->col_dl;
+    }
     hipError_t e = kf::launch_integrate(a, s->cfg.storage, s->fast_math, s->st, s->stream, tev(s, 4), tev(s, 5));
     if (e == hipSuccess) e = kf::launch_brick_dilate(brick_args(s), s->stream);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("integrate launch: ") + hipGetErrorString(e));
@@ -679,6 +685,9 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
               alloc((void **)&s->zflags, s->nbricks) &&
               alloc((void **)&s->dtiles, (size_t)((cfg->width + 7) / 8) * ((cfg->height + 7) / 8) * 8) &&
               alloc((void **)&s->cflags, s->ncoarse) &&
+              alloc((void **)&s->col_pos, (size_t)cfg->volume_size[0] * cfg->volume_size[1] * 16) &&
+              alloc((void **)&s->col_range, (size_t)cfg->volume_size[0] * cfg->volume_size[1] * 8) &&
+              alloc((void **)&s->col_dl, 16) &&
               alloc((void **)&s->stage_u16, p0 * 2) &&
               alloc((void **)&s->stage_f32, p0 * 4) && alloc((void **)&s->refV, p0 * 16) &&
               alloc((void **)&s->refN, p0 * 16) && alloc(&s->tsdf, vb) && alloc(&s->weight, vb) &&
@@ -1006,6 +1015,7 @@ int kf_set_option(kf_session *s, int option, int value)
         s->int_rows = value;
         break;
     case KF_OPT_POSE_FP64: s->pose_fp64 = value ? 1 : 0; break;
+    case KF_OPT_INTEGRATE_COLUMNS: s->int_columns = value ? 1 : 0; break;
     case KF_OPT_ICP_REFERENCE_ORDER:
         if (value) {
             const int rc = alloc_track_records(s);
@@ -1047,6 +1057,7 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_INTEGRATE_ZLEN: *value = s->int_zlen; break;
     case KF_OPT_INTEGRATE_ROWS: *value = s->int_rows; break;
     case KF_OPT_POSE_FP64: *value = s->pose_fp64; break;
+    case KF_OPT_INTEGRATE_COLUMNS: *value = s->int_columns; break;
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     return KF_OK;

@@ -69,6 +69,13 @@ struct IntegrateArgs {
     int NB[3];            // brick grid
     const float2 *dtiles; // (min, max) nonzero depth_m per 8x8 pixel tile
     int tiles_x, tiles_y;
+    // per-column set-up computed once per frame by k_integrate_columns
+    // (nullptr: each thread computes its column's): the column's camera-frame
+    // position at z = 0, its frustum range before the segment clip, and the
+    // z step (the same for every column)
+    float4 *col_pos;
+    int2 *col_range;
+    float *col_dl;
 };
 
 struct BrickArgs {
@@ -191,7 +198,7 @@ hipError_t launch_icp_finish(DevState *st, int frame, float *lib, int lib_max, h
 // state flags of launch_state_set (kStateSetPose is implied by a non-NULL pose)
 constexpr int kStateSetPose = 1, kStateArmWindow = 2, kStateResetPose = 4;
 hipError_t launch_state_set(DevState *st, const float *pose, int flags, hipStream_t s);
-hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s,
+hipError_t launch_integrate(IntegrateArgs a, int storage, int fast, const DevState *st, hipStream_t s,
                             hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 hipError_t launch_brick_rebuild(const BrickArgs &a, int storage, hipStream_t s);
 hipError_t launch_brick_dilate(const BrickArgs &a, hipStream_t s);

@@ -1574,8 +1574,12 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a, const DevSta
 // fails that test by more than the fast path's rounding, so only [zs, ze) is
 // walked; inside it each voxel is still tested exactly.  zs stays on the
 // segment's chunk grid (multiples of 8 from z0).
-__device__ __forceinline__ void column_frustum(const ColumnSetup &c, float fx, float fy, float cxh, float cyh,
-                                               float dmax_x, float dmax_y, int z0, int z1, int &zs, int &ze)
+// column_frustum_range returns the whole column's range [floor(lo) - 1,
+// ceil(hi) + 2) as integers clamped to +-2^30 (beyond every z, so the
+// segment clip below compares as the unclamped values would); frustum_clip
+// clips it to the segment [z0, z1) and aligns zs to the segment's chunk grid.
+__device__ __forceinline__ int2 column_frustum_range(const ColumnSetup &c, float fx, float fy, float cxh, float cyh,
+                                                     float dmax_x, float dmax_y)
 {
     const double P0 = c.pos[0], P1 = c.pos[1], P2 = c.pos[2];
     const double D0 = c.dl[0], D1 = c.dl[1], D2 = c.dl[2];
@@ -1596,14 +1600,51 @@ __device__ __forceinline__ void column_frustum(const ColumnSetup &c, float fx, f
     clip((double)fy * P1 + ((double)cyh + m) * P2, (double)fy * D1 + ((double)cyh + m) * D2);
     clip(-((double)fy * P1 + ((double)cyh - (double)dmax_y - m) * P2),
          -((double)fy * D1 + ((double)cyh - (double)dmax_y - m) * D2));
-    lo = fmax(floor(lo) - 1.0, (double)z0);
-    hi = fmin(ceil(hi) + 2.0, (double)z1);
+    // fmax / fmin ignore a NaN operand: a NaN bound clips nothing
+    const double big = 1073741824.0;
+    const double L = floor(lo) - 1.0, H = ceil(hi) + 2.0;
+    return make_int2(L == L ? (int)fmin(fmax(L, -big), big) : -(1 << 30),
+                     H == H ? (int)fmin(fmax(H, -big), big) : (1 << 30));
+}
+
+__device__ __forceinline__ void frustum_clip(int2 r, int z0, int z1, int &zs, int &ze)
+{
+    const int lo = max(r.x, z0), hi = min(r.y, z1);
     if (!(lo < hi)) {
         zs = ze = z0;
         return;
     }
-    zs = z0 + (((int)lo - z0) / kIntU) * kIntU;
-    ze = (int)hi;
+    zs = z0 + ((lo - z0) / kIntU) * kIntU;
+    ze = hi;
+}
+
+__device__ __forceinline__ void column_frustum(const ColumnSetup &c, float fx, float fy, float cxh, float cyh,
+                                               float dmax_x, float dmax_y, int z0, int z1, int &zs, int &ze)
+{
+    frustum_clip(column_frustum_range(c, fx, fy, cxh, cyh, dmax_x, dmax_y), z0, z1, zs, ze);
+}
+
+// The per-column set-up of the fast integrate for one frame: one thread per
+// (x, y) column computes what every z segment's thread of that column would
+// (column_setup at z = 0, column_frustum_range), so the integrate threads
+// load 24 bytes instead of inverting the pose and solving the frustum in
+// double each; same values bit for bit.
+__global__ __launch_bounds__(256) void k_integrate_columns(IntegrateArgs a, const DevState *__restrict__ st)
+{
+    if (a.gated && !st->integrate_now) return;
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= a.N[0] || y >= a.N[1]) return;
+    ColumnSetup c;
+    column_setup(a, st->pose, x, y, c);
+    const float fx = a.cam[0], fy = a.cam[1], cxh = a.cam[2] + 0.5f, cyh = a.cam[3] + 0.5f;
+    const size_t i = (size_t)y * a.N[0] + x;
+    a.col_pos[i] = make_float4(c.pos[0], c.pos[1], c.pos[2], 0.0f);
+    a.col_range[i] = column_frustum_range(c, fx, fy, cxh, cyh, (float)(a.dw - 1), (float)(a.dh - 1));
+    if (x == 0 && y == 0) {
+        a.col_dl[0] = c.dl[0];
+        a.col_dl[1] = c.dl[1];
+        a.col_dl[2] = c.dl[2];
+    }
 }
 
 // In a brick holding only (0, 0) a voxel changes only if it is live with
@@ -1671,13 +1712,27 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
     const int y = col_in ? by * a.rows + threadIdx.y : 0;
 
     ColumnSetup c;
-    column_setup(a, st->pose, x, y, c);
     const float fx = a.cam[0], fy = a.cam[1], cxh = a.cam[2] + 0.5f, cyh = a.cam[3] + 0.5f;
     const float dmax_x = (float)(a.dw - 1), dmax_y = (float)(a.dh - 1);
     const int z0 = bz * a.zlen;
     const int z1 = min(z0 + a.zlen, a.N[2]);
     int zs, ze;
-    column_frustum(c, fx, fy, cxh, cyh, dmax_x, dmax_y, z0, z1, zs, ze);
+    if (a.col_pos) {
+        // the frame's per-column set-up (k_integrate_columns); the fast path
+        // uses only the position at z = 0 and the z step
+        const size_t ci = (size_t)y * a.N[0] + x;
+        const float4 p = a.col_pos[ci];
+        c.pos[0] = p.x;
+        c.pos[1] = p.y;
+        c.pos[2] = p.z;
+        c.dl[0] = a.col_dl[0];
+        c.dl[1] = a.col_dl[1];
+        c.dl[2] = a.col_dl[2];
+        frustum_clip(a.col_range[ci], z0, z1, zs, ze);
+    } else {
+        column_setup(a, st->pose, x, y, c);
+        column_frustum(c, fx, fy, cxh, cyh, dmax_x, dmax_y, z0, z1, zs, ze);
+    }
     if (!col_in) zs = ze = z0;
     // every lane walks the same chunks, from the wavefront's first chunk to
     // its last (zs is on the segment's chunk grid for every lane), so the
@@ -2997,11 +3052,19 @@ hipError_t launch_batch_raycast(const BatchTable &tb, const BatchTick *tp, int s
     return hipGetLastError();
 }
 
-hipError_t launch_integrate(const IntegrateArgs &a, int storage, int fast, const DevState *st, hipStream_t s,
+hipError_t launch_integrate(IntegrateArgs a, int storage, int fast, const DevState *st, hipStream_t s,
                             hipEvent_t t0, hipEvent_t t1)
 {
     const dim3 g = integrate_grid(a), b(64, a.rows);
     const bool off32 = (size_t)a.N[0] * a.N[1] * a.N[2] < (size_t(1) << 32);
+    if (fast && a.col_pos) {
+        // the timed interval covers the column set-up and the integrate
+        launch_timed(k_integrate_columns, dim3((a.N[0] + 63) / 64, (a.N[1] + 3) / 4), dim3(64, 4), s, t0,
+                     (hipEvent_t) nullptr, a, st);
+        t0 = nullptr;
+    } else {
+        a.col_pos = nullptr;
+    }
 #define KF_INT(ST)                                                                     \
     if (fast && off32) launch_timed(k_integrate_fast<ST, true>, g, b, s, t0, t1, a, st);  \
     else if (fast) launch_timed(k_integrate_fast<ST, false>, g, b, s, t0, t1, a, st);     \

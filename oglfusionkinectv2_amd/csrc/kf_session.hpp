@@ -36,6 +36,9 @@ struct kf_session {
     unsigned long long *counts = nullptr;
     uint8_t *bflags = nullptr, *dflags = nullptr, *cflags = nullptr, *zflags = nullptr;
     float2 *dtiles = nullptr;  // 8x8-pixel (min, max) of the level-0 depth in metres
+    float4 *col_pos = nullptr;  // integrate's per-column set-up (KF_OPT_INTEGRATE_COLUMNS)
+    int2 *col_range = nullptr;
+    float *col_dl = nullptr;
     // stage timing (KF_OPT_STAGE_TIMING): start / stop events of pyramid, ICP
     // (first iteration .. finish), integrate, raycast, carried by the dispatch packets
     int timing = 0;
@@ -57,6 +60,7 @@ struct kf_session {
     int int_zlen = 0;        // integrate z voxels per thread (0: about 2M column segments per launch)
     int int_rows = 4;        // integrate block rows (64 x int_rows threads)
     int pose_fp64 = 0;       // ICP pose updates composed in FP64 (KF_OPT_POSE_FP64)
+    int int_columns = 1;     // per-column integrate set-up once per frame (KF_OPT_INTEGRATE_COLUMNS)
     int cus = 0;             // compute units of the device
     // frame graph: ICP + gate + integrate + raycast, captured once
     hipGraph_t graph = nullptr;
