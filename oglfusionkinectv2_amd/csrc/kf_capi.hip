@@ -318,6 +318,10 @@ kf::IcpArgs icp_args(kf_session *s, int level, int first, int save_pose)
     // the other groups' sums are not needed unless they are to be inspected
     a.group_only = (s->cfg.reduce_rule == KF_RULE_LAST_ROW && !s->icp_all_groups && !s->debug) ? 7 : -1;
     a.pose_fp64 = s->pose_fp64;
+    // the reference-order mode keeps the library cos / sin (its bits are the
+    // oracle's); the other modes, whose sums already differ in order, take
+    // the series
+    a.fast_trig = s->icp_ref_order ? 0 : 1;
     // small row groups: one block takes all rows of a group (the same
     // partial rows whether one group or all eight are summed)
     if (rows_per_group <= s->icp_solo_rows) a.rows_step = 1;

@@ -39,6 +39,7 @@ struct IcpArgs {
     int rows_step;              // blocks per row group
     int group_only;             // -1: all 8 row groups; g: only row group g (the one the combine rule reads)
     int pose_fp64;              // compose the pose update in FP64 (KF_OPT_POSE_FP64)
+    int fast_trig;              // angles' cos / sin by series for |x| <= 0.5 (off in the reference-order mode)
     int32_t *dbg_result;        // optional per-pixel output
     float *dbg_error, *dbg_J;
 };

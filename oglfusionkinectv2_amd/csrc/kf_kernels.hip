@@ -398,6 +398,38 @@ __device__ void solve6(const double *Cin, const double *b, double *x, double *ws
     jacobi6(b, x, ws);
 }
 
+// cos and sin of an ICP update angle in FP64.  fast and |t| <= 0.5 (every
+// ICP increment of a tracked frame): the Taylor series of sin to degree 15
+// and of cos to degree 16 in Horner form (truncation error below 1e-19 on
+// that interval, rounding within an ulp or two of the library's values);
+// otherwise the library routines.
+__device__ __forceinline__ void icp_sincos(double t, bool fast, double &c, double &s)
+{
+    if (fast && fabs(t) <= 0.5) {
+        const double t2 = t * t;
+        double ps = -1.0 / 1307674368000.0;
+        ps = fma(ps, t2, 1.0 / 6227020800.0);
+        ps = fma(ps, t2, -1.0 / 39916800.0);
+        ps = fma(ps, t2, 1.0 / 362880.0);
+        ps = fma(ps, t2, -1.0 / 5040.0);
+        ps = fma(ps, t2, 1.0 / 120.0);
+        ps = fma(ps, t2, -1.0 / 6.0);
+        s = fma(ps * t2, t, t);
+        double pc = 1.0 / 20922789888000.0;
+        pc = fma(pc, t2, -1.0 / 87178291200.0);
+        pc = fma(pc, t2, 1.0 / 479001600.0);
+        pc = fma(pc, t2, -1.0 / 3628800.0);
+        pc = fma(pc, t2, 1.0 / 40320.0);
+        pc = fma(pc, t2, -1.0 / 720.0);
+        pc = fma(pc, t2, 1.0 / 24.0);
+        pc = fma(pc, t2, -0.5);
+        c = fma(pc, t2, 1.0);
+    } else {
+        c = cos(t);
+        s = sin(t);
+    }
+}
+
 // glm::eulerAngleXYZ(x3,x4,x5) in double, cast to float, translation x0..2,
 // then m_pose = delta * m_pose in float (gFusion.cpp:700-705); cs[k] / sn[k]
 // are cos / sin of x[3 + k] in double
@@ -952,7 +984,8 @@ __device__ __forceinline__ void icp_apply(const IcpArgs &a, DevState *__restrict
 {
     const int lane = threadIdx.x & 63;
     const double my_ang = lane == 1 ? x[4] : lane == 2 ? x[5] : x[3];
-    const double c1 = cos(my_ang), s1 = sin(my_ang);
+    double c1, s1;
+    icp_sincos(my_ang, a.fast_trig != 0, c1, s1);
     const double cs[3] = {c1, __shfl(c1, 1, 64), __shfl(c1, 2, 64)};
     const double sn[3] = {s1, __shfl(s1, 1, 64), __shfl(s1, 2, 64)};
     if (lane == 0) {
@@ -1300,7 +1333,8 @@ __device__ __forceinline__ void icp_track_body(const IcpTrackArgs &a, DevState *
                 float ae = 0.0f;
                 const double nrm = icp_solve(L, gsum, solve_ws, x, ae);
                 const double my_ang = lane == 1 ? x[4] : lane == 2 ? x[5] : x[3];
-                const double c1 = cos(my_ang), s1 = sin(my_ang);
+                double c1, s1;
+                icp_sincos(my_ang, L.fast_trig != 0, c1, s1);
                 const double cs[3] = {c1, __shfl(c1, 1, 64), __shfl(c1, 2, 64)};
                 const double sn[3] = {s1, __shfl(s1, 1, 64), __shfl(s1, 2, 64)};
                 if (lane == 0) {
