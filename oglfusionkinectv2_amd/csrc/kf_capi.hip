@@ -233,7 +233,7 @@ kf::RaycastArgs raycast_args(kf_session *s, int lanes = -1, int xcd_group = -1)
     a.tiles_y = (c.height + tile - 1) / tile;
     a.xcd_group = std::max(1, xcd_group < 0 ? s->xcd_group : xcd_group);
     a.blocks = (a.tiles_x * a.tiles_y + 8 * a.xcd_group - 1) / (8 * a.xcd_group) * (8 * a.xcd_group);
-    a.off32 = s->voxels < (size_t(1) << 32) ? 1 : 0;
+    a.off32 = s->voxels * voxel_plane_bytes(c.storage) <= (size_t)0xFFFFFFFFu ? 1 : 0;
     a.refV = s->refV;
     a.refN = s->refN;
     a.dflags = s->dflags;
@@ -1450,7 +1450,7 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
         kf::integrate_grid_dims(bs.integ, bs.int_grid);
         ip[i + 1] = ip[i] + bs.int_grid[0] * bs.int_grid[1] * bs.int_grid[2];
         cp[i + 1] = cp[i] + (int)si->ncoarse;
-        if (si->voxels >= (size_t(1) << 32)) b->off32 = 0;
+        if (!kf::integrate_planes_fit_32(bs.integ, si->cfg.storage)) b->off32 = 0;
     }
     // stream order: launches queued earlier still read the previous tables
     KF_HIP(hipMemcpyAsync(b->d_sess, h, sizeof(kf::BatchSession) * n, hipMemcpyHostToDevice, b->stream));

@@ -101,7 +101,7 @@ struct RaycastArgs {
     unsigned long long *stats;  // optional: evaluated, skipped, cells skipped, hits, max eval, max iters
     int dim_pow2;               // all volume_dim are powers of two
     float inv_dim[3];
-    int off32;                  // voxel count < 2^32 (32-bit offsets)
+    int off32;                  // value-plane bytes < 2^32 (32-bit byte offsets, buffer loads)
     int xcd_group;              // tiles per XCD cluster (block -> tile mapping)
     int lanes;                  // lanes per ray: 1 (16x16-pixel tiles) or 4 (8x8-pixel tiles)
     uint32_t *ray_counts;       // optional (with stats): per pixel, evaluated samples | skip events << 16
@@ -165,6 +165,13 @@ struct BatchTable {
 };
 
 void integrate_grid_dims(const IntegrateArgs &a, int g[3]);
+// every byte offset into a TSDF plane of this volume fits in 32 bits: the
+// fast integrate then addresses the planes through buffer resources
+inline bool integrate_planes_fit_32(const IntegrateArgs &a, int storage)
+{
+    const size_t vb = storage == KF_STORAGE_F32 ? 4 : 2;
+    return (size_t)a.N[0] * a.N[1] * a.N[2] * vb <= (size_t)0xFFFFFFFFu;
+}
 // tb: the member tables; tp: the tick record in device memory
 hipError_t launch_batch_pyramid(const BatchTable &tb, const BatchTick *tp, int blocks_per_session, hipStream_t s);
 hipError_t launch_batch_icp(const BatchTable &tb, const BatchTick *tp, int level, int first, int save_pose,

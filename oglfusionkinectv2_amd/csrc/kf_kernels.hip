@@ -1689,9 +1689,10 @@ __global__ __launch_bounds__(256) void k_integrate_columns(IntegrateArgs a, cons
 // that box lies within reach of the chunk's pos.z range, every voxel of the
 // chunk keeps (0, 0) and the chunk needs no work.  Boxes wider than 2x2 tiles
 // and chunks near the camera plane are always processed.
-__device__ __forceinline__ bool chunk_unreachable(const IntegrateArgs &a, const ColumnSetup &c, float p0, float p1,
-                                                  float p2, float fx, float fy, float cxh, float cyh, float mu_lo,
-                                                  float mu_hi)
+template <class Tiles>
+__device__ __forceinline__ bool chunk_unreachable(const IntegrateArgs &a, const Tiles &tiles, const ColumnSetup &c,
+                                                  float p0, float p1, float p2, float fx, float fy, float cxh,
+                                                  float cyh, float mu_lo, float mu_hi)
 {
     // straight-line: the four tile loads are always issued (at indices
     // clamped into the tile grid), so the tests of several chunks can be in
@@ -1707,19 +1708,65 @@ __device__ __forceinline__ bool chunk_unreachable(const IntegrateArgs &a, const 
     const float y0 = fminf(ya, yb) - 1.0f, y1 = fmaxf(ya, yb) + 1.0f;
     const bool fin = x0 == x0 && x1 == x1 && y0 == y0 && y1 == y1;
     const bool off = x1 < 0.0f || y1 < 0.0f || x0 >= W || y0 >= H;  // whole box off the image
-    const float cx0 = fin ? fminf(fmaxf(x0, 0.0f), W - 1.0f) : 0.0f, cx1 = fin ? fminf(fmaxf(x1, 0.0f), W - 1.0f) : 0.0f;
-    const float cy0 = fin ? fminf(fmaxf(y0, 0.0f), H - 1.0f) : 0.0f, cy1 = fin ? fminf(fmaxf(y1, 0.0f), H - 1.0f) : 0.0f;
+    // clamped into the image (med3 = min(max(., 0), limit) for the finite
+    // values used; a NaN box is never used)
+    const float cx0 = fin ? __builtin_amdgcn_fmed3f(x0, 0.0f, W - 1.0f) : 0.0f;
+    const float cx1 = fin ? __builtin_amdgcn_fmed3f(x1, 0.0f, W - 1.0f) : 0.0f;
+    const float cy0 = fin ? __builtin_amdgcn_fmed3f(y0, 0.0f, H - 1.0f) : 0.0f;
+    const float cy1 = fin ? __builtin_amdgcn_fmed3f(y1, 0.0f, H - 1.0f) : 0.0f;
     const int tx0 = (int)cx0 >> 3, tx1 = (int)cx1 >> 3, ty0 = (int)cy0 >> 3, ty1 = (int)cy1 >> 3;
     const bool small = tx1 - tx0 <= 1 && ty1 - ty0 <= 1;
     const int txb = tx1 > tx0 ? tx0 + 1 : tx0, tyb = ty1 > ty0 ? ty0 + 1 : ty0;
-    const float2 t00 = a.dtiles[ty0 * a.tiles_x + tx0], t01 = a.dtiles[ty0 * a.tiles_x + txb];
-    const float2 t10 = a.dtiles[tyb * a.tiles_x + tx0], t11 = a.dtiles[tyb * a.tiles_x + txb];
+    const float2 t00 = tiles(ty0 * a.tiles_x + tx0), t01 = tiles(ty0 * a.tiles_x + txb);
+    const float2 t10 = tiles(tyb * a.tiles_x + tx0), t11 = tiles(tyb * a.tiles_x + txb);
     const float dmin = fminf(fminf(t00.x, t01.x), fminf(t10.x, t11.x));
     const float dmax = fmaxf(fmaxf(t00.y, t01.y), fmaxf(t10.y, t11.y));
     const float m = 1e-4f;
     const bool reach = zhi + m > dmin - mu_lo && zlo - m < dmax + mu_hi;
     return okz && fin && (off || (small && !reach));
 }
+
+// The fast integrate's reads of the small per-frame arrays: through buffer
+// resources (32-bit offsets) when the volume's planes fit in 32-bit byte
+// offsets, else plain global loads.
+template <bool BUF>
+struct IntegrateInputs;
+template <>
+struct IntegrateInputs<true> {
+    __amdgpu_buffer_rsrc_t depth, tiles, zflags;
+    __device__ __forceinline__ IntegrateInputs(const IntegrateArgs &a)
+        : depth(__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(a.depth_m), (short)0, a.dw * a.dh * 4,
+                                                  0x00020000)),
+          tiles(__builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(a.dtiles), (short)0,
+                                                  a.tiles_x * a.tiles_y * 8, 0x00020000)),
+          zflags(__builtin_amdgcn_make_buffer_rsrc(a.zflags, (short)0, a.NB[0] * a.NB[1] * a.NB[2], 0x00020000))
+    {
+    }
+    // pixel index < 0 reads 0 (beyond the buffer); such values are not used
+    __device__ __forceinline__ float depth_at(int pix) const
+    {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(depth, pix * 4, 0, 0));
+    }
+    __device__ __forceinline__ float2 operator()(int t) const
+    {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(tiles, t * 8, 0, 0);
+        return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+    }
+    __device__ __forceinline__ bool zflag(uint32_t b) const
+    {
+        return __builtin_amdgcn_raw_buffer_load_b8(zflags, (int)b, 0, 0) != 0;
+    }
+};
+template <>
+struct IntegrateInputs<false> {
+    const float *d;
+    const float2 *t;
+    const uint8_t *z;
+    __device__ __forceinline__ IntegrateInputs(const IntegrateArgs &a) : d(a.depth_m), t(a.dtiles), z(a.zflags) {}
+    __device__ __forceinline__ float depth_at(int pix) const { return d[pix < 0 ? 0 : pix]; }
+    __device__ __forceinline__ float2 operator()(int i) const { return t[i]; }
+    __device__ __forceinline__ bool zflag(size_t b) const { return z[b] != 0; }
+};
 
 // Reduced-arithmetic integrate (KF_OPT_FAST_MATH).  Same update rule as
 // k_integrate, with the per-voxel arithmetic cut to about a third: one
@@ -1730,6 +1777,10 @@ __device__ __forceinline__ bool chunk_unreachable(const IntegrateArgs &a, const 
 // from its first chunk as in the reference.  Results agree with
 // the correctly rounded evaluation to a few ulps; voxels on a pixel edge or a
 // branch threshold can take the other side (counted as flips by the tests).
+// OFF32: every byte offset into a TSDF plane fits in 32 bits; the planes,
+// the depth image, the depth tiles and the zero flags are then read and
+// written through buffer resources (one 32-bit offset per voxel, the z step
+// of a chunk in the scalar offset).
 // COUNT: the same walk, loads and decisions with no store and no flag
 // update; counts[0] += voxels read, counts[1] += voxels written (the bytes
 // this kernel moves for the volume are 2 x sizeof(V) per voxel of each).
@@ -1738,6 +1789,7 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
                                                     int by, int bz, unsigned long long *counts = nullptr)
 {
     using Idx = typename std::conditional<OFF32, uint32_t, size_t>::type;
+    const IntegrateInputs<OFF32> in(a);
     if (a.gated && !st->integrate_now) return;
     // columns beyond the volume take part with an empty range (the whole
     // wavefront takes part in the range reduction below)
@@ -1768,11 +1820,11 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
         column_frustum(c, fx, fy, cxh, cyh, dmax_x, dmax_y, z0, z1, zs, ze);
     }
     if (!col_in) zs = ze = z0;
-    // every lane walks the same chunks, from the wavefront's first chunk to
+    // every lane tests the same chunks, from the wavefront's first chunk to
     // its last (zs is on the segment's chunk grid for every lane), so the
-    // lanes of an iteration work on one z: coalesced voxel and flag accesses
-    // and neighbouring columns' similar skip decisions side by side; a lane
-    // works only inside its own [zs, ze)
+    // lanes of an iteration test one z: coalesced flag and tile loads and
+    // neighbouring columns' similar skip decisions side by side; a lane's
+    // chunks outside its own [zs, ze) are never needed
     int wzs = zs < ze ? zs : INT_MAX, wze = zs < ze ? ze : INT_MIN;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
@@ -1784,53 +1836,91 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
     const float inv_mu = 1.0f / a.mu, neg_mu = -a.mu;
     typename S::V *__restrict__ tv = (typename S::V *)a.tsdf;
     typename S::V *__restrict__ wv = (typename S::V *)a.weight;
+    // OFF32: the planes' byte offsets fit in 32 bits (buffer accesses)
+    const uint32_t plane_bytes = OFF32 ? (uint32_t)((size_t)a.N[0] * a.N[1] * a.N[2] * sizeof(typename S::V)) : 0u;
+    const BufPlane<S> TB(tv, plane_bytes), WB(wv, plane_bytes);
     const Idx plane = (Idx)a.N[0] * (Idx)a.N[1];
-    const Idx col = (Idx)y * (Idx)a.N[0] + (Idx)x;
-    const size_t brow = (size_t)(y >> 3) * a.NB[0] + (x >> 3);
     const size_t bplane = (size_t)a.NB[0] * a.NB[1];
+    const size_t brow_own = (size_t)(y >> 3) * a.NB[0] + (x >> 3);  // this lane's column of bricks
     unsigned long long n_read = 0, n_written = 0;  // COUNT only
 
-    // windows of up to 32 chunks: first the skip tests of every chunk of the
+    // windows of up to 16 chunks: first the skip tests of every chunk of the
     // window, one z at a time for the whole wavefront (so neighbouring
     // columns' tile and flag loads coincide) -- a brick with a clear zero
     // flag holds only (0, 0), so a chunk in one that no depth can reach keeps
-    // (0, 0) and needs no work; then each lane's needed chunks in z order, the
-    // k-th of every lane in the k-th pass, so a pass has as many lanes at
-    // work as have a k-th chunk (a surface crosses neighbouring columns at
-    // different chunks)
-    for (int wz = wzs; wz < wze; wz += 32 * kIntU) {
-        const int wend = min(wze, wz + 32 * kIntU);
+    // (0, 0) and needs no work; then the needed chunks of all lanes as one
+    // work list, 64 per pass (a surface crosses neighbouring columns at
+    // different chunks and many columns cross none, so per-lane passes left
+    // about half the lanes idle)
+    constexpr int kWin = 16;  // chunks per window
+    for (int wz = wzs; wz < wze; wz += kWin * kIntU) {
+        const int wend = min(wze, wz + kWin * kIntU);
         uint32_t need = 0u, zfs = 0u;
         for (int i = 0, zb = wz; zb < wend; ++i, zb += kIntU) {
             const bool mine = zb >= zs && zb < ze;
-            const bool zf = a.zflags[(size_t)(zb >> 3) * bplane + brow] != 0;
-            const bool unr = chunk_unreachable(a, c, fmaf((float)zb, c.dl[0], c.pos[0]),
+            const bool zf = in.zflag((Idx)((zb >> 3) * (Idx)bplane + (Idx)brow_own));
+            const bool unr = chunk_unreachable(a, in, c, fmaf((float)zb, c.dl[0], c.pos[0]),
                                                fmaf((float)zb, c.dl[1], c.pos[1]), fmaf((float)zb, c.dl[2], c.pos[2]),
                                                fx, fy, cxh, cyh, mu_lo, mu_hi);
             if (mine && !(unr && !zf)) need |= 1u << i;
             if (zf) zfs |= 1u << i;
         }
-      while (need != 0u) {
-        const int j = __builtin_ctz(need);
-        need &= need - 1u;
+        // the window's needed chunks as one chunk-major work list: item
+        // (lane, chunk i, brick-nonzero flag) at slot base_i + (rank of the
+        // lane among the lanes needing chunk i); then 64 items per pass, each
+        // lane taking the column of the item's lane, so every pass but the
+        // last has all lanes at work and neighbouring lanes mostly hold
+        // neighbouring columns at the same z
+        __shared__ uint16_t work[4][kWin * 64];
+        uint16_t *wl = work[threadIdx.y];
+        const int lane = (int)threadIdx.x;
+        const int nch = (wend - wz + kIntU - 1) / kIntU;
+        uint32_t nitems = 0;
+        for (int i = 0; i < nch; ++i) {
+            const bool bit = (need >> i) & 1u;
+            const unsigned long long bm = __ballot(bit);
+            if (bit) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+                wl[nitems + rank] = (uint16_t)(lane | (i << 6) | (((zfs >> i) & 1u) << 11));
+            }
+            nitems += (uint32_t)__popcll(bm);
+        }
+        __builtin_amdgcn_wave_barrier();
+        const float cp0 = c.pos[0], cp1 = c.pos[1], cp2 = c.pos[2];
+      for (uint32_t q0 = 0; q0 < nitems; q0 += 64) {
+        const uint32_t q = q0 + (uint32_t)lane;
+        const uint32_t it = q < nitems ? (uint32_t)wl[q] : (uint32_t)lane;
+        const int sl = (int)(it & 63u), j = (int)((it >> 6) & 31u);
+        // the item's column: its lane's position at z = 0 (all lanes take part)
+        const float sp0 = __shfl(cp0, sl, 64), sp1 = __shfl(cp1, sl, 64), sp2 = __shfl(cp2, sl, 64);
+        if (q >= nitems) continue;
+        // (the wavefront's row: lanes beyond the volume hold y = 0)
         const int zb = wz + j * kIntU;
-        float p0 = fmaf((float)zb, c.dl[0], c.pos[0]);
-        float p1 = fmaf((float)zb, c.dl[1], c.pos[1]);
-        float p2 = fmaf((float)zb, c.dl[2], c.pos[2]);
+        const int sx = bx * 64 + sl, sy = by * a.rows + (int)threadIdx.y;
+        const Idx col = (Idx)sy * (Idx)a.N[0] + (Idx)sx;
+        const size_t brow = (size_t)(sy >> 3) * a.NB[0] + (sx >> 3);
+        float p0 = fmaf((float)zb, c.dl[0], sp0);
+        float p1 = fmaf((float)zb, c.dl[1], sp1);
+        float p2 = fmaf((float)zb, c.dl[2], sp2);
         const size_t bidx = (size_t)(zb >> 3) * bplane + brow;
-        const bool zf = (zfs >> j) & 1u;
+        const bool zf = (it >> 11) & 1u;
         int pix[kIntU];
         float pz[kIntU], fac[kIntU];
+        const int kend = z1 - zb;  // voxels of the chunk inside the segment
 #pragma unroll
         for (int k = 0; k < kIntU; ++k) {
             const float r = __builtin_amdgcn_rcpf(p2);
             const float qx = p0 * r, qy = p1 * r;
             const float px = fmaf(fx, qx, cxh), py = fmaf(fy, qy, cyh);
-            // bitwise & keeps the predicate branch-free
-            const bool ok = ((zb + k) < z1) & !(p2 < 0.0001f) & (px >= 0.0f) & (px <= dmax_x) & (py >= 0.0f) &
-                            (py <= dmax_y);
-            const int ipx = (int)fminf(fmaxf(px, 0.0f), dmax_x), ipy = (int)fminf(fmaxf(py, 0.0f), dmax_y);
-            pix[k] = ok ? ipy * a.dw + ipx : -1;
+            // px in [0, dmax_x] exactly when clamping leaves it unchanged
+            // (false for NaN); bitwise & keeps the predicate branch-free
+            const float cpx = __builtin_amdgcn_fmed3f(px, 0.0f, dmax_x);
+            const float cpy = __builtin_amdgcn_fmed3f(py, 0.0f, dmax_y);
+            const bool ok = (k < kend) & !(p2 < 0.0001f) & (cpx == px) & (cpy == py);
+            // 24-bit product: pixel indices < 2^24
+            const int pv = (int)__umul24((uint32_t)(int)cpy, (uint32_t)a.dw) + (int)cpx;
+            pix[k] = ok ? pv : -1;
             pz[k] = p2;
             fac[k] = __builtin_amdgcn_sqrtf(fmaf(qx, qx, fmaf(qy, qy, 1.0f)));
             p0 += c.dl[0];
@@ -1839,7 +1929,7 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
         }
         float dep[kIntU];
 #pragma unroll
-        for (int k = 0; k < kIntU; ++k) dep[k] = a.depth_m[pix[k] < 0 ? 0 : pix[k]];
+        for (int k = 0; k < kIntU; ++k) dep[k] = in.depth_at(pix[k]);
         int kind[kIntU];
         float sdf[kIntU];
 #pragma unroll
@@ -1852,15 +1942,25 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
         // stored bits, zero-extended to 32 bits (no register packing of
         // 16-bit values, so the conditional loads stay independent and the
         // conversions wait for them only after all are issued)
+        // voxel k of the chunk: element vc + k * plane; with OFF32 its byte
+        // offset is voff + k * pbytes, the second term a scalar offset
+        const Idx vc = col + (Idx)zb * plane;
+        const uint32_t voff = OFF32 ? (uint32_t)vc * (uint32_t)sizeof(typename S::V) : 0u;
+        const uint32_t pbytes = OFF32 ? (uint32_t)plane * (uint32_t)sizeof(typename S::V) : 0u;
         uint32_t ot[kIntU], ow[kIntU];
 #pragma unroll
         for (int k = 0; k < kIntU; ++k) {
             ot[k] = 0u;
             ow[k] = 0u;
             if (kind[k] == 2 && zf) {
-                const Idx vi = col + (Idx)(zb + k) * plane;
-                ot[k] = S::bits(tv[vi]);
-                ow[k] = S::bits(wv[vi]);
+                const Idx vi = vc + (Idx)k * plane;
+                if (OFF32) {
+                    ot[k] = TB.bits(voff, (uint32_t)k * pbytes);
+                    ow[k] = WB.bits(voff, (uint32_t)k * pbytes);
+                } else {
+                    ot[k] = S::bits(tv[vi]);
+                    ow[k] = S::bits(wv[vi]);
+                }
                 if (COUNT) ++n_read;
             }
         }
@@ -1868,13 +1968,13 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
 #pragma unroll
         for (int k = 0; k < kIntU; ++k) {
             if (kind[k] == 0) continue;
-            const Idx vi = col + (Idx)(zb + k) * plane;
+            const Idx vi = vc + (Idx)k * plane;
             float t = 0.0f, w = 0.0f;
             if (kind[k] == 2) {
                 const float t0 = S::value(S::from_bits(ot[k])), w0 = S::weight(S::from_bits(ow[k]));
                 const float wd = fmaf(w0, t0, sdf[k]) * __builtin_amdgcn_rcpf(w0 + 1.0f);
                 if (wd < 0.2f) {
-                    t = fminf(fmaxf(wd, -0.1f), 0.2f);
+                    t = gmax(wd, -0.1f);  // = min(max(wd, -0.1), 0.2) since wd < 0.2
                     w = fminf(w0 + 1.0f, a.max_weight);
                 }
             }
@@ -1883,6 +1983,9 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
             if (zf || !zero) {
                 if (COUNT) {
                     ++n_written;
+                } else if (OFF32) {
+                    TB.store(voff, (uint32_t)k * pbytes, nt);
+                    WB.store(voff, (uint32_t)k * pbytes, nw);
                 } else {
                     tv[vi] = nt;
                     wv[vi] = nw;
@@ -1896,6 +1999,7 @@ __device__ __forceinline__ void integrate_fast_body(const IntegrateArgs &a, cons
             if (nz && !zf) a.zflags[bidx] = 1;
         }
       }
+        __builtin_amdgcn_wave_barrier();  // the list is read before the next window writes it
     }
     if (COUNT) {
         atomicAdd(&counts[0], n_read);
@@ -2039,8 +2143,14 @@ struct VolView {
     int N0, N1, N2;
     size_t plane;
     uint32_t plane32;
-    int off32;        // every voxel index fits in 32 bits
+    int off32;        // every byte offset into the value plane fits in 32 bits
     uint32_t *touch;  // TOUCH: one bit per voxel, set for every voxel read
+    static constexpr uint32_t kSz = sizeof(typename S::V);
+    __device__ __forceinline__ VolView(const RaycastArgs &a, uint32_t *touch_bits = nullptr)
+        : v((const typename S::V *)a.tsdf), N0(a.N[0]), N1(a.N[1]), N2(a.N[2]), plane((size_t)a.N[0] * a.N[1]),
+          plane32((uint32_t)((size_t)a.N[0] * a.N[1])), off32(a.off32), touch(touch_bits)
+    {
+    }
     __device__ __forceinline__ void mark(size_t i) const
     {
         if (TOUCH) atomicOr(&touch[i >> 5], 1u << (i & 31));
@@ -2049,6 +2159,25 @@ struct VolView {
     {
         mark(i);
         return S::raw(v[i]);
+    }
+    // off32: the value plane is read at 32-bit byte offsets from uniform
+    // base addresses (the base stays in scalar registers and each access
+    // adds one 32-bit offset: no 64-bit address arithmetic per load)
+    __device__ __forceinline__ float ld32(uint32_t i) const
+    {
+        if (TOUCH) return ld(i);
+        return S::raw(*(const typename S::V *)((const char *)v + i * kSz));
+    }
+    // the two consecutive elements at byte offset voff from the uniform
+    // address base (one load)
+    struct alignas(kSz) Pair {
+        typename S::V e0, e1;
+    };
+    __device__ __forceinline__ static void pair(const char *base, uint32_t voff, float &e0, float &e1)
+    {
+        const Pair q = *(const Pair *)(base + voff);
+        e0 = S::raw(q.e0);
+        e1 = S::raw(q.e1);
     }
     __device__ __forceinline__ float raw(int x, int y, int z) const
     {
@@ -2126,14 +2255,14 @@ __device__ __forceinline__ float sample_eval(const VolView<S, TOUCH> &V, const S
         const uint32_t dx = (uint32_t)(hi[0] - lo[0]);
         const uint32_t dy = __umul24((uint32_t)(hi[1] - lo[1]), (uint32_t)V.N0);
         const uint32_t dz = __umul24((uint32_t)(hi[2] - lo[2]), V.plane32);
-        c000 = V.ld(b);
-        c100 = V.ld(b + dx);
-        c010 = V.ld(b + dy);
-        c110 = V.ld(b + dy + dx);
-        c001 = V.ld(b + dz);
-        c101 = V.ld(b + dz + dx);
-        c011 = V.ld(b + dz + dy);
-        c111 = V.ld(b + dz + dy + dx);
+        c000 = V.ld32(b);
+        c100 = V.ld32(b + dx);
+        c010 = V.ld32(b + dy);
+        c110 = V.ld32(b + dy + dx);
+        c001 = V.ld32(b + dz);
+        c101 = V.ld32(b + dz + dx);
+        c011 = V.ld32(b + dz + dy);
+        c111 = V.ld32(b + dz + dy + dx);
     } else {  // corners may be out of bounds: they read 0 (image load semantics)
         c000 = V.raw(lo[0], lo[1], lo[2]);
         c100 = V.raw(hi[0], lo[1], lo[2]);
@@ -2255,35 +2384,33 @@ __device__ void gradient(const VolView<S, TOUCH> &V, const RaycastArgs &a, const
     if (interior) {
         // the 32 voxels of the stencil from twelve row reads: x = b - 1 .. b + 2
         // at y, z in {b, b + 1}; x in {b, b + 1} at y in {b - 1, b + 2} and
-        // at z in {b - 1, b + 2}
-        const typename S::V *c =
-            V.v + (((uint32_t)bb[2] * V.plane32 + (uint32_t)bb[1] * (uint32_t)V.N0) + (uint32_t)bb[0]);
-        const ptrdiff_t py = (ptrdiff_t)V.N0, pz = (ptrdiff_t)V.plane32;
+        // at z in {b - 1, b + 2}.  Byte offsets from the corner (b - 1, b - 1,
+        // b - 1) (interior: b >= 1); each row's (y, z) displacement is folded
+        // into a uniform base address.
+        constexpr uint32_t sz = VolView<S, TOUCH>::kSz;
+        const uint32_t corner = (((uint32_t)bb[2] - 1u) * V.plane32 + ((uint32_t)bb[1] - 1u) * (uint32_t)V.N0) +
+                                ((uint32_t)bb[0] - 1u);
+        const uint32_t vo = corner * sz;
+        const size_t rowb = (size_t)V.N0 * sz, plb = (size_t)V.plane32 * sz;
+        const char *v0 = (const char *)V.v;
+        auto row = [&](int yi, int zi) { return v0 + (size_t)yi * rowb + (size_t)zi * plb; };
         float A[2][2][4], B[2][2][2], C[2][2][2];
 #pragma unroll
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int z = 0; z < 2; ++z) {
-                const typename S::V *r = c + z * pz + y * py - 1;
+                const typename S::V *r = (const typename S::V *)(row(y + 1, z + 1) + vo);
 #pragma unroll
                 for (int x = 0; x < 4; ++x) A[y][z][x] = S::raw(r[x]);
             }
 #pragma unroll
         for (int z = 0; z < 2; ++z)
 #pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                const typename S::V *r = c + z * pz + (y ? 2 : -1) * py;
-                B[z][y][0] = S::raw(r[0]);
-                B[z][y][1] = S::raw(r[1]);
-            }
+            for (int y = 0; y < 2; ++y) V.pair(row(y ? 3 : 0, z + 1), vo + sz, B[z][y][0], B[z][y][1]);
 #pragma unroll
         for (int y = 0; y < 2; ++y)
 #pragma unroll
-            for (int z = 0; z < 2; ++z) {
-                const typename S::V *r = c + (z ? 2 : -1) * pz + y * py;
-                C[y][z][0] = S::raw(r[0]);
-                C[y][z][1] = S::raw(r[1]);
-            }
+            for (int z = 0; z < 2; ++z) V.pair(row(y + 1, z ? 3 : 0), vo + sz, C[y][z][0], C[y][z][1]);
         auto VF = [&](int X, int Y, int Z) -> float {
             if ((Y == kLU || Y == kUL) && (Z == kLU || Z == kUL)) return A[Y - kLU][Z - kLU][X];
             if (Y == kLL || Y == kUU) return B[Z - kLU][Y == kLL ? 0 : 1][X - kLU];
@@ -2319,8 +2446,7 @@ __device__ __forceinline__ void raycast_view(const RaycastArgs &a, const DevStat
 template <class S, bool TOUCH>
 __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float (&view)[16], int x, int y)
 {
-    VolView<S, TOUCH> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
-                 (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32, a.touch};
+    const VolView<S, TOUCH> V(a, a.touch);
 
     // TOUCH: the flag bytes read are marked after the voxel bits (dilated
     // brick flags, then coarse cells)
@@ -2459,20 +2585,20 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
                 if (__all(all_in)) {
                     // every active lane: all loads first, then the blends
                     float c[kBatch][8];
+                    constexpr uint32_t sz = VolView<S, TOUCH>::kSz;
+                    const size_t rowb = (size_t)V.N0 * sz, plb = (size_t)V.plane32 * sz;
+                    // the four row bases are uniform; each sample adds one
+                    // 32-bit byte offset
+                    const char *rb0 = (const char *)V.v;
+                    const char *rb1 = rb0 + rowb, *rb2 = rb0 + plb, *rb3 = rb0 + (plb + rowb);
 #pragma unroll
                     for (int j = 0; j < kBatch; ++j) {
-                        const typename S::V *r0 = V.v + fq[j].base;
-                        const typename S::V *r1 = r0 + V.N0;
-                        const typename S::V *r2 = r0 + V.plane32;
-                        const typename S::V *r3 = r2 + V.N0;
-                        c[j][0] = S::raw(r0[0]);
-                        c[j][1] = S::raw(r0[1]);
-                        c[j][2] = S::raw(r1[0]);
-                        c[j][3] = S::raw(r1[1]);
-                        c[j][4] = S::raw(r2[0]);
-                        c[j][5] = S::raw(r2[1]);
-                        c[j][6] = S::raw(r3[0]);
-                        c[j][7] = S::raw(r3[1]);
+                        // rows (y, z), (y + 1, z), (y, z + 1), (y + 1, z + 1)
+                        const uint32_t vo = fq[j].base * sz;
+                        V.pair(rb0, vo, c[j][0], c[j][1]);
+                        V.pair(rb1, vo, c[j][2], c[j][3]);
+                        V.pair(rb2, vo, c[j][4], c[j][5]);
+                        V.pair(rb3, vo, c[j][6], c[j][7]);
                     }
 #pragma unroll
                     for (int j = 0; j < kBatch; ++j)
@@ -2615,8 +2741,7 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
         inverse_camera_matrix(a.cam, iK);
         mul4(P, iK, view);  // m_pose * invK (gFusion.cpp:1298)
     }
-    VolView<S> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
-                 (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32};
+    const VolView<S> V(a);
 
     const float o[3] = {view[12], view[13], view[14]};
     float d[3];
@@ -3090,7 +3215,7 @@ hipError_t launch_integrate(IntegrateArgs a, int storage, int fast, const DevSta
                             hipEvent_t t0, hipEvent_t t1)
 {
     const dim3 g = integrate_grid(a), b(64, a.rows);
-    const bool off32 = (size_t)a.N[0] * a.N[1] * a.N[2] < (size_t(1) << 32);
+    const bool off32 = integrate_planes_fit_32(a, storage);
     if (fast && a.col_pos) {
         // the timed interval covers the column set-up and the integrate
         launch_timed(k_integrate_columns, dim3((a.N[0] + 63) / 64, (a.N[1] + 3) / 4), dim3(64, 4), s, t0,
@@ -3136,7 +3261,7 @@ hipError_t launch_count_integrate_fast(const IntegrateArgs &a, int storage, cons
                                        unsigned long long *counts, hipStream_t s)
 {
     const dim3 g = integrate_grid(a), b(64, a.rows);
-    const bool off32 = (size_t)a.N[0] * a.N[1] * a.N[2] < (size_t(1) << 32);
+    const bool off32 = integrate_planes_fit_32(a, storage);
 #define KF_INT(ST)                                                                               \
     if (off32) hipLaunchKernelGGL((k_count_integrate_fast<ST, true>), g, b, 0, s, a, st, counts); \
     else hipLaunchKernelGGL((k_count_integrate_fast<ST, false>), g, b, 0, s, a, st, counts);
@@ -3175,8 +3300,7 @@ __global__ __launch_bounds__(256) void k_intensity_projection(RaycastArgs a, con
         inverse_camera_matrix(a.cam, iK);
         mul4(P, iK, view);  // m_pose * invK (gFusion.cpp:1376)
     }
-    VolView<S> V{(const typename S::V *)a.tsdf, a.N[0], a.N[1], a.N[2], (size_t)a.N[0] * a.N[1],
-                 (uint32_t)((size_t)a.N[0] * a.N[1]), a.off32};
+    const VolView<S> V(a);
     const float o[3] = {view[12], view[13], view[14]};
     float d[3];
     rotate3(view, (float)x, (float)y, 1.0f, d);

@@ -55,4 +55,34 @@ struct StoreF16 {
     __device__ static V from_bits(uint32_t b) { return (V)b; }
 };
 
+// One volume plane addressed by 32-bit byte offsets through a buffer
+// resource (the base address lives in scalar registers, so an access costs
+// one offset computation instead of a 64-bit address; the hardware range
+// check reads 0 beyond the plane and drops stores there).  For planes whose
+// size in bytes fits in 32 bits.
+template <class S>
+struct BufPlane {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ BufPlane(const void *p, uint32_t bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000))
+    {
+    }
+    // stored bits of the element at byte offset voff + soff (soff uniform:
+    // it rides in the instruction's scalar offset), zero-extended
+    __device__ __forceinline__ uint32_t bits(uint32_t voff, uint32_t soff) const
+    {
+        if constexpr (sizeof(typename S::V) == 4)
+            return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+        else
+            return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+    }
+    __device__ __forceinline__ void store(uint32_t voff, uint32_t soff, typename S::V v) const
+    {
+        if constexpr (sizeof(typename S::V) == 4)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, (int)voff, (int)soff, 0);
+    }
+};
+
 }  // namespace kf

@@ -8,6 +8,20 @@
 
 #include "kf_march.hpp"
 
+#ifdef KF_MARCH_RCP_PERTURB
+// the reciprocal moved by up to 4 ulps either way (the device's is about 1
+// ulp): the march must still equal plain stepping bit for bit
+static std::mt19937 perturb_rng(777);
+float kf::march_rcp_perturbed(float d)
+{
+    float r = 1.0f / d;
+    const int k = (int)(perturb_rng() % 9) - 4;
+    for (int i = 0; i < k; ++i) r = std::nextafter(r, INFINITY);
+    for (int i = 0; i > k; --i) r = std::nextafter(r, 0.0f);
+    return r;
+}
+#endif
+
 static int plain(float &t, float &t_prev, float s, float T)
 {
     int n = 0;
@@ -49,10 +63,12 @@ int main(int argc, char **argv)
             for (int k = 0; k < (int)(rng() % 50); ++k) t = std::nextafter(t, 0.0f);
             ++crossings;
         }
-        const double span = std::exp(U(rng) * std::log(3000.0)) * (double)s;
+        const double steps_max = rng() % 10 == 0 ? 200000.0 : 3000.0;  // some long runs over several binades
+        const double span = std::exp(U(rng) * std::log(steps_max)) * (double)s;
         float T = (float)((double)t + span * (0.2 + U(rng)));
         if (rng() % 7 == 0) T = t;  // at least one step
         if (rng() % 11 == 0) T = std::nextafter(t + s, INFINITY);
+        if (rng() % 13 == 0) T = t - s;  // a target behind t: one step
         float ta = t, pa = 0.0f, tb = t, pb = 0.0f;
         const int na = plain(ta, pa, s, T);
         const int nb = kf::march_to(tb, pb, s, T);

@@ -554,13 +554,17 @@ def test_brick_flags_cover_every_negative_voxel(seq20):
     np.testing.assert_array_equal(a_v, g.getVerts())
 
 
-def test_fast_math_integrate_within_tolerance(o32, seq20):
-    print("case: fast-math integrate", flush=True)
+@pytest.mark.parametrize("volume", [128, 96, 72])
+def test_fast_math_integrate_within_tolerance(o32, seq20, volume):
+    """Also at widths that are not multiples of the 64-column wavefront: the
+    wavefronts of the last block column hold lanes beyond the volume, and a
+    lane that works on another lane's column must use that column's row."""
+    print("case: fast-math integrate %d^3" % volume, flush=True)
     depth, gt = seq20
-    g, cfg = make(storage="f32")
+    g, cfg = make(volume=volume, storage="f32")
     g.Reset(gt[0])
     g.setOption(K.OPT_FAST_MATH, 1)
-    vol = o32.new_volume((128, 128, 128), oracle.F32X2)
+    vol = o32.new_volume((volume, volume, volume), oracle.F32X2)
     for f in range(6):
         g.setPose(gt[f])
         g.depthToVertex(depth[f])
@@ -568,6 +572,7 @@ def test_fast_math_integrate_within_tolerance(o32, seq20):
         o32.integrate(vol, oracle.F32X2, (2.0,) * 3, depth[f].astype(np.float32), gt[f],
                       (365.5, 365.5, 255.5, 211.5), cfg.mu, 100.0)
     v, w = g.getVolume()
+    assert v.shape == (volume, volume, volume)
     dv = np.abs(v.astype(np.float64) - vol[..., 0])
     flips = (w != vol[..., 1])
     print("fast-math: max |dv| %.3g, >1e-3: %.3g, weight flips %.3g" % (dv[~flips].max(), (dv > 1e-3).mean(),
@@ -576,7 +581,10 @@ def test_fast_math_integrate_within_tolerance(o32, seq20):
     # every other voxel within 1e-3 of truncation
     flips = flips | (dv > 1e-3)
     assert flips.mean() < 1e-4
-    assert np.percentile(dv[~flips], 99.9) < 1e-5
+    # the hardware reciprocal / square root: a few ulps per update, up to
+    # ~1e-5 of truncation after six fused frames (measured 0.6-1.1e-5 at the
+    # 99.9th percentile at 72^3 - 128^3)
+    assert np.percentile(dv[~flips], 99.9) < 2e-5
 
 
 def test_graph_replay_matches_direct_launches(seq20):
