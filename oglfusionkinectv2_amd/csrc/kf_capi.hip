@@ -233,7 +233,11 @@ kf::RaycastArgs raycast_args(kf_session *s, int lanes = -1, int xcd_group = -1)
     a.tiles_y = (c.height + tile - 1) / tile;
     a.xcd_group = std::max(1, xcd_group < 0 ? s->xcd_group : xcd_group);
     a.blocks = (a.tiles_x * a.tiles_y + 8 * a.xcd_group - 1) / (8 * a.xcd_group) * (8 * a.xcd_group);
-    a.off32 = s->voxels * voxel_plane_bytes(c.storage) <= (size_t)0xFFFFFFFFu ? 1 : 0;
+    // 32-bit byte offsets, and N0 N1 < 2^24 for the 24-bit index products
+    a.off32 = s->voxels * voxel_plane_bytes(c.storage) <= (size_t)0xFFFFFFFFu &&
+                      (size_t)c.volume_size[0] * c.volume_size[1] < ((size_t)1 << 24)
+                  ? 1
+                  : 0;
     a.refV = s->refV;
     a.refN = s->refN;
     a.dflags = s->dflags;
@@ -1319,6 +1323,7 @@ struct kf_batch {
     int storage = 0, fast = 0, off32 = 1;
     int pyr_blocks = 0, ray_blocks = 0, nblk[3] = {0, 0, 0};
     int ray_lanes = 1;
+    int ray_dim_pow2 = 1;  // every member's volume dimensions are powers of two
     int total_int = 0, total_coarse = 0;
     std::vector<long> frames;
     // Membership changes upload new tables with stream-ordered copies from a
@@ -1467,6 +1472,8 @@ int batch_fill(kf_batch *b, kf_session *const *sessions, int n)
     b->pyr_blocks = h[0].pyr.blocks_total;
     b->ray_blocks = h[0].ray.blocks;
     b->ray_lanes = h[0].ray.lanes;
+    b->ray_dim_pow2 = 1;
+    for (int i = 0; i < n; ++i) b->ray_dim_pow2 = b->ray_dim_pow2 && h[i].ray.dim_pow2;
     for (int l = 0; l < 3; ++l) b->nblk[l] = (h[0].icp[l].group_only >= 0 ? 1 : 8) * h[0].icp[l].rows_step;
     b->total_int = ip[n];
     b->total_coarse = cp[n];
@@ -1514,7 +1521,8 @@ int enqueue_tick(kf_batch *b, int k)
     if (e == hipSuccess)
         e = kf::launch_batch_integrate(tb, tp, b->storage, b->fast, b->off32, b->total_int, b->int_rows, b->stream);
     if (e == hipSuccess) e = kf::launch_batch_bricks(tb, tp, b->total_coarse, b->stream);
-    if (e == hipSuccess) e = kf::launch_batch_raycast(tb, tp, b->storage, b->ray_lanes, b->ray_blocks, b->stream);
+    if (e == hipSuccess) e = kf::launch_batch_raycast(tb, tp, b->storage, b->ray_lanes, b->ray_blocks, b->ray_dim_pow2,
+                                                            b->stream);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
     return KF_OK;
 }

@@ -101,7 +101,7 @@ struct RaycastArgs {
     unsigned long long *stats;  // optional: evaluated, skipped, cells skipped, hits, max eval, max iters
     int dim_pow2;               // all volume_dim are powers of two
     float inv_dim[3];
-    int off32;                  // value-plane bytes < 2^32 (32-bit byte offsets, buffer loads)
+    int off32;                  // value-plane bytes < 2^32 and N0 N1 < 2^24 (32-bit offsets, 24-bit products)
     int xcd_group;              // tiles per XCD cluster (block -> tile mapping)
     int lanes;                  // lanes per ray: 1 (16x16-pixel tiles) or 4 (8x8-pixel tiles)
     uint32_t *ray_counts;       // optional (with stats): per pixel, evaluated samples | skip events << 16
@@ -181,7 +181,7 @@ hipError_t launch_batch_integrate(const BatchTable &tb, const BatchTick *tp, int
                                   int total_blocks, int rows, hipStream_t s);
 hipError_t launch_batch_bricks(const BatchTable &tb, const BatchTick *tp, int total_coarse, hipStream_t s);
 hipError_t launch_batch_raycast(const BatchTable &tb, const BatchTick *tp, int storage, int lanes,
-                                int blocks_per_session, hipStream_t s);
+                                int blocks_per_session, int dim_pow2, hipStream_t s);
 
 // Optional t0 / t1: events recorded by the kernel's own dispatch packet at its
 // start / end (exact kernel duration, no separate marker packets).

@@ -2203,17 +2203,22 @@ struct Sample {
 // p * N / dim - 0.5 on axis k.  When every dim is a power of two the
 // division is an exact scaling, done as a multiply by 1/dim (the same result
 // bit for bit).
+// DP: 1 = every dim a power of two, 0 = not, -1 = read a.dim_pow2 (the
+// raycast kernel is instantiated for both cases, so its hot loop carries no
+// branch on it)
+template <int DP = -1>
 __device__ __forceinline__ float voxel_coord(const RaycastArgs &a, float p, int k)
 {
-    if (a.dim_pow2) return ((p * (float)a.N[k]) * a.inv_dim[k]) - 0.5f;
+    if (DP == 1 || (DP < 0 && a.dim_pow2)) return ((p * (float)a.N[k]) * a.inv_dim[k]) - 0.5f;
     return (p * (float)a.N[k] / a.dim[k]) - 0.5f;
 }
 
+template <int DP = -1>
 __device__ __forceinline__ void sample_setup(const RaycastArgs &a, float p0, float p1, float p2, Sample &sm)
 {
-    sm.s[0] = voxel_coord(a, p0, 0);
-    sm.s[1] = voxel_coord(a, p1, 1);
-    sm.s[2] = voxel_coord(a, p2, 2);
+    sm.s[0] = voxel_coord<DP>(a, p0, 0);
+    sm.s[1] = voxel_coord<DP>(a, p1, 1);
+    sm.s[2] = voxel_coord<DP>(a, p2, 2);
     sm.valid = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -2249,7 +2254,7 @@ __device__ __forceinline__ float sample_eval(const VolView<S, TOUCH> &V, const S
                         (unsigned)lo[2] < (unsigned)V.N2 && (unsigned)hi[2] < (unsigned)V.N2;
     if (V.off32 && inside) {
         // all 8 corners in the volume: 32-bit element offsets from one base
-        // 24-bit products: lo, hi < N_k <= 2048 and plane32 = N0 N1 < 2^24
+        // 24-bit products: lo, hi < N_k and plane32 = N0 N1 < 2^24 (off32)
         const uint32_t b = (__umul24((uint32_t)lo[2], V.plane32) + __umul24((uint32_t)lo[1], (uint32_t)V.N0)) +
                            (uint32_t)lo[0];
         const uint32_t dx = (uint32_t)(hi[0] - lo[0]);
@@ -2288,6 +2293,7 @@ struct FastSample {
     bool inside;    // 0 <= b_k <= N_k - 2 on every axis
 };
 
+template <int DP = -1>
 __device__ __forceinline__ void sample_fast(const RaycastArgs &a, float p0, float p1, float p2, uint32_t plane32,
                                             FastSample &q)
 {
@@ -2296,7 +2302,7 @@ __device__ __forceinline__ void sample_fast(const RaycastArgs &a, float p0, floa
     uint32_t bi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float sk = voxel_coord(a, p[k], k);
+        const float sk = voxel_coord<DP>(a, p[k], k);
         const float fl = floorf(sk);
         q.b[k] = floor_index(fl);
         q.f[k] = sk - fl;
@@ -2306,16 +2312,16 @@ __device__ __forceinline__ void sample_fast(const RaycastArgs &a, float p0, floa
     }
     q.inside = in;
     // 24-bit products: the base is used only when the sample is inside, where
-    // b_k < N_k <= 2048 and plane32 = N0 N1 < 2^24
+    // b_k < N_k, and only with off32, where plane32 = N0 N1 < 2^24
     q.base = (__umul24(bi[2], plane32) + __umul24(bi[1], (uint32_t)a.N[0])) + bi[0];
 }
 
 
-template <class S, bool TOUCH>
+template <class S, bool TOUCH, int DP = -1>
 __device__ __forceinline__ float interp_vol(const VolView<S, TOUCH> &V, const RaycastArgs &a, float p0, float p1, float p2)
 {
     Sample sm;
-    sample_setup(a, p0, p1, p2, sm);
+    sample_setup<DP>(a, p0, p1, p2, sm);
     return sample_eval(V, sm);
 }
 
@@ -2443,7 +2449,7 @@ __device__ __forceinline__ void raycast_view(const RaycastArgs &a, const DevStat
 }
 
 // The ray of pixel (x, y): march, hit refinement, normal, map writes.
-template <class S, bool TOUCH>
+template <class S, bool TOUCH, int DP>
 __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float (&view)[16], int x, int y)
 {
     const VolView<S, TOUCH> V(a, a.touch);
@@ -2478,7 +2484,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
     unsigned n_eval = 0, n_skip = 0, n_brick = 0;  // statistics mode only
     if (tnear < tfar) {
         float t = tnear, stepsize = a.large_step;
-        float f_t = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
+        float f_t = interp_vol<S, TOUCH, DP>(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
         float f_tt = 0.0f;
         if (f_t >= 0.0f) {
             // Empty-space skipping.  A sample whose lower trilinear corner lies
@@ -2503,7 +2509,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
             for (; t < tfar && guard < kMaxMarchSteps; ++guard) {
                 const bool can_skip = a.skip && stepsize == a.step;
                 Sample sm;
-                sample_setup(a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t, sm);
+                sample_setup<DP>(a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t, sm);
                 if (can_skip && sm.valid) {
                     // two-level occupancy: 32^3 coarse cells, then 8^3 bricks
                     uint8_t cf = cf_n, ff = ff_n;
@@ -2573,11 +2579,12 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
                         fq[0].f[k] = sm.f[k];
                         fq[0].inside = fq[0].inside && sm.lo[k] >= 0 && sm.hi[k] == sm.lo[k] + 1;
                     }
-                    fq[0].base = ((uint32_t)sm.lo[2] * V.plane32 + (uint32_t)sm.lo[1] * (uint32_t)a.N[0]) +
+                    // 24-bit products, as in sample_fast (used only when inside)
+                    fq[0].base = (__umul24((uint32_t)sm.lo[2], V.plane32) + __umul24((uint32_t)sm.lo[1], (uint32_t)a.N[0])) +
                                  (uint32_t)sm.lo[0];
 #pragma unroll
                     for (int j = 1; j < kBatch; ++j)
-                        sample_fast(a, o[0] + d[0] * tj[j], o[1] + d[1] * tj[j], o[2] + d[2] * tj[j], V.plane32,
+                        sample_fast<DP>(a, o[0] + d[0] * tj[j], o[1] + d[1] * tj[j], o[2] + d[2] * tj[j], V.plane32,
                                     fq[j]);
 #pragma unroll
                     for (int j = 0; j < kBatch; ++j) all_in = all_in && fq[j].inside;
@@ -2609,7 +2616,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
 #pragma unroll
                     for (int j = 1; j < kBatch; ++j) {
                         Sample sj;
-                        sample_setup(a, o[0] + d[0] * tj[j], o[1] + d[1] * tj[j], o[2] + d[2] * tj[j], sj);
+                        sample_setup<DP>(a, o[0] + d[0] * tj[j], o[1] + d[1] * tj[j], o[2] + d[2] * tj[j], sj);
                         fv[j] = sample_eval(V, sj);
                     }
                 }
@@ -2619,7 +2626,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
                     // voxel of the volume (checked all the same); the corner
                     // index is floor(s) clamped at 0, as in sample_setup
                     FastSample fn;
-                    sample_fast(a, o[0] + d[0] * t_next, o[1] + d[1] * t_next, o[2] + d[2] * t_next, V.plane32, fn);
+                    sample_fast<DP>(a, o[0] + d[0] * t_next, o[1] + d[1] * t_next, o[2] + d[2] * t_next, V.plane32, fn);
                     const bool valid = fn.b[0] != INT_MIN && fn.b[1] != INT_MIN && fn.b[2] != INT_MIN;
                     const int b0 = max(fn.b[0], 0) >> 3, b1 = max(fn.b[1], 0) >> 3, b2 = max(fn.b[2], 0) >> 3;
                     if (valid && b0 < a.NB[0] && b1 < a.NB[1] && b2 < a.NB[2]) {
@@ -2673,7 +2680,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
                 guard += kBatch - 1;
             }
             if (f_tt < 0.0f) {
-                if (!have_prev) f_t = interp_vol(V, a, o[0] + d[0] * t_prev, o[1] + d[1] * t_prev, o[2] + d[2] * t_prev);
+                if (!have_prev) f_t = interp_vol<S, TOUCH, DP>(V, a, o[0] + d[0] * t_prev, o[1] + d[1] * t_prev, o[2] + d[2] * t_prev);
                 t = t + stepsize * f_tt / (f_t - f_tt);
                 hp[0] = o[0] + d[0] * t;
                 hp[1] = o[1] + d[1] * t;
@@ -2921,7 +2928,7 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
 // One thread per pixel; 16x16 pixel tiles per block.  Block ids are remapped
 // so that the blocks an XCD receives (id mod 8) cover one contiguous band of
 // tiles, keeping the voxels neighbouring rays share in that XCD's L2.
-template <class S, bool TOUCH = false>
+template <class S, bool TOUCH = false, int DP = -1>
 __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevState *__restrict__ st, int blk)
 {
     // block b runs on XCD b % 8; XCD x receives the clusters of xcd_group
@@ -2936,13 +2943,13 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
     if (x >= a.w || y >= a.h) return;
     float view[16];
     raycast_view(a, st, view);
-    raycast_pixel<S, TOUCH>(a, view, x, y);
+    raycast_pixel<S, TOUCH, DP>(a, view, x, y);
 }
 
-template <class S>
+template <class S, int DP>
 __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *__restrict__ st)
 {
-    raycast_body<S>(a, st, blockIdx.x);
+    raycast_body<S, false, DP>(a, st, blockIdx.x);
 }
 
 // k_raycast with every voxel and flag byte read marked in a.touch (counts the
@@ -3053,13 +3060,13 @@ __global__ __launch_bounds__(256) void k_batch_dilate(BatchTable tb, const Batch
 
 // blocks_per_session is a multiple of 8, so a block's XCD (global id mod 8)
 // equals its session-local id mod 8 and the per-session tile clustering holds
-template <class S>
+template <class S, int DP>
 __global__ __launch_bounds__(256) void k_batch_raycast(BatchTable tb, const BatchTick *__restrict__ tp, int per)
 {
     const int i = blockIdx.x / per;
     if (!batch_active(tp, i)) return;
     const RaycastArgs a = tb.sess[i].ray;
-    raycast_body<S>(a, tb.sess[i].st, blockIdx.x % per);
+    raycast_body<S, false, DP>(a, tb.sess[i].st, blockIdx.x % per);
 }
 
 template <class S>
@@ -3190,7 +3197,7 @@ hipError_t launch_batch_bricks(const BatchTable &tb, const BatchTick *tp, int to
 }
 
 hipError_t launch_batch_raycast(const BatchTable &tb, const BatchTick *tp, int storage, int lanes, int per,
-                                hipStream_t s)
+                                int dim_pow2, hipStream_t s)
 {
     const dim3 g(tb.n * per), b(256);
     if (lanes == 4) {
@@ -3202,12 +3209,18 @@ hipError_t launch_batch_raycast(const BatchTable &tb, const BatchTick *tp, int s
         }
         return hipGetLastError();
     }
+    // dim_pow2: every member's dims are powers of two (else the kernel
+    // reads each member's flag)
+#define KF_BRC(ST)                                                                            \
+    if (dim_pow2) hipLaunchKernelGGL((k_batch_raycast<ST, 1>), g, b, 0, s, tb, tp, per);     \
+    else hipLaunchKernelGGL((k_batch_raycast<ST, -1>), g, b, 0, s, tb, tp, per);
     switch (storage) {
-    case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_batch_raycast<StoreRG16I>, g, b, 0, s, tb, tp, per); break;
-    case KF_STORAGE_F32: hipLaunchKernelGGL(k_batch_raycast<StoreF32>, g, b, 0, s, tb, tp, per); break;
-    case KF_STORAGE_F16: hipLaunchKernelGGL(k_batch_raycast<StoreF16>, g, b, 0, s, tb, tp, per); break;
+    case KF_STORAGE_RG16I: KF_BRC(StoreRG16I); break;
+    case KF_STORAGE_F32: KF_BRC(StoreF32); break;
+    case KF_STORAGE_F16: KF_BRC(StoreF16); break;
     default: return hipErrorInvalidValue;
     }
+#undef KF_BRC
     return hipGetLastError();
 }
 
@@ -3368,12 +3381,16 @@ hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st,
         }
         return hipGetLastError();
     }
+#define KF_RC(ST)                                                                  \
+    if (a.dim_pow2) launch_timed(k_raycast<ST, 1>, g, b, s, t0, t1, a, st);        \
+    else launch_timed(k_raycast<ST, 0>, g, b, s, t0, t1, a, st);
     switch (storage) {
-    case KF_STORAGE_RG16I: launch_timed(k_raycast<StoreRG16I>, g, b, s, t0, t1, a, st); break;
-    case KF_STORAGE_F32: launch_timed(k_raycast<StoreF32>, g, b, s, t0, t1, a, st); break;
-    case KF_STORAGE_F16: launch_timed(k_raycast<StoreF16>, g, b, s, t0, t1, a, st); break;
+    case KF_STORAGE_RG16I: KF_RC(StoreRG16I); break;
+    case KF_STORAGE_F32: KF_RC(StoreF32); break;
+    case KF_STORAGE_F16: KF_RC(StoreF16); break;
     default: return hipErrorInvalidValue;
     }
+#undef KF_RC
     return hipGetLastError();
 }
 
