@@ -458,7 +458,7 @@ def main():
         ray_bytes = rv * 4 + rf + 424 * 512 * 32
         dom = max(kern, key=kern.get)
         rooflines = {
-            "raycast": roofline("k_raycast<StoreF32>", ray_bytes, kern["raycast"], traffic),
+            "raycast": roofline("k_raycast<StoreF32, 1>", ray_bytes, kern["raycast"], traffic),
             "integrate": roofline("k_integrate_fast<StoreF32, true>", int_bytes, kern["integrate"], traffic),
         }
         rooflines["integrate"]["reference_semantics_bytes"] = int(int_ref_bytes)

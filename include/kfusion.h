@@ -329,6 +329,17 @@ int kf_raycast_stats(kf_session *s, uint64_t stats[6]);
  * at 65535. */
 int kf_raycast_ray_counts(kf_session *s, uint32_t *counts_host);
 
+/* Runs raycast() once more recording, for every wavefront, when it started and
+ * ended and where it ran: records_host[8 * k .. 8 * k + 7] for wavefront k =
+ * 4 * block + wave = {start (low, high 32 bits), end (low, high) of the
+ * 100 MHz GPU clock, the HW_ID register (SIMD, CU, shader engine), the XCC
+ * id, the pixel tile of the block, 0xABCD}; wavefronts that hold no pixel
+ * leave their record zero.  *n_records receives the number of records
+ * (4 x blocks); records_host must hold 8 x that many words (at most
+ * 8 x 4 x 4096).  Results identical to raycast().  No counterpart in the
+ * reference (a measurement aid). */
+int kf_raycast_wave_trace(kf_session *s, uint32_t *records_host, int *n_records);
+
 /* Algorithmic traffic of raycast() for the current volume and pose: the
  * number of distinct voxels whose value the march and the normals read, and
  * of distinct occupancy-flag bytes it read (no timing; the maps written are

@@ -142,6 +142,7 @@ def lib():
         "kf_read_stage_times": [sess, C.POINTER(C.c_float)],
         "kf_raycast_stats": [sess, C.POINTER(C.c_uint64)],
         "kf_raycast_ray_counts": [sess, C.POINTER(C.c_uint32)],
+        "kf_raycast_wave_trace": [sess, C.POINTER(C.c_uint32), C.POINTER(C.c_int)],
         "kf_count_raycast_traffic": [sess, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
         "kf_track_sdf": [sess, C.POINTER(C.c_int)],
         "kf_recover_pose": [sess, C.POINTER(C.c_int)],

@@ -1240,6 +1240,26 @@ int kf_raycast_ray_counts(kf_session *s, uint32_t *counts_host)
     return KF_OK;
 }
 
+int kf_raycast_wave_trace(kf_session *s, uint32_t *records_host, int *n_records)
+{
+    if (!s || !records_host || !n_records) return fail(KF_ERR_INVALID, "NULL argument");
+    kf::RaycastArgs a = raycast_args(s, 1);
+    const size_t n = (size_t)a.blocks * 4;
+    if (n > 4 * 4096) return fail(KF_ERR_INVALID, "too many wavefronts for the trace");
+    uint32_t *d = nullptr;
+    KF_HIP(hipMalloc(&d, n * 8 * sizeof(uint32_t)));
+    a.wave_trace = d;
+    hipError_t e = hipMemsetAsync(d, 0, n * 8 * sizeof(uint32_t), s->stream);
+    if (e == hipSuccess) e = kf::launch_raycast(a, s->cfg.storage, s->st, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(records_host, d, n * 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    const hipError_t ef = hipFree(d);
+    KF_HIP(e);
+    KF_HIP(ef);
+    *n_records = (int)n;
+    return KF_OK;
+}
+
 int kf_count_raycast_traffic(kf_session *s, uint64_t *voxels_read, uint64_t *flag_bytes_read)
 {
     if (!s) return fail(KF_ERR_INVALID, "session is NULL");

@@ -106,6 +106,7 @@ struct RaycastArgs {
     int lanes;                  // lanes per ray: 1 (16x16-pixel tiles) or 4 (8x8-pixel tiles)
     uint32_t *ray_counts;       // optional (with stats): per pixel, evaluated samples | skip events << 16
     uint32_t *touch;            // k_raycast_touch: bitmap of voxels, then flag bytes, read
+    uint32_t *wave_trace;       // optional: 8 words per wavefront (clock at start and end, HW_ID, XCC id, tile)
 };
 
 // Point-to-volume tracking (persistent k_sdf_track, one launch per TrackSDF /

@@ -2931,8 +2931,10 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
 template <class S, bool TOUCH = false, int DP = -1>
 __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevState *__restrict__ st, int blk)
 {
-    // block b runs on XCD b % 8; XCD x receives the clusters of xcd_group
-    // consecutive tiles x, x + 8, x + 16, ... (1 = fully interleaved)
+    // blocks are dealt to the 8 XCDs round-robin (from a starting XCD that
+    // varies between launches), so blocks whose numbers agree mod 8 share an
+    // XCD; residue x receives the clusters of xcd_group consecutive tiles x,
+    // x + 8, x + 16, ... (1 = fully interleaved)
     const int nb = a.tiles_x * a.tiles_y;
     const int G = a.xcd_group;
     const int j = blk / 8;
@@ -2941,9 +2943,24 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
     const int x = (tile % a.tiles_x) * 16 + (threadIdx.x & 15);
     const int y = (tile / a.tiles_x) * 16 + (threadIdx.x >> 4);
     if (x >= a.w || y >= a.h) return;
+    const unsigned long long wt0 = a.wave_trace ? __builtin_amdgcn_s_memrealtime() : 0ull;
     float view[16];
     raycast_view(a, st, view);
     raycast_pixel<S, TOUCH, DP>(a, view, x, y);
+    if (a.wave_trace) {
+        // diagnostic: where and how long this wavefront ran (lanes 0-7 of
+        // each wavefront write its record with vector stores)
+        const unsigned long long wt1 = __builtin_amdgcn_s_memrealtime();
+        unsigned hw, xcc;
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)" : "=s"(hw));
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+        const int l = (int)threadIdx.x & 63;
+        uint32_t *rec = a.wave_trace + ((size_t)blk * 4 + (threadIdx.x >> 6)) * 8;
+        const uint32_t v = l == 0 ? (uint32_t)wt0 : l == 1 ? (uint32_t)(wt0 >> 32) : l == 2 ? (uint32_t)wt1
+                         : l == 3 ? (uint32_t)(wt1 >> 32) : l == 4 ? hw : l == 5 ? xcc : l == 6 ? (uint32_t)tile
+                                                                                                    : 0xABCDu;
+        if (l < 8) rec[l] = v;
+    }
 }
 
 template <class S, int DP>
@@ -3058,8 +3075,9 @@ __global__ __launch_bounds__(256) void k_batch_dilate(BatchTable tb, const Batch
     brick_dilate_body(a, b - tb.coa_prefix[i]);
 }
 
-// blocks_per_session is a multiple of 8, so a block's XCD (global id mod 8)
-// equals its session-local id mod 8 and the per-session tile clustering holds
+// blocks_per_session is a multiple of 8, so a block's global and
+// session-local numbers agree mod 8 (the residue that decides its XCD) and
+// the per-session tile clustering holds
 template <class S, int DP>
 __global__ __launch_bounds__(256) void k_batch_raycast(BatchTable tb, const BatchTick *__restrict__ tp, int per)
 {

@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--xcd-group", type=int, default=-1, help="raycast XCD cluster (-1: the tuned default)")
     ap.add_argument("--sessions", type=int, default=1, help="> 1: a saturated batch of this many sessions")
     ap.add_argument("--time", type=int, default=0, help="time raycast per frame with events and print the mean")
+    ap.add_argument("--wave-trace", type=str, default="",
+                    help="comma-separated frames after which one more raycast is traced per wavefront and reported")
     args = ap.parse_args()
 
     import torch
@@ -59,8 +61,11 @@ def main():
         g.setOption(K.OPT_RAYCAST_XCD_GROUP, args.xcd_group)
     g.Reset(poses[0])
     fb = depth[0].nbytes
+    traced = set(int(f) for f in args.wave_trace.split(",") if f.strip())
     for i in range(args.frames):
         K.check(L.kf_frame_u16_dev(g.session, C.c_void_p(dev.data_ptr() + i * fb)))
+        if i in traced:
+            wave_trace_report(g, i)
         if args.stats and i % args.stats == args.stats - 1:
             st = g.raycastStats()
             vr, vw = g.countIntegrateTraffic()
@@ -103,6 +108,48 @@ def main():
             print("icp rows_per_block %d: %.1f us per Track (%d iterations)"
                   % (rpb, e0.elapsed_time(e1) * 1000 / args.time, g.status()["iterations"]), flush=True)
     copy_calibration(torch, args.copy_mib)
+
+
+def wave_trace_report(g, frame):
+    """One more raycast traced per wavefront (kf_raycast_wave_trace) and the
+    per-ray sample counts of the same state (kf_raycast_ray_counts): how long
+    each wavefront ran against its loop iterations (the largest over its rays
+    of fine-step batches of four samples + empty cells entered), how many
+    wavefronts shared each SIMD, and for the slowest wavefronts the time they
+    ran alone on their SIMD once the others had ended."""
+    tr = g.raycastWaveTrace()
+    ev, cells = g.raycastRayCounts()
+    h, w = ev.shape
+    hp, wp = (h + 15) // 16 * 16, (w + 15) // 16 * 16
+    it = np.zeros((hp, wp))
+    it[:h, :w] = np.ceil(ev / 4.0) + cells
+    tiles_x = wp // 16
+    ty, tx = tr["tile"] // tiles_x, tr["tile"] % tiles_x
+    iters = np.array([it[ty[k] * 16 + tr["wave"][k] * 4:ty[k] * 16 + tr["wave"][k] * 4 + 4,
+                         tx[k] * 16:tx[k] * 16 + 16].max() for k in range(len(tr))])
+    t0 = (tr["start"] - tr["start"].min()).astype(np.float64) * 0.01  # us (100 MHz)
+    t1 = (tr["end"] - tr["start"].min()).astype(np.float64) * 0.01
+    dur = t1 - t0
+    hw = tr["hw_id"].astype(np.int64)
+    simd = ((tr["xcc"].astype(np.int64) * 8 + ((hw >> 13) & 7)) * 2 + ((hw >> 12) & 1)) * 16 + ((hw >> 8) & 15)
+    simd = simd * 4 + ((hw >> 4) & 3)
+    _, inv, cnt = np.unique(simd, return_inverse=True, return_counts=True)
+    A = np.vstack([iters, np.ones_like(iters)]).T
+    slope, icpt = np.linalg.lstsq(A, dur, rcond=None)[0]
+    print("wave trace frame %d: %d wavefronts on %d SIMDs (wavefronts per SIMD: %s), kernel span %.1f us, "
+          "starts within %.1f us; duration p50 %.1f p99 %.1f max %.1f us; iterations p50 %d p99 %d max %d; "
+          "fit duration = %.2f us x iterations %+.1f us"
+          % (frame, len(tr), len(cnt), {int(a): int(b) for a, b in zip(*np.unique(cnt, return_counts=True))}, t1.max(), t0.max(),
+             np.percentile(dur, 50), np.percentile(dur, 99), dur.max(), np.percentile(iters, 50),
+             np.percentile(iters, 99), iters.max(), slope, icpt), flush=True)
+    for k in np.argsort(-dur)[:6]:
+        mates = np.nonzero((inv == inv[k]) & (np.arange(len(tr)) != k))[0]
+        last = t1[mates].max() if len(mates) else t0[k]
+        alone = max(0.0, t1[k] - max(last, t0[k]))
+        print("  slowest: tile %d wave %d, %d iterations in %.1f us; %d other wavefronts on its SIMD ended by "
+              "%.1f us; alone for the last %.1f us (%.2f us per iteration over the whole run)"
+              % (tr["tile"][k], tr["wave"][k], iters[k], dur[k], len(mates), last, alone,
+                 dur[k] / max(iters[k], 1)), flush=True)
 
 
 def copy_calibration(torch, mib):

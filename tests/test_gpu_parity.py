@@ -508,6 +508,35 @@ def test_raycast_traffic_and_ray_counts():
     assert vox1 >= len(distinct)
 
 
+def test_raycast_wave_trace():
+    """The per-wavefront trace leaves the maps as raycast() makes them and
+    holds one record per wavefront with pixels: the start before the end,
+    every wavefront started within the launch, a tile per block, and blocks
+    whose numbers agree mod 8 on the same XCC (round-robin dispatch from a
+    starting XCC that varies between launches: the order the raycast's tile
+    clustering relies on for L2 locality)."""
+    print("case: raycast wave trace f32 128^3", flush=True)
+    depth, gt = synth.sequence(0, 4)
+    g, cfg = make(volume=128, storage="f32")
+    g.Reset(gt[0])
+    for f in range(4):
+        g.frame(depth[f].astype(np.float32))
+    g.raycast()
+    v0, n0 = g.getVerts(), g.getNorms()
+    tr = g.raycastWaveTrace()
+    np.testing.assert_array_equal(g.getVerts(), v0)
+    np.testing.assert_array_equal(g.getNorms(), n0)
+    tiles_x, rows = 512 // 16, 424 // 4  # 16 x 4-pixel wavefronts, 106 rows of them
+    assert len(tr) == tiles_x * rows
+    assert (tr["end"] >= tr["start"]).all()
+    span = int(tr["end"].max() - tr["start"].min())
+    assert 0 < span < 100_000_000  # under a second of 100 MHz ticks
+    assert len(np.unique(tr["tile"])) == tiles_x * ((424 + 15) // 16)
+    if len(np.unique(tr["xcc"])) == 8:  # eight XCDs in one partition
+        rot = (tr["xcc"].astype(np.int64) - tr["block"]) % 8
+        assert (rot == rot[0]).all()
+
+
 def test_integrate_fast_traffic_counts():
     """The counting walk of the default integrate stores nothing; the voxels
     it counts as read / written are within the reference semantics' (the same
