@@ -8,8 +8,11 @@ result the same bits, so the choice affects time only:
   integrate_zlen       z voxels one integrate thread walks (voxels per
                        thread; 0 = enough segments for about 2M per launch)
   integrate_rows       y rows of an integrate block (64 x rows columns)
-  raycast_xcd_group    raycast tiles per XCD cluster (block b runs on XCD
-                       b % 8 and takes that many consecutive tiles)
+  integrate_columns    per-column set-up once per frame in its own launch
+                       (1) or in every segment's thread (0)
+  raycast_xcd_group    raycast tiles per XCD cluster (blocks whose numbers
+                       agree mod 8 run on one XCD and take that many
+                       consecutive tiles)
   raycast_lanes        lanes per ray (1: 16x16-ray tiles, 4: 8x8-ray tiles)
   brick_rebuild_period frames between exact rebuilds of the brick flags
 
@@ -46,12 +49,13 @@ from . import _lib as K  # noqa: E402
 KNOBS = [
     ("integrate_zlen", K.OPT_INTEGRATE_ZLEN, "integrate"),
     ("integrate_rows", K.OPT_INTEGRATE_ROWS, "integrate"),
+    ("integrate_columns", K.OPT_INTEGRATE_COLUMNS, "integrate"),
     ("raycast_xcd_group", K.OPT_RAYCAST_XCD_GROUP, "raycast"),
     ("raycast_lanes", K.OPT_RAYCAST_LANES, "raycast"),
     ("brick_rebuild_period", K.OPT_BRICK_REBUILD_PERIOD, "frame"),
 ]
-DEFAULTS = {"integrate_zlen": 0, "integrate_rows": 4, "raycast_xcd_group": 4, "raycast_lanes": 1,
-            "brick_rebuild_period": 16}
+DEFAULTS = {"integrate_zlen": 0, "integrate_rows": 4, "integrate_columns": 1, "raycast_xcd_group": 4,
+            "raycast_lanes": 1, "brick_rebuild_period": 16}
 STORAGE_CODE = {"f32": 1, "f16": 2}
 FRAME_WINDOW = 128  # frames timed for the rebuild period (a multiple of 8, 16, 32 and 64)
 
@@ -63,6 +67,8 @@ def candidates(knob, volume):
         return [0] + [z for z in (16, 32, 64, 128, 256, 512) if z <= volume]
     if knob == "integrate_rows":
         return [1, 2, 4]
+    if knob == "integrate_columns":
+        return [0, 1]
     if knob == "raycast_xcd_group":
         return [1, 2, 4, 8, 16]
     if knob == "raycast_lanes":
@@ -87,13 +93,14 @@ def write_inc(rows, path):
              "// oglfusionkinectv2_amd/autotune.py from measurements on MI355X (gfx950).",
              "// {storage, volume, integrate z voxels per thread (0 = automatic), integrate",
              "//  block rows, raycast tiles per XCD cluster, raycast lanes per ray, brick",
-             "//  rebuild period}"]
+             "//  rebuild period, integrate per-column set-up launch}"]
     for r in rows:
         c = r["chosen"]
-        lines.append("    {%d, %d, %d, %d, %d, %d, %d},  // %s %d^3: integrate %.1f us, raycast %.1f us"
+        lines.append("    {%d, %d, %d, %d, %d, %d, %d, %d},  // %s %d^3: integrate %.1f us, raycast %.1f us"
                      % (STORAGE_CODE[r["storage"]], r["volume"], c["integrate_zlen"], c["integrate_rows"],
-                        c["raycast_xcd_group"], c["raycast_lanes"], c["brick_rebuild_period"], r["storage"],
-                        r["volume"], 1e3 * r["stage_ms"]["integrate"], 1e3 * r["stage_ms"]["raycast"]))
+                        c["raycast_xcd_group"], c["raycast_lanes"], c["brick_rebuild_period"],
+                        c["integrate_columns"], r["storage"], r["volume"], 1e3 * r["stage_ms"]["integrate"],
+                        1e3 * r["stage_ms"]["raycast"]))
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -109,7 +116,8 @@ def read_inc(path):
             vals = [int(v) for v in s[1:s.index("}")].split(",")]
             st = {v: k for k, v in STORAGE_CODE.items()}[vals[0]]
             out.append(dict(storage=st, volume=vals[1], integrate_zlen=vals[2], integrate_rows=vals[3],
-                            raycast_xcd_group=vals[4], raycast_lanes=vals[5], brick_rebuild_period=vals[6]))
+                            raycast_xcd_group=vals[4], raycast_lanes=vals[5], brick_rebuild_period=vals[6],
+                            integrate_columns=vals[7] if len(vals) > 7 else 1))
     return out
 
 
