@@ -411,6 +411,10 @@ def main():
     legs = {}
     kr = run_leg(legs, "kernels", lambda: run_kernels(torch, L, g, frames_dev, poses, args.warmup, args.steps))
     kern, counts, ray_counts, acc = (kr if isinstance(kr, tuple) else ({}, [], [], {}))
+    # the raycast kernel the frame loop launches: the hybrid schedule (heavy
+    # tiles four lanes per ray) when on, else the one-lane march
+    ray_kernel = ("k_raycast_hybrid<StoreF32, 1>" if g.getOption(K.OPT_RAYCAST_HYBRID)
+                  and g.getOption(K.OPT_RAYCAST_LANES) == 1 else "k_raycast<StoreF32, 1>")
     run_leg(legs, "beside_the_loop", lambda: row_timings(torch, L, g, frames_dev, nframes))
     g.close()
     depth_f32 = [depth[i].astype(np.float32) for i in range(nframes)]
@@ -458,7 +462,7 @@ def main():
         ray_bytes = rv * 4 + rf + 424 * 512 * 32
         dom = max(kern, key=kern.get)
         rooflines = {
-            "raycast": roofline("k_raycast<StoreF32, 1>", ray_bytes, kern["raycast"], traffic),
+            "raycast": roofline(ray_kernel, ray_bytes, kern["raycast"], traffic),
             "integrate": roofline("k_integrate_fast<StoreF32, true>", int_bytes, kern["integrate"], traffic),
         }
         rooflines["integrate"]["reference_semantics_bytes"] = int(int_ref_bytes)

@@ -284,6 +284,18 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         one thread per column, not in every z segment's
  *                         thread; 0: per thread.  Same bits either way. */
 #define KF_OPT_INTEGRATE_COLUMNS 20
+/*   KF_OPT_RAYCAST_HYBRID 1 (default): raycast() marches the tiles whose rays
+ *                         took the most loop iterations in the session's
+ *                         previous raycast with four lanes per ray and every
+ *                         other tile with one (one launch; the same bits as
+ *                         one lane everywhere); 0: one lane everywhere.
+ *                         Applies with KF_OPT_RAYCAST_LANES 1. */
+#define KF_OPT_RAYCAST_HYBRID 21
+/*   KF_OPT_RAYCAST_HEAVY_TILES  read-only (kf_get_option; synchronises the
+ *                         session stream): the number of 16x16 tiles the
+ *                         next hybrid raycast marches with four lanes per
+ *                         ray. */
+#define KF_OPT_RAYCAST_HEAVY_TILES 22
 /*   KF_OPT_TRACK_MODE     tracking of the frame loop (kinect.h:143-144):
  *                         KF_TRACK_POINT_TO_PLANE (Track, default),
  *                         KF_TRACK_POINT_TO_VOLUME (TrackSDF) or both (Track

@@ -52,6 +52,8 @@ OPT_INTEGRATE_ZLEN = 17
 OPT_INTEGRATE_ROWS = 18
 OPT_POSE_FP64 = 19
 OPT_INTEGRATE_COLUMNS = 20
+OPT_RAYCAST_HYBRID = 21
+OPT_RAYCAST_HEAVY_TILES = 22
 
 TRACK_POINT_TO_PLANE = 1
 TRACK_POINT_TO_VOLUME = 2

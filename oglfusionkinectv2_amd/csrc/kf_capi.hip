@@ -107,6 +107,7 @@ size_t session_bytes(const kf_config *c)
                       ((c->volume_size[2] + 7) / 8);
     b += 4 * nb;                                   // brick flags, dilated, zero, coarse (bound)
     b += (size_t)c->volume_size[0] * c->volume_size[1] * 24 + 16;  // integrate's per-column set-up
+    b += kf::ray_sched_bytes(((c->width + 15) / 16) * ((c->height + 15) / 16));  // hybrid raycast schedule
     return b;
 }
 
@@ -124,7 +125,7 @@ void free_all(kf_session *s)
         if (p) (void)hipFree(p);
     };
     drop_graph(s);
-    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags); f(s->cflags); f(s->zflags); f(s->dtiles); f(s->col_pos); f(s->col_range); f(s->col_dl);
+    f(s->depth0); f(s->depth_m); f(s->stage_u16); f(s->stage_f32); f(s->bflags); f(s->dflags); f(s->cflags); f(s->zflags); f(s->dtiles); f(s->col_pos); f(s->col_range); f(s->col_dl); f(s->ray_sched);
     for (int l = 0; l < 3; ++l) {
         f(s->vert[l]);
         f(s->norm[l]);
@@ -233,6 +234,15 @@ kf::RaycastArgs raycast_args(kf_session *s, int lanes = -1, int xcd_group = -1)
     a.tiles_y = (c.height + tile - 1) / tile;
     a.xcd_group = std::max(1, xcd_group < 0 ? s->xcd_group : xcd_group);
     a.blocks = (a.tiles_x * a.tiles_y + 8 * a.xcd_group - 1) / (8 * a.xcd_group) * (8 * a.xcd_group);
+    if (s->ray_hybrid && a.lanes == 1 && s->ray_sched) {
+        // tile costs, heavy list (count + tiles), flags, finished-block count
+        const int nt = a.tiles_x * a.tiles_y;
+        a.tile_cost = (uint32_t *)s->ray_sched;
+        a.heavy = (int *)(s->ray_sched + 4 * (size_t)nt);
+        a.blocks_done = (unsigned *)(s->ray_sched + 4 * ((size_t)nt + kf::kRayHeavyMax + 1));
+        a.heavy_flag = s->ray_sched + 4 * ((size_t)nt + kf::kRayHeavyMax + 2);
+        a.heavy_max = kf::kRayHeavyMax;
+    }
     // 32-bit byte offsets, and N0 N1 < 2^24 for the 24-bit index products
     a.off32 = s->voxels * voxel_plane_bytes(c.storage) <= (size_t)0xFFFFFFFFu &&
                       (size_t)c.volume_size[0] * c.volume_size[1] < ((size_t)1 << 24)
@@ -697,6 +707,7 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
               alloc((void **)&s->col_pos, (size_t)cfg->volume_size[0] * cfg->volume_size[1] * 16) &&
               alloc((void **)&s->col_range, (size_t)cfg->volume_size[0] * cfg->volume_size[1] * 8) &&
               alloc((void **)&s->col_dl, 16) &&
+              alloc((void **)&s->ray_sched, kf::ray_sched_bytes(((cfg->width + 15) / 16) * ((cfg->height + 15) / 16))) &&
               alloc((void **)&s->stage_u16, p0 * 2) &&
               alloc((void **)&s->stage_f32, p0 * 4) && alloc((void **)&s->refV, p0 * 16) &&
               alloc((void **)&s->refN, p0 * 16) && alloc(&s->tsdf, vb) && alloc(&s->weight, vb) &&
@@ -725,6 +736,8 @@ int kf_session_create(const kf_config *cfg, void *stream, kf_session **out)
              hipMemsetAsync(s->dtiles, 0, (size_t)((cfg->width + 7) / 8) * ((cfg->height + 7) / 8) * 8,
                             s->stream) == hipSuccess &&
              hipMemsetAsync(s->cflags, 0, s->ncoarse, s->stream) == hipSuccess &&
+             hipMemsetAsync(s->ray_sched, 0, kf::ray_sched_bytes(((cfg->width + 15) / 16) * ((cfg->height + 15) / 16)),
+                            s->stream) == hipSuccess &&
              hipMemsetAsync(s->st, 0, sizeof(kf::DevState), s->stream) == hipSuccess &&
              hipMemsetAsync(s->icp_rows, 0, 2 * (size_t)(cfg->height + 8) * 32 * 8, s->stream) == hipSuccess &&
              hipMemsetAsync(s->library, 0, (size_t)kf::kPoseLibraryMax * 64, s->stream) == hipSuccess &&
@@ -778,6 +791,9 @@ int kf_reset_volume(kf_session *s)
     KF_HIP(hipMemsetAsync(s->dflags, 0, s->nbricks, s->stream));
     KF_HIP(hipMemsetAsync(s->zflags, 0, s->nbricks, s->stream));
     KF_HIP(hipMemsetAsync(s->cflags, 0, s->ncoarse, s->stream));
+    // the raycast's heavy-tile schedule starts over with the volume
+    KF_HIP(hipMemsetAsync(s->ray_sched, 0,
+                          kf::ray_sched_bytes(((s->cfg.width + 15) / 16) * ((s->cfg.height + 15) / 16)), s->stream));
     return KF_OK;
 }
 
@@ -1025,6 +1041,8 @@ int kf_set_option(kf_session *s, int option, int value)
         break;
     case KF_OPT_POSE_FP64: s->pose_fp64 = value ? 1 : 0; break;
     case KF_OPT_INTEGRATE_COLUMNS: s->int_columns = value ? 1 : 0; break;
+    case KF_OPT_RAYCAST_HYBRID: s->ray_hybrid = value ? 1 : 0; break;
+    case KF_OPT_RAYCAST_HEAVY_TILES: return fail(KF_ERR_INVALID, "the heavy-tile count is read-only");
     case KF_OPT_ICP_REFERENCE_ORDER:
         if (value) {
             const int rc = alloc_track_records(s);
@@ -1067,6 +1085,15 @@ int kf_get_option(kf_session *s, int option, int *value)
     case KF_OPT_INTEGRATE_ROWS: *value = s->int_rows; break;
     case KF_OPT_POSE_FP64: *value = s->pose_fp64; break;
     case KF_OPT_INTEGRATE_COLUMNS: *value = s->int_columns; break;
+    case KF_OPT_RAYCAST_HYBRID: *value = s->ray_hybrid; break;
+    case KF_OPT_RAYCAST_HEAVY_TILES: {
+        const int nt = ((s->cfg.width + 15) / 16) * ((s->cfg.height + 15) / 16);
+        int h = 0;
+        KF_HIP(hipMemcpyAsync(&h, s->ray_sched + 4 * (size_t)nt, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        KF_HIP(hipStreamSynchronize(s->stream));
+        *value = h;
+        break;
+    }
     default: return fail(KF_ERR_INVALID, "unknown option");
     }
     return KF_OK;

@@ -107,7 +107,25 @@ struct RaycastArgs {
     uint32_t *ray_counts;       // optional (with stats): per pixel, evaluated samples | skip events << 16
     uint32_t *touch;            // k_raycast_touch: bitmap of voxels, then flag bytes, read
     uint32_t *wave_trace;       // optional: 8 words per wavefront (clock at start and end, HW_ID, XCC id, tile)
+    // hybrid schedule (k_raycast_hybrid; heavy_max 0: off): per 16x16 tile
+    // the largest one-lane loop iteration count of the last launch, the
+    // tiles this launch marches with four lanes per ray (heavy[0] = count,
+    // heavy[1..] = tile numbers) and their flags, and the launch's count of
+    // finished blocks
+    uint32_t *tile_cost;
+    int *heavy;
+    uint8_t *heavy_flag;
+    unsigned *blocks_done;
+    int heavy_max;              // tiles marched four lanes per ray at most (a multiple of 8)
 };
+
+constexpr int kRayHeavyMax = 48;   // hybrid raycast: tiles per launch marched four lanes per ray
+constexpr int kRayHeavyFloor = 16; // ... only tiles whose last march took at least this many iterations
+// block b < 4 kRayHeavyMax of a hybrid launch takes quarter (b / 8) % 4 of
+// heavy tile (b / 32) 8 + b % 8: every heavy tile once when this holds
+static_assert(kRayHeavyMax % 8 == 0, "heavy tiles come in groups of 8 (one per XCD)");
+// bytes of a session's hybrid-schedule state for `tiles` 16x16 tiles
+inline size_t ray_sched_bytes(int tiles) { return 4 * ((size_t)tiles + kRayHeavyMax + 2) + (size_t)tiles; }
 
 // Point-to-volume tracking (persistent k_sdf_track, one launch per TrackSDF /
 // recoverPose).

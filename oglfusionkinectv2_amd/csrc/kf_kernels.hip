@@ -2449,8 +2449,10 @@ __device__ __forceinline__ void raycast_view(const RaycastArgs &a, const DevStat
 }
 
 // The ray of pixel (x, y): march, hit refinement, normal, map writes.
+// Returns the ray's loop iterations counted as empty cells entered +
+// fine-step batches of four samples.
 template <class S, bool TOUCH, int DP>
-__device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float (&view)[16], int x, int y)
+__device__ __forceinline__ unsigned raycast_pixel(const RaycastArgs &a, const float (&view)[16], int x, int y)
 {
     const VolView<S, TOUCH> V(a, a.touch);
 
@@ -2713,6 +2715,7 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
         atomicMax(&a.stats[5], (unsigned long long)(n_eval + n_brick));
         if (a.ray_counts) a.ray_counts[i] = min(n_eval, 0xffffu) | (min(n_brick, 0xffffu) << 16);
     }
+    return n_brick + (n_eval + 3u) / 4u;
 }
 
 // The same march with four lanes per ray (8x8-pixel tiles, 64 rays per
@@ -2726,28 +2729,15 @@ __device__ __forceinline__ void raycast_pixel(const RaycastArgs &a, const float 
 constexpr int kCoopB = 4;                 // samples per lane per round
 constexpr int kCoopN = 4 * kCoopB;        // consecutive samples per round
 
-template <class S>
-__device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const DevState *__restrict__ st, int blk)
+// The march of the ray of pixel (x, y) by a group of four lanes (lane q of
+// the group, whose first lane in the wavefront is gbase); the group's lane 0
+// writes the maps.  Returns the ray's iterations as the one-lane march counts
+// them (empty cells entered + fine-step batches of four samples), the same
+// on all four lanes.
+template <class S, int DP = -1>
+__device__ __forceinline__ unsigned raycast_coop_ray(const RaycastArgs &a, const float (&view)[16], int x, int y,
+                                                     int q, int gbase)
 {
-    const int nb = a.tiles_x * a.tiles_y;
-    const int G = a.xcd_group;
-    const int j = blk / 8;
-    const int tile = ((blk % 8) + 8 * (j / G)) * G + (j % G);
-    if (tile >= nb) return;
-    const int q = threadIdx.x & 3, r = threadIdx.x >> 2;
-    const int gbase = (int)(threadIdx.x & 63) & ~3;  // the group's first lane in the wavefront
-    const int x = (tile % a.tiles_x) * 8 + (r & 7);
-    const int y = (tile / a.tiles_x) * 8 + (r >> 3);
-    if (x >= a.w || y >= a.h) return;  // the whole group
-
-    float view[16];
-    {
-        float iK[16], P[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) P[i] = st->pose[i];
-        inverse_camera_matrix(a.cam, iK);
-        mul4(P, iK, view);  // m_pose * invK (gFusion.cpp:1298)
-    }
     const VolView<S> V(a);
 
     const float o[3] = {view[12], view[13], view[14]};
@@ -2772,7 +2762,7 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
     unsigned n_eval = 0, n_skip = 0, n_brick = 0;  // statistics mode only
     if (tnear < tfar) {
         float t = tnear, stepsize = a.large_step;
-        float f_t = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
+        float f_t = interp_vol<S, false, DP>(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
         float f_tt = 0.0f;
         if (f_t >= 0.0f) {
             const float ds[3] = {d[0] * (float)a.N[0] / a.dim[0], d[1] * (float)a.N[1] / a.dim[1],
@@ -2787,7 +2777,7 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
             for (; t < tfar && guard < kMaxMarchSteps; ++guard) {
                 const bool can_skip = a.skip && stepsize == a.step;
                 Sample sm;
-                sample_setup(a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t, sm);
+                sample_setup<DP>(a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t, sm);
                 if (can_skip && sm.valid) {
                     // empty-space skipping exactly as the one-lane march
                     const int b0 = sm.lo[0] >> 3, b1 = sm.lo[1] >> 3, b2 = sm.lo[2] >> 3;
@@ -2848,7 +2838,7 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
 #pragma unroll
                 for (int b = 0; b < kCoopB; ++b) {
                     Sample sb;
-                    sample_setup(a, o[0] + d[0] * tb[b], o[1] + d[1] * tb[b], o[2] + d[2] * tb[b], sb);
+                    sample_setup<DP>(a, o[0] + d[0] * tb[b], o[1] + d[1] * tb[b], o[2] + d[2] * tb[b], sb);
                     fv[b] = sample_eval(V, sb);
                 }
                 int mine = kCoopN;  // first sample of this lane that ends the march
@@ -2890,7 +2880,7 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
                 break;
             }
             if (f_tt < 0.0f) {
-                if (!have_prev) f_t = interp_vol(V, a, o[0] + d[0] * t_prev, o[1] + d[1] * t_prev, o[2] + d[2] * t_prev);
+                if (!have_prev) f_t = interp_vol<S, false, DP>(V, a, o[0] + d[0] * t_prev, o[1] + d[1] * t_prev, o[2] + d[2] * t_prev);
                 t = t + stepsize * f_tt / (f_t - f_tt);
                 hp[0] = o[0] + d[0] * t;
                 hp[1] = o[1] + d[1] * t;
@@ -2900,7 +2890,8 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
             }
         }
     }
-    if (q != 0) return;
+    const unsigned cost = n_brick + (n_eval + 3u) / 4u;
+    if (q != 0) return cost;
     const int i = y * a.w + x;
     if (hit && th > 0.0f) {
         a.refV[i] = make_float4(hp[0], hp[1], hp[2], 1.0f);
@@ -2923,7 +2914,29 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
         atomicMax(&a.stats[4], (unsigned long long)n_eval);
         atomicMax(&a.stats[5], (unsigned long long)(n_eval + n_brick));
     }
+    return cost;
 }
+
+// Four lanes per ray over 8x8-pixel tiles, 64 rays per block
+template <class S>
+__device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const DevState *__restrict__ st, int blk)
+{
+    const int nb = a.tiles_x * a.tiles_y;
+    const int G = a.xcd_group;
+    const int j = blk / 8;
+    const int tile = ((blk % 8) + 8 * (j / G)) * G + (j % G);
+    if (tile >= nb) return;
+    const int q = threadIdx.x & 3, r = threadIdx.x >> 2;
+    const int gbase = (int)(threadIdx.x & 63) & ~3;  // the group's first lane in the wavefront
+    const int x = (tile % a.tiles_x) * 8 + (r & 7);
+    const int y = (tile / a.tiles_x) * 8 + (r >> 3);
+    if (x >= a.w || y >= a.h) return;  // the whole group
+    float view[16];
+    raycast_view(a, st, view);
+    (void)raycast_coop_ray<S>(a, view, x, y, q, gbase);
+}
+
+
 
 // One thread per pixel; 16x16 pixel tiles per block.  Block ids are remapped
 // so that the blocks an XCD receives (id mod 8) cover one contiguous band of
@@ -2946,7 +2959,7 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
     const unsigned long long wt0 = a.wave_trace ? __builtin_amdgcn_s_memrealtime() : 0ull;
     float view[16];
     raycast_view(a, st, view);
-    raycast_pixel<S, TOUCH, DP>(a, view, x, y);
+    (void)raycast_pixel<S, TOUCH, DP>(a, view, x, y);
     if (a.wave_trace) {
         // diagnostic: where and how long this wavefront ran (lanes 0-7 of
         // each wavefront write its record with vector stores)
@@ -2967,6 +2980,156 @@ template <class S, int DP>
 __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *__restrict__ st)
 {
     raycast_body<S, false, DP>(a, st, blockIdx.x);
+}
+
+// Hybrid schedule.  The raycast's time is set by its slowest wavefronts,
+// which hold grazing rays and cluster in a few tiles; their fine-step
+// samples spread over four lanes per ray (raycast_coop_ray, the same bits)
+// cut those tiles' chains, while every other tile keeps one lane per ray.
+// Which tiles: those whose rays took the most loop iterations in the previous
+// launch of this session (consecutive frames see nearly the same scene).  The
+// grid is 4 x heavy_max blocks for the quarters of the heavy tiles (the four
+// quarters of a tile get block numbers equal mod 8, so they share an XCD),
+// dispatched first, then the one-lane blocks, which skip heavy tiles.  Every
+// wavefront records its tile's largest one-lane iteration count; the last
+// block to finish picks the next launch's heavy tiles: at most heavy_max of
+// those at or above max(0.7 x the largest count, kRayHeavyFloor), the
+// heaviest ones when more qualify, and clears the counts.
+constexpr int kSelPerThread = 16;  // tiles one thread of the selecting block holds: up to 4096 tiles
+
+__device__ __forceinline__ unsigned block_sum_u(unsigned v, unsigned *sh)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const unsigned r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
+// Run by the last block of a hybrid launch.  The flags and the list are
+// derived from the same register copy of the counts, so every flagged tile is
+// in the list (the threshold admits at most heavy_max tiles) and each tile is
+// marched exactly once in the next launch, whatever counts were read; the
+// counts themselves only steer the schedule.
+__device__ __forceinline__ void select_heavy_tiles(const RaycastArgs &a)
+{
+    __shared__ unsigned sh[4];
+    __shared__ int sh_pos[4];
+    // every count read once (device-coherent loads, all issued before use);
+    // tiles beyond 256 x kSelPerThread are never marched four lanes per ray
+    const int ntiles = a.tiles_x * a.tiles_y;
+    const int nt = min(ntiles, 256 * kSelPerThread);
+    unsigned c[kSelPerThread];
+#pragma unroll
+    for (int k = 0; k < kSelPerThread; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
+        c[k] = i < nt ? __hip_atomic_load(&a.tile_cost[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    }
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < kSelPerThread; ++k) m = max(m, c[k]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    const unsigned mx = max(max(sh[0], sh[1]), max(sh[2], sh[3]));
+    __syncthreads();
+    // the smallest threshold in [max(0.7 x the largest count, floor),
+    // largest + 1] with at most heavy_max tiles at or above it (a bisection:
+    // the number at or above a threshold falls as the threshold rises)
+    unsigned lo = max((unsigned)kRayHeavyFloor, (mx * 7u + 9u) / 10u), hi = mx + 1u;
+    if (lo > hi) lo = hi;
+    while (lo < hi) {
+        const unsigned mid = (lo + hi) / 2u;
+        unsigned n = 0;
+#pragma unroll
+        for (int k = 0; k < kSelPerThread; ++k) n += c[k] >= mid ? 1u : 0u;
+        if (block_sum_u(n, sh) <= (unsigned)a.heavy_max)
+            hi = mid;
+        else
+            lo = mid + 1u;
+    }
+    const unsigned thr = lo;
+    // flags and the list in tile order (a block-wide scan per 256 tiles)
+    int base = 0;
+#pragma unroll
+    for (int k = 0; k < kSelPerThread; ++k) {
+        if (256 * k >= nt) break;  // uniform
+        const int i = (int)threadIdx.x + 256 * k;
+        const bool heavy = i < nt && c[k] >= thr;
+        const unsigned long long bm = __ballot(heavy);
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+        if ((threadIdx.x & 63) == 0) sh_pos[threadIdx.x >> 6] = __popcll(bm);
+        __syncthreads();
+        int before = base;
+        for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += sh_pos[w];
+        if (i < nt) {
+            a.heavy_flag[i] = heavy ? 1 : 0;
+            if (heavy && before + rank < a.heavy_max) a.heavy[1 + before + rank] = i;
+            a.tile_cost[i] = 0u;
+        }
+        base += sh_pos[0] + sh_pos[1] + sh_pos[2] + sh_pos[3];
+        __syncthreads();
+    }
+    for (int i = nt + (int)threadIdx.x; i < ntiles; i += 256) {
+        a.heavy_flag[i] = 0;
+        a.tile_cost[i] = 0u;
+    }
+    if (threadIdx.x == 0) {
+        a.heavy[0] = min(base, a.heavy_max);
+        *a.blocks_done = 0u;
+    }
+}
+
+template <class S, int DP>
+__global__ __launch_bounds__(256) void k_raycast_hybrid(RaycastArgs a, const DevState *__restrict__ st)
+{
+    const int nheavy_blocks = 4 * a.heavy_max;
+    const int b = blockIdx.x;
+    unsigned cost = 0;
+    int tile = -1;  // the 16x16 tile this wavefront's rays belong to
+    float view[16];
+    if (b >= nheavy_blocks || (b / 32) * 8 + (b % 8) < min(a.heavy[0], a.heavy_max)) raycast_view(a, st, view);
+    if (b < nheavy_blocks) {
+        // quarter qd of heavy tile h, 64 rays of four lanes
+        const int h = (b / 32) * 8 + (b % 8), qd = (b / 8) % 4;
+        if (h < min(a.heavy[0], a.heavy_max)) {
+            tile = a.heavy[1 + h];
+            const int q = threadIdx.x & 3, r = threadIdx.x >> 2;
+            const int gbase = (int)(threadIdx.x & 63) & ~3;
+            const int x = (tile % a.tiles_x) * 16 + (qd & 1) * 8 + (r & 7);
+            const int y = (tile / a.tiles_x) * 16 + (qd >> 1) * 8 + (r >> 3);
+            if (x < a.w && y < a.h) cost = raycast_coop_ray<S, DP>(a, view, x, y, q, gbase);
+        }
+    } else {
+        const int blk = b - nheavy_blocks;
+        const int nb = a.tiles_x * a.tiles_y;
+        const int G = a.xcd_group;
+        const int j = blk / 8;
+        const int t = ((blk % 8) + 8 * (j / G)) * G + (j % G);
+        if (t < nb && !a.heavy_flag[t]) {
+            tile = t;
+            const int x = (t % a.tiles_x) * 16 + (threadIdx.x & 15);
+            const int y = (t / a.tiles_x) * 16 + (threadIdx.x >> 4);
+            if (x < a.w && y < a.h) cost = raycast_pixel<S, false, DP>(a, view, x, y);
+        }
+    }
+    // the wavefront's largest count into its tile's (tile is uniform per wavefront)
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) cost = max(cost, (unsigned)__shfl_xor((int)cost, o, 64));
+    if ((threadIdx.x & 63) == 0 && tile >= 0 && cost > 0u) atomicMax(&a.tile_cost[tile], cost);
+    // The block barrier waits for the block's count updates to be performed
+    // (device-scope atomics, at the device's coherence point), so a block
+    // counts itself finished only after its counts are visible; the last
+    // block reads them with device-coherent loads.  No release fence, which
+    // on this GPU would write back the L2 at every block's end.
+    __shared__ int sh_last;
+    __syncthreads();
+    if (threadIdx.x == 0) sh_last = atomicAdd(a.blocks_done, 1u) == gridDim.x - 1u;
+    __syncthreads();
+    if (sh_last) select_heavy_tiles(a);
 }
 
 // k_raycast with every voxel and flag byte read marked in a.touch (counts the
@@ -3399,8 +3562,13 @@ hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st,
         }
         return hipGetLastError();
     }
+    // the hybrid schedule, unless a diagnostic needs the one-lane kernel's counters
+    const bool hybrid = a.heavy_max > 0 && a.tile_cost && !a.stats && !a.ray_counts && !a.wave_trace;
+    const dim3 gh(a.blocks + 4 * a.heavy_max);
 #define KF_RC(ST)                                                                  \
-    if (a.dim_pow2) launch_timed(k_raycast<ST, 1>, g, b, s, t0, t1, a, st);        \
+    if (hybrid && a.dim_pow2) launch_timed(k_raycast_hybrid<ST, 1>, gh, b, s, t0, t1, a, st); \
+    else if (hybrid) launch_timed(k_raycast_hybrid<ST, 0>, gh, b, s, t0, t1, a, st);         \
+    else if (a.dim_pow2) launch_timed(k_raycast<ST, 1>, g, b, s, t0, t1, a, st);              \
     else launch_timed(k_raycast<ST, 0>, g, b, s, t0, t1, a, st);
     switch (storage) {
     case KF_STORAGE_RG16I: KF_RC(StoreRG16I); break;

@@ -39,6 +39,8 @@ struct kf_session {
     float4 *col_pos = nullptr;  // integrate's per-column set-up (KF_OPT_INTEGRATE_COLUMNS)
     int2 *col_range = nullptr;
     float *col_dl = nullptr;
+    unsigned char *ray_sched = nullptr;  // hybrid raycast schedule: tile costs, heavy list, flags, block count
+    int ray_hybrid = 1;                  // KF_OPT_RAYCAST_HYBRID
     // stage timing (KF_OPT_STAGE_TIMING): start / stop events of pyramid, ICP
     // (first iteration .. finish), integrate, raycast, carried by the dispatch packets
     int timing = 0;

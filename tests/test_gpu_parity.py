@@ -469,6 +469,42 @@ def test_raycast_four_lanes_equal_one_lane(o32, seq20, storage):
     assert (out[0][0][..., 3] > 0).mean() > 0.5
 
 
+@pytest.mark.parametrize("volume,storage", [(256, "f32"), (128, "f16"), (128, "rg16i")])
+def test_raycast_hybrid_equals_one_lane(seq20, volume, storage):
+    """The hybrid schedule (the previous raycast's heaviest tiles marched four
+    lanes per ray) must leave every frame's maps, pose and volume as the
+    one-lane march makes them; the schedule picks at least one tile once a
+    raycast has run, at most the list's capacity, and starts over with the
+    volume."""
+    print("case: raycast hybrid vs one lane %d^3 %s" % (volume, storage), flush=True)
+    depth, gt = seq20
+    runs = []
+    for hybrid in (0, 1):
+        g, cfg = make(volume=volume, storage=storage, fast=1)
+        g.setOption(K.OPT_RAYCAST_HYBRID, hybrid)
+        g.Reset(gt[0])
+        per_frame = []
+        for f in range(10):
+            g.frame(depth[f].astype(np.float32))
+            per_frame.append((g.getVerts(), g.getNorms(), g.getPose()))
+            if hybrid:
+                assert 1 <= g.getOption(K.OPT_RAYCAST_HEAVY_TILES) <= 48
+        runs.append((per_frame, g.getVolume()))
+        if hybrid:
+            with pytest.raises(RuntimeError):
+                g.setOption(K.OPT_RAYCAST_HEAVY_TILES, 1)
+            g.resetVolume()
+            assert g.getOption(K.OPT_RAYCAST_HEAVY_TILES) == 0
+        g.close()
+    (off, vol_off), (on, vol_on) = runs
+    for f in range(10):
+        for a, b in zip(off[f], on[f]):
+            np.testing.assert_array_equal(a, b)
+    for a, b in zip(vol_off, vol_on):
+        np.testing.assert_array_equal(a, b)
+    assert (off[-1][0][..., 3] > 0).mean() > 0.5
+
+
 def test_raycast_traffic_and_ray_counts():
     """The counting variants of raycast (distinct voxels / flag bytes read,
     per-ray sample counts) leave the maps as raycast() makes them, agree with
