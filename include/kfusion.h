@@ -265,7 +265,8 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         about 2M column segments per launch.  Same results.
  *   KF_OPT_INTEGRATE_ROWS y rows of an integrate block of 64 x rows threads:
  *                         1, 2 or 4 (default).  Same results.
- *   Both, with KF_OPT_RAYCAST_XCD_GROUP, KF_OPT_RAYCAST_LANES and
+ *   Both, with KF_OPT_INTEGRATE_COLUMNS, KF_OPT_RAYCAST_XCD_GROUP,
+ *   KF_OPT_RAYCAST_LANES, KF_OPT_RAYCAST_HYBRID and
  *   KF_OPT_BRICK_REBUILD_PERIOD, are picked per volume size by the autotuner
  *   (oglfusionkinectv2_amd/autotune.py), whose table kf_session_create
  *   applies; kf_set_option overrides it. */

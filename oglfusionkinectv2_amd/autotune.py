@@ -14,6 +14,8 @@ result the same bits, so the choice affects time only:
                        agree mod 8 run on one XCD and take that many
                        consecutive tiles)
   raycast_lanes        lanes per ray (1: 16x16-ray tiles, 4: 8x8-ray tiles)
+  raycast_hybrid       with one lane per ray, the previous launch's heaviest
+                       tiles marched four lanes per ray (1) or not (0)
   brick_rebuild_period frames between exact rebuilds of the brick flags
 
 Method: one session per (storage, volume) on synthetic sequence 0; every
@@ -52,10 +54,11 @@ KNOBS = [
     ("integrate_columns", K.OPT_INTEGRATE_COLUMNS, "integrate"),
     ("raycast_xcd_group", K.OPT_RAYCAST_XCD_GROUP, "raycast"),
     ("raycast_lanes", K.OPT_RAYCAST_LANES, "raycast"),
+    ("raycast_hybrid", K.OPT_RAYCAST_HYBRID, "raycast"),
     ("brick_rebuild_period", K.OPT_BRICK_REBUILD_PERIOD, "frame"),
 ]
 DEFAULTS = {"integrate_zlen": 0, "integrate_rows": 4, "integrate_columns": 1, "raycast_xcd_group": 4,
-            "raycast_lanes": 1, "brick_rebuild_period": 16}
+            "raycast_lanes": 1, "raycast_hybrid": 1, "brick_rebuild_period": 16}
 STORAGE_CODE = {"f32": 1, "f16": 2}
 FRAME_WINDOW = 128  # frames timed for the rebuild period (a multiple of 8, 16, 32 and 64)
 
@@ -73,6 +76,8 @@ def candidates(knob, volume):
         return [1, 2, 4, 8, 16]
     if knob == "raycast_lanes":
         return [1, 4]
+    if knob == "raycast_hybrid":
+        return [0, 1]
     if knob == "brick_rebuild_period":
         return [8, 16, 32, 64]
     raise KeyError(knob)
@@ -93,13 +98,14 @@ def write_inc(rows, path):
              "// oglfusionkinectv2_amd/autotune.py from measurements on MI355X (gfx950).",
              "// {storage, volume, integrate z voxels per thread (0 = automatic), integrate",
              "//  block rows, raycast tiles per XCD cluster, raycast lanes per ray, brick",
-             "//  rebuild period, integrate per-column set-up launch}"]
+             "//  rebuild period, integrate per-column set-up launch, hybrid raycast schedule}"]
     for r in rows:
         c = r["chosen"]
-        lines.append("    {%d, %d, %d, %d, %d, %d, %d, %d},  // %s %d^3: integrate %.1f us, raycast %.1f us"
+        lines.append("    {%d, %d, %d, %d, %d, %d, %d, %d, %d},  // %s %d^3: integrate %.1f us, raycast %.1f us"
                      % (STORAGE_CODE[r["storage"]], r["volume"], c["integrate_zlen"], c["integrate_rows"],
                         c["raycast_xcd_group"], c["raycast_lanes"], c["brick_rebuild_period"],
-                        c["integrate_columns"], r["storage"], r["volume"], 1e3 * r["stage_ms"]["integrate"],
+                        c["integrate_columns"], c["raycast_hybrid"], r["storage"], r["volume"],
+                        1e3 * r["stage_ms"]["integrate"],
                         1e3 * r["stage_ms"]["raycast"]))
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
@@ -117,7 +123,8 @@ def read_inc(path):
             st = {v: k for k, v in STORAGE_CODE.items()}[vals[0]]
             out.append(dict(storage=st, volume=vals[1], integrate_zlen=vals[2], integrate_rows=vals[3],
                             raycast_xcd_group=vals[4], raycast_lanes=vals[5], brick_rebuild_period=vals[6],
-                            integrate_columns=vals[7] if len(vals) > 7 else 1))
+                            integrate_columns=vals[7] if len(vals) > 7 else 1,
+                            raycast_hybrid=vals[8] if len(vals) > 8 else 1))
     return out
 
 

@@ -639,11 +639,11 @@ int kf_session_bytes(const kf_config *cfg, size_t *bytes)
 // with the nearest size (largest side), RG16I the FP16 rows (both 4 B per voxel).
 namespace {
 struct TunedRow {
-    int storage, volume, int_zlen, int_rows, xcd_group, ray_lanes, rebuild_period, int_columns;
+    int storage, volume, int_zlen, int_rows, xcd_group, ray_lanes, rebuild_period, int_columns, ray_hybrid;
 };
 static const TunedRow kTuned[] = {
 #include "kf_tuned.inc"
-    {-1, 0, 0, 4, 4, 1, 16, 1},  // end marker
+    {-1, 0, 0, 4, 4, 1, 16, 1, 1},  // end marker
 };
 
 void apply_tuned(kf_session *s)
@@ -662,6 +662,7 @@ void apply_tuned(kf_session *s)
     s->ray_lanes = best->ray_lanes;
     s->rebuild_period = best->rebuild_period;
     s->int_columns = best->int_columns;
+    s->ray_hybrid = best->ray_hybrid;
 }
 }  // namespace
 

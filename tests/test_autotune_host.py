@@ -26,7 +26,7 @@ def test_candidates_fit_the_volume():
 def test_inc_round_trip(tmp_path):
     rows = [dict(storage="f32", volume=512, stage_ms={"integrate": 0.1, "raycast": 0.2},
                  chosen=dict(integrate_zlen=64, integrate_rows=2, raycast_xcd_group=8, raycast_lanes=1,
-                             brick_rebuild_period=32, integrate_columns=0)),
+                             brick_rebuild_period=32, integrate_columns=0, raycast_hybrid=0)),
             dict(storage="f16", volume=128, stage_ms={"integrate": 0.01, "raycast": 0.05},
                  chosen=dict(autotune.DEFAULTS))]
     p = tmp_path / "t.inc"
