@@ -346,11 +346,16 @@ int kf_raycast_ray_counts(kf_session *s, uint32_t *counts_host);
  * ended and where it ran: records_host[8 * k .. 8 * k + 7] for wavefront k =
  * 4 * block + wave = {start (low, high 32 bits), end (low, high) of the
  * 100 MHz GPU clock, the HW_ID register (SIMD, CU, shader engine), the XCC
- * id, the pixel tile of the block, 0xABCD}; wavefronts that hold no pixel
- * leave their record zero.  *n_records receives the number of records
- * (4 x blocks); records_host must hold 8 x that many words (at most
- * 8 x 4 x 4096).  Results identical to raycast().  No counterpart in the
- * reference (a measurement aid). */
+ * id, the 16x16 pixel tile, 0xAB << 24 | lanes per ray << 16 | the
+ * wavefront's largest loop iteration count (empty cells entered + fine-step
+ * batches of four samples, as one lane counts them; at most 65535)}; with the
+ * hybrid schedule (KF_OPT_RAYCAST_HYBRID) the first 4 x 48 blocks march
+ * quarters of the heavy tiles four lanes per ray and the one-lane blocks of
+ * those tiles hold no pixel.  Wavefronts that hold no pixel leave their
+ * record zero.  *n_records receives the number of records (4 x blocks);
+ * records_host must hold 8 x that many words (at most 8 x 4 x 4096).  Results
+ * identical to raycast().  No counterpart in the reference (a measurement
+ * aid). */
 int kf_raycast_wave_trace(kf_session *s, uint32_t *records_host, int *n_records);
 
 /* Algorithmic traffic of raycast() for the current volume and pose: the

@@ -1273,7 +1273,8 @@ int kf_raycast_wave_trace(kf_session *s, uint32_t *records_host, int *n_records)
 {
     if (!s || !records_host || !n_records) return fail(KF_ERR_INVALID, "NULL argument");
     kf::RaycastArgs a = raycast_args(s, 1);
-    const size_t n = (size_t)a.blocks * 4;
+    // the hybrid launch has 4 x heavy_max blocks before the one-lane grid
+    const size_t n = ((size_t)a.blocks + (a.heavy_max > 0 && a.tile_cost ? 4 * (size_t)a.heavy_max : 0)) * 4;
     if (n > 4 * 4096) return fail(KF_ERR_INVALID, "too many wavefronts for the trace");
     uint32_t *d = nullptr;
     KF_HIP(hipMalloc(&d, n * 8 * sizeof(uint32_t)));

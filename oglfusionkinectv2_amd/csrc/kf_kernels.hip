@@ -2938,6 +2938,33 @@ __device__ __forceinline__ void raycast_coop_body(const RaycastArgs &a, const De
 
 
 
+// Diagnostic: the record of this wavefront in a.wave_trace (8 words at
+// 4 x blk + wave: start and end clock, HW_ID, XCC id, tile, and 0xAB << 24 |
+// lanes per ray << 16 | the wavefront's largest loop iteration count, capped
+// at 65535).  The first active lane writes the first seven words with vector
+// stores; every active lane folds its count into the last word (atomicMax
+// over words with the same top 16 bits, zeroed before the launch).
+__device__ __forceinline__ void wave_trace_record(const RaycastArgs &a, int blk, unsigned long long wt0, int tile,
+                                                  int lanes, unsigned cost)
+{
+    const unsigned long long wt1 = __builtin_amdgcn_s_memrealtime();
+    unsigned hw, xcc;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)" : "=s"(hw));
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+    uint32_t *rec = a.wave_trace + ((size_t)blk * 4 + (threadIdx.x >> 6)) * 8;
+    const unsigned long long act = __ballot(1);
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)act) - 1) {
+        rec[0] = (uint32_t)wt0;
+        rec[1] = (uint32_t)(wt0 >> 32);
+        rec[2] = (uint32_t)wt1;
+        rec[3] = (uint32_t)(wt1 >> 32);
+        rec[4] = hw;
+        rec[5] = xcc;
+        rec[6] = (uint32_t)tile;
+    }
+    atomicMax(&rec[7], 0xAB000000u | ((uint32_t)lanes << 16) | min(cost, 0xFFFFu));
+}
+
 // One thread per pixel; 16x16 pixel tiles per block.  Block ids are remapped
 // so that the blocks an XCD receives (id mod 8) cover one contiguous band of
 // tiles, keeping the voxels neighbouring rays share in that XCD's L2.
@@ -2959,21 +2986,8 @@ __device__ __forceinline__ void raycast_body(const RaycastArgs &a, const DevStat
     const unsigned long long wt0 = a.wave_trace ? __builtin_amdgcn_s_memrealtime() : 0ull;
     float view[16];
     raycast_view(a, st, view);
-    (void)raycast_pixel<S, TOUCH, DP>(a, view, x, y);
-    if (a.wave_trace) {
-        // diagnostic: where and how long this wavefront ran (lanes 0-7 of
-        // each wavefront write its record with vector stores)
-        const unsigned long long wt1 = __builtin_amdgcn_s_memrealtime();
-        unsigned hw, xcc;
-        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_HW_ID)" : "=s"(hw));
-        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
-        const int l = (int)threadIdx.x & 63;
-        uint32_t *rec = a.wave_trace + ((size_t)blk * 4 + (threadIdx.x >> 6)) * 8;
-        const uint32_t v = l == 0 ? (uint32_t)wt0 : l == 1 ? (uint32_t)(wt0 >> 32) : l == 2 ? (uint32_t)wt1
-                         : l == 3 ? (uint32_t)(wt1 >> 32) : l == 4 ? hw : l == 5 ? xcc : l == 6 ? (uint32_t)tile
-                                                                                                    : 0xABCDu;
-        if (l < 8) rec[l] = v;
-    }
+    const unsigned cost = raycast_pixel<S, TOUCH, DP>(a, view, x, y);
+    if (a.wave_trace) wave_trace_record(a, blk, wt0, tile, 1, cost);
 }
 
 template <class S, int DP>
@@ -3088,7 +3102,9 @@ __global__ __launch_bounds__(256) void k_raycast_hybrid(RaycastArgs a, const Dev
 {
     const int nheavy_blocks = 4 * a.heavy_max;
     const int b = blockIdx.x;
+    const unsigned long long wt0 = a.wave_trace ? __builtin_amdgcn_s_memrealtime() : 0ull;
     unsigned cost = 0;
+    bool in_img = false;  // this lane's ray is a pixel of the image
     int tile = -1;  // the 16x16 tile this wavefront's rays belong to
     float view[16];
     if (b >= nheavy_blocks || (b / 32) * 8 + (b % 8) < min(a.heavy[0], a.heavy_max)) raycast_view(a, st, view);
@@ -3101,7 +3117,8 @@ __global__ __launch_bounds__(256) void k_raycast_hybrid(RaycastArgs a, const Dev
             const int gbase = (int)(threadIdx.x & 63) & ~3;
             const int x = (tile % a.tiles_x) * 16 + (qd & 1) * 8 + (r & 7);
             const int y = (tile / a.tiles_x) * 16 + (qd >> 1) * 8 + (r >> 3);
-            if (x < a.w && y < a.h) cost = raycast_coop_ray<S, DP>(a, view, x, y, q, gbase);
+            in_img = x < a.w && y < a.h;
+            if (in_img) cost = raycast_coop_ray<S, DP>(a, view, x, y, q, gbase);
         }
     } else {
         const int blk = b - nheavy_blocks;
@@ -3113,13 +3130,15 @@ __global__ __launch_bounds__(256) void k_raycast_hybrid(RaycastArgs a, const Dev
             tile = t;
             const int x = (t % a.tiles_x) * 16 + (threadIdx.x & 15);
             const int y = (t / a.tiles_x) * 16 + (threadIdx.x >> 4);
-            if (x < a.w && y < a.h) cost = raycast_pixel<S, false, DP>(a, view, x, y);
+            in_img = x < a.w && y < a.h;
+            if (in_img) cost = raycast_pixel<S, false, DP>(a, view, x, y);
         }
     }
     // the wavefront's largest count into its tile's (tile is uniform per wavefront)
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) cost = max(cost, (unsigned)__shfl_xor((int)cost, o, 64));
     if ((threadIdx.x & 63) == 0 && tile >= 0 && cost > 0u) atomicMax(&a.tile_cost[tile], cost);
+    if (a.wave_trace && tile >= 0 && __any(in_img)) wave_trace_record(a, b, wt0, tile, b < nheavy_blocks ? 4 : 1, cost);
     // The block barrier waits for the block's count updates to be performed
     // (device-scope atomics, at the device's coherence point), so a block
     // counts itself finished only after its counts are visible; the last
@@ -3563,7 +3582,7 @@ hipError_t launch_raycast(const RaycastArgs &a, int storage, const DevState *st,
         return hipGetLastError();
     }
     // the hybrid schedule, unless a diagnostic needs the one-lane kernel's counters
-    const bool hybrid = a.heavy_max > 0 && a.tile_cost && !a.stats && !a.ray_counts && !a.wave_trace;
+    const bool hybrid = a.heavy_max > 0 && a.tile_cost && !a.stats && !a.ray_counts;
     const dim3 gh(a.blocks + 4 * a.heavy_max);
 #define KF_RC(ST)                                                                  \
     if (hybrid && a.dim_pow2) launch_timed(k_raycast_hybrid<ST, 1>, gh, b, s, t0, t1, a, st); \

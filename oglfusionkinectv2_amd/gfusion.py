@@ -415,20 +415,23 @@ class GFusion:
 
     def raycastWaveTrace(self):
         """Per-wavefront records of one more raycast(): a structured array with
-        start / end (100 MHz GPU clock ticks), HW_ID, XCC id, pixel tile, and
-        the block and wave numbers; wavefronts that held no pixel are left
-        out."""
+        start / end (100 MHz GPU clock ticks), HW_ID, XCC id, 16x16 pixel
+        tile, lanes per ray (4 for the hybrid schedule's heavy-tile quarters),
+        the wavefront's largest loop iteration count, and the block and wave
+        numbers; wavefronts that held no pixel are left out."""
         n = C.c_int()
         raw = np.zeros(8 * 4 * 4096, np.uint32)
         K.check(K.lib().kf_raycast_wave_trace(self.session, raw.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(n)))
         r = raw[:8 * n.value].reshape(-1, 8)
-        k = np.nonzero(r[:, 7] == 0xABCD)[0]
+        k = np.nonzero((r[:, 7] >> 24) == 0xAB)[0]
         r = r[k].astype(np.uint64)
         out = np.zeros(len(k), dtype=[("start", np.uint64), ("end", np.uint64), ("hw_id", np.uint32),
-                                      ("xcc", np.uint32), ("tile", np.int32), ("block", np.int32), ("wave", np.int32)])
+                                      ("xcc", np.uint32), ("tile", np.int32), ("lanes", np.int32),
+                                      ("iters", np.int32), ("block", np.int32), ("wave", np.int32)])
         out["start"] = r[:, 0] | (r[:, 1] << np.uint64(32))
         out["end"] = r[:, 2] | (r[:, 3] << np.uint64(32))
         out["hw_id"], out["xcc"], out["tile"] = r[:, 4], r[:, 5], r[:, 6]
+        out["lanes"], out["iters"] = (r[:, 7] >> np.uint64(16)) & np.uint64(0xFF), r[:, 7] & np.uint64(0xFFFF)
         out["block"], out["wave"] = k // 4, k % 4
         return out
 

@@ -111,22 +111,15 @@ def main():
 
 
 def wave_trace_report(g, frame):
-    """One more raycast traced per wavefront (kf_raycast_wave_trace) and the
-    per-ray sample counts of the same state (kf_raycast_ray_counts): how long
+    """One more raycast traced per wavefront (kf_raycast_wave_trace): how long
     each wavefront ran against its loop iterations (the largest over its rays
-    of fine-step batches of four samples + empty cells entered), how many
-    wavefronts shared each SIMD, and for the slowest wavefronts the time they
-    ran alone on their SIMD once the others had ended."""
+    of fine-step batches of four samples + empty cells entered, as one lane
+    counts them), how many wavefronts shared each SIMD, the hybrid schedule's
+    four-lane wavefronts apart from the one-lane ones, and for the slowest
+    wavefronts the time they ran alone on their SIMD once the others had
+    ended."""
     tr = g.raycastWaveTrace()
-    ev, cells = g.raycastRayCounts()
-    h, w = ev.shape
-    hp, wp = (h + 15) // 16 * 16, (w + 15) // 16 * 16
-    it = np.zeros((hp, wp))
-    it[:h, :w] = np.ceil(ev / 4.0) + cells
-    tiles_x = wp // 16
-    ty, tx = tr["tile"] // tiles_x, tr["tile"] % tiles_x
-    iters = np.array([it[ty[k] * 16 + tr["wave"][k] * 4:ty[k] * 16 + tr["wave"][k] * 4 + 4,
-                         tx[k] * 16:tx[k] * 16 + 16].max() for k in range(len(tr))])
+    iters = tr["iters"].astype(np.float64)
     t0 = (tr["start"] - tr["start"].min()).astype(np.float64) * 0.01  # us (100 MHz)
     t1 = (tr["end"] - tr["start"].min()).astype(np.float64) * 0.01
     dur = t1 - t0
@@ -134,22 +127,29 @@ def wave_trace_report(g, frame):
     simd = ((tr["xcc"].astype(np.int64) * 8 + ((hw >> 13) & 7)) * 2 + ((hw >> 12) & 1)) * 16 + ((hw >> 8) & 15)
     simd = simd * 4 + ((hw >> 4) & 3)
     _, inv, cnt = np.unique(simd, return_inverse=True, return_counts=True)
-    A = np.vstack([iters, np.ones_like(iters)]).T
-    slope, icpt = np.linalg.lstsq(A, dur, rcond=None)[0]
     print("wave trace frame %d: %d wavefronts on %d SIMDs (wavefronts per SIMD: %s), kernel span %.1f us, "
-          "starts within %.1f us; duration p50 %.1f p99 %.1f max %.1f us; iterations p50 %d p99 %d max %d; "
-          "fit duration = %.2f us x iterations %+.1f us"
-          % (frame, len(tr), len(cnt), {int(a): int(b) for a, b in zip(*np.unique(cnt, return_counts=True))}, t1.max(), t0.max(),
-             np.percentile(dur, 50), np.percentile(dur, 99), dur.max(), np.percentile(iters, 50),
-             np.percentile(iters, 99), iters.max(), slope, icpt), flush=True)
-    for k in np.argsort(-dur)[:6]:
+          "starts within %.1f us" % (frame, len(tr), len(cnt),
+                                     {int(a): int(b) for a, b in zip(*np.unique(cnt, return_counts=True))},
+                                     t1.max(), t0.max()), flush=True)
+    for lanes in (1, 4):
+        m = tr["lanes"] == lanes
+        if not m.any():
+            continue
+        A = np.vstack([iters[m], np.ones(m.sum())]).T
+        slope, icpt = np.linalg.lstsq(A, dur[m], rcond=None)[0]
+        print("  %d lane(s) per ray: %d wavefronts, %d tiles; end p50 %.1f max %.1f us; duration p50 %.1f p99 %.1f "
+              "max %.1f us; iterations p50 %d p99 %d max %d; fit duration = %.2f us x iterations %+.1f us"
+              % (lanes, m.sum(), len(np.unique(tr["tile"][m])), np.percentile(t1[m], 50), t1[m].max(),
+                 np.percentile(dur[m], 50), np.percentile(dur[m], 99), dur[m].max(), np.percentile(iters[m], 50),
+                 np.percentile(iters[m], 99), iters[m].max(), slope, icpt), flush=True)
+    for k in np.argsort(-t1)[:6]:
         mates = np.nonzero((inv == inv[k]) & (np.arange(len(tr)) != k))[0]
         last = t1[mates].max() if len(mates) else t0[k]
         alone = max(0.0, t1[k] - max(last, t0[k]))
-        print("  slowest: tile %d wave %d, %d iterations in %.1f us; %d other wavefronts on its SIMD ended by "
-              "%.1f us; alone for the last %.1f us (%.2f us per iteration over the whole run)"
-              % (tr["tile"][k], tr["wave"][k], iters[k], dur[k], len(mates), last, alone,
-                 dur[k] / max(iters[k], 1)), flush=True)
+        print("  last to end: tile %d wave %d (%d lane(s) per ray, block %d), %d iterations from %.1f to %.1f us; "
+              "%d other wavefronts on its SIMD ended by %.1f us; alone for the last %.1f us"
+              % (tr["tile"][k], tr["wave"][k], tr["lanes"][k], tr["block"][k], iters[k], t0[k], t1[k], len(mates),
+                 last, alone), flush=True)
 
 
 def copy_calibration(torch, mib):

@@ -554,6 +554,7 @@ def test_raycast_wave_trace():
     print("case: raycast wave trace f32 128^3", flush=True)
     depth, gt = synth.sequence(0, 4)
     g, cfg = make(volume=128, storage="f32")
+    g.setOption(K.OPT_RAYCAST_HYBRID, 0)
     g.Reset(gt[0])
     for f in range(4):
         g.frame(depth[f].astype(np.float32))
@@ -564,6 +565,7 @@ def test_raycast_wave_trace():
     np.testing.assert_array_equal(g.getNorms(), n0)
     tiles_x, rows = 512 // 16, 424 // 4  # 16 x 4-pixel wavefronts, 106 rows of them
     assert len(tr) == tiles_x * rows
+    assert (tr["lanes"] == 1).all() and tr["iters"].max() > 0
     assert (tr["end"] >= tr["start"]).all()
     span = int(tr["end"].max() - tr["start"].min())
     assert 0 < span < 100_000_000  # under a second of 100 MHz ticks
@@ -571,6 +573,39 @@ def test_raycast_wave_trace():
     if len(np.unique(tr["xcc"])) == 8:  # eight XCDs in one partition
         rot = (tr["xcc"].astype(np.int64) - tr["block"]) % 8
         assert (rot == rot[0]).all()
+
+
+def test_raycast_wave_trace_hybrid():
+    """With the hybrid schedule the trace covers the heavy tiles' four-lane
+    quarter blocks (the first 4 x 48 blocks) and the one-lane tiles, every
+    tile once: the heavy tiles' one-lane blocks hold no pixel, and each heavy
+    tile has 8 or 16 four-lane wavefronts (16 rays each; 8 in the last tile
+    row, half of which lies below the image)."""
+    print("case: raycast wave trace hybrid f32 256^3", flush=True)
+    depth, gt = synth.sequence(0, 6)
+    g, cfg = make(volume=256, storage="f32", fast=1)
+    g.setOption(K.OPT_RAYCAST_HYBRID, 1)
+    g.Reset(gt[0])
+    for f in range(6):
+        g.frame(depth[f].astype(np.float32))
+    g.raycast()
+    v0, n0 = g.getVerts(), g.getNorms()
+    nheavy = g.getOption(K.OPT_RAYCAST_HEAVY_TILES)
+    assert 1 <= nheavy <= 48
+    tr = g.raycastWaveTrace()
+    np.testing.assert_array_equal(g.getVerts(), v0)
+    np.testing.assert_array_equal(g.getNorms(), n0)
+    four, one = tr[tr["lanes"] == 4], tr[tr["lanes"] == 1]
+    assert len(four) + len(one) == len(tr)
+    heavy = np.unique(four["tile"])
+    assert len(heavy) == nheavy
+    assert (four["block"] < 4 * 48).all() and (one["block"] >= 4 * 48).all()
+    assert not np.isin(one["tile"], heavy).any()
+    tiles_x, tiles_y = 512 // 16, (424 + 15) // 16
+    assert len(np.unique(tr["tile"])) == tiles_x * tiles_y
+    for t in heavy:
+        assert (four["tile"] == t).sum() == (8 if t // tiles_x == tiles_y - 1 else 16)
+    assert (tr["end"] >= tr["start"]).all() and tr["iters"].max() > 0
 
 
 def test_integrate_fast_traffic_counts():
