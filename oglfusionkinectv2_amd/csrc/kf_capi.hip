@@ -155,8 +155,10 @@ kf::PyramidArgs pyramid_args(kf_session *s)
     for (int k = 0; k < 4; ++k) a.cam[k] = s->cfg.cam[k];
     a.tiles_x = (a.w + 7) / 8;
     a.tiles_y = (a.h + 7) / 8;
+    // levels 2, 1, 0 in grid order: the coarse levels' threads average 16
+    // and 4 input pixels per depth and take longest, so they start first
     int start = 0;
-    for (int l = 0; l < 3; ++l) {
+    for (int l = 2; l >= 0; --l) {
         a.block_start[l] = start;
         // level 0: one wavefront per 8x8 tile, four per block
         start += l == 0 ? (a.tiles_x * a.tiles_y + 3) / 4 : (level_w(s, l) * level_h(s, l) + 255) / 256;
@@ -406,10 +408,14 @@ int run_track(kf_session *s, int frame)
     return KF_OK;
 }
 
-// TrackSDF / recoverPose (gFusion.cpp:835-1038, 1616-1653): one persistent launch
-int run_sdf(kf_session *s, int mode, int frame)
+// TrackSDF / recoverPose (gFusion.cpp:835-1038, 1616-1653): one persistent
+// launch; dilate: it first does the brick-flag dilation of the integrate
+// before it (k_brick_dilate's work), saving that launch
+int run_sdf(kf_session *s, int mode, int frame, int dilate = 0)
 {
     kf::SdfArgs a{};
+    a.dilate = dilate;
+    if (dilate) a.bricks = brick_args(s);
     int base = 0;
     for (int l = 0; l < 3; ++l) {
         a.vert[l] = s->vert[l];
@@ -446,7 +452,7 @@ int run_tracking(kf_session *s, int frame)
     return rc;
 }
 
-int run_integrate(kf_session *s, int gated)
+int run_integrate(kf_session *s, int gated, int dilate = 1)
 {
     kf::IntegrateArgs a = integrate_args(s, gated);
     if (s->fast_math && s->int_columns) {
@@ -455,7 +461,7 @@ int run_integrate(kf_session *s, int gated)
         a.col_dl = s->col_dl;
     }
     hipError_t e = kf::launch_integrate(a, s->cfg.storage, s->fast_math, s->st, s->stream, tev(s, 4), tev(s, 5));
-    if (e == hipSuccess) e = kf::launch_brick_dilate(brick_args(s), s->stream);
+    if (e == hipSuccess && dilate) e = kf::launch_brick_dilate(brick_args(s), s->stream);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("integrate launch: ") + hipGetErrorString(e));
     return KF_OK;
 }
@@ -469,10 +475,11 @@ int run_raycast(kf_session *s);
 int run_frame_direct(kf_session *s)
 {
     int rc = run_tracking(s, 1);
-    if (!rc) rc = run_integrate(s, 1);
-    // if (!tracked) recoverPose() (kinect.cpp:686-689): a no-op launch unless
-    // the tracking failed with a non-empty pose library
-    if (!rc && s->recover) rc = run_sdf(s, kf::kSdfRecoverIfLost, 0);
+    // with recoverPose on, its launch does the integrate's flag dilation
+    if (!rc) rc = run_integrate(s, 1, s->recover ? 0 : 1);
+    // if (!tracked) recoverPose() (kinect.cpp:686-689): only the dilation
+    // unless the tracking failed with a non-empty pose library
+    if (!rc && s->recover) rc = run_sdf(s, kf::kSdfRecoverIfLost, 0, 1);
     if (!rc) rc = run_raycast(s);
     return rc;
 }

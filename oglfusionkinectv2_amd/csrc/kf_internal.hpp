@@ -149,6 +149,8 @@ struct SdfArgs {
     int slot_base[3];       // first slot of each level
     const float *library;   // pose library [kPoseLibraryMax][16]
     unsigned spin_limit;    // grid-barrier wait bound (polls)
+    int dilate;             // 1: first the dilated brick flags and coarse cells of `bricks` (k_brick_dilate's work)
+    BrickArgs bricks;
 };
 hipError_t launch_sdf_track(const SdfArgs &a, int storage, DevState *st, int nblk, hipStream_t s);
 

@@ -22,6 +22,7 @@
 #include <type_traits>
 
 #include "kf_device.hpp"
+#include "kf_dilate.hpp"
 #include "kf_internal.hpp"
 #include "kf_march.hpp"
 #include "kf_store.hpp"
@@ -88,8 +89,9 @@ __device__ __forceinline__ float wave_max(float v)
 template <typename DT>
 __device__ __forceinline__ void pyramid_body(const PyramidArgs &a, const DT *__restrict__ in, int b)
 {
-    int l = 0;
-    if (b >= a.block_start[2]) l = 2;
+    // grid order: level 2, level 1, level 0 (block_start[2] = 0)
+    int l = 2;
+    if (b >= a.block_start[0]) l = 0;
     else if (b >= a.block_start[1]) l = 1;
     const int w = a.w >> l, h = a.h >> l;
     int x, y;
@@ -2097,36 +2099,6 @@ __global__ __launch_bounds__(512) void k_brick_rebuild(BrickArgs a)
         a.flags[bi] = any[threadIdx.x] ? 1 : 0;
         a.zflags[bi] = any[8 + threadIdx.x] ? 1 : 0;
     }
-}
-
-// One thread per 32^3 coarse cell c (4^3 bricks): the dilated flag of each of
-// its bricks b, dflags(b) = OR of flags over b + {0,1}^3 (a trilinear sample
-// whose lower corner lies in brick b reads only voxels of those 8 bricks), and
-// cflags(c) = OR of those 64 dilated flags.  The 5x5x5 brick flags the cell
-// needs are kept as 25 five-bit row masks.
-__device__ __forceinline__ void brick_dilate_body(const BrickArgs &a, int i)
-{
-    // one wavefront per 32^3 coarse cell: lane l holds brick (4 cx + l % 4,
-    // 4 cy + (l / 4) % 4, 4 cz + l / 16); its dilated flag is the OR over the
-    // bricks b .. b + 1 in each axis (the trilinear corners of a sample in b)
-    const int n = a.NC[0] * a.NC[1] * a.NC[2];
-    if (i >= n) return;  // the whole wavefront
-    const int lane = threadIdx.x & 63;
-    const int cx = i % a.NC[0], cy = (i / a.NC[0]) % a.NC[1], cz = i / (a.NC[0] * a.NC[1]);
-    const int bx = 4 * cx + (lane & 3), by = 4 * cy + ((lane >> 2) & 3), bz = 4 * cz + (lane >> 4);
-    const bool in = bx < a.NB[0] && by < a.NB[1] && bz < a.NB[2];
-    uint32_t d = 0;
-    if (in) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int X = bx + (k & 1), Y = by + ((k >> 1) & 1), Z = bz + (k >> 2);
-            if (X < a.NB[0] && Y < a.NB[1] && Z < a.NB[2])
-                d |= a.flags[((size_t)Z * a.NB[1] + Y) * a.NB[0] + X];
-        }
-        a.dflags[((size_t)bz * a.NB[1] + by) * a.NB[0] + bx] = d ? 1 : 0;
-    }
-    const bool any = __any(d != 0);
-    if (lane == 0) a.cflags[i] = any ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_brick_dilate(BrickArgs a)

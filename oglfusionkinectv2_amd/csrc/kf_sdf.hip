@@ -27,6 +27,7 @@
 #include <stdint.h>
 
 #include "kf_device.hpp"
+#include "kf_dilate.hpp"
 #include "kf_internal.hpp"
 #include "kf_store.hpp"
 
@@ -299,6 +300,15 @@ __global__ __launch_bounds__(kSdfThreads) void k_sdf_track(SdfArgs a, DevState *
     const int blk = blockIdx.x, nblk = gridDim.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const typename S::V *__restrict__ vol = (const typename S::V *)a.tsdf;
+
+    // the frame loop's dilation of the brick flags (independent of the
+    // tracking below, which reads the TSDF only): four coarse cells per
+    // wavefront per round
+    if (a.dilate) {
+        const int waves = nblk * (kSdfThreads / 64);
+        const int nc = a.bricks.NC[0] * a.bricks.NC[1] * a.bricks.NC[2];
+        for (int i0 = blk * (kSdfThreads / 64) + wave; i0 < nc; i0 += 4 * waves) brick_dilate_cells4(a.bricks, i0, waves);
+    }
 
     // which entries to run (uniform over the grid: the state read here was
     // written by earlier launches)
