@@ -317,6 +317,13 @@ int kf_read_stage_times(kf_session *s, float ms[4]);
 /* Recompute the brick occupancy and zero flags from the volume (done
  * automatically by kf_reset / kf_write_tsdf and periodically by kf_frame_*). */
 int kf_rebuild_bricks(kf_session *s);
+/* Copies the 8^3 brick flags to the host (synchronises the session stream):
+ * occupancy (the brick holds a value that can stop a ray or switch its step:
+ * negative or >= 0.79) and zero (the brick holds a voxel other than (0, 0)),
+ * NB0 x NB1 x NB2 bytes each (NB_k = ceil(volume_size[k] / 8)), x fastest.
+ * Right after a rebuild both are exact; between rebuilds they are supersets
+ * of the exact flags.  No counterpart in the reference (a test aid). */
+int kf_read_brick_flags(kf_session *s, uint8_t *occupancy_host, uint8_t *zero_host);
 
 /* Synchronise the session stream / get its stream (hipStream_t). */
 int kf_synchronize(kf_session *s);

@@ -141,6 +141,7 @@ def lib():
         "kf_set_option": [sess, i32, i32],
         "kf_get_option": [sess, i32, C.POINTER(C.c_int)],
         "kf_rebuild_bricks": [sess],
+        "kf_read_brick_flags": [sess, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)],
         "kf_read_stage_times": [sess, C.POINTER(C.c_float)],
         "kf_raycast_stats": [sess, C.POINTER(C.c_uint64)],
         "kf_raycast_ray_counts": [sess, C.POINTER(C.c_uint32)],

@@ -60,7 +60,7 @@ KNOBS = [
 DEFAULTS = {"integrate_zlen": 0, "integrate_rows": 4, "integrate_columns": 1, "raycast_xcd_group": 4,
             "raycast_lanes": 1, "raycast_hybrid": 1, "brick_rebuild_period": 16}
 STORAGE_CODE = {"f32": 1, "f16": 2}
-FRAME_WINDOW = 128  # frames timed for the rebuild period (a multiple of 8, 16, 32 and 64)
+FRAME_WINDOW = 128  # frames timed for the rebuild period (a multiple of 4, 8, 16, 32 and 64)
 
 
 def candidates(knob, volume):
@@ -79,7 +79,7 @@ def candidates(knob, volume):
     if knob == "raycast_hybrid":
         return [0, 1]
     if knob == "brick_rebuild_period":
-        return [8, 16, 32, 64]
+        return [4, 8, 16, 32, 64]
     raise KeyError(knob)
 
 

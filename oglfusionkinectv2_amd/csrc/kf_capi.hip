@@ -286,10 +286,14 @@ kf::BrickArgs brick_args(kf_session *s)
     return a;
 }
 
-// exact brick flags (occupancy and zero) from both planes, then the dilated copy
-int rebuild_bricks(kf_session *s)
+// exact brick flags (occupancy and zero) from both planes, then the dilated
+// copy; incremental: read only bricks whose zero flag is set (valid while
+// every writer since the last rebuild has kept the flags as supersets: the
+// frame loop's periodic rebuild, not after kf_write_tsdf)
+int rebuild_bricks(kf_session *s, int incremental = 0)
 {
     kf::BrickArgs a = brick_args(s);
+    a.incremental = incremental;
     hipError_t e = kf::launch_brick_rebuild(a, s->cfg.storage, s->stream);
     if (e == hipSuccess) e = kf::launch_brick_dilate(a, s->stream);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("brick rebuild: ") + hipGetErrorString(e));
@@ -519,7 +523,7 @@ int run_frame_body(kf_session *s)
     }
     if (rc) return rc;
     s->frames++;
-    if (s->rebuild_period > 0 && s->frames % s->rebuild_period == 0) rc = rebuild_bricks(s);
+    if (s->rebuild_period > 0 && s->frames % s->rebuild_period == 0) rc = rebuild_bricks(s, 1);
     return rc;
 }
 
@@ -1123,6 +1127,15 @@ int kf_rebuild_bricks(kf_session *s)
 {
     if (!s) return fail(KF_ERR_INVALID, "session is NULL");
     return rebuild_bricks(s);
+}
+
+int kf_read_brick_flags(kf_session *s, uint8_t *occupancy_host, uint8_t *zero_host)
+{
+    if (!s || !occupancy_host || !zero_host) return fail(KF_ERR_INVALID, "NULL argument");
+    KF_HIP(hipMemcpyAsync(occupancy_host, s->bflags, s->nbricks, hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipMemcpyAsync(zero_host, s->zflags, s->nbricks, hipMemcpyDeviceToHost, s->stream));
+    KF_HIP(hipStreamSynchronize(s->stream));
+    return KF_OK;
 }
 
 int kf_read_map(kf_session *s, int which, int level, void *host, size_t bytes)
@@ -1753,6 +1766,7 @@ int kf_batch_frame_u16_dev(kf_batch *b, const uint16_t *const *depth_mm_dev, con
         si->frames = ++b->frames[i];
         if (si->rebuild_period > 0 && si->frames % si->rebuild_period == 0) {
             kf::BrickArgs ba = brick_args(si);
+            ba.incremental = 1;
             e = kf::launch_brick_rebuild(ba, si->cfg.storage, b->stream);
             if (e == hipSuccess) e = kf::launch_brick_dilate(ba, b->stream);
             if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("brick rebuild: ") + hipGetErrorString(e));

@@ -83,6 +83,9 @@ struct BrickArgs {
     int N[3], NB[3], NC[3];
     const void *tsdf, *weight;
     uint8_t *flags, *dflags, *cflags, *zflags;
+    // k_brick_rebuild: 1 = only bricks whose zero flag is set are read (the
+    // flags are exact supersets since the last rebuild); 0 = every brick
+    int incremental;
 };
 
 struct RaycastArgs {

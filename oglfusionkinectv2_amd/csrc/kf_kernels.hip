@@ -2065,16 +2065,24 @@ __global__ __launch_bounds__(256) void k_count_integrate(IntegrateArgs a, const 
 // (8 bricks along x) and one brick layer in z; each wavefront reads one row of
 // 64 consecutive voxels per z of both planes (coalesced), a ballot splits it
 // into the 8 bricks, and LDS ORs the 8 rows.
+// Incremental (a.incremental): between rebuilds every writer of the volume
+// only sets flags (integrate sets both where it writes a negative / nonzero
+// value; a reset clears volume and flags together), so a brick whose zero
+// flag is clear holds only (0, 0) and keeps (0, 0) flags: only bricks with
+// the zero flag set are read and rewritten.
 template <class S>
 __global__ __launch_bounds__(512) void k_brick_rebuild(BrickArgs a)
 {
     __shared__ unsigned int any[16];
     if (threadIdx.x < 16) any[threadIdx.x] = 0u;
-    __syncthreads();
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
     const int x = blockIdx.x * 64 + lane, y = blockIdx.y * 8 + row, bz = blockIdx.z;
+    bool scan = x < a.N[0] && y < a.N[1];
+    if (a.incremental && scan) scan = a.zflags[((size_t)bz * a.NB[1] + blockIdx.y) * a.NB[0] + (x >> 3)] != 0;
+    // the zero flags are read before the block rewrites any
+    __syncthreads();
     bool hit = false, nz = false;
-    if (x < a.N[0] && y < a.N[1]) {
+    if (scan) {
         const typename S::V *v = (const typename S::V *)a.tsdf;
         const typename S::V *w = (const typename S::V *)a.weight;
         const size_t plane = (size_t)a.N[0] * a.N[1];
@@ -2096,8 +2104,12 @@ __global__ __launch_bounds__(512) void k_brick_rebuild(BrickArgs a)
     const int bx = blockIdx.x * 8 + (int)threadIdx.x;
     if (threadIdx.x < 8 && bx < a.NB[0]) {
         const size_t bi = ((size_t)bz * a.NB[1] + blockIdx.y) * a.NB[0] + bx;
-        a.flags[bi] = any[threadIdx.x] ? 1 : 0;
-        a.zflags[bi] = any[8 + threadIdx.x] ? 1 : 0;
+        // (incremental: a brick with a clear zero flag was not read and
+        // keeps its (0, 0) flags)
+        if (!a.incremental || a.zflags[bi]) {
+            a.flags[bi] = any[threadIdx.x] ? 1 : 0;
+            a.zflags[bi] = any[8 + threadIdx.x] ? 1 : 0;
+        }
     }
 }
 

@@ -452,5 +452,14 @@ class GFusion:
     def rebuildBricks(self):
         K.check(K.lib().kf_rebuild_bricks(self.session))
 
+    def getBrickFlags(self):
+        """(occupancy, zero) 8^3 brick flags as (NB2, NB1, NB0) uint8 arrays."""
+        nb = [(n + 7) // 8 for n in self.configuration.volumeSize]
+        occ = np.zeros(nb[::-1], np.uint8)
+        zero = np.zeros(nb[::-1], np.uint8)
+        K.check(K.lib().kf_read_brick_flags(self.session, occ.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                            zero.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return occ, zero
+
     def synchronize(self):
         K.check(K.lib().kf_synchronize(self.session))

@@ -654,6 +654,58 @@ def test_brick_flags_cover_every_negative_voxel(seq20):
     np.testing.assert_array_equal(a_v, g.getVerts())
 
 
+def _exact_brick_flags(v, w, storage):
+    """Exact 8^3 brick flags of (Z, Y, X) planes: occupancy (a value < 0 or
+    >= 0.79) and zero (a voxel other than (0, 0), compared as stored bits)."""
+    if storage == "rg16i":
+        occ = (v < 0) | (v >= 25886)  # 0.79 x 32766
+        nz = (v != 0) | (w != 0)
+    else:
+        bits = np.uint32 if storage == "f32" else np.uint16
+        val = v.astype(np.float32)
+        occ = (val < 0) | (val >= np.float32(0.79))
+        nz = (v.view(bits) != 0) | (w.view(bits) != 0)
+    Z, Y, X = v.shape
+    nb = ((Z + 7) // 8, (Y + 7) // 8, (X + 7) // 8)
+
+    def per_brick(m):
+        p = np.zeros((nb[0] * 8, nb[1] * 8, nb[2] * 8), bool)
+        p[:Z, :Y, :X] = m
+        return p.reshape(nb[0], 8, nb[1], 8, nb[2], 8).any(axis=(1, 3, 5)).astype(np.uint8)
+
+    return per_brick(occ), per_brick(nz)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "rg16i"])
+def test_incremental_brick_rebuild_is_exact(seq20, storage):
+    """The frame loop's periodic rebuild reads only the bricks whose zero flag
+    is set; right after it both flag sets equal the exact flags of the volume
+    (as after the full rebuild of kf_rebuild_bricks), and between rebuilds
+    they cover the exact ones."""
+    print("case: incremental brick rebuild %s" % storage, flush=True)
+    depth, gt = seq20
+    g, cfg = make(volume=128, storage=storage, fast=1)
+    g.setOption(K.OPT_BRICK_REBUILD_PERIOD, 4)
+    g.Reset(gt[0])
+    for f in range(12):  # a new session: rebuilds after frames 4, 8 and 12
+        g.frame(depth[f].astype(np.float32))
+        occ, zero = g.getBrickFlags()
+        eo, ez = _exact_brick_flags(*g.getVolume(), storage)
+        assert ez.sum() > 0
+        if (f + 1) % 4 == 0:
+            np.testing.assert_array_equal(occ, eo)
+            np.testing.assert_array_equal(zero, ez)
+        else:
+            assert (occ >= eo).all() and (zero >= ez).all()
+    g.frame(depth[12].astype(np.float32))
+    g.rebuildBricks()
+    occ, zero = g.getBrickFlags()
+    eo, ez = _exact_brick_flags(*g.getVolume(), storage)
+    np.testing.assert_array_equal(occ, eo)
+    np.testing.assert_array_equal(zero, ez)
+    g.close()
+
+
 @pytest.mark.parametrize("volume", [128, 96, 72])
 def test_fast_math_integrate_within_tolerance(o32, seq20, volume):
     """Also at widths that are not multiples of the 64-column wavefront: the
