@@ -2163,6 +2163,14 @@ struct VolView {
         e0 = S::raw(q.e0);
         e1 = S::raw(q.e1);
     }
+    // the pair's stored bits as one dword (binary16 volumes: element x in
+    // the low half, x + 1 in the high half)
+    __device__ __forceinline__ static uint32_t pair_bits(const char *base, uint32_t voff)
+    {
+        static_assert(sizeof(Pair) == 4, "two 16-bit elements");
+        const Pair q = *(const Pair *)(base + voff);
+        return __builtin_bit_cast(uint32_t, q);
+    }
     __device__ __forceinline__ float raw(int x, int y, int z) const
     {
         if ((unsigned)x >= (unsigned)N0 || (unsigned)y >= (unsigned)N1 || (unsigned)z >= (unsigned)N2) return 0.0f;
@@ -2225,6 +2233,36 @@ __device__ __forceinline__ float trilerp(float c000, float c100, float c010, flo
                     ((c001 * (1 - f[0]) + c101 * f[0]) * (1 - f[1]) + (c011 * (1 - f[0]) + c111 * f[0]) * f[1]) *
                         f[2];
     return S::kScaled ? r * 0.00003051944088f : r;
+}
+
+// F16 volumes: the eight corner products of the blend read the stored halves
+// directly.  v_fma_mix_f32 with a binary16 first source (the low or high
+// half of a dword), an f32 weight and -0.0 as addend rounds the exact product
+// once, sign of zero included: the bits of converting the half to f32 (exact)
+// and multiplying, without the conversion instruction.
+__device__ __forceinline__ float mul_half_lo(uint32_t h2, float w)
+{
+    float r;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h2), "v"(w), "v"(-0.0f));
+    return r;
+}
+__device__ __forceinline__ float mul_half_hi(uint32_t h2, float w)
+{
+    float r;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h2), "v"(w), "v"(-0.0f));
+    return r;
+}
+
+// trilerp of four x-pairs of binary16 corners (low half x = b, high half
+// x = b + 1): rows (y, z), (y + 1, z), (y, z + 1), (y + 1, z + 1); the same
+// expression, in the same order, as trilerp<StoreF16>
+__device__ __forceinline__ float trilerp_f16_pairs(const uint32_t (&r)[4], const float *f)
+{
+    const float a0 = mul_half_lo(r[0], 1 - f[0]) + mul_half_hi(r[0], f[0]);
+    const float a1 = mul_half_lo(r[1], 1 - f[0]) + mul_half_hi(r[1], f[0]);
+    const float a2 = mul_half_lo(r[2], 1 - f[0]) + mul_half_hi(r[2], f[0]);
+    const float a3 = mul_half_lo(r[3], 1 - f[0]) + mul_half_hi(r[3], f[0]);
+    return (a0 * (1 - f[1]) + a1 * f[1]) * (1 - f[2]) + (a2 * (1 - f[1]) + a3 * f[1]) * f[2];
 }
 
 template <class S, bool TOUCH>
@@ -2577,26 +2615,41 @@ __device__ __forceinline__ unsigned raycast_pixel(const RaycastArgs &a, const fl
                 }
                 if (__all(all_in)) {
                     // every active lane: all loads first, then the blends
-                    float c[kBatch][8];
                     constexpr uint32_t sz = VolView<S, TOUCH>::kSz;
                     const size_t rowb = (size_t)V.N0 * sz, plb = (size_t)V.plane32 * sz;
                     // the four row bases are uniform; each sample adds one
                     // 32-bit byte offset
                     const char *rb0 = (const char *)V.v;
                     const char *rb1 = rb0 + rowb, *rb2 = rb0 + plb, *rb3 = rb0 + (plb + rowb);
+                    if constexpr (std::is_same<S, StoreF16>::value) {
+                        // binary16: the pairs stay packed, the blend reads the halves
+                        uint32_t r[kBatch][4];
 #pragma unroll
-                    for (int j = 0; j < kBatch; ++j) {
-                        // rows (y, z), (y + 1, z), (y, z + 1), (y + 1, z + 1)
-                        const uint32_t vo = fq[j].base * sz;
-                        V.pair(rb0, vo, c[j][0], c[j][1]);
-                        V.pair(rb1, vo, c[j][2], c[j][3]);
-                        V.pair(rb2, vo, c[j][4], c[j][5]);
-                        V.pair(rb3, vo, c[j][6], c[j][7]);
+                        for (int j = 0; j < kBatch; ++j) {
+                            const uint32_t vo = fq[j].base * sz;
+                            r[j][0] = V.pair_bits(rb0, vo);
+                            r[j][1] = V.pair_bits(rb1, vo);
+                            r[j][2] = V.pair_bits(rb2, vo);
+                            r[j][3] = V.pair_bits(rb3, vo);
+                        }
+#pragma unroll
+                        for (int j = 0; j < kBatch; ++j) fv[j] = trilerp_f16_pairs(r[j], fq[j].f);
+                    } else {
+                        float c[kBatch][8];
+#pragma unroll
+                        for (int j = 0; j < kBatch; ++j) {
+                            // rows (y, z), (y + 1, z), (y, z + 1), (y + 1, z + 1)
+                            const uint32_t vo = fq[j].base * sz;
+                            V.pair(rb0, vo, c[j][0], c[j][1]);
+                            V.pair(rb1, vo, c[j][2], c[j][3]);
+                            V.pair(rb2, vo, c[j][4], c[j][5]);
+                            V.pair(rb3, vo, c[j][6], c[j][7]);
+                        }
+#pragma unroll
+                        for (int j = 0; j < kBatch; ++j)
+                            fv[j] = trilerp<S>(c[j][0], c[j][1], c[j][2], c[j][3], c[j][4], c[j][5], c[j][6],
+                                               c[j][7], fq[j].f);
                     }
-#pragma unroll
-                    for (int j = 0; j < kBatch; ++j)
-                        fv[j] = trilerp<S>(c[j][0], c[j][1], c[j][2], c[j][3], c[j][4], c[j][5], c[j][6], c[j][7],
-                                           fq[j].f);
                 } else {
                     fv[0] = sample_eval(V, sm);
 #pragma unroll

@@ -505,6 +505,35 @@ def test_raycast_hybrid_equals_one_lane(seq20, volume, storage):
     assert (off[-1][0][..., 3] > 0).mean() > 0.5
 
 
+@pytest.mark.parametrize("volume", [128, 256])
+def test_raycast_f16_equals_f32_of_the_same_values(seq20, volume):
+    """An F16 volume's maps are those of an F32 session holding the same
+    values converted to float (exactly): the F16 blend reads the stored
+    halves with mixed-precision multiplies, which must round as the F32 path
+    does (default launch shapes, so the hybrid schedule at 256^3)."""
+    print("case: raycast f16 vs f32 of the same values %d^3" % volume, flush=True)
+    depth, gt = seq20
+    g16, _ = make(volume=volume, storage="f16", fast=1)
+    g16.Reset(gt[0])
+    for f in range(8):
+        g16.frame(depth[f].astype(np.float32))
+    v16, w16 = g16.getVolume()
+    pose = g16.getPose()
+    g16.raycast()
+    g16.raycast()
+    v_a, n_a = g16.getVerts(), g16.getNorms()
+    g32, _ = make(volume=volume, storage="f32", fast=1)
+    g32.Reset(np.eye(4, dtype=np.float32))
+    g32.setVolume(v16.astype(np.float32), w16.astype(np.float32))
+    g32.setPose(pose)
+    g32.raycast()
+    np.testing.assert_array_equal(g32.getVerts(), v_a)
+    np.testing.assert_array_equal(g32.getNorms(), n_a)
+    assert (v_a[..., 3] > 0).mean() > 0.5
+    g16.close()
+    g32.close()
+
+
 def test_raycast_traffic_and_ray_counts():
     """The counting variants of raycast (distinct voxels / flag bytes read,
     per-ray sample counts) leave the maps as raycast() makes them, agree with
