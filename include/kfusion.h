@@ -287,9 +287,11 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
 #define KF_OPT_INTEGRATE_COLUMNS 20
 /*   KF_OPT_RAYCAST_HYBRID 1 (default): raycast() marches the tiles whose rays
  *                         took the most loop iterations in the session's
- *                         previous raycast with four lanes per ray and every
+ *                         earlier raycasts with four lanes per ray and every
  *                         other tile with one (one launch; the same bits as
- *                         one lane everywhere); 0: one lane everywhere.
+ *                         one lane everywhere); each depth-pyramid launch
+ *                         (kf_frame_*, kf_depth_to_vertex_*) picks the tiles
+ *                         for the raycasts after it; 0: one lane everywhere.
  *                         Applies with KF_OPT_RAYCAST_LANES 1. */
 #define KF_OPT_RAYCAST_HYBRID 21
 /*   KF_OPT_RAYCAST_HEAVY_TILES  read-only (kf_get_option; synchronises the

@@ -211,6 +211,21 @@ kf::IntegrateArgs integrate_args(kf_session *s, int gated, long target_segments 
 }
 
 // lanes: rays per lane group (1 or 4); -1 takes the session's option
+// the session's hybrid raycast schedule (tiles = 0 when the hybrid schedule
+// is off: KF_OPT_RAYCAST_HYBRID 0 or four lanes per ray)
+kf::RaySched ray_sched(kf_session *s)
+{
+    kf::RaySched r{};
+    if (!s->ray_hybrid || s->ray_lanes != 1 || !s->ray_sched) return r;
+    const int nt = ((s->cfg.width + 15) / 16) * ((s->cfg.height + 15) / 16);
+    r.tile_cost = (uint32_t *)s->ray_sched;
+    r.heavy = (int *)(s->ray_sched + 4 * (size_t)nt);
+    r.heavy_flag = s->ray_sched + 4 * ((size_t)nt + kf::kRayHeavyMax + 1);
+    r.tiles = nt;
+    r.heavy_max = kf::kRayHeavyMax;
+    return r;
+}
+
 kf::RaycastArgs raycast_args(kf_session *s, int lanes = -1, int xcd_group = -1)
 {
     const kf_config &c = s->cfg;
@@ -236,14 +251,14 @@ kf::RaycastArgs raycast_args(kf_session *s, int lanes = -1, int xcd_group = -1)
     a.tiles_y = (c.height + tile - 1) / tile;
     a.xcd_group = std::max(1, xcd_group < 0 ? s->xcd_group : xcd_group);
     a.blocks = (a.tiles_x * a.tiles_y + 8 * a.xcd_group - 1) / (8 * a.xcd_group) * (8 * a.xcd_group);
-    if (s->ray_hybrid && a.lanes == 1 && s->ray_sched) {
-        // tile costs, heavy list (count + tiles), flags, finished-block count
-        const int nt = a.tiles_x * a.tiles_y;
-        a.tile_cost = (uint32_t *)s->ray_sched;
-        a.heavy = (int *)(s->ray_sched + 4 * (size_t)nt);
-        a.blocks_done = (unsigned *)(s->ray_sched + 4 * ((size_t)nt + kf::kRayHeavyMax + 1));
-        a.heavy_flag = s->ray_sched + 4 * ((size_t)nt + kf::kRayHeavyMax + 2);
-        a.heavy_max = kf::kRayHeavyMax;
+    if (a.lanes == 1) {
+        const kf::RaySched rs = ray_sched(s);
+        if (rs.tiles > 0) {
+            a.tile_cost = rs.tile_cost;
+            a.heavy = rs.heavy;
+            a.heavy_flag = rs.heavy_flag;
+            a.heavy_max = rs.heavy_max;
+        }
     }
     // 32-bit byte offsets, and N0 N1 < 2^24 for the 24-bit index products
     a.off32 = s->voxels * voxel_plane_bytes(c.storage) <= (size_t)0xFFFFFFFFu &&
@@ -305,6 +320,13 @@ hipEvent_t tev(kf_session *s, int i) { return s->timing ? s->tev[i] : nullptr; }
 int run_pyramid_from_stage(kf_session *s, const void *dev, int is_u16)
 {
     kf::PyramidArgs a = pyramid_args(s);
+    // one more block (the first) selects the hybrid raycast's next heavy
+    // tiles from the session's earlier raycasts
+    a.sched = ray_sched(s);
+    if (a.sched.tiles > 0) {
+        for (int l = 0; l < 3; ++l) a.block_start[l] += 1;
+        a.blocks_total += 1;
+    }
     hipError_t e = is_u16 ? kf::launch_pyramid_u16(a, (const uint16_t *)dev, s->stream, tev(s, 0), tev(s, 1))
                           : kf::launch_pyramid_f32(a, (const float *)dev, s->stream, tev(s, 0), tev(s, 1));
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("pyramid launch: ") + hipGetErrorString(e));

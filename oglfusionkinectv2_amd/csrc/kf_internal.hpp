@@ -11,6 +11,18 @@ namespace kf {
 
 struct DevState;
 
+// The hybrid raycast's schedule of a session: per 16x16 tile the largest
+// one-lane loop iteration count since the last selection, the tiles selected
+// for four lanes per ray (heavy[0] = count, heavy[1..] = tile numbers) and
+// their flags.  tiles = 0: no schedule.
+struct RaySched {
+    uint32_t *tile_cost;
+    int *heavy;
+    uint8_t *heavy_flag;
+    int tiles;
+    int heavy_max;
+};
+
 struct PyramidArgs {
     int w, h;               // level-0 size
     float cam[4];           // level-0 fx, fy, cx, cy
@@ -22,6 +34,9 @@ struct PyramidArgs {
     float4 *norm[3];
     float2 *dtiles;         // per 8x8 level-0 tile: (min, max) of its nonzero depth_m
     int tiles_x, tiles_y;
+    // k_pyramid with sched.tiles > 0: block 0 selects the hybrid raycast's
+    // next heavy tiles (the pyramid's blocks start at 1)
+    RaySched sched;
 };
 
 struct IcpArgs {
@@ -110,15 +125,12 @@ struct RaycastArgs {
     uint32_t *ray_counts;       // optional (with stats): per pixel, evaluated samples | skip events << 16
     uint32_t *touch;            // k_raycast_touch: bitmap of voxels, then flag bytes, read
     uint32_t *wave_trace;       // optional: 8 words per wavefront (clock at start and end, HW_ID, XCC id, tile)
-    // hybrid schedule (k_raycast_hybrid; heavy_max 0: off): per 16x16 tile
-    // the largest one-lane loop iteration count of the last launch, the
-    // tiles this launch marches with four lanes per ray (heavy[0] = count,
-    // heavy[1..] = tile numbers) and their flags, and the launch's count of
-    // finished blocks
+    // hybrid schedule (k_raycast_hybrid; heavy_max 0: off), as RaySched:
+    // the launch adds its tiles' counts and marches the listed tiles with
+    // four lanes per ray
     uint32_t *tile_cost;
     int *heavy;
     uint8_t *heavy_flag;
-    unsigned *blocks_done;
     int heavy_max;              // tiles marched four lanes per ray at most (a multiple of 8)
 };
 
@@ -128,7 +140,8 @@ constexpr int kRayHeavyFloor = 16; // ... only tiles whose last march took at le
 // heavy tile (b / 32) 8 + b % 8: every heavy tile once when this holds
 static_assert(kRayHeavyMax % 8 == 0, "heavy tiles come in groups of 8 (one per XCD)");
 // bytes of a session's hybrid-schedule state for `tiles` 16x16 tiles
-inline size_t ray_sched_bytes(int tiles) { return 4 * ((size_t)tiles + kRayHeavyMax + 2) + (size_t)tiles; }
+// (tile_cost[tiles], heavy[1 + kRayHeavyMax], heavy_flag[tiles])
+inline size_t ray_sched_bytes(int tiles) { return 4 * ((size_t)tiles + kRayHeavyMax + 1) + (size_t)tiles; }
 
 // Point-to-volume tracking (persistent k_sdf_track, one launch per TrackSDF /
 // recoverPose).

@@ -159,9 +159,102 @@ __device__ __forceinline__ void pyramid_body(const PyramidArgs &a, const DT *__r
     a.norm[l][idx] = n;
 }
 
+constexpr int kSelPerThread = 16;  // tiles one thread of the selecting block holds: up to 4096 tiles
+
+__device__ __forceinline__ unsigned block_sum_u(unsigned v, unsigned *sh)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const unsigned r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
+// The hybrid raycast's next heavy tiles, by block 0 (256 threads) of a
+// single-session pyramid launch, which runs after the session's earlier
+// raycasts and before its next one, beside the pyramid's blocks: at most
+// heavy_max tiles at or above max(0.7 x the largest count, kRayHeavyFloor),
+// the heaviest ones when more qualify; the counts are cleared.  The flags
+// and the list are derived from the same register copy of the counts, so
+// every flagged tile is in the list (the threshold admits at most heavy_max
+// tiles) and each tile is marched exactly once by the next raycast,
+// whatever counts were read; the counts themselves only steer the schedule.
+__device__ __forceinline__ void select_heavy_tiles(const RaySched &a)
+{
+    __shared__ unsigned sh[4];
+    __shared__ int sh_pos[4];
+    // every count read once (all loads issued before use); tiles beyond
+    // 256 x kSelPerThread are never marched four lanes per ray
+    const int ntiles = a.tiles;
+    const int nt = min(ntiles, 256 * kSelPerThread);
+    unsigned c[kSelPerThread];
+#pragma unroll
+    for (int k = 0; k < kSelPerThread; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
+        c[k] = i < nt ? a.tile_cost[i] : 0u;
+    }
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < kSelPerThread; ++k) m = max(m, c[k]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    const unsigned mx = max(max(sh[0], sh[1]), max(sh[2], sh[3]));
+    __syncthreads();
+    // the smallest threshold in [max(0.7 x the largest count, floor),
+    // largest + 1] with at most heavy_max tiles at or above it (a bisection:
+    // the number at or above a threshold falls as the threshold rises)
+    unsigned lo = max((unsigned)kRayHeavyFloor, (mx * 7u + 9u) / 10u), hi = mx + 1u;
+    if (lo > hi) lo = hi;
+    while (lo < hi) {
+        const unsigned mid = (lo + hi) / 2u;
+        unsigned n = 0;
+#pragma unroll
+        for (int k = 0; k < kSelPerThread; ++k) n += c[k] >= mid ? 1u : 0u;
+        if (block_sum_u(n, sh) <= (unsigned)a.heavy_max)
+            hi = mid;
+        else
+            lo = mid + 1u;
+    }
+    const unsigned thr = lo;
+    // flags and the list in tile order (a block-wide scan per 256 tiles)
+    int base = 0;
+#pragma unroll
+    for (int k = 0; k < kSelPerThread; ++k) {
+        if (256 * k >= nt) break;  // uniform
+        const int i = (int)threadIdx.x + 256 * k;
+        const bool heavy = i < nt && c[k] >= thr;
+        const unsigned long long bm = __ballot(heavy);
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+        if ((threadIdx.x & 63) == 0) sh_pos[threadIdx.x >> 6] = __popcll(bm);
+        __syncthreads();
+        int before = base;
+        for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += sh_pos[w];
+        if (i < nt) {
+            a.heavy_flag[i] = heavy ? 1 : 0;
+            if (heavy && before + rank < a.heavy_max) a.heavy[1 + before + rank] = i;
+            a.tile_cost[i] = 0u;
+        }
+        base += sh_pos[0] + sh_pos[1] + sh_pos[2] + sh_pos[3];
+        __syncthreads();
+    }
+    for (int i = nt + (int)threadIdx.x; i < ntiles; i += 256) {
+        a.heavy_flag[i] = 0;
+        a.tile_cost[i] = 0u;
+    }
+    if (threadIdx.x == 0) a.heavy[0] = min(base, a.heavy_max);
+}
+
 template <typename DT>
 __global__ __launch_bounds__(256) void k_pyramid(PyramidArgs a, const DT *__restrict__ in)
 {
+    if (a.sched.tiles > 0 && blockIdx.x == 0) {
+        select_heavy_tiles(a.sched);
+        return;
+    }
     pyramid_body(a, in, blockIdx.x);
 }
 
@@ -3042,98 +3135,10 @@ __global__ __launch_bounds__(256) void k_raycast(RaycastArgs a, const DevState *
 // grid is 4 x heavy_max blocks for the quarters of the heavy tiles (the four
 // quarters of a tile get block numbers equal mod 8, so they share an XCD),
 // dispatched first, then the one-lane blocks, which skip heavy tiles.  Every
-// wavefront records its tile's largest one-lane iteration count; the last
-// block to finish picks the next launch's heavy tiles: at most heavy_max of
-// those at or above max(0.7 x the largest count, kRayHeavyFloor), the
-// heaviest ones when more qualify, and clears the counts.
-constexpr int kSelPerThread = 16;  // tiles one thread of the selecting block holds: up to 4096 tiles
-
-__device__ __forceinline__ unsigned block_sum_u(unsigned v, unsigned *sh)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    const unsigned r = sh[0] + sh[1] + sh[2] + sh[3];
-    __syncthreads();
-    return r;
-}
-
-// Run by the last block of a hybrid launch.  The flags and the list are
-// derived from the same register copy of the counts, so every flagged tile is
-// in the list (the threshold admits at most heavy_max tiles) and each tile is
-// marched exactly once in the next launch, whatever counts were read; the
-// counts themselves only steer the schedule.
-__device__ __forceinline__ void select_heavy_tiles(const RaycastArgs &a)
-{
-    __shared__ unsigned sh[4];
-    __shared__ int sh_pos[4];
-    // every count read once (device-coherent loads, all issued before use);
-    // tiles beyond 256 x kSelPerThread are never marched four lanes per ray
-    const int ntiles = a.tiles_x * a.tiles_y;
-    const int nt = min(ntiles, 256 * kSelPerThread);
-    unsigned c[kSelPerThread];
-#pragma unroll
-    for (int k = 0; k < kSelPerThread; ++k) {
-        const int i = (int)threadIdx.x + 256 * k;
-        c[k] = i < nt ? __hip_atomic_load(&a.tile_cost[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-    }
-    unsigned m = 0;
-#pragma unroll
-    for (int k = 0; k < kSelPerThread; ++k) m = max(m, c[k]);
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-    __syncthreads();
-    const unsigned mx = max(max(sh[0], sh[1]), max(sh[2], sh[3]));
-    __syncthreads();
-    // the smallest threshold in [max(0.7 x the largest count, floor),
-    // largest + 1] with at most heavy_max tiles at or above it (a bisection:
-    // the number at or above a threshold falls as the threshold rises)
-    unsigned lo = max((unsigned)kRayHeavyFloor, (mx * 7u + 9u) / 10u), hi = mx + 1u;
-    if (lo > hi) lo = hi;
-    while (lo < hi) {
-        const unsigned mid = (lo + hi) / 2u;
-        unsigned n = 0;
-#pragma unroll
-        for (int k = 0; k < kSelPerThread; ++k) n += c[k] >= mid ? 1u : 0u;
-        if (block_sum_u(n, sh) <= (unsigned)a.heavy_max)
-            hi = mid;
-        else
-            lo = mid + 1u;
-    }
-    const unsigned thr = lo;
-    // flags and the list in tile order (a block-wide scan per 256 tiles)
-    int base = 0;
-#pragma unroll
-    for (int k = 0; k < kSelPerThread; ++k) {
-        if (256 * k >= nt) break;  // uniform
-        const int i = (int)threadIdx.x + 256 * k;
-        const bool heavy = i < nt && c[k] >= thr;
-        const unsigned long long bm = __ballot(heavy);
-        const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
-        if ((threadIdx.x & 63) == 0) sh_pos[threadIdx.x >> 6] = __popcll(bm);
-        __syncthreads();
-        int before = base;
-        for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += sh_pos[w];
-        if (i < nt) {
-            a.heavy_flag[i] = heavy ? 1 : 0;
-            if (heavy && before + rank < a.heavy_max) a.heavy[1 + before + rank] = i;
-            a.tile_cost[i] = 0u;
-        }
-        base += sh_pos[0] + sh_pos[1] + sh_pos[2] + sh_pos[3];
-        __syncthreads();
-    }
-    for (int i = nt + (int)threadIdx.x; i < ntiles; i += 256) {
-        a.heavy_flag[i] = 0;
-        a.tile_cost[i] = 0u;
-    }
-    if (threadIdx.x == 0) {
-        a.heavy[0] = min(base, a.heavy_max);
-        *a.blocks_done = 0u;
-    }
-}
-
+// wavefront records its tile's largest one-lane iteration count; the
+// session's next pyramid launch picks the next launch's heavy tiles from them
+// (select_heavy_tiles, beside the pyramid's blocks, off the raycast's
+// critical path).
 template <class S, int DP>
 __global__ __launch_bounds__(256) void k_raycast_hybrid(RaycastArgs a, const DevState *__restrict__ st)
 {
@@ -3176,16 +3181,6 @@ __global__ __launch_bounds__(256) void k_raycast_hybrid(RaycastArgs a, const Dev
     for (int o = 32; o >= 1; o >>= 1) cost = max(cost, (unsigned)__shfl_xor((int)cost, o, 64));
     if ((threadIdx.x & 63) == 0 && tile >= 0 && cost > 0u) atomicMax(&a.tile_cost[tile], cost);
     if (a.wave_trace && tile >= 0 && __any(in_img)) wave_trace_record(a, b, wt0, tile, b < nheavy_blocks ? 4 : 1, cost);
-    // The block barrier waits for the block's count updates to be performed
-    // (device-scope atomics, at the device's coherence point), so a block
-    // counts itself finished only after its counts are visible; the last
-    // block reads them with device-coherent loads.  No release fence, which
-    // on this GPU would write back the L2 at every block's end.
-    __shared__ int sh_last;
-    __syncthreads();
-    if (threadIdx.x == 0) sh_last = atomicAdd(a.blocks_done, 1u) == gridDim.x - 1u;
-    __syncthreads();
-    if (sh_last) select_heavy_tiles(a);
 }
 
 // k_raycast with every voxel and flag byte read marked in a.touch (counts the

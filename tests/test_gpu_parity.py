@@ -487,7 +487,7 @@ def test_raycast_hybrid_equals_one_lane(seq20, volume, storage):
         for f in range(10):
             g.frame(depth[f].astype(np.float32))
             per_frame.append((g.getVerts(), g.getNorms(), g.getPose()))
-            if hybrid:
+            if hybrid and f >= 2:  # picked from the previous frames' raycasts
                 assert 1 <= g.getOption(K.OPT_RAYCAST_HEAVY_TILES) <= 48
         runs.append((per_frame, g.getVolume()))
         if hybrid:
