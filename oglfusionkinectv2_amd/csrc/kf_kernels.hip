@@ -317,64 +317,6 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// ldlt6 spread over a wavefront: lane i < 6 holds row i of A (Arow) and b[i]
-// (bi); every lane returns the same x.  Each element is formed by the same
-// operations in the same order as in ldlt6 (so the result has the same bits),
-// but the rows of a column are divided in parallel: six dependent divisions
-// instead of twenty-one.  All 64 lanes call it.
-__device__ __forceinline__ bool ldlt6_wave(const double (&Arow)[6], double bi, double (&x)[6])
-{
-    const int lane = threadIdx.x & 63;
-    double diag[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) diag[i] = readlane_d(Arow[i], i);
-    double dmax = 0.0;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) dmax = fmax(dmax, fabs(diag[i]));
-    if (!(dmax > 0.0)) return false;
-    const double tol = dmax * 1e-12;
-    bool ok = true;
-    double L[6], D[6];  // L: this lane's row of L (entries below the diagonal)
-#pragma unroll
-    for (int k = 0; k < 6; ++k) L[k] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        double Lj[6];  // row j of L, k < j
-#pragma unroll
-        for (int k = 0; k < j; ++k) Lj[k] = readlane_d(L[k], j);
-        double d = diag[j];
-#pragma unroll
-        for (int k = 0; k < j; ++k) d -= Lj[k] * Lj[k] * D[k];
-        ok = ok && (d > tol);
-        D[j] = d;
-        if (lane > j && lane < 6) {
-            double sacc = Arow[j];
-#pragma unroll
-            for (int k = 0; k < j; ++k) sacc -= L[k] * Lj[k] * D[k];
-            L[j] = sacc / d;
-        }
-    }
-    if (!ok) return false;
-    double y[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        double sacc = bi;
-#pragma unroll
-        for (int k = 0; k < i; ++k) sacc -= L[k] * y[k];
-        y[i] = readlane_d(sacc, i);
-    }
-#pragma unroll
-    for (int i = 0; i < 6; ++i) y[i] /= D[i];
-#pragma unroll
-    for (int i = 5; i >= 0; --i) {
-        double sacc = y[i];
-#pragma unroll
-        for (int k = i + 1; k < 6; ++k) sacc -= readlane_d(L[i], k) * x[k];
-        x[i] = sacc;
-    }
-    return true;
-}
-
 // LDS scratch of the one-thread 6x6 solves below (doubles), so that a solve
 // does not inflate the register budget of the kernel that calls it.
 constexpr int kSolveWs = 120;
@@ -528,7 +470,7 @@ __device__ __forceinline__ void icp_sincos(double t, bool fast, double &c, doubl
 // glm::eulerAngleXYZ(x3,x4,x5) in double, cast to float, translation x0..2,
 // then m_pose = delta * m_pose in float (gFusion.cpp:700-705); cs[k] / sn[k]
 // are cos / sin of x[3 + k] in double
-__device__ void pose_update_cs(const double x[6], const double cs[3], const double sn[3], float pose[16])
+__device__ __forceinline__ void pose_delta(const double x[6], const double cs[3], const double sn[3], float d[16])
 {
     const double ca = cs[0], sa = sn[0];
     const double cb = cs[1], sb = sn[1];
@@ -538,8 +480,9 @@ __device__ void pose_update_cs(const double x[6], const double cs[3], const doub
         {sa * sb * cc + ca * sc, -sa * sb * sc + ca * cc, -sa * cb},
         {-ca * sb * cc + sa * sc, ca * sb * sc + sa * cc, ca * cb},
     };
-    float d[16];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
+#pragma unroll
         for (int r = 0; r < 3; ++r) d[c * 4 + r] = (float)Rm[r][c];
         d[c * 4 + 3] = 0.0f;
     }
@@ -547,8 +490,15 @@ __device__ void pose_update_cs(const double x[6], const double cs[3], const doub
     d[13] = (float)x[1];
     d[14] = (float)x[2];
     d[15] = 1.0f;
+}
+
+__device__ void pose_update_cs(const double x[6], const double cs[3], const double sn[3], float pose[16])
+{
+    float d[16];
+    pose_delta(x, cs, sn, d);
     mul4(d, pose, pose);
 }
+
 
 
 // The same update composed in FP64 (the precision variant): delta in double
@@ -968,29 +918,30 @@ __device__ __forceinline__ double icp_solve(const IcpArgs &a, double (*gsum)[kTe
         count = (double)((a.h > 7) ? (a.h - 7 + 7) / 8 : 0) * (double)a.w;
     }
     ae = sqrtf((float)v[0] / (float)count);  // gFusion.cpp:1167
-    // lane i < 6: row i of C and b[i]
-    double Arow[6] = {0, 0, 0, 0, 0, 0}, bi = 0.0;
+    // every lane solves the whole system from its own copy of the sums (the
+    // same bits on every lane; no cross-lane traffic on the solve's chain)
+    double A[36], bb[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k)
-        if (lane == k) bi = v[1 + k];
+    for (int k = 0; k < 6; ++k) bb[k] = v[1 + k];
     {
         int q = 7;
 #pragma unroll
         for (int p = 0; p < 6; ++p)
 #pragma unroll
             for (int s2 = p; s2 < 6; ++s2) {
-                if (lane == p) Arow[s2] = v[q];
-                if (lane == s2) Arow[p] = v[q];
+                A[p * 6 + s2] = v[q];
+                A[s2 * 6 + p] = v[q];
                 ++q;
             }
     }
-    if (!ldlt6_wave(Arow, bi, x)) {
+    if (!ldlt6(A, bb, x)) {
         // singular / indefinite: minimum-norm solve on lane 0 in LDS
         double *Cl = solve_ws + kSolveWs, *bl = Cl + 36, *xl = bl + 6;
-        if (lane < 6) {
+        if (lane == 0) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) Cl[lane * 6 + k] = Arow[k];
-            bl[lane] = bi;
+            for (int k = 0; k < 36; ++k) Cl[k] = A[k];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) bl[k] = bb[k];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
