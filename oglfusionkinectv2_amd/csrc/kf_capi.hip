@@ -278,8 +278,9 @@ kf::RaycastArgs raycast_args(kf_session *s, int lanes = -1, int xcd_group = -1)
     for (int k = 0; k < 3; ++k) {
         int e;
         const float m = std::frexp(c.volume_dim[k], &e);
-        if (m != 0.5f) a.dim_pow2 = 0;
-        a.inv_dim[k] = 1.0f / c.volume_dim[k];
+        // a power of two in 2^-32 .. 2^32 (e - 1 is its exponent)
+        if (m != 0.5f || e - 1 < -32 || e - 1 > 32) a.dim_pow2 = 0;
+        a.vscale[k] = (float)c.volume_size[k] * (1.0f / c.volume_dim[k]);  // exact when dim_pow2
     }
     return a;
 }

@@ -118,7 +118,7 @@ struct RaycastArgs {
     int skip;               // empty-space skipping on / off
     unsigned long long *stats;  // optional: evaluated, skipped, cells skipped, hits, max eval, max iters
     int dim_pow2;               // all volume_dim are powers of two
-    float inv_dim[3];
+    float vscale[3];            // N / dim, exact when dim_pow2 (a power-of-two scaling)
     int off32;                  // value-plane bytes < 2^32 and N0 N1 < 2^24 (32-bit offsets, 24-bit products)
     int xcd_group;              // tiles per XCD cluster (block -> tile mapping)
     int lanes;                  // lanes per ray: 1 (16x16-pixel tiles) or 4 (8x8-pixel tiles)

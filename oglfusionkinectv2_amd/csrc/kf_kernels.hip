@@ -2236,16 +2236,20 @@ struct Sample {
     bool valid;
 };
 
-// p * N / dim - 0.5 on axis k.  When every dim is a power of two the
-// division is an exact scaling, done as a multiply by 1/dim (the same result
-// bit for bit).
+// p * N / dim - 0.5 on axis k.  When every dim is a power of two, 2^-m
+// (the host allows only 2^-32 .. 2^32), the division is an exact scaling and
+// p * N / dim = round(p N) 2^-m = round(p (N 2^-m)): rounding commutes with a
+// power-of-two scaling while the result stays normal, and a result below the
+// normal range (|p N 2^-m| < 2^-126) leaves s = -0.5 on either side, since
+// the subtraction of 0.5 absorbs it.  So one multiply by the exact constant
+// N 2^-m (vscale) gives the bits of the reference's multiply and divide.
 // DP: 1 = every dim a power of two, 0 = not, -1 = read a.dim_pow2 (the
 // raycast kernel is instantiated for both cases, so its hot loop carries no
 // branch on it)
 template <int DP = -1>
 __device__ __forceinline__ float voxel_coord(const RaycastArgs &a, float p, int k)
 {
-    if (DP == 1 || (DP < 0 && a.dim_pow2)) return ((p * (float)a.N[k]) * a.inv_dim[k]) - 0.5f;
+    if (DP == 1 || (DP < 0 && a.dim_pow2)) return (p * a.vscale[k]) - 0.5f;
     return (p * (float)a.N[k] / a.dim[k]) - 0.5f;
 }
 
