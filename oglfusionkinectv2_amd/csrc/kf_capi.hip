@@ -133,7 +133,7 @@ void free_all(kf_session *s)
     f(s->refV); f(s->refN); f(s->tsdf); f(s->weight); f(s->st); f(s->block_partials); f(s->icp_rows); f(s->counts);
     f(s->dbg_result); f(s->dbg_error); f(s->dbg_J);
     f(s->library); f(s->sdf_partials); f(s->sdf_slots);
-    f(s->mc_tab); f(s->mc_count); f(s->mc_offset); f(s->mc_part); f(s->mc_total); f(s->mc_verts); f(s->mip);
+    f(s->mc_tab); f(s->mc_count); f(s->mc_offset); f(s->mc_part); f(s->mc_total); f(s->mc_verts); f(s->mip); f(s->mip_zflags);
     for (int i = 0; i < 8; ++i)
         if (s->tev[i]) (void)hipEventDestroy(s->tev[i]);
     for (int k = 0; k < 2; ++k) {
@@ -994,8 +994,22 @@ int kf_intensity_projection(kf_session *s)
         const size_t p0 = (size_t)s->cfg.width * s->cfg.height;
         if (hipMalloc((void **)&s->mip, p0 * 16) != hipSuccess) return fail(KF_ERR_OOM, "intensity image");
     }
+    if (!s->mip_zflags && hipMalloc((void **)&s->mip_zflags, s->nbricks + s->ncoarse) != hipSuccess)
+        return fail(KF_ERR_OOM, "intensity projection flags");
     kf::RaycastArgs a = raycast_args(s);
-    hipError_t e = kf::launch_intensity_projection(a, s->cfg.storage, s->st, s->mip, s->stream);
+    hipError_t e = hipSuccess;
+    if (a.skip) {
+        // the zero flags dilated as the raycast's flags are: a brick's flag is
+        // clear when it and its +1 neighbours hold only (0, 0)
+        kf::BrickArgs z = brick_args(s);
+        z.flags = s->zflags;
+        z.dflags = s->mip_zflags;
+        z.cflags = s->mip_zflags + s->nbricks;
+        e = kf::launch_brick_dilate(z, s->stream);
+        a.dflags = z.dflags;
+        a.cflags = z.cflags;
+    }
+    if (e == hipSuccess) e = kf::launch_intensity_projection(a, s->cfg.storage, s->st, s->mip, s->stream);
     if (e != hipSuccess) return fail(KF_ERR_HIP, std::string("intensity projection: ") + hipGetErrorString(e));
     return KF_OK;
 }

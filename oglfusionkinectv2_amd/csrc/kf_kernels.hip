@@ -3486,12 +3486,25 @@ hipError_t launch_count_integrate(const IntegrateArgs &a, const DevState *st, un
 // intensityProjection (gFusion.cpp:1368-1409, mip.cs:33-107): per pixel, the
 // sum of the positive trilinear samples of the TSDF along the ray at t =
 // tnear, tnear + largeStep, ... < tfar; written as (sum, 0, 0, 1).
+//
+// kMipLanes lanes share a ray.  Every lane of the group steps t through the
+// same float additions; lane l evaluates the samples j = l (mod kMipLanes),
+// kMipBatch of them per round with all their loads issued before the first
+// is used, and every lane of the group then adds the round's positive values
+// in sample order (broadcast within the group), so the sum is the sequential
+// one, bit for bit.  With a.skip, a.dflags holds the dilated zero flags: a
+// sample whose lower corner lies in a brick with a clear flag reads only
+// (0, 0) voxels, its value is 0, it adds nothing and is not evaluated.
+constexpr int kMipLanes = 4, kMipBatch = 2;
+
 template <class S>
 __global__ __launch_bounds__(256) void k_intensity_projection(RaycastArgs a, const DevState *__restrict__ st,
                                                               float4 *__restrict__ out)
 {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= a.w || y >= a.h) return;
+    constexpr int L = kMipLanes, B = kMipBatch, R = 256 / L;  // R rays: 8 x (R / 8) pixels
+    const int lane = threadIdx.x & (L - 1), ray = threadIdx.x / L;
+    const int x = blockIdx.x * 8 + (ray & 7), y = blockIdx.y * (R / 8) + (ray >> 3);
+    if (x >= a.w || y >= a.h) return;  // the whole group of L lanes
     float view[16];
     {
         float iK[16], P[16];
@@ -3518,20 +3531,88 @@ __global__ __launch_bounds__(256) void k_intensity_projection(RaycastArgs a, con
     const float tnear = gmax(largest_tmin, a.near_plane);
     const float tfar = gmin(smallest_tmax, a.far_plane);
     float sum = 0.0f;
-    if (tnear < tfar) {
-        int guard = 0;
-        for (float t = tnear; t < tfar && guard < kMaxMarchSteps; t += a.large_step, ++guard) {
-            const float f = interp_vol(V, a, o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t);
-            if (f > 0.0f) sum += f;
+    // tnear, tfar and t are the same in every lane of the group, so the
+    // group enters, loops and leaves together
+    float t = tnear;
+    int guard = 0;
+    while (t < tfar && guard < kMaxMarchSteps) {
+        // this lane's samples of the round: j = b L + lane
+        float tj[B];
+        bool ok[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            tj[b] = 0.0f;
+            ok[b] = false;
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                if (l == lane) {
+                    tj[b] = t;
+                    ok[b] = t < tfar && guard < kMaxMarchSteps;
+                }
+                t += a.large_step;
+                ++guard;
+            }
+        }
+        FastSample q[B];
+        bool ev[B], fast = a.off32 != 0;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            sample_fast(a, o[0] + d[0] * tj[b], o[1] + d[1] * tj[b], o[2] + d[2] * tj[b], V.plane32, q[b]);
+            ev[b] = ok[b];
+            if (a.skip && ok[b] && q[b].inside)
+                ev[b] = a.dflags[((q[b].b[2] >> 3) * a.NB[1] + (q[b].b[1] >> 3)) * a.NB[0] + (q[b].b[0] >> 3)] != 0;
+            fast = fast && (!ev[b] || q[b].inside);
+        }
+        float fv[B];
+        if (__all(fast)) {
+            // corners b and b + 1 on every axis: four x-pairs per sample,
+            // from the four uniform row bases
+            constexpr uint32_t sz = VolView<S>::kSz;
+            const size_t rowb = (size_t)V.N0 * sz, plb = (size_t)V.plane32 * sz;
+            const char *rb0 = (const char *)V.v;
+            const char *rb1 = rb0 + rowb, *rb2 = rb0 + plb, *rb3 = rb0 + (plb + rowb);
+            float c[B][8];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) c[b][k] = 0.0f;
+                if (ev[b]) {
+                    const uint32_t vo = q[b].base * sz;
+                    V.pair(rb0, vo, c[b][0], c[b][1]);
+                    V.pair(rb1, vo, c[b][2], c[b][3]);
+                    V.pair(rb2, vo, c[b][4], c[b][5]);
+                    V.pair(rb3, vo, c[b][6], c[b][7]);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+                fv[b] = ev[b] ? trilerp<S>(c[b][0], c[b][1], c[b][2], c[b][3], c[b][4], c[b][5], c[b][6], c[b][7],
+                                           q[b].f)
+                              : 0.0f;
+        } else {
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                fv[b] = 0.0f;
+                if (ev[b]) fv[b] = interp_vol(V, a, o[0] + d[0] * tj[b], o[1] + d[1] * tj[b], o[2] + d[2] * tj[b]);
+            }
+        }
+        // the round's samples in order: j = b L + l is lane l's sample b
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                const float f = __shfl(fv[b], l, L);
+                if (f > 0.0f) sum += f;
+            }
         }
     }
-    out[(size_t)y * a.w + x] = make_float4(sum, 0.0f, 0.0f, 1.0f);
+    if (lane == 0) out[(size_t)y * a.w + x] = make_float4(sum, 0.0f, 0.0f, 1.0f);
 }
 
 hipError_t launch_intensity_projection(const RaycastArgs &a, int storage, const DevState *st, float4 *out,
                                        hipStream_t s)
 {
-    const dim3 g((a.w + 15) / 16, (a.h + 15) / 16), b(256);
+    const dim3 g((a.w + 7) / 8, (a.h + 256 / kMipLanes / 8 - 1) / (256 / kMipLanes / 8)), b(256);
     switch (storage) {
     case KF_STORAGE_RG16I: hipLaunchKernelGGL(k_intensity_projection<StoreRG16I>, g, b, 0, s, a, st, out); break;
     case KF_STORAGE_F32: hipLaunchKernelGGL(k_intensity_projection<StoreF32>, g, b, 0, s, a, st, out); break;

@@ -92,6 +92,7 @@ struct kf_session {
     size_t mc_cap = 0;                  // triangles mc_verts holds
     unsigned long long mc_triangles = 0;  // triangles of the last kf_mesh_generate
     float4 *mip = nullptr;              // intensityProjection image (RGBA32F, W x H)
+    uint8_t *mip_zflags = nullptr;      // its dilated zero flags (nbricks) + coarse cells (ncoarse)
 };
 
 // sets the calling thread's error message (kf_last_error); returns code

@@ -152,13 +152,17 @@ def test_mesh_512_after_frames_is_consistent(seq20):
     assert np.all(near.sum(axis=2) >= 2)  # on an edge: two integer coordinates
 
 
-def test_intensity_projection_matches_oracle(o32, seq20):
+@pytest.mark.parametrize("storage", ["rg16i", "f32", "f16"])
+def test_intensity_projection_matches_oracle(o32, seq20, storage):
     """intensityProjection (mip.cs) on a fused 128^3 volume: the same FP32
-    sample expressions and additions on both sides."""
-    print("case: intensity projection 128^3", flush=True)
+    sample expressions and additions on both sides.  The GPU splits each ray
+    over four lanes and adds their samples back in march order, and skips
+    samples in bricks that hold only (0, 0): neither may change a bit, with
+    the skipping on or off."""
+    print("case: intensity projection 128^3 %s" % storage, flush=True)
     depth, gt = seq20
-    cfg = kinect_config(128, 2.0, "rg16i")
-    s = oracle.OracleSession(o32, 128, 2.0, cfg.mu, oracle.RG16I)
+    cfg = kinect_config(128, 2.0, storage)
+    s = oracle.OracleSession(o32, 128, 2.0, cfg.mu, STORAGE_ORACLE[storage])
     s.reset(gt[0])
     for f in range(5):
         s.frame(depth[f].astype(np.float32))
@@ -167,10 +171,14 @@ def test_intensity_projection_matches_oracle(o32, seq20):
     g.setConfig(cfg)
     g.allocateBuffers()
     g.Reset(gt[0])
-    g.setVolume(*planes(vol, "rg16i"))
+    g.setVolume(*planes(vol, storage))
     g.setPose(s.pose())
+    _, zero = g.getBrickFlags()
+    assert 0.1 < (zero == 0).mean() < 0.9  # bricks of (0, 0) to skip, and others
     img = g.intensityProjection()
-    want = o32.intensity_projection(vol, oracle.RG16I, (2.0,) * 3, s.pose(), (365.5, 365.5, 255.5, 211.5),
+    want = o32.intensity_projection(vol, STORAGE_ORACLE[storage], (2.0,) * 3, s.pose(), (365.5, 365.5, 255.5, 211.5),
                                     0.1, 3.0, np.float32(0.75) * np.float32(cfg.mu))
     assert (want[..., 0] > 0).mean() > 0.2
     np.testing.assert_array_equal(img, want)
+    g.setOption(K.OPT_SKIP_EMPTY, 0)
+    np.testing.assert_array_equal(g.intensityProjection(), want)
