@@ -40,6 +40,7 @@ def main():
     ap.add_argument("--time", type=int, default=0, help="time raycast per frame with events and print the mean")
     ap.add_argument("--wave-trace", type=str, default="",
                     help="comma-separated frames after which one more raycast is traced per wavefront and reported")
+    ap.add_argument("--mip", type=int, default=0, help="after the frames, run intensityProjection this many times")
     args = ap.parse_args()
 
     import torch
@@ -76,6 +77,11 @@ def main():
                      st["max_iter"], st["hits"] / 217088, vr, vw, err), flush=True)
     g.synchronize()
     print("frames done:", args.frames, "status", g.status(), flush=True)
+    for _ in range(args.mip):
+        K.check(L.kf_intensity_projection(g.session))
+    if args.mip:
+        g.synchronize()
+        print("intensity projections done:", args.mip, flush=True)
     if args.time:
         for grp in (1, 2, 4, 8, 16, 27):
             g.setOption(K.OPT_RAYCAST_XCD_GROUP, grp)
