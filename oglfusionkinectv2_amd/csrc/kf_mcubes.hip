@@ -52,10 +52,37 @@ struct McArgs {
 };
 
 // corner k of the cube in classification-bit order, and edge k's end corners
-__constant__ int8_t kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1},
-                                     {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
-__constant__ int8_t kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
-                                    {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+constexpr int8_t kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1},
+                                  {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
+constexpr int8_t kEdge[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                 {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+// kCorner and kEdge as arithmetic (the emit kernel's indices differ per
+// lane, where a table lookup would be a memory load per vertex)
+__host__ __device__ constexpr int corner_x(int c) { return ((c & 3) == 1 || (c & 3) == 2) ? 1 : 0; }
+__host__ __device__ constexpr int corner_y(int c) { return c >> 2; }
+__host__ __device__ constexpr int corner_z(int c) { return (c & 3) >= 2 ? 1 : 0; }
+__host__ __device__ constexpr int edge_lo(int e) { return e < 8 ? e : e - 8; }
+__host__ __device__ constexpr int edge_hi(int e) { return e < 8 ? ((e & 4) | ((e + 1) & 3)) : e - 4; }
+constexpr bool corner_edge_formulas_match()
+{
+    for (int c = 0; c < 8; ++c)
+        if (corner_x(c) != kCorner[c][0] || corner_y(c) != kCorner[c][1] || corner_z(c) != kCorner[c][2])
+            return false;
+    for (int e = 0; e < 12; ++e)
+        if (edge_lo(e) != kEdge[e][0] || edge_hi(e) != kEdge[e][1]) return false;
+    return true;
+}
+static_assert(corner_edge_formulas_match(), "corner / edge formulas differ from the tables");
+
+// g[c] for a per-lane index c in 0..7 (selects, no indexed register access)
+__device__ __forceinline__ float pick8(const float (&g)[8], int c)
+{
+    float r = g[0];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) r = c == i ? g[i] : r;
+    return r;
+}
 
 // (x, y, z) of a Morton key with `levels` digits; digit = x + 2 z + 4 y
 __device__ __forceinline__ void morton_decode(uint32_t m, int levels, int &x, int &y, int &z)
@@ -77,17 +104,30 @@ __device__ __forceinline__ float mc_field(const typename S::V *__restrict__ v, c
     return S::raw(v[((size_t)z * a.N[1] + y) * a.N[0] + x]);
 }
 
-// classifyCubes (histoPyramids.cs:104-170): cube index of the cube at voxel
-// (x, y, z), 0 for the 10-voxel border and for cubes without a value > 0 in
-// the first seven corners
+// A wavefront lane owns the 2x2x2 cubes (x0, y0, z0) + {0,1}^3 (Morton
+// digit 0 of its cube keys); they read the 3x3x3 voxels (x0, y0, z0) +
+// {0,1,2}^3, loaded once, all loads issued before any is used.
 template <class S>
-__device__ __forceinline__ int mc_classify(const typename S::V *__restrict__ v, const McArgs &a, int x, int y, int z)
+__device__ __forceinline__ void load_block27(const typename S::V *__restrict__ v, const McArgs &a, int x0, int y0,
+                                             int z0, float (&F)[27])
+{
+#pragma unroll
+    for (int i = 0; i < 27; ++i) F[i] = mc_field<S>(v, a, x0 + i % 3, y0 + (i / 3) % 3, z0 + i / 9);
+}
+__host__ __device__ constexpr int f27(int dx, int dy, int dz) { return dx + 3 * dy + 9 * dz; }
+
+// classifyCubes (histoPyramids.cs:104-170): cube index of the cube at voxel
+// (x, y, z) = (x0, y0, z0) + (ox, oy, oz), 0 for the 10-voxel border and for
+// cubes without a value > 0 in the first seven corners; its field values
+// come from the lane's 3x3x3 block
+__device__ __forceinline__ int mc_classify27(const float (&F)[27], const McArgs &a, int x, int y, int z, int ox,
+                                             int oy, int oz)
 {
     if (x <= 10 || x >= a.N[0] - 10 || y <= 10 || y >= a.N[1] - 10 || z <= 10 || z >= a.N[2] - 10) return 0;
     // field[k] at cubeOffsets[k] = (k & 1, k >> 2, (k >> 1) & 1) in (x, y, z)
     float f[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) f[k] = mc_field<S>(v, a, x + (k & 1), y + (k >> 2), z + ((k >> 1) & 1));
+    for (int k = 0; k < 8; ++k) f[k] = F[f27(ox + (k & 1), oy + (k >> 2), oz + ((k >> 1) & 1))];
     bool pos = false;
 #pragma unroll
     for (int k = 0; k < 7; ++k) pos = pos || f[k] > 0.0f;
@@ -133,10 +173,15 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_mc_count(McArgs a)
     uint32_t n = 0;
     if (brick_live(a, bx, by, bz)) {
         const typename S::V *__restrict__ v = (const typename S::V *)a.tsdf;
-        for (int j = 0; j < 8; ++j) {
-            int lx, ly, lz;
-            morton_decode((uint32_t)(8 * lane + j), 3, lx, ly, lz);
-            const int ci = mc_classify<S>(v, a, 8 * bx + lx, 8 * by + ly, 8 * bz + lz);
+        int lx, ly, lz;
+        morton_decode((uint32_t)(8 * lane), 3, lx, ly, lz);
+        const int x0 = 8 * bx + lx, y0 = 8 * by + ly, z0 = 8 * bz + lz;
+        float F[27];
+        load_block27<S>(v, a, x0, y0, z0, F);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {  // Morton digit j = x + 2 z + 4 y
+            const int ox = j & 1, oz = (j >> 1) & 1, oy = j >> 2;
+            const int ci = mc_classify27(F, a, x0 + ox, y0 + oy, z0 + oz, ox, oy, oz);
             if (ci > 0) n += a.ntri[ci];
         }
         n = wave_sum_u32(n);
@@ -154,33 +199,46 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_mc_emit(McArgs a)
     int bx, by, bz;
     morton_decode((uint32_t)m, a.levels, bx, by, bz);
     const typename S::V *__restrict__ v = (const typename S::V *)a.tsdf;
+    int lx, ly, lz;
+    morton_decode((uint32_t)(8 * lane), 3, lx, ly, lz);
+    const int x0 = 8 * bx + lx, y0 = 8 * by + ly, z0 = 8 * bz + lz;
+    float F[27];
+    load_block27<S>(v, a, x0, y0, z0, F);
     int ci[8];
     uint32_t n = 0;
-    for (int j = 0; j < 8; ++j) {
-        int lx, ly, lz;
-        morton_decode((uint32_t)(8 * lane + j), 3, lx, ly, lz);
-        const int c = mc_classify<S>(v, a, 8 * bx + lx, 8 * by + ly, 8 * bz + lz);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {  // Morton digit j = x + 2 z + 4 y
+        const int ox = j & 1, oz = (j >> 1) & 1, oy = j >> 2;
+        const int c = mc_classify27(F, a, x0 + ox, y0 + oy, z0 + oz, ox, oy, oz);
         ci[j] = c > 0 ? c : 0;
         n += a.ntri[ci[j]];
     }
     unsigned long long t = (unsigned long long)a.offset[m] + wave_exclusive_u32(n, lane);
+#pragma unroll
     for (int j = 0; j < 8; ++j) {
         if (!ci[j]) continue;
-        int lx, ly, lz;
-        morton_decode((uint32_t)(8 * lane + j), 3, lx, ly, lz);
-        const int cx = 8 * bx + lx, cy = 8 * by + ly, cz = 8 * bz + lz;
-        const uint8_t *tri = a.tri + 16 * ci[j];
+        const int ox = j & 1, oz = (j >> 1) & 1, oy = j >> 2;
+        const int cx = x0 + ox, cy = y0 + oy, cz = z0 + oz;
+        // the cube's eight corner values in kCorner order, from the block
+        float g[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) g[c] = F[f27(ox + corner_x(c), oy + corner_y(c), oz + corner_z(c))];
+        const uint4 row = *(const uint4 *)(a.tri + 16 * ci[j]);
+        const uint32_t rw[4] = {row.x, row.y, row.z, row.w};
         const int nt = a.ntri[ci[j]];
         for (int k = 0; k < nt; ++k, ++t) {
             if (t >= a.cap) continue;
+#pragma unroll
             for (int q = 0; q < 3; ++q) {
                 // traverseHistoPyramids.cs:196-250
-                const int e = tri[3 * k + q];
-                const int c0 = kEdge[e][0], c1 = kEdge[e][1];
-                const int p0[3] = {cx + kCorner[c0][0], cy + kCorner[c0][1], cz + kCorner[c0][2]};
-                const int p1[3] = {cx + kCorner[c1][0], cy + kCorner[c1][1], cz + kCorner[c1][2]};
-                const float v0 = mc_field<S>(v, a, p0[0], p0[1], p0[2]);
-                float diff = (a.iso - v0) / (mc_field<S>(v, a, p1[0], p1[1], p1[2]) - v0);
+                const int b = 3 * k + q;  // byte b of the triangle row
+                const uint32_t w = b < 4 ? rw[0] : (b < 8 ? rw[1] : (b < 12 ? rw[2] : rw[3]));
+                const int e = (int)((w >> (8 * (b & 3))) & 0xffu);
+                const int c0 = edge_lo(e), c1 = edge_hi(e);
+                const int p0[3] = {cx + corner_x(c0), cy + corner_y(c0), cz + corner_z(c0)};
+                const int p1[3] = {cx + corner_x(c1), cy + corner_y(c1), cz + corner_z(c1)};
+                const float v0 = pick8(g, c0);
+                float diff = (a.iso - v0) / (pick8(g, c1) - v0);
                 if (diff > 1.0f || diff < 0.0f) diff = 0.5f;
                 float4 o;
                 o.x = (float)p0[0] * (1.0f - diff) + (float)p1[0] * diff;

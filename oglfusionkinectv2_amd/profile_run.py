@@ -41,6 +41,7 @@ def main():
     ap.add_argument("--wave-trace", type=str, default="",
                     help="comma-separated frames after which one more raycast is traced per wavefront and reported")
     ap.add_argument("--mip", type=int, default=0, help="after the frames, run intensityProjection this many times")
+    ap.add_argument("--mesh", type=int, default=0, help="after the frames, run the surface export this many times")
     args = ap.parse_args()
 
     import torch
@@ -82,6 +83,12 @@ def main():
     if args.mip:
         g.synchronize()
         print("intensity projections done:", args.mip, flush=True)
+    tri = C.c_uint64()
+    for _ in range(args.mesh):
+        K.check(L.kf_mesh_generate(g.session, C.c_float(0.0), C.byref(tri)))
+    if args.mesh:
+        g.synchronize()
+        print("surface exports done:", args.mesh, "triangles", tri.value, flush=True)
     if args.time:
         for grp in (1, 2, 4, 8, 16, 27):
             g.setOption(K.OPT_RAYCAST_XCD_GROUP, grp)
