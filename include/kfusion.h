@@ -157,7 +157,11 @@ int kf_write_pose_eig(kf_session *s, const float pose_eig[16]);
 /* intensityProjection() (gFusion.cpp:1368-1409, mip.cs): for every pixel the
  * sum of the positive TSDF samples along its ray, one every 0.75 mu from the
  * box / near-plane entry to min(box exit, far plane), as (sum, 0, 0, 1) into
- * the KF_MAP_INTENSITY image (kf_read_map). */
+ * the KF_MAP_INTENSITY image (kf_read_map).  The first call allocates the
+ * image (W x H x 16 B) and a flag buffer (one byte per 8^3 brick and per 32^3
+ * cell).  With KF_OPT_SKIP_EMPTY, samples whose trilinear corners lie only in
+ * bricks holding (0, 0) are not evaluated (they are 0 and add nothing: the
+ * image is the same bit for bit). */
 int kf_intensity_projection(kf_session *s);
 
 /* integrate() (gFusion.cpp:1234-1287).  mode = KF_INTEGRATE_ALWAYS or _GATED. */
@@ -202,7 +206,9 @@ int kf_read_track_pixels(kf_session *s, int level, int32_t *result, float *error
  *                         instead of correctly rounded division.  Default 1;
  *                         0 reproduces the CPU oracle bit for bit.
  *   KF_OPT_SKIP_EMPTY     raycast skips samples in 8^3 bricks that hold no
- *                         negative value (exact: same hits).  Default 1.
+ *                         negative value (exact: same hits); the intensity
+ *                         projection skips samples in bricks of (0, 0)
+ *                         (exact: same image).  Default 1.
  *   KF_OPT_GRAPH          kf_frame_* replays a captured hipGraph.  Default 1.
  *   KF_OPT_BRICK_REBUILD_PERIOD  frames between exact brick-flag rebuilds
  *                         (flags are otherwise only set): the raycast's
