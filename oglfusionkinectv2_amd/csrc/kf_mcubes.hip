@@ -213,40 +213,75 @@ __global__ __launch_bounds__(64 * kMcWaves) void k_mc_emit(McArgs a)
         ci[j] = c > 0 ? c : 0;
         n += a.ntri[ci[j]];
     }
-    unsigned long long t = (unsigned long long)a.offset[m] + wave_exclusive_u32(n, lane);
+    // The wavefront's triangles are offset[m] + 0 .. T - 1, lane l's at
+    // excl(l) .. incl(l) - 1, in cube order and then triangle order.  Each
+    // pass gives every lane one of them: the owner lane (binary search over
+    // the inclusive prefixes), the cube (the owner's per-cube counts, 3 bits
+    // each: at most 5 triangles per cube) and the triangle within it; the
+    // cube's eight corners are read again (cached: this wavefront has just
+    // read them) and the triangle's three vertices written.
+    const uint32_t excl = wave_exclusive_u32(n, lane), incl = excl + n;
+    const uint32_t T = __shfl(incl, 63, 64);
+    const unsigned long long base = a.offset[m];
+    uint32_t cnt3 = 0, ci_lo = 0, ci_hi = 0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        if (!ci[j]) continue;
-        const int ox = j & 1, oz = (j >> 1) & 1, oy = j >> 2;
-        const int cx = x0 + ox, cy = y0 + oy, cz = z0 + oz;
-        // the cube's eight corner values in kCorner order, from the block
+        cnt3 |= (uint32_t)a.ntri[ci[j]] << (3 * j);
+        if (j < 4) ci_lo |= (uint32_t)ci[j] << (8 * j);
+        else ci_hi |= (uint32_t)ci[j] << (8 * (j - 4));
+    }
+    for (uint32_t p = 0; p < T; p += 64) {  // uniform: every lane takes part in the shuffles
+        const uint32_t i = p + (uint32_t)lane;
+        int o = 0;  // lanes with incl <= i
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const uint32_t v = __shfl(incl, o + step - 1, 64);
+            if (v <= i) o += step;
+        }
+        o = min(o, 63);  // lanes with i >= T (they write nothing)
+        const uint32_t oe = __shfl(excl, o, 64), oc = __shfl(cnt3, o, 64);
+        const uint32_t olo = __shfl(ci_lo, o, 64), ohi = __shfl(ci_hi, o, 64);
+        if (i >= T) continue;
+        uint32_t r = i - oe;
+        int j = -1;
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const uint32_t c = (oc >> (3 * jj)) & 7u;
+            if (j < 0) {
+                if (r < c) j = jj;
+                else r -= c;
+            }
+        }
+        const int cube = j < 4 ? (int)((olo >> (8 * j)) & 0xffu) : (int)((ohi >> (8 * (j - 4))) & 0xffu);
+        const int k = (int)r;
+        const unsigned long long t = base + i;
+        if (t >= a.cap) continue;
+        int ox0, oy0, oz0;
+        morton_decode((uint32_t)(8 * o), 3, ox0, oy0, oz0);
+        const int cx = 8 * bx + ox0 + (j & 1), cy = 8 * by + oy0 + (j >> 2), cz = 8 * bz + oz0 + ((j >> 1) & 1);
         float g[8];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) g[c] = F[f27(ox + corner_x(c), oy + corner_y(c), oz + corner_z(c))];
-        const uint4 row = *(const uint4 *)(a.tri + 16 * ci[j]);
+        for (int c = 0; c < 8; ++c) g[c] = mc_field<S>(v, a, cx + corner_x(c), cy + corner_y(c), cz + corner_z(c));
+        const uint4 row = *(const uint4 *)(a.tri + 16 * cube);
         const uint32_t rw[4] = {row.x, row.y, row.z, row.w};
-        const int nt = a.ntri[ci[j]];
-        for (int k = 0; k < nt; ++k, ++t) {
-            if (t >= a.cap) continue;
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                // traverseHistoPyramids.cs:196-250
-                const int b = 3 * k + q;  // byte b of the triangle row
-                const uint32_t w = b < 4 ? rw[0] : (b < 8 ? rw[1] : (b < 12 ? rw[2] : rw[3]));
-                const int e = (int)((w >> (8 * (b & 3))) & 0xffu);
-                const int c0 = edge_lo(e), c1 = edge_hi(e);
-                const int p0[3] = {cx + corner_x(c0), cy + corner_y(c0), cz + corner_z(c0)};
-                const int p1[3] = {cx + corner_x(c1), cy + corner_y(c1), cz + corner_z(c1)};
-                const float v0 = pick8(g, c0);
-                float diff = (a.iso - v0) / (pick8(g, c1) - v0);
-                if (diff > 1.0f || diff < 0.0f) diff = 0.5f;
-                float4 o;
-                o.x = (float)p0[0] * (1.0f - diff) + (float)p1[0] * diff;
-                o.y = (float)p0[1] * (1.0f - diff) + (float)p1[1] * diff;
-                o.z = (float)p0[2] * (1.0f - diff) + (float)p1[2] * diff;
-                o.w = 0.0f;
-                a.verts[3 * t + q] = o;
-            }
+        for (int q = 0; q < 3; ++q) {
+            // traverseHistoPyramids.cs:196-250
+            const int b = 3 * k + q;  // byte b of the triangle row
+            const uint32_t w = b < 4 ? rw[0] : (b < 8 ? rw[1] : (b < 12 ? rw[2] : rw[3]));
+            const int e = (int)((w >> (8 * (b & 3))) & 0xffu);
+            const int c0 = edge_lo(e), c1 = edge_hi(e);
+            const int p0[3] = {cx + corner_x(c0), cy + corner_y(c0), cz + corner_z(c0)};
+            const int p1[3] = {cx + corner_x(c1), cy + corner_y(c1), cz + corner_z(c1)};
+            const float v0 = pick8(g, c0);
+            float diff = (a.iso - v0) / (pick8(g, c1) - v0);
+            if (diff > 1.0f || diff < 0.0f) diff = 0.5f;
+            float4 out;
+            out.x = (float)p0[0] * (1.0f - diff) + (float)p1[0] * diff;
+            out.y = (float)p0[1] * (1.0f - diff) + (float)p1[1] * diff;
+            out.z = (float)p0[2] * (1.0f - diff) + (float)p1[2] * diff;
+            out.w = 0.0f;
+            a.verts[3 * t + q] = out;
         }
     }
 }
